@@ -1,0 +1,253 @@
+"""cudebye_amd: MI355X (gfx950) Debye scattering engine with the method and file surface of
+cuDebye v1.5 (Martin-Rudolph/cuDebye).
+
+This module is a thin ctypes front end of libdebye.so (C ABI in include/debye.h).  Every
+computation runs in the HIP kernels of that library; if the library is missing or has
+not been built, importing the library raises - there is no CPU fallback.
+
+    import cudebye_amd as cd
+    with cd.Context(0) as ctx:
+        m = ctx.model(x, y, z, group_n)
+        H = ctx.histogram(m, rstep=1e-3, n_bins=cd.channels(rmax, 1e-3))
+        I = ctx.intensity(m, params, rstep, n_bins, q, first_n=8)
+    cd.run_deb("model.deb")          # the reference program, writes .debI / .debx files
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdebye.so")
+CLI_PATH = os.path.join(_HERE, "bin", "debye")
+
+DBX_OK = 0
+ERRORS = {-1: "DBX_ERR_ARG", -2: "DBX_ERR_HIP", -3: "DBX_ERR_IO", -4: "DBX_ERR_PARSE",
+          -5: "DBX_ERR_RANGE", -6: "DBX_ERR_NOMEM", -7: "DBX_ERR_BUDGET", -8: "DBX_ERR_STATE"}
+DBX_ERR_RANGE = -5
+DBX_RUN_CRLF = 1
+DBX_RUN_QUIET = 2
+
+_lib = None
+
+
+class DebyeError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class GroupParams(C.Structure):
+    """dbx_group_params: occ, Beq, a1..a5, b1..b5, c of one subgroup."""
+    _fields_ = [("occ", C.c_double), ("beq", C.c_double), ("a", C.c_double * 5),
+                ("b", C.c_double * 5), ("c", C.c_double)]
+
+    @classmethod
+    def make(cls, occ, beq, a, b, c):
+        g = cls()
+        g.occ, g.beq, g.c = float(occ), float(beq), float(c)
+        for i in range(5):
+            g.a[i] = float(a[i]) if i < len(a) else 0.0
+            g.b[i] = float(b[i]) if i < len(b) else 0.0
+        return g
+
+
+def lib():
+    """The loaded libdebye.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m cudebye_amd._build` "
+                              "(or __graft_entry__.build()); there is no CPU fallback")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, i64, u64, f64, f32, sz = (C.c_void_p, C.c_int, C.c_int64, C.c_uint64, C.c_double,
+                                           C.c_float, C.c_size_t)
+        P = C.POINTER
+        sig = {
+            "dbx_open": (i32, [i32, sz, P(vp)]),
+            "dbx_close": (i32, [vp]),
+            "dbx_last_error": (C.c_char_p, [vp]),
+            "dbx_stream": (vp, [vp]),
+            "dbx_q_from_2theta": (i32, [f64, f64, f64, f64, vp, vp, i32]),
+            "dbx_channels": (i64, [f64, f64]),
+            "dbx_model_create": (i32, [vp, vp, vp, vp, vp, i32, P(vp)]),
+            "dbx_model_destroy": (i32, [vp, vp]),
+            "dbx_model_info": (i32, [vp, P(i64), P(i32), P(i32)]),
+            "dbx_histogram": (i32, [vp, vp, f64, i64, vp, i32, P(u64)]),
+            "dbx_prefactor": (i32, [vp, vp, vp, vp, i32, vp]),
+            "dbx_debye": (i32, [vp, vp, i32, i32, i64, f64, i32, vp, i32, vp, vp, vp]),
+            "dbx_intensity": (i32, [vp, vp, vp, f64, i64, i32, vp, i32, vp, vp]),
+            "dbx_run_deb": (i32, [C.c_char_p, i32]),
+            "dbx_selftest_sqrt": (i32, [C.c_uint32, C.c_uint32, P(u64)]),
+            "dbx_selftest_bins": (i32, [vp, vp, i32, f32, vp]),
+            "dbx_last_pass1_stats": (i32, [vp, P(i64), P(u64)]),
+            "dbx_last_kernel_times": (i32, [vp, P(f64), P(f64)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+EXPORTS = ("dbx_open", "dbx_close", "dbx_last_error", "dbx_stream", "dbx_q_from_2theta",
+           "dbx_channels", "dbx_model_create", "dbx_model_destroy", "dbx_model_info",
+           "dbx_histogram", "dbx_prefactor", "dbx_debye", "dbx_intensity", "dbx_run_deb",
+           "dbx_selftest_sqrt", "dbx_selftest_bins", "dbx_last_pass1_stats",
+           "dbx_last_kernel_times")
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def q_from_2theta(lam, tt_min, tt_max, tt_step):
+    """(2theta, K) grids of cuDebye-v15.cu:690-698."""
+    L = lib()
+    n = L.dbx_q_from_2theta(lam, tt_min, tt_max, tt_step, None, None, 0)
+    if n < 1:
+        raise DebyeError(n, "bad 2theta grid")
+    tt, q = np.zeros(n), np.zeros(n)
+    L.dbx_q_from_2theta(lam, tt_min, tt_max, tt_step, _p(tt), _p(q), n)
+    return tt, q
+
+
+def channels(rmax, rstep) -> int:
+    n = int(lib().dbx_channels(rmax, rstep))
+    if n < 0:
+        raise DebyeError(n, "bad distance grid")
+    return n
+
+
+def run_deb(path, crlf=False, quiet=True) -> int:
+    """Runs a .deb file like `cuDebye.exe path` (writes path+'I' and/or path+'x<r>x<c>')."""
+    flags = (DBX_RUN_CRLF if crlf else 0) | (DBX_RUN_QUIET if quiet else 0)
+    return int(lib().dbx_run_deb(os.fspath(path).encode(), flags))
+
+
+def selftest_sqrt(lo, hi):
+    """(not correctly rounded, differs from sqrtf) counts over float bit patterns [lo, hi)."""
+    n = (C.c_uint64 * 2)()
+    rc = lib().dbx_selftest_sqrt(lo, hi, n)
+    if rc != DBX_OK:
+        raise DebyeError(rc, "selftest_sqrt")
+    return int(n[0]), int(n[1])
+
+
+def selftest_bins(a, b, rp):
+    a = np.ascontiguousarray(a, np.float32).reshape(-1, 3)
+    b = np.ascontiguousarray(b, np.float32).reshape(-1, 3)
+    out = np.zeros(len(a), np.int32)
+    rc = lib().dbx_selftest_bins(_p(a), _p(b), len(a), float(np.float32(rp)), _p(out))
+    if rc != DBX_OK:
+        raise DebyeError(rc, "selftest_bins")
+    return out
+
+
+class Model:
+    def __init__(self, ctx, handle, n_atoms, n_groups):
+        self.ctx, self.handle, self.n_atoms, self.n_groups = ctx, handle, n_atoms, n_groups
+
+    @property
+    def n_pairs(self):
+        return self.n_groups * (self.n_groups + 1) // 2
+
+    def close(self):
+        if self.handle:
+            lib().dbx_model_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """A device context (dbx_open).  Use as a context manager or call close()."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = C.c_void_p()
+        rc = lib().dbx_open(device, hbm_budget, C.byref(h))
+        if rc != DBX_OK:
+            raise DebyeError(rc, f"cannot open device {device}")
+        self.handle = h
+
+    def _check(self, rc, what, allow=()):
+        if rc != DBX_OK and rc not in allow:
+            raise DebyeError(rc, f"{what}: {lib().dbx_last_error(self.handle).decode()}")
+        return rc
+
+    def close(self):
+        if self.handle:
+            lib().dbx_close(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def model(self, x, y, z, group_n) -> Model:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        z = np.ascontiguousarray(z, np.float32)
+        gn = np.ascontiguousarray(group_n, np.int64)
+        assert len(x) == len(y) == len(z) == int(gn.sum())
+        h = C.c_void_p()
+        self._check(lib().dbx_model_create(self.handle, _p(x), _p(y), _p(z), _p(gn), len(gn),
+                                           C.byref(h)), "model_create")
+        return Model(self, h, int(gn.sum()), len(gn))
+
+    def histogram(self, m: Model, rstep, n_bins, return_oor=False):
+        """uint64[P, n_bins] partial histograms (reference pair order)."""
+        out = np.zeros((m.n_pairs, n_bins), np.uint64)
+        oor = C.c_uint64(0)
+        self._check(lib().dbx_histogram(self.handle, m.handle, rstep, n_bins, _p(out), 0,
+                                        C.byref(oor)), "histogram",
+                    allow=(DBX_ERR_RANGE,) if return_oor else ())
+        return (out, int(oor.value)) if return_oor else out
+
+    def pass1_stats(self):
+        nj, miss = C.c_int64(0), C.c_uint64(0)
+        lib().dbx_last_pass1_stats(self.handle, C.byref(nj), C.byref(miss))
+        return int(nj.value), int(miss.value)
+
+    def kernel_times(self):
+        """(pass-1 histogram kernel ms, pass-2 structure ms) of the last calls (HIP events)."""
+        a, b = C.c_double(0), C.c_double(0)
+        self._check(lib().dbx_last_kernel_times(self.handle, C.byref(a), C.byref(b)), "kernel_times")
+        return a.value, b.value
+
+    def prefactor(self, ga: GroupParams, gb: GroupParams, q):
+        q = np.ascontiguousarray(q, np.float64)
+        ff = np.zeros(len(q))
+        self._check(lib().dbx_prefactor(self.handle, C.byref(ga), C.byref(gb), _p(q), len(q),
+                                        _p(ff)), "prefactor")
+        return ff
+
+    def debye(self, hist, rstep, q, ffoobb, first_n=8, with_S=False):
+        hist = np.ascontiguousarray(hist, np.uint64)
+        P, nb = hist.shape
+        q = np.ascontiguousarray(q, np.float64)
+        ff = np.ascontiguousarray(ffoobb, np.float64).reshape(P, len(q))
+        I = np.zeros(len(q))
+        S = np.zeros((P, len(q))) if with_S else None
+        self._check(lib().dbx_debye(self.handle, _p(hist), 0, P, nb, rstep, first_n, _p(q), len(q),
+                                    _p(ff), _p(I), None if S is None else _p(S)), "debye")
+        return (I, S) if with_S else I
+
+    def intensity(self, m: Model, params, rstep, n_bins, q, first_n=8, with_hist=False):
+        arr = (GroupParams * m.n_groups)(*params)
+        q = np.ascontiguousarray(q, np.float64)
+        I = np.zeros(len(q))
+        H = np.zeros((m.n_pairs, n_bins), np.uint64) if with_hist else None
+        self._check(lib().dbx_intensity(self.handle, m.handle, arr, rstep, n_bins, first_n, _p(q),
+                                        len(q), _p(I), None if H is None else _p(H)), "intensity")
+        return (I, H) if with_hist else I

@@ -1,0 +1,448 @@
+// C ABI of libdebye (include/debye.h): context, models, pass 1, prefactor, pass 2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "internal.h"
+
+namespace dbx {
+
+void *scratch(dbx_ctx *ctx, int slot, size_t bytes)
+{
+    if (bytes == 0) bytes = 256;
+    if (ctx->slot_bytes[slot] < bytes) {
+        if (ctx->slot_ptr[slot]) {
+            DBX_HIP(hipStreamSynchronize(ctx->stream));
+            DBX_HIP(hipFree(ctx->slot_ptr[slot]));
+            ctx->slot_ptr[slot] = nullptr;
+            ctx->slot_bytes[slot] = 0;
+        }
+        size_t want = bytes + bytes / 4;
+        void *p = nullptr;
+        DBX_HIP(hipMalloc(&p, want));
+        ctx->slot_ptr[slot] = p;
+        ctx->slot_bytes[slot] = want;
+    }
+    return ctx->slot_ptr[slot];
+}
+
+template <class F>
+static int guarded(dbx_ctx *ctx, F &&f)
+{
+    if (!ctx) return DBX_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    try {
+        DBX_HIP(hipSetDevice(ctx->device));
+        f();
+        ctx->err.clear();
+        return DBX_OK;
+    } catch (const Error &e) {
+        ctx->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        ctx->err = "host allocation failed";
+        return DBX_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return DBX_ERR_ARG;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// spatial tiling of a model (host)
+// ---------------------------------------------------------------------------------
+namespace {
+
+struct Pt {
+    float x, y, z;
+    int64_t id;
+};
+
+// Recursive median bisection of pts[lo, hi) along the longest bounding-box axis into
+// tiles of at most kTileAtoms atoms; the left part always receives a whole number of
+// full tiles, so only the last tile of a subgroup can be short.
+void kd_split(std::vector<Pt> &pts, int64_t lo, int64_t hi, std::vector<std::pair<int64_t, int64_t>> &out)
+{
+    struct Range { int64_t lo, hi; };
+    std::vector<Range> stack{{lo, hi}};
+    std::vector<Range> leaves;
+    while (!stack.empty()) {
+        Range r = stack.back();
+        stack.pop_back();
+        const int64_t n = r.hi - r.lo;
+        if (n <= kTileAtoms) {
+            leaves.push_back(r);
+            continue;
+        }
+        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = r.lo; i < r.hi; i++) {
+            const float v[3] = {pts[i].x, pts[i].y, pts[i].z};
+            for (int a = 0; a < 3; a++) {
+                mn[a] = std::min(mn[a], v[a]);
+                mx[a] = std::max(mx[a], v[a]);
+            }
+        }
+        int ax = 0;
+        for (int a = 1; a < 3; a++)
+            if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
+        const int64_t nt = (n + kTileAtoms - 1) / kTileAtoms;
+        const int64_t nl = (nt / 2) * kTileAtoms;
+        auto key = [ax](const Pt &p) { return ax == 0 ? p.x : (ax == 1 ? p.y : p.z); };
+        std::nth_element(pts.begin() + r.lo, pts.begin() + r.lo + nl, pts.begin() + r.hi,
+                         [&](const Pt &a, const Pt &b) {
+                             float ka = key(a), kb = key(b);
+                             return ka < kb || (ka == kb && a.id < b.id);
+                         });
+        // depth first, left before right
+        stack.push_back({r.lo + nl, r.hi});
+        stack.push_back({r.lo, r.lo + nl});
+    }
+    for (auto &l : leaves) out.push_back({l.lo, l.hi});
+}
+
+}  // namespace
+
+static void build_model(dbx_ctx *ctx, const float *x, const float *y, const float *z,
+                        const int64_t *group_n, int S, ModelDev &m)
+{
+    DBX_REQUIRE(S > 0 && S <= 64, DBX_ERR_ARG, "n_groups must be in [1, 64]");
+    int64_t N = 0;
+    for (int g = 0; g < S; g++) {
+        DBX_REQUIRE(group_n[g] >= 0, DBX_ERR_ARG, "negative subgroup size");
+        N += group_n[g];
+    }
+    DBX_REQUIRE(N < ((int64_t)1 << 31) - kTileAtoms * 64, DBX_ERR_ARG, "too many atoms");
+    for (int64_t i = 0; i < N; i++)
+        DBX_REQUIRE(std::isfinite(x[i]) && std::isfinite(y[i]) && std::isfinite(z[i]), DBX_ERR_ARG,
+                    "non-finite coordinate at atom " + std::to_string(i));
+    m.n_groups = S;
+    m.n_atoms = N;
+    m.group_n.assign(group_n, group_n + S);
+    m.group_tile0.resize(S);
+    m.group_tile1.resize(S);
+    m.bbox.assign(6 * S, 0.f);
+    std::vector<float4> pos_h;
+    pos_h.reserve(N);
+    m.tiles_h.clear();
+    int64_t start = 0;
+    for (int g = 0; g < S; g++) {
+        const int64_t n = group_n[g];
+        std::vector<Pt> pts(n);
+        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = 0; i < n; i++) {
+            pts[i] = {x[start + i], y[start + i], z[start + i], start + i};
+            const float v[3] = {pts[i].x, pts[i].y, pts[i].z};
+            for (int a = 0; a < 3; a++) {
+                mn[a] = std::min(mn[a], v[a]);
+                mx[a] = std::max(mx[a], v[a]);
+            }
+        }
+        for (int a = 0; a < 3; a++) {
+            m.bbox[6 * g + a] = n ? mn[a] : 0.f;
+            m.bbox[6 * g + 3 + a] = n ? mx[a] : 0.f;
+        }
+        std::vector<std::pair<int64_t, int64_t>> leaves;
+        if (n) kd_split(pts, 0, n, leaves);
+        m.group_tile0[g] = (int32_t)m.tiles_h.size();
+        for (auto &lf : leaves) {
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int64_t i = lf.first; i < lf.second; i++) {
+                const double v[3] = {pts[i].x, pts[i].y, pts[i].z};
+                for (int a = 0; a < 3; a++) {
+                    lo[a] = std::min(lo[a], v[a]);
+                    hi[a] = std::max(hi[a], v[a]);
+                }
+            }
+            Tile t;
+            t.cx = (float)(0.5 * (lo[0] + hi[0]));
+            t.cy = (float)(0.5 * (lo[1] + hi[1]));
+            t.cz = (float)(0.5 * (lo[2] + hi[2]));
+            double r2 = 0;
+            for (int64_t i = lf.first; i < lf.second; i++) {
+                const double dx = pts[i].x - (double)t.cx, dy = pts[i].y - (double)t.cy,
+                             dz = pts[i].z - (double)t.cz;
+                r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+            }
+            t.rad = (float)(std::sqrt(r2) * (1 + 1e-6) + 1e-5);
+            t.start = (int32_t)pos_h.size();
+            t.count = (int32_t)(lf.second - lf.first);
+            t.group = g;
+            t.pad = 0;
+            for (int64_t i = lf.first; i < lf.second; i++)
+                pos_h.push_back(make_float4(pts[i].x, pts[i].y, pts[i].z, 0.f));
+            m.tiles_h.push_back(t);
+        }
+        m.group_tile1[g] = (int32_t)m.tiles_h.size();
+        start += n;
+    }
+    const size_t pb = sizeof(float4) * std::max<size_t>(1, pos_h.size());
+    const size_t tb = sizeof(Tile) * std::max<size_t>(1, m.tiles_h.size());
+    DBX_HIP(hipMalloc(&m.pos, pb));
+    DBX_HIP(hipMalloc(&m.tiles, tb));
+    if (!pos_h.empty())
+        DBX_HIP(hipMemcpyAsync(m.pos, pos_h.data(), sizeof(float4) * pos_h.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (!m.tiles_h.empty())
+        DBX_HIP(hipMemcpyAsync(m.tiles, m.tiles_h.data(), sizeof(Tile) * m.tiles_h.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    DBX_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+static void free_model(ModelDev &m)
+{
+    if (m.pos) (void)hipFree(m.pos);
+    if (m.tiles) (void)hipFree(m.tiles);
+    m.pos = nullptr;
+    m.tiles = nullptr;
+}
+
+}  // namespace dbx
+
+using namespace dbx;
+
+extern "C" {
+
+int dbx_open(int device, size_t hbm_budget, dbx_ctx **out)
+{
+    if (!out) return DBX_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return DBX_ERR_HIP;
+    if (hipSetDevice(device) != hipSuccess) return DBX_ERR_HIP;
+    dbx_ctx *c = new dbx_ctx;
+    c->device = device;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        delete c;
+        return DBX_ERR_HIP;
+    }
+    c->budget = hbm_budget ? hbm_budget : (size_t)(0.9 * (double)fr);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&c->counters, 64) != hipSuccess) {
+        delete c;
+        return DBX_ERR_HIP;
+    }
+    for (int i = 0; i < 4; i++)
+        if (hipEventCreate(&c->ev[i]) != hipSuccess) {
+            delete c;
+            return DBX_ERR_HIP;
+        }
+    *out = c;
+    return DBX_OK;
+}
+
+int dbx_close(dbx_ctx *ctx)
+{
+    if (!ctx) return DBX_ERR_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (int s = 0; s < 8; s++)
+        if (ctx->slot_ptr[s]) (void)hipFree(ctx->slot_ptr[s]);
+    if (ctx->counters) (void)hipFree(ctx->counters);
+    for (int i = 0; i < 4; i++)
+        if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return DBX_OK;
+}
+
+const char *dbx_last_error(const dbx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void *dbx_stream(dbx_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int dbx_q_from_2theta(double lambda, double tt_min, double tt_max, double tt_step, double *tt,
+                      double *q, int cap)
+{
+    const double PI = 3.14159265358979323846;  // cuDebye-v15.cu:46
+    if (!(tt_step > 0) || !(lambda > 0) || !std::isfinite(tt_min) || !std::isfinite(tt_max))
+        return DBX_ERR_ARG;
+    const double span = (tt_max - tt_min) / tt_step;
+    if (!(span >= 0) || span > 2.0e9) return DBX_ERR_ARG;
+    const int n = (int)span + 1;  // cuDebye-v15.cu:690
+    if (tt && q && cap >= n) {
+        for (int i = 0; i < n; i++) {
+            tt[i] = tt_min + (double)i * tt_step;                   // :695
+            q[i] = 4 * PI * std::sin(tt[i] / 2 * PI / 180) / lambda;  // :696
+        }
+    }
+    return n;
+}
+
+int64_t dbx_channels(double rmax, double rstep)
+{
+    if (!(rstep > 0) || !(rmax >= 0)) return DBX_ERR_ARG;
+    const double c = rmax / rstep;
+    if (c > 2.0e9) return DBX_ERR_ARG;
+    return (int64_t)(int)c + 2;  // cuDebye-v15.cu:734
+}
+
+int dbx_model_create(dbx_ctx *ctx, const float *x, const float *y, const float *z,
+                     const int64_t *group_n, int n_groups, dbx_model **out)
+{
+    if (!out || !group_n || (!x && n_groups > 0)) return DBX_ERR_ARG;
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        dbx_model *m = new dbx_model;
+        try {
+            build_model(ctx, x, y, z, group_n, n_groups, m->d);
+        } catch (...) {
+            free_model(m->d);
+            delete m;
+            throw;
+        }
+        *out = m;
+    });
+}
+
+int dbx_model_destroy(dbx_ctx *ctx, dbx_model *m)
+{
+    if (!m) return DBX_ERR_ARG;
+    return guarded(ctx, [&] {
+        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        free_model(m->d);
+        delete m;
+    });
+}
+
+int dbx_last_pass1_stats(const dbx_ctx *ctx, int64_t *n_jobs, uint64_t *window_misses)
+{
+    if (!ctx) return DBX_ERR_ARG;
+    if (n_jobs) *n_jobs = ctx->last_p1_jobs;
+    if (window_misses) *window_misses = ctx->last_p1_misses;
+    return DBX_OK;
+}
+
+int dbx_last_kernel_times(dbx_ctx *ctx, double *pass1_ms, double *pass2_ms)
+{
+    if (!ctx) return DBX_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (pass1_ms) *pass1_ms = ctx->last_p1_kernel_ms;
+        if (pass2_ms) {
+            float t = 0.f;
+            if (ctx->p2_timed) {
+                DBX_HIP(hipEventSynchronize(ctx->ev[3]));
+                DBX_HIP(hipEventElapsedTime(&t, ctx->ev[2], ctx->ev[3]));
+            }
+            *pass2_ms = t;
+        }
+    });
+}
+
+int dbx_model_info(const dbx_model *m, int64_t *n_atoms, int *n_groups, int *n_pairs)
+{
+    if (!m) return DBX_ERR_ARG;
+    if (n_atoms) *n_atoms = m->d.n_atoms;
+    if (n_groups) *n_groups = m->d.n_groups;
+    if (n_pairs) *n_pairs = m->d.n_groups * (m->d.n_groups + 1) / 2;
+    return DBX_OK;
+}
+
+int dbx_histogram(dbx_ctx *ctx, const dbx_model *m, double rstep, int64_t n_bins, uint64_t *out,
+                  int out_is_device, uint64_t *n_out_of_range)
+{
+    if (!m || !out) return DBX_ERR_ARG;
+    int rc = DBX_OK;
+    int grc = guarded(ctx, [&] {
+        const int P = m->d.n_groups * (m->d.n_groups + 1) / 2;
+        const size_t bytes = sizeof(uint64_t) * (size_t)P * n_bins;
+        uint64_t *d = out_is_device ? out : (uint64_t *)scratch(ctx, kSlotHist, bytes);
+        P1Stats st = pass1_histograms(ctx, m->d, rstep, n_bins, d);
+        if (!out_is_device) {
+            DBX_HIP(hipMemcpyAsync(out, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+            DBX_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        if (n_out_of_range) *n_out_of_range = st.out_of_range;
+        if (st.out_of_range) {
+            rc = DBX_ERR_RANGE;
+            ctx->err = std::to_string(st.out_of_range) +
+                       " pair distances fall at or beyond the last bin (rmax too small)";
+        }
+    });
+    if (grc != DBX_OK) return grc;
+    return rc;
+}
+
+int dbx_prefactor(dbx_ctx *ctx, const dbx_group_params *ga, const dbx_group_params *gb,
+                  const double *q, int nq, double *ffoobb)
+{
+    if (!ga || !gb || !q || !ffoobb || nq <= 0) return DBX_ERR_ARG;
+    return guarded(ctx, [&] {
+        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * 2 * (size_t)nq);
+        DBX_HIP(hipMemcpyAsync(d, q, sizeof(double) * nq, hipMemcpyHostToDevice, ctx->stream));
+        prefactor_dev(ctx, *ga, *gb, d, nq, d + nq);
+        DBX_HIP(hipMemcpyAsync(ffoobb, d + nq, sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream));
+        DBX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int64_t n_bins,
+              double rstep, int first_n, const double *q, int nq, const double *ffoobb,
+              double *I_out, double *S_out)
+{
+    if (!hist || !q || !ffoobb || !I_out || P <= 0 || nq <= 0 || n_bins <= 0) return DBX_ERR_ARG;
+    return guarded(ctx, [&] {
+        const size_t hb = sizeof(uint64_t) * (size_t)P * n_bins;
+        const uint64_t *dh = hist;
+        if (!hist_is_device) {
+            uint64_t *t = (uint64_t *)scratch(ctx, kSlotHist, hb);
+            DBX_HIP(hipMemcpyAsync(t, hist, hb, hipMemcpyHostToDevice, ctx->stream));
+            dh = t;
+        }
+        const size_t nqd = (size_t)nq;
+        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 2)));
+        double *dq = d, *dff = d + nqd, *dS = dff + P * nqd, *dI = dS + P * nqd;
+        DBX_HIP(hipMemcpyAsync(dq, q, sizeof(double) * nqd, hipMemcpyHostToDevice, ctx->stream));
+        DBX_HIP(hipMemcpyAsync(dff, ffoobb, sizeof(double) * P * nqd, hipMemcpyHostToDevice,
+                               ctx->stream));
+        debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS);
+        combine_intensity(ctx, dff, dS, P, nq, dI);
+        DBX_HIP(hipMemcpyAsync(I_out, dI, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
+        if (S_out)
+            DBX_HIP(hipMemcpyAsync(S_out, dS, sizeof(double) * P * nqd, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        DBX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int dbx_intensity(dbx_ctx *ctx, const dbx_model *m, const dbx_group_params *params, double rstep,
+                  int64_t n_bins, int first_n, const double *q, int nq, double *I_out,
+                  uint64_t *hist_out)
+{
+    if (!m || !params || !q || !I_out || nq <= 0) return DBX_ERR_ARG;
+    int rc = DBX_OK;
+    int grc = guarded(ctx, [&] {
+        const int S = m->d.n_groups;
+        const int P = S * (S + 1) / 2;
+        const size_t hb = sizeof(uint64_t) * (size_t)P * n_bins;
+        uint64_t *dh = (uint64_t *)scratch(ctx, kSlotHist, hb);
+        P1Stats st = pass1_histograms(ctx, m->d, rstep, n_bins, dh);
+        if (st.out_of_range) {
+            rc = DBX_ERR_RANGE;
+            ctx->err = std::to_string(st.out_of_range) +
+                       " pair distances fall at or beyond the last bin (rmax too small)";
+        }
+        const size_t nqd = (size_t)nq;
+        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 2)));
+        double *dq = d, *dff = d + nqd, *dS = dff + P * nqd, *dI = dS + P * nqd;
+        DBX_HIP(hipMemcpyAsync(dq, q, sizeof(double) * nqd, hipMemcpyHostToDevice, ctx->stream));
+        int p = 0;
+        for (int a = 0; a < S; a++)
+            for (int b = a; b < S; b++, p++) prefactor_dev(ctx, params[a], params[b], dq, nq, dff + p * nqd);
+        debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS);
+        combine_intensity(ctx, dff, dS, P, nq, dI);
+        DBX_HIP(hipMemcpyAsync(I_out, dI, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
+        if (hist_out) DBX_HIP(hipMemcpyAsync(hist_out, dh, hb, hipMemcpyDeviceToHost, ctx->stream));
+        DBX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+    if (grc != DBX_OK) return grc;
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,26 @@
+// `debye file.deb [--crlf] [--quiet]`: the command-line program of cuDebye
+// (cuDebye.exe <file.deb>, cuDebye-v15.cu:599-614) on top of libdebye.
+#include <cstdio>
+#include <cstring>
+
+#include "../../include/debye.h"
+
+int main(int argc, char **argv)
+{
+    const char *path = nullptr;
+    int flags = 0;
+    for (int i = 1; i < argc; i++) {
+        if (!std::strcmp(argv[i], "--crlf"))
+            flags |= DBX_RUN_CRLF;
+        else if (!std::strcmp(argv[i], "--quiet"))
+            flags |= DBX_RUN_QUIET;
+        else
+            path = argv[i];
+    }
+    if (!path) {
+        std::fprintf(stderr, "usage: %s file.deb [--crlf] [--quiet]\n", argv[0]);
+        return 2;
+    }
+    const int rc = dbx_run_deb(path, flags);
+    return rc == DBX_OK ? 0 : 1;
+}

@@ -1,0 +1,282 @@
+// The .deb driver: the reference program `cuDebye.exe file.deb` (cuDebye-v15.cu:599-1003)
+// on top of the C ABI.  Parses the .deb (cuDebye-v15.cu:624-833), runs every subgroup pair
+// on the device, prints the reference's sanity checks (cuDebye-v15.cu:360-410) and writes
+// <path>x<r>x<c> (cuDebye-v15.cu:413-430) and <path>I (cuDebye-v15.cu:973-998).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace {
+
+struct DebFile {
+    int device = 0, block = 0, grid = 0, hist = 0, S = 0;
+    double lambda = 0, tt_min = 0, tt_max = 0, tt_step = 0, rmax = 0, rstep = 0;
+    std::vector<int64_t> n;
+    std::vector<std::string> type;
+    std::vector<dbx_group_params> gp;
+    std::vector<float> x, y, z;
+};
+
+bool blank(const char *s)
+{
+    for (; *s; s++)
+        if (!std::strchr(" \t\r\v\f", *s)) return false;
+    return true;
+}
+
+// line reader over a whole file; strips "\n" and a trailing "\r"
+struct Lines {
+    std::string buf;
+    size_t pos = 0;
+    bool next(std::string &out)
+    {
+        if (pos >= buf.size()) return false;
+        size_t e = buf.find('\n', pos);
+        if (e == std::string::npos) e = buf.size();
+        out.assign(buf, pos, e - pos);
+        if (!out.empty() && out.back() == '\r') out.pop_back();
+        pos = e + 1;
+        return true;
+    }
+};
+
+int parse_deb(const char *path, DebFile &d, std::string &err)
+{
+    FILE *f = std::fopen(path, "rb");
+    if (!f) {
+        err = std::string(">>") + path + "<< does not exist";  // cuDebye-v15.cu:620
+        return DBX_ERR_IO;
+    }
+    Lines L;
+    char tmp[1 << 16];
+    size_t got;
+    while ((got = std::fread(tmp, 1, sizeof(tmp), f)) > 0) L.buf.append(tmp, got);
+    std::fclose(f);
+    std::string ln;
+    // header lines 1-4 (cuDebye-v15.cu:644, 689, 730, 761)
+    if (!L.next(ln) || std::sscanf(ln.c_str(), "%d %d %d", &d.device, &d.block, &d.grid) != 3) {
+        err = "header line 1 must be 'device blockDimXY gridDimXY'";
+        return DBX_ERR_PARSE;
+    }
+    if (!L.next(ln) ||
+        std::sscanf(ln.c_str(), "%lf %lf %lf %lf", &d.lambda, &d.tt_min, &d.tt_max, &d.tt_step) != 4) {
+        err = "header line 2 must be 'lambda 2theta_min 2theta_max 2theta_step'";
+        return DBX_ERR_PARSE;
+    }
+    if (!L.next(ln) || std::sscanf(ln.c_str(), "%lf %lf %d", &d.rmax, &d.rstep, &d.hist) != 3) {
+        err = "header line 3 must be 'rmax rstep hist'";
+        return DBX_ERR_PARSE;
+    }
+    if (d.hist != 1 && d.hist != 2) d.hist = 0;  // cuDebye-v15.cu:751-753
+    if (!L.next(ln) || std::sscanf(ln.c_str(), "%d", &d.S) != 1 || d.S < 1 || d.S > 64) {
+        err = "header line 4 must hold the number of subgroups (1..64)";
+        return DBX_ERR_PARSE;
+    }
+    int64_t N = 0;
+    for (int g = 0; g < d.S; g++) {
+        if (!L.next(ln)) {
+            err = "missing subgroup line " + std::to_string(g);
+            return DBX_ERR_PARSE;
+        }
+        long long n;
+        char type[64];
+        dbx_group_params p;
+        int k = std::sscanf(ln.c_str(),
+                            "%lld %63s %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf %lf", &n, type,
+                            &p.occ, &p.beq, &p.a[0], &p.a[1], &p.a[2], &p.a[3], &p.a[4], &p.b[0],
+                            &p.b[1], &p.b[2], &p.b[3], &p.b[4], &p.c);
+        if (k != 15 || n < 0) {
+            err = "malformed subgroup line " + std::to_string(g) +
+                  " (want 'n type occ Beq a1..a5 b1..b5 c')";
+            return DBX_ERR_PARSE;
+        }
+        d.n.push_back(n);
+        d.type.push_back(type);
+        d.gp.push_back(p);
+        N += n;
+    }
+    d.x.reserve(N);
+    d.y.reserve(N);
+    d.z.reserve(N);
+    int64_t i = 0;
+    while (L.next(ln)) {
+        if (blank(ln.c_str())) continue;  // cuDebye-v15.cu:821
+        if (i >= N) {
+            err = "more atom positions than the subgroups declare";  // cuDebye-v15.cu:830-833
+            return DBX_ERR_PARSE;
+        }
+        const char *p = ln.c_str();
+        char *e;
+        float v[3];
+        for (int t = 0; t < 3; t++) {
+            v[t] = std::strtof(p, &e);
+            if (e == p) {
+                err = "malformed position line for atom " + std::to_string(i);
+                return DBX_ERR_PARSE;
+            }
+            p = e;
+        }
+        d.x.push_back(v[0]);
+        d.y.push_back(v[1]);
+        d.z.push_back(v[2]);
+        i++;
+    }
+    if (i != N) {
+        err = "Number of extracted Atoms (" + std::to_string(i) +
+              ") doesn't match the number specified in the preamble (" + std::to_string(N) + ")";
+        return DBX_ERR_PARSE;
+    }
+    return DBX_OK;
+}
+
+__attribute__((format(printf, 2, 3))) void logf(bool quiet, const char *fmt, ...)
+{
+    if (quiet) return;
+    va_list ap;
+    va_start(ap, fmt);
+    std::vprintf(fmt, ap);
+    va_end(ap);
+}
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" int dbx_run_deb(const char *path, int flags)
+{
+    if (!path) return DBX_ERR_ARG;
+    const bool quiet = flags & DBX_RUN_QUIET;
+    const char *eol = (flags & DBX_RUN_CRLF) ? "\r\n" : "\n";
+    double t0 = now_s();
+    DebFile d;
+    std::string err;
+    int rc = parse_deb(path, d, err);
+    if (rc != DBX_OK) {
+        std::fprintf(stderr, "ERROR: %s\n", err.c_str());
+        return rc;
+    }
+    const int nTT = dbx_q_from_2theta(d.lambda, d.tt_min, d.tt_max, d.tt_step, nullptr, nullptr, 0);
+    const int64_t channels = dbx_channels(d.rmax, d.rstep);
+    if (nTT < 1 || channels < 2) {
+        std::fprintf(stderr, "ERROR: invalid 2theta or distance grid in the header\n");
+        return DBX_ERR_ARG;
+    }
+    std::vector<double> tt(nTT), K(nTT);
+    dbx_q_from_2theta(d.lambda, d.tt_min, d.tt_max, d.tt_step, tt.data(), K.data(), nTT);
+    logf(quiet, "Loaded %s: %lld atoms, %d subgroups, %d 2theta values (K %g..%g), %lld channels, "
+        "hist=%d (%.3f s)\n",
+        path, (long long)d.x.size(), d.S, nTT, K[0], K[nTT - 1], (long long)channels, d.hist,
+        now_s() - t0);
+
+    dbx_ctx *ctx = nullptr;
+    rc = dbx_open(d.device, 0, &ctx);
+    if (rc != DBX_OK) {
+        std::fprintf(stderr, "ERROR: cannot open device %d\n", d.device);
+        return rc;
+    }
+    dbx_model *m = nullptr;
+    double t1 = now_s();
+    rc = dbx_model_create(ctx, d.x.data(), d.y.data(), d.z.data(), d.n.data(), d.S, &m);
+    if (rc != DBX_OK) {
+        std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
+        dbx_close(ctx);
+        return rc;
+    }
+    const int P = d.S * (d.S + 1) / 2;
+    std::vector<uint64_t> H((size_t)P * channels);
+    std::vector<double> I(nTT, 0.0);
+    const int first_n = d.block > 0 ? d.block : 1;
+    if (d.hist != 1)
+        rc = dbx_intensity(ctx, m, d.gp.data(), d.rstep, channels, first_n, K.data(), nTT, I.data(),
+                           H.data());
+    else
+        rc = dbx_histogram(ctx, m, d.rstep, channels, H.data(), 0, nullptr);
+    const double t_calc = now_s() - t1;
+    if (rc != DBX_OK) {
+        std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
+        if (rc != DBX_ERR_RANGE) {
+            dbx_model_destroy(ctx, m);
+            dbx_close(ctx);
+            return rc;
+        }
+    }
+    // per-pair checks (cuDebye-v15.cu:360-410) and .debx files (cuDebye-v15.cu:413-430)
+    int p = 0;
+    for (int row = 0; row < d.S; row++) {
+        for (int col = row; col < d.S; col++, p++) {
+            const uint64_t *h = &H[(size_t)p * channels];
+            uint64_t sum = 0, zero = 0, near = 0;
+            int64_t nnz = 0;
+            for (int64_t k = 0; k < channels; k++) {
+                if (!h[k]) continue;
+                nnz++;
+                sum += h[k];
+                const double r = (double)k * d.rstep;
+                if (r <= 0.0005) zero += h[k];
+                if (r < 0.25) near += h[k];
+            }
+            const uint64_t na = d.n[row], nb = d.n[col];
+            const uint64_t expect = row == col ? (na * na - na) / 2 + na : na * nb;
+            logf(quiet, ">>>> Subgroups %d and %d: %lld unique distances, %llu observed, %llu expected\n", row,
+                col, (long long)nnz, (unsigned long long)sum, (unsigned long long)expect);
+            if (sum != expect)
+                logf(quiet, "WARNING: %llu Distances Expected and %llu Observed!!!\n",
+                    (unsigned long long)expect, (unsigned long long)sum);
+            if (row == col) {
+                if (h[0] != na)
+                    logf(quiet,
+                         "WARNING: Number of Expected Zero Distances Missing or not Correct!!!\n"
+                         "Number of Atoms within a Tolerance of 0.0005 A is %llu\n",
+                         (unsigned long long)zero);
+                if (near > na)
+                    logf(quiet, "WARNING: Possible Atom Collision!!! Found around %llu interatomic Distances "
+                        "lower than the Radius of an Hydrogen Atom.\n",
+                        (unsigned long long)(near - na));
+            } else if (near > 0) {
+                logf(quiet, "WARNING: Possible Atom Collision!!! Found around %llu interatomic Distances lower "
+                    "than the Radius of an Hydrogen Atom.\n",
+                    (unsigned long long)near);
+            }
+            if (d.hist > 0) {
+                const std::string fn = std::string(path) + "x" + std::to_string(row) + "x" + std::to_string(col);
+                FILE *f = std::fopen(fn.c_str(), "wb");
+                if (!f) {
+                    std::fprintf(stderr, "ERROR: cannot write %s\n", fn.c_str());
+                    rc = DBX_ERR_IO;
+                    continue;
+                }
+                for (int64_t k = 0; k < channels; k++)
+                    if (h[k])
+                        std::fprintf(f, "%.7g\t%llu%s", (double)k * d.rstep, (unsigned long long)h[k], eol);
+                std::fclose(f);
+            }
+        }
+    }
+    logf(quiet, "Time for Calculation of Distances and Evaluation of the Debye Double Sum: %.3f s\n", t_calc);
+    if (d.hist != 1) {
+        const std::string fn = std::string(path) + "I";
+        FILE *f = std::fopen(fn.c_str(), "wb");
+        if (!f) {
+            std::fprintf(stderr, "ERROR: cannot write %s\n", fn.c_str());
+            rc = DBX_ERR_IO;
+        } else {
+            for (int i = 0; i < nTT; i++) std::fprintf(f, "%g\t%g%s", tt[i], I[i], eol);
+            std::fclose(f);
+        }
+    }
+    dbx_model_destroy(ctx, m);
+    dbx_close(ctx);
+    return rc;
+}

@@ -1,0 +1,136 @@
+// Internal definitions shared by the host code and the HIP kernels of libdebye.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/debye.h"
+
+namespace dbx {
+
+// ---------------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define DBX_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            throw ::dbx::Error(e_ == hipErrorOutOfMemory ? DBX_ERR_NOMEM : DBX_ERR_HIP, \
+                               std::string(#call) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+#define DBX_REQUIRE(cond, code, msg)                   \
+    do {                                               \
+        if (!(cond)) throw ::dbx::Error((code), (msg)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------------
+// device layout of a model
+// ---------------------------------------------------------------------------------
+// Atoms are stored per subgroup as float4 (x, y, z, 0) in a spatial order: every
+// subgroup is cut by recursive median bisection into tiles of at most 64 atoms (all
+// tiles of a subgroup but its last are full).  Coordinates are the caller's float
+// values bit for bit; only their order changes, which leaves every histogram unchanged.
+constexpr int kTileAtoms = 64;
+
+struct Tile {          // 32 bytes
+    float cx, cy, cz;  // centre (midpoint of the tile's bounding box)
+    float rad;         // >= max distance of a tile atom from the centre (rounded up)
+    int32_t start;     // first atom (index into the float4 position array)
+    int32_t count;     // atoms in the tile (1..64)
+    int32_t group;
+    int32_t pad;
+};
+
+// One pass-1 work item: the pairs of A tiles [a0, a1) (subgroup row) with the B atoms
+// (subgroup col, tiles [b0, b1)) whose distance from the A tile's centre lies in shell
+// `shell` of pair `pair`.  Every pair lands in one LDS window of kP1Bins bins starting
+// at bin k_lo (pairs outside it, which only occur when the window is narrower than the
+// geometry needs, go straight to the global histogram).
+struct P1Job {
+    int32_t pair;
+    int32_t a0, a1;
+    int32_t b0, b1;
+    int32_t same;   // 1: one subgroup, count b >= a (upper triangle incl. self)
+    int32_t shell;  // index into the pair's shell-edge table
+    int32_t k_lo;   // first bin of the LDS window
+    int32_t e2_off; // offset of the pair's shell-edge table in the edge array
+    int32_t pad[3];
+};
+
+constexpr int kP1Threads = 1024;   // 16 waves of 64
+constexpr int kP1Bins = 36864;     // LDS window: 144 KiB of uint32 counters
+
+struct ModelDev {
+    int n_groups = 0;
+    int64_t n_atoms = 0;
+    std::vector<int64_t> group_n;
+    std::vector<int32_t> group_tile0, group_tile1;  // tile range per subgroup
+    std::vector<Tile> tiles_h;                      // host copy (job planning)
+    std::vector<float> bbox;                        // per subgroup: min xyz, max xyz
+    float4 *pos = nullptr;                          // device, spatially ordered
+    Tile *tiles = nullptr;                          // device
+};
+
+}  // namespace dbx
+
+struct dbx_model {
+    dbx::ModelDev d;
+};
+
+struct dbx_ctx {
+    int device = 0;
+    size_t budget = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::recursive_mutex mu;
+    // grow-only device scratch, one buffer per slot (dbx::kSlot*)
+    void *slot_ptr[8] = {};
+    size_t slot_bytes[8] = {};
+    uint64_t *counters = nullptr;  // [0] out-of-range pairs, [1] pairs outside the LDS window
+    int cu_count = 0;
+    int64_t last_p1_jobs = 0;
+    uint64_t last_p1_misses = 0;
+    float last_p1_kernel_ms = 0.f;  // hipEvent time of the last pass-1 histogram kernel
+    float last_p2_ms = 0.f;         // hipEvent time of the last pass-2 structure computation
+    hipEvent_t ev[4] = {};
+    bool p2_timed = false;
+};
+
+namespace dbx {
+
+enum { kSlotPass1 = 0, kSlotPass2 = 1, kSlotPass2b = 2, kSlotHist = 3, kSlotQ = 4 };
+// device buffer of at least `bytes` for `slot`; contents are not preserved when it grows
+void *scratch(dbx_ctx *ctx, int slot, size_t bytes);
+
+// pass 1 (pass1.hip): histograms of all subgroup pairs into d_hist[P][n_bins]
+// (zeroed here).  Returns the number of pairs with bin >= n_bins.
+struct P1Stats {
+    uint64_t out_of_range = 0;
+    uint64_t window_misses = 0;
+    int n_jobs = 0;
+};
+P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t n_bins,
+                         uint64_t *d_hist, bool sync_stats = true);
+
+// pass 2 (pass2.hip)
+void prefactor_dev(dbx_ctx *ctx, const dbx_group_params &ga, const dbx_group_params &gb,
+                   const double *d_q, int nq, double *d_ff);
+// S_p(q) for all p into d_S[P][nq] from device histograms d_hist[P][n_bins].
+void debye_structure(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
+                     int first_n, const double *d_q, int nq, double *d_S);
+// I[q] = sum_p ff[p][q] S[p][q]
+void combine_intensity(dbx_ctx *ctx, const double *d_ff, const double *d_S, int P, int nq,
+                       double *d_I);
+
+}  // namespace dbx

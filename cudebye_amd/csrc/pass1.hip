@@ -1,0 +1,305 @@
+// Pass 1: all-pairs interatomic-distance histograms per subgroup pair.
+//
+// Replaces splitCalcDist and the dist / dist2 / care_dist2 kernels of cuDebye-v15.cu:88-272.
+// The per-pair expression is the reference's (cuDebye-v15.cu:101-115):
+//     dx, dy, dz = FP32 differences;  s = (dx*dx + dy*dy) + dz*dz, every operation rounded
+//     (no fused multiply-add: the reference's .debx outputs are reproduced only by that form);
+//     k = (int) roundf(sqrtf(s) * rp),  rp = (float)(1/rstep),  sqrtf correctly rounded.
+// The pair set is the reference's: one subgroup -> pairs col <= row including self pairs
+// (cuDebye-v15.cu:137), two subgroups -> the full n_row x n_col rectangle.
+//
+// MI355X design.  A workgroup (16 waves) owns a window of kP1Bins uint32 counters in LDS.
+// Each wave holds 64 atoms of an A tile in its lanes and streams B atoms as wave-uniform
+// values (v_readlane), so every B atom is one 64-lane step of ~25 VALU operations and one
+// LDS atomic per lane.  To keep all bins of a workgroup inside its LDS window, B atoms are
+// taken shell by shell: a job handles the B atoms whose distance from the A tile's centre
+// lies in [s*h, (s+1)*h); every pair of such an atom with an A atom then lies within
+// [s*h - rad_A, (s+1)*h + rad_A].  At the end of a job the nonzero LDS counters are added
+// to the uint64 histogram in HBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "binrule.h"
+#include "internal.h"
+
+#pragma clang fp contract(off)
+
+namespace dbx {
+
+__device__ __forceinline__ float rd_lane(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+__global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
+    const float4 *__restrict__ pos, const Tile *__restrict__ tiles, const P1Job *__restrict__ jobs,
+    const float *__restrict__ edge2, float rp, unsigned long long *__restrict__ hist,
+    int64_t n_bins, unsigned long long *__restrict__ counters)
+{
+    __shared__ uint32_t h[kP1Bins];
+    const P1Job job = jobs[blockIdx.x];
+    for (int i = threadIdx.x; i < kP1Bins; i += kP1Threads) h[i] = 0u;
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const float e_lo = edge2[job.e2_off + job.shell];
+    const float e_hi = edge2[job.e2_off + job.shell + 1];
+    const float r_lo = e_lo > 0.f ? sqrtf(e_lo) : 0.f;
+    const float r_hi = sqrtf(e_hi);  // +inf for the last shell
+    const int k_lo = job.k_lo;
+    unsigned long long *hrow = hist + (int64_t)job.pair * n_bins;
+    uint32_t miss = 0, oor = 0;
+
+    for (int ia = job.a0; ia < job.a1; ++ia) {
+        const Tile ta = tiles[ia];
+        const bool a_ok = lane < ta.count;
+        const float4 pa = pos[ta.start + (a_ok ? lane : 0)];
+        const int jbeg = job.same ? ia : job.b0;
+        for (int jb = jbeg + wave * 64; jb < job.b1; jb += (kP1Threads / 64) * 64) {
+            // cull 64 B tiles at once: keep those with an atom possibly in the shell
+            bool cand = false;
+            const int j = jb + lane;
+            if (j < job.b1) {
+                const Tile tb = tiles[j];
+                const float dx = tb.cx - ta.cx, dy = tb.cy - ta.cy, dz = tb.cz - ta.cz;
+                const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
+                const float mg = 1e-3f + 1e-5f * (d + tb.rad);
+                cand = (d + tb.rad + mg >= r_lo) && (d - tb.rad - mg < r_hi);
+            }
+            uint64_t cm = __ballot(cand);
+            while (cm) {
+                const int l = __builtin_ctzll(cm);
+                cm &= cm - 1;
+                const int j2 = jb + l;
+                const Tile tb = tiles[j2];
+                const bool b_ok = lane < tb.count;
+                const float4 pb = pos[tb.start + (b_ok ? lane : 0)];
+                const float ex = pb.x - ta.cx, ey = pb.y - ta.cy, ez = pb.z - ta.cz;
+                const float d2c = (ex * ex + ey * ey) + ez * ez;
+                uint64_t bm = __ballot(b_ok && d2c >= e_lo && d2c < e_hi);
+                const bool diag = job.same && (j2 == ia);
+                while (bm) {
+                    const int m = __builtin_ctzll(bm);
+                    bm &= bm - 1;
+                    const float bx = rd_lane(pb.x, m), by = rd_lane(pb.y, m), bz = rd_lane(pb.z, m);
+                    if (a_ok && (!diag || lane <= m)) {
+                        const int k = pair_bin(pa.x, pa.y, pa.z, bx, by, bz, rp);
+                        const unsigned kk = (unsigned)(k - k_lo);
+                        if (kk < (unsigned)kP1Bins) {
+                            atomicAdd(&h[kk], 1u);
+                        } else {
+                            ++miss;
+                            if (k >= 0 && (int64_t)k < n_bins)
+                                atomicAdd(&hrow[k], 1ull);
+                            else
+                                ++oor;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    // wave totals of the rare counters
+    for (int off = 32; off > 0; off >>= 1) {
+        miss += __shfl_down(miss, off);
+        oor += __shfl_down(oor, off);
+    }
+    if (lane == 0) {
+        if (miss) atomicAdd(&counters[1], (unsigned long long)miss);
+        if (oor) atomicAdd(&counters[0], (unsigned long long)oor);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < kP1Bins; i += kP1Threads) {
+        const uint32_t v = h[i];
+        if (v) {
+            const int64_t k = (int64_t)k_lo + i;
+            if (k < n_bins)
+                atomicAdd(&hrow[k], (unsigned long long)v);
+            else
+                atomicAdd(&counters[0], (unsigned long long)v);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// host: job planning
+// ---------------------------------------------------------------------------------
+namespace {
+
+double point_box_min(const Tile &t, const float *bb)
+{
+    double d2 = 0;
+    const double c[3] = {t.cx, t.cy, t.cz};
+    for (int a = 0; a < 3; a++) {
+        double lo = bb[a], hi = bb[3 + a];
+        double e = c[a] < lo ? lo - c[a] : (c[a] > hi ? c[a] - hi : 0.0);
+        d2 += e * e;
+    }
+    return std::sqrt(d2);
+}
+
+double point_box_max(const Tile &t, const float *bb)
+{
+    double d2 = 0;
+    const double c[3] = {t.cx, t.cy, t.cz};
+    for (int a = 0; a < 3; a++) {
+        double e = std::max(std::fabs(c[a] - bb[a]), std::fabs(bb[3 + a] - c[a]));
+        d2 += e * e;
+    }
+    return std::sqrt(d2);
+}
+
+}  // namespace
+
+P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t n_bins,
+                         uint64_t *d_hist, bool sync_stats)
+{
+    DBX_REQUIRE(rstep > 0 && std::isfinite(rstep), DBX_ERR_ARG, "rstep must be positive");
+    DBX_REQUIRE(n_bins > 0 && n_bins < (int64_t)1 << 40, DBX_ERR_ARG, "bad n_bins");
+    const int S = m.n_groups;
+    const int P = S * (S + 1) / 2;
+    const float rp = (float)(1 / rstep);  // cuDebye-v15.cu:739
+    const double usable = (kP1Bins - 6) * rstep;
+
+    std::vector<P1Job> jobs;
+    std::vector<double> cost;
+    std::vector<float> edge2;
+    int p = 0;
+    for (int ga = 0; ga < S; ga++) {
+        for (int gb = ga; gb < S; gb++, p++) {
+            const int a0 = m.group_tile0[ga], a1 = m.group_tile1[ga];
+            const int b0 = m.group_tile0[gb], b1 = m.group_tile1[gb];
+            if (a0 == a1 || b0 == b1) continue;
+            const float *bbA = &m.bbox[6 * ga];
+            const float *bbB = &m.bbox[6 * gb];
+            double radA = 0;
+            for (int t = a0; t < a1; t++) radA = std::max(radA, (double)m.tiles_h[t].rad);
+            double D2 = 0;
+            for (int a = 0; a < 3; a++) {
+                double e = std::max(std::fabs((double)bbB[3 + a] - bbA[a]),
+                                    std::fabs((double)bbA[3 + a] - bbB[a]));
+                D2 += e * e;
+            }
+            const double D = std::sqrt(D2) + 1e-3;
+            const double mg = 2e-3 + 4e-6 * (D + radA);
+            double h = usable - 2 * radA - 2 * mg;
+            bool degraded = false;
+            if (h < 0.25 * usable) {
+                h = 0.25 * usable;
+                degraded = true;
+            }
+            int ns = (int)(D / h) + 1;
+            const int e2_off = (int)edge2.size();
+            edge2.push_back(-1.0f);
+            for (int s = 1; s < ns; s++) edge2.push_back((float)((s * h) * (s * h)));
+            edge2.push_back(INFINITY);
+
+            // A chunks: enough pairs per job to amortise the LDS window flush
+            const double nB = (double)m.group_n[gb];
+            const double shell_frac = std::min(1.0, h / std::max(D, h));
+            const double per_tile = 64.0 * std::max(1.0, nB * shell_frac);
+            int CA = (int)std::ceil(6.0e6 / per_tile);
+            CA = std::max(1, std::min(CA, 32));
+            // sample of B tiles for cost estimates
+            const int nbt = b1 - b0;
+            const int nchunks = (a1 - a0 + CA - 1) / CA;
+            int stride = (int)std::max<int64_t>(1, ((int64_t)nchunks * nbt) / 2000000);
+            std::vector<double> shell_cost(ns);
+            for (int c0 = a0; c0 < a1; c0 += CA) {
+                const int c1 = std::min(a1, c0 + CA);
+                double dmin = 1e300, dmax = 0, rmaxc = 0, cx = 0, cy = 0, cz = 0, na = 0;
+                for (int t = c0; t < c1; t++) {
+                    const Tile &T = m.tiles_h[t];
+                    dmin = std::min(dmin, point_box_min(T, bbB));
+                    dmax = std::max(dmax, point_box_max(T, bbB));
+                    rmaxc = std::max(rmaxc, (double)T.rad);
+                    cx += T.cx * T.count; cy += T.cy * T.count; cz += T.cz * T.count;
+                    na += T.count;
+                }
+                cx /= na; cy /= na; cz /= na;
+                std::fill(shell_cost.begin(), shell_cost.end(), 0.0);
+                const int jfirst = (gb == ga) ? c0 : b0;
+                for (int j = jfirst; j < b1; j += stride) {
+                    const Tile &T = m.tiles_h[j];
+                    double dx = T.cx - cx, dy = T.cy - cy, dz = T.cz - cz;
+                    int s = (int)(std::sqrt(dx * dx + dy * dy + dz * dz) / h);
+                    if (s >= ns) s = ns - 1;
+                    shell_cost[s] += (double)T.count * stride;
+                }
+                for (int s = 0; s < ns; s++) {
+                    if (s * h > dmax + mg + 1e-3 || (s + 1) * h < dmin - mg - 1e-3) continue;
+                    P1Job J;
+                    std::memset(&J, 0, sizeof(J));
+                    J.pair = p;
+                    J.a0 = c0;
+                    J.a1 = c1;
+                    J.b0 = b0;
+                    J.b1 = b1;
+                    J.same = (ga == gb);
+                    J.shell = s;
+                    J.e2_off = e2_off;
+                    double klo;
+                    if (!degraded)
+                        klo = std::floor((s * h - rmaxc - mg) / rstep) - 2;
+                    else
+                        klo = std::floor(((s + 0.5) * h) / rstep) - kP1Bins / 2;
+                    if (klo < 0) klo = 0;
+                    J.k_lo = (int32_t)std::min<double>(klo, (double)INT32_MAX - kP1Bins);
+                    jobs.push_back(J);
+                    // neighbouring shells share straddling tiles: spread the sample cost
+                    double c = shell_cost[s];
+                    if (s > 0) c += 0.25 * shell_cost[s - 1];
+                    if (s + 1 < ns) c += 0.25 * shell_cost[s + 1];
+                    cost.push_back(na * (c + 1.0));
+                }
+            }
+        }
+    }
+
+    P1Stats st;
+    st.n_jobs = (int)jobs.size();
+    DBX_HIP(hipMemsetAsync(d_hist, 0, sizeof(uint64_t) * (size_t)P * n_bins, ctx->stream));
+    DBX_HIP(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(uint64_t), ctx->stream));
+    if (jobs.empty()) {
+        if (sync_stats) DBX_HIP(hipStreamSynchronize(ctx->stream));
+        return st;
+    }
+    // largest jobs first
+    std::vector<int> order(jobs.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    std::vector<P1Job> sorted(jobs.size());
+    for (size_t i = 0; i < order.size(); i++) sorted[i] = jobs[order[i]];
+
+    const size_t jb = sizeof(P1Job) * sorted.size();
+    const size_t eb = sizeof(float) * edge2.size();
+    char *buf = (char *)scratch(ctx, kSlotPass1, jb + eb + 256);
+    P1Job *d_jobs = (P1Job *)buf;
+    float *d_e2 = (float *)(buf + ((jb + 255) & ~(size_t)255));
+    DBX_HIP(hipMemcpyAsync(d_jobs, sorted.data(), jb, hipMemcpyHostToDevice, ctx->stream));
+    DBX_HIP(hipMemcpyAsync(d_e2, edge2.data(), eb, hipMemcpyHostToDevice, ctx->stream));
+    DBX_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    hipLaunchKernelGGL(p1_histogram_kernel, dim3((unsigned)sorted.size()), dim3(kP1Threads), 0,
+                       ctx->stream, m.pos, m.tiles, d_jobs, d_e2, rp,
+                       (unsigned long long *)d_hist, n_bins, (unsigned long long *)ctx->counters);
+    DBX_HIP(hipGetLastError());
+    DBX_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    if (sync_stats) {
+        uint64_t c[2];
+        DBX_HIP(hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
+        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        st.out_of_range = c[0];
+        st.window_misses = c[1];
+        ctx->last_p1_misses = c[1];
+        DBX_HIP(hipEventElapsedTime(&ctx->last_p1_kernel_ms, ctx->ev[0], ctx->ev[1]));
+    }
+    ctx->last_p1_jobs = st.n_jobs;
+    return st;
+}
+
+}  // namespace dbx

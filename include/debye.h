@@ -1,0 +1,153 @@
+/*
+ * debye.h - C ABI of libdebye.so, an MI355X (gfx950) Debye scattering engine with the
+ * method and file surface of cuDebye v1.5 (Martin-Rudolph/cuDebye, cuDebye-v15.cu).
+ *
+ * The reference is one CUDA executable whose in-process seams are four host functions
+ * (splitCalcDist, reduceCheck, atomicProp, intensity) plus main()'s parser and writers.
+ * Each entry point below names the reference code it replaces.
+ *
+ * Conventions
+ *   - Every function returns an int status: DBX_OK (0) or a negative DBX_ERR_* code;
+ *     dbx_last_error() gives the message of the last failure on a context.  Nothing here
+ *     calls exit() (the reference's gpuErrchk does, cuDebye-v15.cu:58-66).
+ *   - Host buffers belong to the caller.  Device memory belongs to the context and is
+ *     released by dbx_model_destroy / dbx_close.
+ *   - A context serialises its synchronous calls; distinct contexts are independent.
+ *   - Subgroup pairs are numbered in the reference's order (cuDebye-v15.cu:941-947):
+ *     (0,0),(0,1),...,(0,S-1),(1,1),...,(S-1,S-1); P = S(S+1)/2.
+ *   - Histograms are uint64 per bin (the reference uses uint32, cuDebye-v15.cu:872).
+ */
+#ifndef DEBYE_H
+#define DEBYE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DBX_OK = 0,
+    DBX_ERR_ARG = -1,      /* invalid argument */
+    DBX_ERR_HIP = -2,      /* HIP runtime failure */
+    DBX_ERR_IO = -3,       /* file missing / unwritable */
+    DBX_ERR_PARSE = -4,    /* malformed .deb, or atom count mismatch (cuDebye-v15.cu:830-833) */
+    DBX_ERR_RANGE = -5,    /* a pair distance fell at or beyond the last bin (rmax too small) */
+    DBX_ERR_NOMEM = -6,    /* device allocation failed */
+    DBX_ERR_BUDGET = -7,   /* request larger than the context's HBM budget */
+    DBX_ERR_STATE = -8     /* unknown ticket / wrong state */
+};
+
+typedef struct dbx_ctx dbx_ctx;
+typedef struct dbx_model dbx_model;
+
+/* Scattering parameters of one subgroup: the subgroup line of a .deb
+ * "n type occ Beq a1..a5 b1..b5 c" (cuDebye-v15.cu:786, README.md "Fifth Line"). */
+typedef struct {
+    double occ;
+    double beq;
+    double a[5];
+    double b[5];
+    double c;
+} dbx_group_params;
+
+/* ---- context ------------------------------------------------------------------- */
+
+/* Opens device `device` (the .deb header's first number, cuDebye-v15.cu:644-646).
+ * hbm_budget: bytes of device memory the context may hold (0 = 90% of free memory). */
+int dbx_open(int device, size_t hbm_budget, dbx_ctx **out);
+int dbx_close(dbx_ctx *ctx);
+const char *dbx_last_error(const dbx_ctx *ctx);
+/* HIP stream the context launches on (hipStream_t as void*). */
+void *dbx_stream(dbx_ctx *ctx);
+
+/* ---- grids (host) ----------------------------------------------------------------- */
+
+/* 2theta and K grids, cuDebye-v15.cu:690-698: nTT = int((max-min)/step)+1,
+ * 2theta_i = min + i*step, K_i = 4*pi*sin(2theta_i/2*pi/180)/lambda.
+ * Returns nTT (>0) and fills tt/q when both are non-NULL and cap >= nTT;
+ * returns DBX_ERR_ARG for a non-positive step or nTT < 1. */
+int dbx_q_from_2theta(double lambda, double tt_min, double tt_max, double tt_step, double *tt,
+                      double *q, int cap);
+/* channels = int(rmax/rstep)+2 (cuDebye-v15.cu:734); <0 on bad input. */
+int64_t dbx_channels(double rmax, double rstep);
+
+/* ---- models ----------------------------------------------------------------------- */
+
+/* Uploads a model: FP32 positions in subgroup order (SoA, as cuDebye-v15.cu:809-827),
+ * group_n[n_groups] atoms per subgroup.  Coordinates are copied bit for bit (only their
+ * order changes inside the device layout).  Non-finite coordinates are rejected. */
+int dbx_model_create(dbx_ctx *ctx, const float *x, const float *y, const float *z,
+                     const int64_t *group_n, int n_groups, dbx_model **out);
+int dbx_model_destroy(dbx_ctx *ctx, dbx_model *m);
+int dbx_model_info(const dbx_model *m, int64_t *n_atoms, int *n_groups, int *n_pairs);
+
+/* ---- pass 1: pair-distance histograms ----------------------------------------------
+ * Replaces splitCalcDist + the dist/dist2/care_dist2 kernels (cuDebye-v15.cu:88-272)
+ * for every subgroup pair.  Bin rule (bit-exact with the reference):
+ *   dx,dy,dz FP32 differences; s = (dx*dx + dy*dy) + dz*dz, each operation rounded;
+ *   k = (int)roundf(sqrtf(s) * (float)(1/rstep)), sqrtf correctly rounded.
+ * Same subgroup: pairs col <= row including self pairs (bin 0 += n); cross: n_a*n_b.
+ * out: uint64[P][n_bins] (host memory, or device memory when out_is_device != 0).
+ * Returns DBX_ERR_RANGE (histogram still written, offending pairs not counted) if any
+ * pair falls at k >= n_bins; *n_out_of_range (optional) receives their number. */
+int dbx_histogram(dbx_ctx *ctx, const dbx_model *m, double rstep, int64_t n_bins, uint64_t *out,
+                  int out_is_device, uint64_t *n_out_of_range);
+
+/* ---- prefactor ----------------------------------------------------------------------
+ * ffoobb[q] = occ_a occ_b exp(nh (B_a+B_b)) f_a f_b with nh = -Q^2/(16 pi^2) and
+ * f = c + sum_5 a_i exp(b_i nh), FP64 (replaces atomicProp/atomicScatter,
+ * cuDebye-v15.cu:447-479). */
+int dbx_prefactor(dbx_ctx *ctx, const dbx_group_params *ga, const dbx_group_params *gb,
+                  const double *q, int nq, double *ffoobb);
+
+/* ---- pass 2: Debye sum ----------------------------------------------------------------
+ * Replaces reduceCheck's compaction + intensity/debyeRowSum (cuDebye-v15.cu:283-346,
+ * 493-577) for all P partials at once:
+ *   S_p(q) = sum_k w_k N_pk sinc(q r_k), r_k = k*rstep, with the reference weight rule:
+ *   the first `first_n` nonzero bins (first_n = the .deb block size) with r_k <= 0.0005
+ *   count N_pk once with sinc 1, every other nonzero bin counts 2 N_pk sin(q r_k)/(q r_k);
+ *   I(q) = sum_p ffoobb_p(q) S_p(q).
+ * hist: uint64[P][n_bins] (host or device memory per hist_is_device).  ffoobb: [P][nq]
+ * host FP64.  I_out[nq] host FP64 (overwritten).  S_out ([P][nq], optional) receives S_p. */
+int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int64_t n_bins,
+              double rstep, int first_n, const double *q, int nq, const double *ffoobb,
+              double *I_out, double *S_out);
+
+/* ---- whole model ----------------------------------------------------------------------
+ * One model end to end on the device: pass 1, compaction, prefactors, pass 2.
+ * params[n_groups]; I_out[nq]; hist_out ([P][n_bins], optional, host) receives the
+ * histograms.  Equivalent to the reference's subgroup-pair loop (cuDebye-v15.cu:941-975). */
+int dbx_intensity(dbx_ctx *ctx, const dbx_model *m, const dbx_group_params *params,
+                  double rstep, int64_t n_bins, int first_n, const double *q, int nq,
+                  double *I_out, uint64_t *hist_out);
+
+/* ---- .deb driver ------------------------------------------------------------------------
+ * The reference program: `cuDebye.exe file.deb` (cuDebye-v15.cu:599-1003).  Parses the
+ * .deb, runs every subgroup pair, writes <path>I (hist 0 or 2) and <path>x<r>x<c>
+ * (hist 1 or 2) with the reference's formats.  flags: DBX_RUN_CRLF writes "\r\n" line
+ * ends like the reference's Windows build; DBX_RUN_QUIET suppresses the log on stdout. */
+enum { DBX_RUN_CRLF = 1, DBX_RUN_QUIET = 2 };
+int dbx_run_deb(const char *path, int flags);
+
+/* ---- diagnostics --------------------------------------------------------------------
+ * Self tests of the device arithmetic behind the pass-1 bin rule (no reference
+ * counterpart: the reference relies on CUDA's sqrtf and roundf).
+ * dbx_selftest_sqrt: for every float bit pattern in [lo, hi) with 2^-96 <= s <= FLT_MAX,
+ *   n_bad[0] counts device square roots that are not correctly rounded, n_bad[1] those
+ *   that differ from the compiler's correctly rounded sqrtf.
+ * dbx_selftest_bins: out[i] = device bin of the pair (a[3i..], b[3i..]) with rp. */
+int dbx_selftest_sqrt(uint32_t lo, uint32_t hi, uint64_t *n_bad);
+int dbx_selftest_bins(const float *a, const float *b, int n, float rp, int32_t *out);
+/* Statistics of the last pass-1 run on ctx: work items launched and pairs whose bin fell
+ * outside their workgroup's LDS window (added to HBM directly; a performance figure). */
+int dbx_last_pass1_stats(const dbx_ctx *ctx, int64_t *n_jobs, uint64_t *window_misses);
+/* Device times (HIP events on the context's stream) of the last pass-1 histogram kernel
+ * and of the last pass-2 structure computation (compaction through the Debye sum), ms. */
+int dbx_last_kernel_times(dbx_ctx *ctx, double *pass1_ms, double *pass2_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DEBYE_H */

@@ -1,0 +1,161 @@
+"""Pass 1 on the GPU against the CPU oracle: bit-exact uint64 histograms.
+
+Fixture-anchored: the reference's own Data/GammaAl2O3_10nm.debx* (all six partials)."""
+import numpy as np
+import pytest
+
+import cudebye_amd as cd
+from cudebye_amd import synth
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cd.Context(0)
+    yield c
+    c.close()
+
+
+def _oracle_hist(m, n_bins=None):
+    st = np.concatenate([[0], np.cumsum(m.group_n)[:-1]])
+    S = len(m.group_n)
+    nb = n_bins or m.n_bins
+    rp = O.rp_of(m.rstep)
+    H = np.zeros((S * (S + 1) // 2, nb), np.uint64)
+    ov = 0
+    p = 0
+    for r in range(S):
+        for c in range(r, S):
+            _, o = O.histogram(m.x, m.y, m.z, st[r], m.group_n[r], st[c], m.group_n[c], r == c, rp, nb,
+                               out=H[p])
+            ov += o
+            p += 1
+    return H, ov
+
+
+def test_sqrt_correctly_rounded_exhaustive():
+    """Every float from 2^-96 up to FLT_MAX (bit patterns 0x0F800000..0x7F7FFFFF): the device
+    square root of pass 1 is correctly rounded and equals the compiler's sqrtf."""
+    bad, mism = cd.selftest_sqrt(0x0F800000, 0x7F800000)
+    assert bad == 0 and mism == 0
+
+
+def test_bin_rule_matches_oracle_random_and_near_ties():
+    rng = np.random.default_rng(11)
+    n = 400000
+    a = rng.uniform(-150, 150, (n, 3)).astype(np.float32)
+    b = rng.uniform(-150, 150, (n, 3)).astype(np.float32)
+    # near-tie pairs: b along x from a at distances (k + 0.5) * rstep +- a few ulp
+    for rstep in (1e-3, 5e-4, 1e-4):
+        rp = O.rp_of(rstep)
+        k = rng.integers(1, 150000, 50000)
+        d = ((k + 0.5) * rstep).astype(np.float32)
+        jit = rng.integers(-3, 4, 50000).astype(np.int32)
+        d = (d.view(np.int32) + jit).view(np.float32)
+        a2 = rng.uniform(-10, 10, (50000, 3)).astype(np.float32)
+        b2 = a2.copy()
+        b2[:, 0] = (a2[:, 0] + d).astype(np.float32)
+        A = np.concatenate([a, a2])
+        B = np.concatenate([b, b2])
+        got = cd.selftest_bins(A, B, rp)
+        want = np.array([O.pair_bin(A[i], B[i], rp) for i in range(0, len(A), 7)])
+        assert np.array_equal(got[::7], want)
+
+
+def test_c1_fcc_sphere_bit_exact(ctx):
+    m = synth.fcc_sphere()
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    H = ctx.histogram(mod, m.rstep, m.n_bins)
+    ref, ov = _oracle_hist(m)
+    assert ov == 0
+    assert np.array_equal(H, ref)
+    n = m.n_atoms
+    assert int(H.sum()) == n * (n + 1) // 2 and int(H[0, 0]) == n
+
+
+@pytest.mark.parametrize("n,S,seed", [(1, 1, 0), (2, 1, 1), (63, 1, 2), (64, 2, 3), (65, 3, 4),
+                                      (300, 2, 5), (1000, 4, 6), (2500, 5, 7)])
+def test_small_random_models_bit_exact(ctx, n, S, seed):
+    m = synth.small_random(n, S, seed)
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    H = ctx.histogram(mod, m.rstep, m.n_bins)
+    ref, ov = _oracle_hist(m)
+    assert ov == 0
+    assert np.array_equal(H, ref)
+
+
+def test_empty_subgroup_and_duplicates(ctx):
+    """A subgroup with no atoms, and coincident atoms (all in bin 0)."""
+    x = np.array([0, 0, 0, 1.5, 1.5], np.float32)
+    y = np.array([0, 0, 0, 0.0, 0.0], np.float32)
+    z = np.zeros(5, np.float32)
+    gn = np.array([3, 0, 2], np.int64)
+    mod = ctx.model(x, y, z, gn)
+    H = ctx.histogram(mod, 1e-3, 2000)
+    # pairs: (0,0): 3*4/2 = 6 all in bin 0; (0,2): 3*2 = 6 at 1.5 A; (2,2): 3 in bin 0
+    assert H.shape == (6, 2000)
+    assert H[0, 0] == 6 and H[0].sum() == 6
+    assert H[1].sum() == 0 and H[3].sum() == 0 and H[4].sum() == 0
+    assert H[2, 1500] == 6 and H[2].sum() == 6
+    assert H[5, 0] == 3 and H[5].sum() == 3
+
+
+def test_rmax_too_small_reports_range_error(ctx):
+    m = synth.small_random(500, 2, 9, spread=20.0)
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    nb = O.channels(15.0, m.rstep)
+    H, oor = ctx.histogram(mod, m.rstep, nb, return_oor=True)
+    ref, ov = _oracle_hist(m, nb)
+    assert oor == ov and ov > 0
+    assert np.array_equal(H, ref)
+    with pytest.raises(cd.DebyeError) as e:
+        ctx.histogram(mod, m.rstep, nb)
+    assert e.value.code == cd.DBX_ERR_RANGE
+
+
+def test_nonfinite_coordinates_rejected(ctx):
+    x = np.array([0, np.nan], np.float32)
+    with pytest.raises(cd.DebyeError):
+        ctx.model(x, x, x, np.array([2], np.int64))
+
+
+def test_gamma_fixture_histograms_bit_exact(ctx, golden_dir, gamma_debx):
+    """The reference's own outputs: Data/GammaAl2O3_10nm.debx0x0 ... debx2x2 (rstep 1e-4)."""
+    g = O.parse_deb(str(golden_dir / "GammaAl2O3_10nm.deb"))
+    nb = O.channels(g.rmax, g.rstep)
+    mod = ctx.model(g.x, g.y, g.z, g.group_n)
+    H = ctx.histogram(mod, g.rstep, nb)
+    for p, (r, c) in enumerate(O.pair_list(g.group_n)):
+        bins, cnt = gamma_debx[f"{r}x{c}"]
+        ref = np.zeros(nb, np.uint64)
+        ref[bins] = cnt
+        assert np.array_equal(H[p], ref), (r, c)
+
+
+def test_c2_spinel_bit_exact_and_totals(ctx):
+    m = synth.spinel_sphere()
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    H = ctx.histogram(mod, m.rstep, m.n_bins)
+    n = m.group_n
+    assert [int(h.sum()) for h in H] == [n[0] * (n[0] + 1) // 2, n[0] * n[1], n[1] * (n[1] + 1) // 2]
+    jobs, misses = ctx.pass1_stats()
+    assert misses == 0
+    ref, ov = _oracle_hist(m)
+    assert ov == 0
+    assert np.array_equal(H, ref)
+
+
+def test_repeatable_and_permutation_invariant(ctx):
+    """Integer histograms are identical across runs and under any atom order within a subgroup."""
+    m = synth.small_random(3000, 2, 21, spread=15.0)
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    H1 = ctx.histogram(mod, m.rstep, m.n_bins)
+    H2 = ctx.histogram(mod, m.rstep, m.n_bins)
+    rng = np.random.default_rng(3)
+    n0 = int(m.group_n[0])
+    perm = np.concatenate([rng.permutation(n0), n0 + rng.permutation(int(m.group_n[1]))])
+    mod2 = ctx.model(m.x[perm], m.y[perm], m.z[perm], m.group_n)
+    H3 = ctx.histogram(mod2, m.rstep, m.n_bins)
+    assert np.array_equal(H1, H2) and np.array_equal(H1, H3)

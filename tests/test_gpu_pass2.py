@@ -1,0 +1,99 @@
+"""Pass 2 on the GPU against the FP64 oracle: prefactors, the Debye sum with the reference
+weight rule, and I(Q) (relative error bound written in each test)."""
+import numpy as np
+import pytest
+
+import cudebye_amd as cd
+from cudebye_amd import synth
+from oracle import oracle as O
+from tests.conftest import read_debi
+
+pytestmark = pytest.mark.gpu
+
+# FP64 end to end on the device: only summation order and the device sin differ from the
+# oracle's compensated FP64 sum.
+TOL_FP64 = 1e-9
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cd.Context(0)
+    yield c
+    c.close()
+
+
+def _gp(p):
+    return cd.GroupParams.make(p["occ"], p["beq"], p["a"], p["b"], p["c"])
+
+
+def _oracle_I(m, H, first_n):
+    class Pm:
+        pass
+    Pm.group_n = m.group_n
+    Pm.occ = np.array([p["occ"] for p in m.params])
+    Pm.beq = np.array([p["beq"] for p in m.params])
+    Pm.a = np.array([p["a"] for p in m.params])
+    Pm.b = np.array([p["b"] for p in m.params])
+    Pm.c = np.array([p["c"] for p in m.params])
+    return O.intensity_from_histograms(H, Pm, m.q, m.rstep, first_n=first_n, with_cond=True)
+
+
+def test_prefactor_matches_oracle(ctx):
+    rng = np.random.default_rng(5)
+    q = np.linspace(0.3, 20, 777)
+    for _ in range(4):
+        pa, pb = synth.cromer_mann(rng), synth.cromer_mann(rng)
+        pa["occ"], pb["beq"] = 0.8333, 2.0
+        got = ctx.prefactor(_gp(pa), _gp(pb), q)
+        ref = O.prefactor(q, pa["occ"], pb["occ"], pa["beq"], pb["beq"], pa["a"], pa["b"], pa["c"],
+                          pb["a"], pb["b"], pb["c"])
+        assert np.max(np.abs(got - ref) / np.abs(ref)) < 1e-14
+
+
+def test_gamma_debI_from_fixture_histograms(ctx, golden_dir, gamma_debx):
+    """Pass-2-only known answer: the reference's .debx histograms + header -> its .debI."""
+    g = O.parse_deb(str(golden_dir / "GammaAl2O3_10nm.deb"))
+    tt, K = cd.q_from_2theta(g.lam, g.tt_min, g.tt_max, g.tt_step)
+    nb = O.channels(g.rmax, g.rstep)
+    pairs = O.pair_list(g.group_n)
+    H = np.zeros((len(pairs), nb), np.uint64)
+    ff = np.zeros((len(pairs), len(K)))
+    for p, (r, c) in enumerate(pairs):
+        bins, cnt = gamma_debx[f"{r}x{c}"]
+        H[p][bins] = cnt
+        ga = cd.GroupParams.make(g.occ[r], g.beq[r], g.a[r], g.b[r], g.c[r])
+        gb = cd.GroupParams.make(g.occ[c], g.beq[c], g.a[c], g.b[c], g.c[c])
+        ff[p] = ctx.prefactor(ga, gb, K)
+    I = ctx.debye(H, g.rstep, K, ff, first_n=g.block)
+    Iref, mag = O.intensity_from_histograms(H, g, K, g.rstep, first_n=g.block, with_cond=True)
+    assert np.max(np.abs(I - Iref) / np.abs(Iref)) < TOL_FP64
+    ref = read_debi(str(golden_dir / "GammaAl2O3_10nm.debI.ref"))
+    got = [f"{a:g}\t{b:g}" for a, b in zip(tt, I)]
+    assert got == ref
+
+
+@pytest.mark.parametrize("maker", ["c1", "rand2", "rand5"])
+def test_intensity_matches_oracle(ctx, maker):
+    m = {"c1": lambda: synth.fcc_sphere(nq=1024),
+         "rand2": lambda: synth.small_random(3000, 2, 31, nq=512),
+         "rand5": lambda: synth.small_random(4000, 5, 32, nq=300)}[maker]()
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    I, H = ctx.intensity(mod, [_gp(p) for p in m.params], m.rstep, m.n_bins, m.q,
+                         first_n=m.first_n, with_hist=True)
+    Iref, mag = _oracle_I(m, H, m.first_n)
+    rel = np.abs(I - Iref) / np.abs(Iref)
+    assert rel.max() < TOL_FP64, (rel.max(), (mag / np.abs(Iref)).max())
+
+
+def test_weight_rule_first_n(ctx):
+    """More low bins than first_n: the extra entries at r <= 0.0005 take weight 2 with sinc."""
+    nb = 100
+    H = np.zeros((1, nb), np.uint64)
+    H[0, [0, 1, 2, 3, 4, 5, 6, 40]] = [10, 3, 4, 5, 6, 7, 8, 9]
+    K = np.array([0.7, 3.0, 11.0])
+    ff = np.ones((1, 3))
+    for first_n in (1, 3, 8):
+        I = ctx.debye(H, 1e-4, K, ff, first_n=first_n)
+        nz = np.nonzero(H[0])[0]
+        Iref = O.debye_sum(nz, H[0][nz].astype(float), 1e-4, K, ff[0], first_n=first_n)
+        assert np.allclose(I, Iref, rtol=1e-13, atol=0)

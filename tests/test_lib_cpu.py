@@ -1,0 +1,61 @@
+"""CPU-side checks of the product library: it loads, exports every entry point that
+include/debye.h declares, and its host-only grid functions follow the reference's rules
+(cuDebye-v15.cu:690-698, 734).  No device call is made here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cudebye_amd as cd
+from oracle import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    with open(os.path.join(ROOT, "include", "debye.h")) as f:
+        text = f.read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(dbx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = cd.lib()
+    names = _declared()
+    assert len(names) >= 17
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(cd.EXPORTS)
+
+
+def test_cli_built():
+    assert os.access(cd.CLI_PATH, os.X_OK)
+
+
+@pytest.mark.parametrize("hdr", [(1.5418, 10.0, 140.0, 0.02), (1.5406, 10.0, 150.0, 140.0 / 4095),
+                                 (0.7093, 2.0, 120.0, 0.005), (1.0, 0.0, 180.0, 1.0)])
+def test_q_grid_matches_oracle(hdr):
+    tt, q = cd.q_from_2theta(*hdr)
+    ott, oq = O.q_from_2theta(*hdr)
+    assert np.array_equal(tt, ott) and np.array_equal(q, oq)
+
+
+@pytest.mark.parametrize("rmax,rstep", [(356.0, 1e-4), (122.0, 1e-4), (104.0, 1e-3), (208.0, 5e-4),
+                                        (0.0, 1e-3)])
+def test_channels_matches_oracle(rmax, rstep):
+    assert cd.channels(rmax, rstep) == O.channels(rmax, rstep)
+
+
+def test_bad_grids_rejected():
+    with pytest.raises(cd.DebyeError):
+        cd.q_from_2theta(1.5, 10.0, 140.0, 0.0)
+    with pytest.raises(cd.DebyeError):
+        cd.channels(10.0, -1.0)
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(cd, "LIB_PATH", str(tmp_path / "nope.so"))
+    monkeypatch.setattr(cd, "_lib", None)
+    with pytest.raises(ImportError):
+        cd.lib()
