@@ -107,8 +107,10 @@ void kd_split(std::vector<Pt> &pts, int64_t lo, int64_t hi, std::vector<std::pai
 
 }  // namespace
 
-static void build_model(dbx_ctx *ctx, const float *x, const float *y, const float *z,
-                        const int64_t *group_n, int S, ModelDev &m)
+// host part of a model: validation, tiling, tile metadata and the tile-ordered positions.
+// Within a subgroup, tiles are ordered by the diagonal of their bounding box.
+static void tile_model_host(const float *x, const float *y, const float *z, const int64_t *group_n,
+                            int S, ModelDev &m, std::vector<float4> &pos_h)
 {
     DBX_REQUIRE(S > 0 && S <= 64, DBX_ERR_ARG, "n_groups must be in [1, 64]");
     int64_t N = 0;
@@ -126,7 +128,7 @@ static void build_model(dbx_ctx *ctx, const float *x, const float *y, const floa
     m.group_tile0.resize(S);
     m.group_tile1.resize(S);
     m.bbox.assign(6 * S, 0.f);
-    std::vector<float4> pos_h;
+    pos_h.clear();
     pos_h.reserve(N);
     m.tiles_h.clear();
     int64_t start = 0;
@@ -149,6 +151,26 @@ static void build_model(dbx_ctx *ctx, const float *x, const float *y, const floa
         std::vector<std::pair<int64_t, int64_t>> leaves;
         if (n) kd_split(pts, 0, n, leaves);
         m.group_tile0[g] = (int32_t)m.tiles_h.size();
+        // leaf radii, then leaves in ascending radius (pass 1 sizes its shells per run of
+        // consecutive tiles by their largest radius)
+        std::vector<std::pair<double, size_t>> lr(leaves.size());
+        for (size_t li = 0; li < leaves.size(); li++) {
+            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+            for (int64_t i = leaves[li].first; i < leaves[li].second; i++) {
+                const double v[3] = {pts[i].x, pts[i].y, pts[i].z};
+                for (int a = 0; a < 3; a++) {
+                    lo[a] = std::min(lo[a], v[a]);
+                    hi[a] = std::max(hi[a], v[a]);
+                }
+            }
+            double e2 = 0;
+            for (int a = 0; a < 3; a++) e2 += (hi[a] - lo[a]) * (hi[a] - lo[a]);
+            lr[li] = {e2, li};
+        }
+        std::stable_sort(lr.begin(), lr.end());
+        std::vector<std::pair<int64_t, int64_t>> sorted_leaves(leaves.size());
+        for (size_t li = 0; li < leaves.size(); li++) sorted_leaves[li] = leaves[lr[li].second];
+        leaves.swap(sorted_leaves);
         for (auto &lf : leaves) {
             double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
             for (int64_t i = lf.first; i < lf.second; i++) {
@@ -180,6 +202,13 @@ static void build_model(dbx_ctx *ctx, const float *x, const float *y, const floa
         m.group_tile1[g] = (int32_t)m.tiles_h.size();
         start += n;
     }
+}
+
+static void build_model(dbx_ctx *ctx, const float *x, const float *y, const float *z,
+                        const int64_t *group_n, int S, ModelDev &m)
+{
+    std::vector<float4> pos_h;
+    tile_model_host(x, y, z, group_n, S, m, pos_h);
     const size_t pb = sizeof(float4) * std::max<size_t>(1, pos_h.size());
     const size_t tb = sizeof(Tile) * std::max<size_t>(1, m.tiles_h.size());
     DBX_HIP(hipMalloc(&m.pos, pb));
@@ -443,6 +472,83 @@ int dbx_intensity(dbx_ctx *ctx, const dbx_model *m, const dbx_group_params *para
     });
     if (grc != DBX_OK) return grc;
     return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// diagnostics (include/debye_debug.h)
+// ---------------------------------------------------------------------------------
+#include "../../include/debye_debug.h"
+
+extern "C" {
+
+int dbx_debug_model_tiles(const dbx_model *m, void *out, int cap)
+{
+    if (!m) return DBX_ERR_ARG;
+    const int n = (int)m->d.tiles_h.size();
+    if (out && cap >= n) std::memcpy(out, m->d.tiles_h.data(), sizeof(Tile) * n);
+    return n;
+}
+
+int64_t dbx_debug_model_positions(dbx_ctx *ctx, const dbx_model *m, float *out4, int64_t cap)
+{
+    if (!m || !ctx) return DBX_ERR_ARG;
+    const int64_t n = m->d.n_atoms;
+    if (out4 && cap >= n && n > 0) {
+        int rc = guarded(ctx, [&] {
+            DBX_HIP(hipMemcpy(out4, m->d.pos, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        });
+        if (rc != DBX_OK) return rc;
+    }
+    return n;
+}
+
+int dbx_debug_pass1_plan(const dbx_model *m, double rstep, void *jobs, int cap_jobs, float *edge2,
+                         int cap_edge2, int *n_jobs, int *n_edge2)
+{
+    if (!m) return DBX_ERR_ARG;
+    try {
+        std::vector<P1Job> j;
+        std::vector<float> e;
+        plan_pass1(m->d, rstep, j, e);
+        if (n_jobs) *n_jobs = (int)j.size();
+        if (n_edge2) *n_edge2 = (int)e.size();
+        if (jobs && cap_jobs >= (int)j.size()) std::memcpy(jobs, j.data(), sizeof(P1Job) * j.size());
+        if (edge2 && cap_edge2 >= (int)e.size()) std::memcpy(edge2, e.data(), sizeof(float) * e.size());
+    } catch (const Error &e) {
+        return e.code;
+    }
+    return DBX_OK;
+}
+
+// Host-only tiling and pass-1 plan of a model given as arrays (no device needed).
+int dbx_debug_plan_host(const float *x, const float *y, const float *z, const int64_t *group_n,
+                        int n_groups, double rstep, void *tiles, int cap_tiles, float *pos4,
+                        int64_t cap_pos, void *jobs, int cap_jobs, float *edge2, int cap_edge2,
+                        int *n_tiles, int *n_jobs, int *n_edge2)
+{
+    if (!x || !group_n) return DBX_ERR_ARG;
+    try {
+        ModelDev m;
+        std::vector<float4> pos_h;
+        tile_model_host(x, y, z, group_n, n_groups, m, pos_h);
+        std::vector<P1Job> j;
+        std::vector<float> e;
+        plan_pass1(m, rstep, j, e);
+        if (n_tiles) *n_tiles = (int)m.tiles_h.size();
+        if (n_jobs) *n_jobs = (int)j.size();
+        if (n_edge2) *n_edge2 = (int)e.size();
+        if (tiles && cap_tiles >= (int)m.tiles_h.size())
+            std::memcpy(tiles, m.tiles_h.data(), sizeof(Tile) * m.tiles_h.size());
+        if (pos4 && cap_pos >= (int64_t)pos_h.size())
+            std::memcpy(pos4, pos_h.data(), sizeof(float4) * pos_h.size());
+        if (jobs && cap_jobs >= (int)j.size()) std::memcpy(jobs, j.data(), sizeof(P1Job) * j.size());
+        if (edge2 && cap_edge2 >= (int)e.size()) std::memcpy(edge2, e.data(), sizeof(float) * e.size());
+    } catch (const Error &e) {
+        return e.code;
+    }
+    return DBX_OK;
 }
 
 }  // extern "C"

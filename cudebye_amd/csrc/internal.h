@@ -122,6 +122,8 @@ struct P1Stats {
 };
 P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t n_bins,
                          uint64_t *d_hist, bool sync_stats = true);
+// host planning of pass 1: work items (largest first) and the shell-edge tables
+void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &jobs, std::vector<float> &edge2);
 
 // pass 2 (pass2.hip)
 void prefactor_dev(dbx_ctx *ctx, const dbx_group_params &ga, const dbx_group_params &gb,

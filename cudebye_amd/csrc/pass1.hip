@@ -156,19 +156,16 @@ double point_box_max(const Tile &t, const float *bb)
 
 }  // namespace
 
-P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t n_bins,
-                         uint64_t *d_hist, bool sync_stats)
+void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
+                std::vector<float> &edge2)
 {
     DBX_REQUIRE(rstep > 0 && std::isfinite(rstep), DBX_ERR_ARG, "rstep must be positive");
-    DBX_REQUIRE(n_bins > 0 && n_bins < (int64_t)1 << 40, DBX_ERR_ARG, "bad n_bins");
     const int S = m.n_groups;
-    const int P = S * (S + 1) / 2;
-    const float rp = (float)(1 / rstep);  // cuDebye-v15.cu:739
     const double usable = (kP1Bins - 6) * rstep;
 
     std::vector<P1Job> jobs;
     std::vector<double> cost;
-    std::vector<float> edge2;
+    edge2.clear();
     int p = 0;
     for (int ga = 0; ga < S; ga++) {
         for (int gb = ga; gb < S; gb++, p++) {
@@ -185,54 +182,58 @@ P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t 
                                     std::fabs((double)bbA[3 + a] - bbB[a]));
                 D2 += e * e;
             }
-            const double D = std::sqrt(D2) + 1e-3;
+            const double D = std::sqrt(D2) + 1e-3;  // bound on |b - c_A|
             const double mg = 2e-3 + 4e-6 * (D + radA);
-            double h = usable - 2 * radA - 2 * mg;
-            bool degraded = false;
-            if (h < 0.25 * usable) {
-                h = 0.25 * usable;
-                degraded = true;
-            }
-            int ns = (int)(D / h) + 1;
-            const int e2_off = (int)edge2.size();
-            edge2.push_back(-1.0f);
-            for (int s = 1; s < ns; s++) edge2.push_back((float)((s * h) * (s * h)));
-            edge2.push_back(INFINITY);
-
-            // A chunks: enough pairs per job to amortise the LDS window flush
+            const double h_min = std::max(0.5, 0.02 * usable);
+            // A chunks (tiles of a subgroup are stored in ascending radius, so a chunk's
+            // largest radius is close to its typical one); CA sized for enough pairs per job
+            // to amortise the LDS window flush
+            const double rad_typ = m.tiles_h[a0 + (a1 - a0) / 2].rad;
+            const double h_typ = std::max(h_min, usable - 2 * rad_typ - 2 * mg);
             const double nB = (double)m.group_n[gb];
-            const double shell_frac = std::min(1.0, h / std::max(D, h));
+            const double shell_frac = std::min(1.0, h_typ / std::max(D, h_typ));
             const double per_tile = 64.0 * std::max(1.0, nB * shell_frac);
             int CA = (int)std::ceil(6.0e6 / per_tile);
             CA = std::max(1, std::min(CA, 32));
-            // sample of B tiles for cost estimates
             const int nbt = b1 - b0;
             const int nchunks = (a1 - a0 + CA - 1) / CA;
-            int stride = (int)std::max<int64_t>(1, ((int64_t)nchunks * nbt) / 2000000);
-            std::vector<double> shell_cost(ns);
+            const int stride = (int)std::max<int64_t>(1, ((int64_t)nchunks * nbt) / 2000000);
             for (int c0 = a0; c0 < a1; c0 += CA) {
                 const int c1 = std::min(a1, c0 + CA);
-                double dmin = 1e300, dmax = 0, rmaxc = 0, cx = 0, cy = 0, cz = 0, na = 0;
+                double dmin = 1e300, dmax = 0, rmaxc = 0, na = 0;
                 for (int t = c0; t < c1; t++) {
                     const Tile &T = m.tiles_h[t];
                     dmin = std::min(dmin, point_box_min(T, bbB));
                     dmax = std::max(dmax, point_box_max(T, bbB));
                     rmaxc = std::max(rmaxc, (double)T.rad);
-                    cx += T.cx * T.count; cy += T.cy * T.count; cz += T.cz * T.count;
                     na += T.count;
                 }
-                cx /= na; cy /= na; cz /= na;
-                std::fill(shell_cost.begin(), shell_cost.end(), 0.0);
+                // shells of this chunk: width h such that [s h - rad, (s+1) h + rad] fits the
+                // LDS window; narrower windows than that (fine bins, wide tiles) fall back to
+                // a window centred on the shell, with the rest of the pairs sent to HBM
+                double h = usable - 2 * rmaxc - 2 * mg;
+                const bool degraded = h < h_min;
+                if (degraded) h = h_min;
+                const int ns = (int)(D / h) + 1;
+                const int e2_off = (int)edge2.size();
+                edge2.push_back(-1.0f);
+                for (int sh = 1; sh < ns; sh++) edge2.push_back((float)((sh * h) * (sh * h)));
+                edge2.push_back(INFINITY);
+                // cost estimate: B tiles by centre distance from each A tile (sampled)
+                std::vector<double> shell_cost(ns, 0.0);
                 const int jfirst = (gb == ga) ? c0 : b0;
-                for (int j = jfirst; j < b1; j += stride) {
-                    const Tile &T = m.tiles_h[j];
-                    double dx = T.cx - cx, dy = T.cy - cy, dz = T.cz - cz;
-                    int s = (int)(std::sqrt(dx * dx + dy * dy + dz * dz) / h);
-                    if (s >= ns) s = ns - 1;
-                    shell_cost[s] += (double)T.count * stride;
+                for (int t = c0; t < c1; t += std::max(1, (c1 - c0) / 4)) {
+                    const Tile &A = m.tiles_h[t];
+                    for (int j = std::max(jfirst, gb == ga ? t : b0); j < b1; j += stride) {
+                        const Tile &T = m.tiles_h[j];
+                        const double dx = T.cx - A.cx, dy = T.cy - A.cy, dz = T.cz - A.cz;
+                        int sh = (int)(std::sqrt(dx * dx + dy * dy + dz * dz) / h);
+                        if (sh >= ns) sh = ns - 1;
+                        shell_cost[sh] += (double)T.count * stride;
+                    }
                 }
-                for (int s = 0; s < ns; s++) {
-                    if (s * h > dmax + mg + 1e-3 || (s + 1) * h < dmin - mg - 1e-3) continue;
+                for (int sh = 0; sh < ns; sh++) {
+                    if (sh * h > dmax + mg + 1e-3 || (sh + 1) * h < dmin - mg - 1e-3) continue;
                     P1Job J;
                     std::memset(&J, 0, sizeof(J));
                     J.pair = p;
@@ -241,40 +242,53 @@ P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t 
                     J.b0 = b0;
                     J.b1 = b1;
                     J.same = (ga == gb);
-                    J.shell = s;
+                    J.shell = sh;
                     J.e2_off = e2_off;
                     double klo;
                     if (!degraded)
-                        klo = std::floor((s * h - rmaxc - mg) / rstep) - 2;
+                        klo = std::floor((sh * h - rmaxc - mg) / rstep) - 2;
                     else
-                        klo = std::floor(((s + 0.5) * h) / rstep) - kP1Bins / 2;
+                        klo = std::floor(((sh + 0.5) * h) / rstep) - kP1Bins / 2;
                     if (klo < 0) klo = 0;
                     J.k_lo = (int32_t)std::min<double>(klo, (double)INT32_MAX - kP1Bins);
                     jobs.push_back(J);
-                    // neighbouring shells share straddling tiles: spread the sample cost
-                    double c = shell_cost[s];
-                    if (s > 0) c += 0.25 * shell_cost[s - 1];
-                    if (s + 1 < ns) c += 0.25 * shell_cost[s + 1];
+                    // neighbouring shells share straddling tiles: spread the sampled cost
+                    double c = shell_cost[sh];
+                    if (sh > 0) c += 0.25 * shell_cost[sh - 1];
+                    if (sh + 1 < ns) c += 0.25 * shell_cost[sh + 1];
                     cost.push_back(na * (c + 1.0));
                 }
             }
         }
     }
 
-    P1Stats st;
-    st.n_jobs = (int)jobs.size();
-    DBX_HIP(hipMemsetAsync(d_hist, 0, sizeof(uint64_t) * (size_t)P * n_bins, ctx->stream));
-    DBX_HIP(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(uint64_t), ctx->stream));
-    if (jobs.empty()) {
-        if (sync_stats) DBX_HIP(hipStreamSynchronize(ctx->stream));
-        return st;
-    }
     // largest jobs first
     std::vector<int> order(jobs.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-    std::vector<P1Job> sorted(jobs.size());
+    sorted.resize(jobs.size());
     for (size_t i = 0; i < order.size(); i++) sorted[i] = jobs[order[i]];
+}
+
+P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t n_bins,
+                         uint64_t *d_hist, bool sync_stats)
+{
+    DBX_REQUIRE(n_bins > 0 && n_bins < (int64_t)1 << 40, DBX_ERR_ARG, "bad n_bins");
+    const int S = m.n_groups;
+    const int P = S * (S + 1) / 2;
+    const float rp = (float)(1 / rstep);  // cuDebye-v15.cu:739
+    std::vector<P1Job> sorted;
+    std::vector<float> edge2;
+    plan_pass1(m, rstep, sorted, edge2);
+
+    P1Stats st;
+    st.n_jobs = (int)sorted.size();
+    DBX_HIP(hipMemsetAsync(d_hist, 0, sizeof(uint64_t) * (size_t)P * n_bins, ctx->stream));
+    DBX_HIP(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(uint64_t), ctx->stream));
+    if (sorted.empty()) {
+        if (sync_stats) DBX_HIP(hipStreamSynchronize(ctx->stream));
+        return st;
+    }
 
     const size_t jb = sizeof(P1Job) * sorted.size();
     const size_t eb = sizeof(float) * edge2.size();
