@@ -65,11 +65,13 @@ struct P1Job {
     int32_t shell;  // index into the pair's shell-edge table
     int32_t k_lo;   // first bin of the LDS window
     int32_t e2_off; // offset of the pair's shell-edge table in the edge array
-    int32_t pad[3];
+    int32_t degraded;  // 1: the window cannot hold the job's geometry (checked kernel)
+    int32_t pad[2];
 };
 
 constexpr int kP1Threads = 1024;   // 16 waves of 64
-constexpr int kP1Bins = 36864;     // LDS window: 144 KiB of uint32 counters
+constexpr int kP1Bins = 36864;     // LDS window: 144 KiB of uint32 counters (+1 sentinel)
+constexpr int kP1Cand = 3072;      // LDS candidate list (A tile, B tile), 12 KiB
 
 struct ModelDev {
     int n_groups = 0;
@@ -97,10 +99,13 @@ struct dbx_ctx {
     // grow-only device scratch, one buffer per slot (dbx::kSlot*)
     void *slot_ptr[8] = {};
     size_t slot_bytes[8] = {};
-    uint64_t *counters = nullptr;  // [0] out-of-range pairs, [1] pairs outside the LDS window
+    // [0] out-of-range pairs, [1] pairs outside the LDS window (checked kernel),
+    // [2] fast jobs queued for the checked kernel; then the retry list (int32)
+    uint64_t *counters = nullptr;
     int cu_count = 0;
     int64_t last_p1_jobs = 0;
     uint64_t last_p1_misses = 0;
+    uint64_t last_p1_retried = 0;
     float last_p1_kernel_ms = 0.f;  // hipEvent time of the last pass-1 histogram kernel
     float last_p2_ms = 0.f;         // hipEvent time of the last pass-2 structure computation
     hipEvent_t ev[4] = {};
@@ -118,12 +123,15 @@ void *scratch(dbx_ctx *ctx, int slot, size_t bytes);
 struct P1Stats {
     uint64_t out_of_range = 0;
     uint64_t window_misses = 0;
+    uint64_t retried = 0;
     int n_jobs = 0;
+    int n_checked = 0;
 };
 P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t n_bins,
                          uint64_t *d_hist, bool sync_stats = true);
 // host planning of pass 1: work items (largest first) and the shell-edge tables
-void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &jobs, std::vector<float> &edge2);
+void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &jobs, std::vector<float> &edge2,
+                int n_cu = 256);
 
 // pass 2 (pass2.hip)
 void prefactor_dev(dbx_ctx *ctx, const dbx_group_params &ga, const dbx_group_params &gb,

@@ -35,15 +35,43 @@ __device__ __forceinline__ float rd_lane(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// One workgroup runs one job: it zeroes its LDS window, culls the job's (A tile, B tile)
+// pairs into an LDS candidate list (B tiles with an atom possibly in the shell), then its
+// 16 waves take candidates one at a time from that list, so work spreads evenly over the
+// waves whatever the shape of the job.  For each candidate the wave loads the B tile into
+// its lanes, keeps the atoms inside the shell (ballot) and streams them past its 64 A atoms.
+//
+// CHECKED = false: every pair's bin is assumed to fall in the window (the planner sized the
+// shells for that).  An out-of-window bin is clamped into a sentinel counter h[kP1Bins]; a
+// job whose sentinel is nonzero flushes nothing and is queued for the checked variant.
+// CHECKED = true: out-of-window pairs go straight to the uint64 histogram in HBM (jobs whose
+// window is narrower than their geometry, and the queued ones).
+template <bool CHECKED>
 __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const float4 *__restrict__ pos, const Tile *__restrict__ tiles, const P1Job *__restrict__ jobs,
-    const float *__restrict__ edge2, float rp, unsigned long long *__restrict__ hist,
-    int64_t n_bins, unsigned long long *__restrict__ counters)
+    int n_fast, int n_checked, const float *__restrict__ edge2, float rp,
+    unsigned long long *__restrict__ hist, int64_t n_bins, unsigned long long *__restrict__ counters,
+    int *__restrict__ retry)
 {
-    __shared__ uint32_t h[kP1Bins];
-    const P1Job job = jobs[blockIdx.x];
-    for (int i = threadIdx.x; i < kP1Bins; i += kP1Threads) h[i] = 0u;
-    __syncthreads();
+    __shared__ uint32_t h[kP1Bins + 1];
+    __shared__ uint32_t cand[kP1Cand];
+    __shared__ int s_ncand, s_next;
+
+    int job_index;
+    if (!CHECKED) {
+        job_index = blockIdx.x;
+    } else {
+        // checked launch: the planned checked jobs, then the fast jobs queued for a re-run
+        if ((int)blockIdx.x < n_checked) {
+            job_index = n_fast + blockIdx.x;
+        } else {
+            const int r = blockIdx.x - n_checked;
+            if (r >= (int)counters[2]) return;
+            job_index = retry[r];
+        }
+    }
+    const P1Job job = jobs[job_index];
+    for (int i = threadIdx.x; i <= kP1Bins; i += kP1Threads) h[i] = 0u;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -53,42 +81,79 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const float r_hi = sqrtf(e_hi);  // +inf for the last shell
     const int k_lo = job.k_lo;
     unsigned long long *hrow = hist + (int64_t)job.pair * n_bins;
+    const int nblk = (job.b1 - job.b0 + 63) >> 6;
+    const int units = (job.a1 - job.a0) * nblk;
     uint32_t miss = 0, oor = 0;
 
-    for (int ia = job.a0; ia < job.a1; ++ia) {
-        const Tile ta = tiles[ia];
-        const bool a_ok = lane < ta.count;
-        const float4 pa = pos[ta.start + (a_ok ? lane : 0)];
-        const int jbeg = job.same ? ia : job.b0;
-        for (int jb = jbeg + wave * 64; jb < job.b1; jb += (kP1Threads / 64) * 64) {
-            // cull 64 B tiles at once: keep those with an atom possibly in the shell
-            bool cand = false;
-            const int j = jb + lane;
-            if (j < job.b1) {
+    int cur_ia = -1;
+    Tile ta = {};
+    bool a_ok = false;
+    float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
+
+    for (int u0 = 0; u0 < units; u0 += kP1Cand / 64) {
+        if (threadIdx.x == 0) {
+            s_ncand = 0;
+            s_next = 0;
+        }
+        __syncthreads();
+        // cull: unit = (A tile, block of 64 B tiles); candidates appended to the LDS list
+        const int u1 = min(units, u0 + kP1Cand / 64);
+        for (int u = u0 + wave; u < u1; u += kP1Threads / 64) {
+            const int ial = u / nblk;
+            const int ia = job.a0 + ial;
+            const int j = job.b0 + (u - ial * nblk) * 64 + lane;
+            const Tile A = tiles[ia];
+            bool c = false;
+            if (j < job.b1 && (!job.same || j >= ia)) {
                 const Tile tb = tiles[j];
-                const float dx = tb.cx - ta.cx, dy = tb.cy - ta.cy, dz = tb.cz - ta.cz;
+                const float dx = tb.cx - A.cx, dy = tb.cy - A.cy, dz = tb.cz - A.cz;
                 const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
                 const float mg = 1e-3f + 1e-5f * (d + tb.rad);
-                cand = (d + tb.rad + mg >= r_lo) && (d - tb.rad - mg < r_hi);
+                c = (d + tb.rad + mg >= r_lo) && (d - tb.rad - mg < r_hi);
             }
-            uint64_t cm = __ballot(cand);
-            while (cm) {
-                const int l = __builtin_ctzll(cm);
-                cm &= cm - 1;
-                const int j2 = jb + l;
-                const Tile tb = tiles[j2];
-                const bool b_ok = lane < tb.count;
-                const float4 pb = pos[tb.start + (b_ok ? lane : 0)];
-                const float ex = pb.x - ta.cx, ey = pb.y - ta.cy, ez = pb.z - ta.cz;
-                const float d2c = (ex * ex + ey * ey) + ez * ez;
-                uint64_t bm = __ballot(b_ok && d2c >= e_lo && d2c < e_hi);
-                const bool diag = job.same && (j2 == ia);
-                while (bm) {
-                    const int m = __builtin_ctzll(bm);
-                    bm &= bm - 1;
-                    const float bx = rd_lane(pb.x, m), by = rd_lane(pb.y, m), bz = rd_lane(pb.z, m);
-                    if (a_ok && (!diag || lane <= m)) {
-                        const int k = pair_bin(pa.x, pa.y, pa.z, bx, by, bz, rp);
+            const uint64_t cm = __ballot(c);
+            if (cm) {
+                int base = 0;
+                if (lane == 0) base = atomicAdd(&s_ncand, __popcll(cm));
+                base = __builtin_amdgcn_readfirstlane(base);
+                const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(cm >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)cm, 0));
+                if (c) cand[base + below] = ((uint32_t)ial << 24) | (uint32_t)(j - job.b0);
+            }
+        }
+        __syncthreads();
+        const int nc = s_ncand;
+        for (;;) {
+            int ci = 0;
+            if (lane == 0) ci = atomicAdd(&s_next, 1);
+            ci = __builtin_amdgcn_readfirstlane(ci);
+            if (ci >= nc) break;
+            const uint32_t e = cand[ci];
+            const int ia = job.a0 + (int)(e >> 24);
+            const int j2 = job.b0 + (int)(e & 0xFFFFFFu);
+            if (ia != cur_ia) {
+                cur_ia = ia;
+                ta = tiles[ia];
+                a_ok = lane < ta.count;
+                pa = pos[ta.start + (a_ok ? lane : 0)];
+            }
+            const Tile tb = tiles[j2];
+            const bool b_ok = lane < tb.count;
+            const float4 pb = pos[tb.start + (b_ok ? lane : 0)];
+            const float ex = pb.x - ta.cx, ey = pb.y - ta.cy, ez = pb.z - ta.cz;
+            const float d2c = (ex * ex + ey * ey) + ez * ez;
+            uint64_t bm = __ballot(b_ok && d2c >= e_lo && d2c < e_hi);
+            const bool diag = job.same && (j2 == ia);
+            while (bm) {
+                const int m = __builtin_ctzll(bm);
+                bm &= bm - 1;
+                const float bx = rd_lane(pb.x, m), by = rd_lane(pb.y, m), bz = rd_lane(pb.z, m);
+                if (a_ok && (!diag || lane <= m)) {
+                    const int k = pair_bin(pa.x, pa.y, pa.z, bx, by, bz, rp);
+                    if (!CHECKED) {
+                        const unsigned kk = min((unsigned)(k - k_lo), (unsigned)kP1Bins);
+                        atomicAdd(&h[kk], 1u);
+                    } else {
                         const unsigned kk = (unsigned)(k - k_lo);
                         if (kk < (unsigned)kP1Bins) {
                             atomicAdd(&h[kk], 1u);
@@ -103,17 +168,26 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                 }
             }
         }
+        __syncthreads();
     }
-    // wave totals of the rare counters
-    for (int off = 32; off > 0; off >>= 1) {
-        miss += __shfl_down(miss, off);
-        oor += __shfl_down(oor, off);
+    __syncthreads();  // zeroing visible to all threads even when the job had no candidates
+    if (CHECKED) {
+        for (int off = 32; off > 0; off >>= 1) {
+            miss += __shfl_down(miss, off);
+            oor += __shfl_down(oor, off);
+        }
+        if (lane == 0) {
+            if (miss) atomicAdd(&counters[1], (unsigned long long)miss);
+            if (oor) atomicAdd(&counters[0], (unsigned long long)oor);
+        }
+    } else if (h[kP1Bins] != 0u) {
+        // a bin fell outside the window: leave this job to the checked variant
+        if (threadIdx.x == 0) {
+            const unsigned long long r = atomicAdd(&counters[2], 1ull);
+            retry[r] = job_index;
+        }
+        return;
     }
-    if (lane == 0) {
-        if (miss) atomicAdd(&counters[1], (unsigned long long)miss);
-        if (oor) atomicAdd(&counters[0], (unsigned long long)oor);
-    }
-    __syncthreads();
     for (int i = threadIdx.x; i < kP1Bins; i += kP1Threads) {
         const uint32_t v = h[i];
         if (v) {
@@ -157,7 +231,7 @@ double point_box_max(const Tile &t, const float *bb)
 }  // namespace
 
 void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
-                std::vector<float> &edge2)
+                std::vector<float> &edge2, int n_cu)
 {
     DBX_REQUIRE(rstep > 0 && std::isfinite(rstep), DBX_ERR_ARG, "rstep must be positive");
     const int S = m.n_groups;
@@ -251,6 +325,7 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                         klo = std::floor(((sh + 0.5) * h) / rstep) - kP1Bins / 2;
                     if (klo < 0) klo = 0;
                     J.k_lo = (int32_t)std::min<double>(klo, (double)INT32_MAX - kP1Bins);
+                    J.degraded = degraded ? 1 : 0;
                     jobs.push_back(J);
                     // neighbouring shells share straddling tiles: spread the sampled cost
                     double c = shell_cost[sh];
@@ -262,12 +337,36 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
         }
     }
 
-    // largest jobs first
-    std::vector<int> order(jobs.size());
+    // split jobs far above the mean share of a CU along their B-tile range (the candidates of
+    // a shell are spread over the whole range: tiles are ordered by size, not by position)
+    double total = 0;
+    for (double c : cost) total += c;
+    const double target = total / (double)(std::max(1, n_cu) * 4);
+    std::vector<P1Job> split_jobs;
+    std::vector<double> split_cost;
+    for (size_t i = 0; i < jobs.size(); i++) {
+        const P1Job &J = jobs[i];
+        const int nb = J.b1 - J.b0;
+        int pieces = (int)std::min<double>(std::ceil(cost[i] / std::max(target, 1.0)), (double)nb);
+        pieces = std::max(1, std::min(pieces, 64));
+        for (int k = 0; k < pieces; k++) {
+            P1Job K = J;
+            K.b0 = J.b0 + (int)((int64_t)nb * k / pieces);
+            K.b1 = J.b0 + (int)((int64_t)nb * (k + 1) / pieces);
+            if (K.b1 <= K.b0) continue;
+            split_jobs.push_back(K);
+            split_cost.push_back(cost[i] / pieces);
+        }
+    }
+    // launch order: exact-window jobs first, largest first; then the degraded ones
+    std::vector<int> order(split_jobs.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
-    sorted.resize(jobs.size());
-    for (size_t i = 0; i < order.size(); i++) sorted[i] = jobs[order[i]];
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        if (split_jobs[x].degraded != split_jobs[y].degraded) return split_jobs[x].degraded < split_jobs[y].degraded;
+        return split_cost[x] > split_cost[y];
+    });
+    sorted.resize(split_jobs.size());
+    for (size_t i = 0; i < order.size(); i++) sorted[i] = split_jobs[order[i]];
 }
 
 P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t n_bins,
@@ -279,38 +378,51 @@ P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t 
     const float rp = (float)(1 / rstep);  // cuDebye-v15.cu:739
     std::vector<P1Job> sorted;
     std::vector<float> edge2;
-    plan_pass1(m, rstep, sorted, edge2);
+    plan_pass1(m, rstep, sorted, edge2, ctx->cu_count);
+    int n_fast = 0;
+    while (n_fast < (int)sorted.size() && !sorted[n_fast].degraded) n_fast++;
+    const int n_checked = (int)sorted.size() - n_fast;
 
     P1Stats st;
     st.n_jobs = (int)sorted.size();
+    st.n_checked = n_checked;
     DBX_HIP(hipMemsetAsync(d_hist, 0, sizeof(uint64_t) * (size_t)P * n_bins, ctx->stream));
-    DBX_HIP(hipMemsetAsync(ctx->counters, 0, 2 * sizeof(uint64_t), ctx->stream));
-    if (sorted.empty()) {
-        if (sync_stats) DBX_HIP(hipStreamSynchronize(ctx->stream));
-        return st;
+    DBX_HIP(hipMemsetAsync(ctx->counters, 0, 4 * sizeof(uint64_t), ctx->stream));
+    if (!sorted.empty()) {
+        const size_t jb = sizeof(P1Job) * sorted.size();
+        const size_t eb = sizeof(float) * edge2.size();
+        const size_t rb = sizeof(int) * (size_t)(n_fast + 1);
+        auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+        char *buf = (char *)scratch(ctx, kSlotPass1, up(jb) + up(eb) + up(rb));
+        P1Job *d_jobs = (P1Job *)buf;
+        float *d_e2 = (float *)(buf + up(jb));
+        int *d_retry = (int *)(buf + up(jb) + up(eb));
+        DBX_HIP(hipMemcpyAsync(d_jobs, sorted.data(), jb, hipMemcpyHostToDevice, ctx->stream));
+        DBX_HIP(hipMemcpyAsync(d_e2, edge2.data(), eb, hipMemcpyHostToDevice, ctx->stream));
+        unsigned long long *cnt = (unsigned long long *)ctx->counters;
+        DBX_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+        if (n_fast > 0)
+            hipLaunchKernelGGL(p1_histogram_kernel<false>, dim3((unsigned)n_fast), dim3(kP1Threads), 0,
+                               ctx->stream, m.pos, m.tiles, d_jobs, n_fast, n_checked, d_e2, rp,
+                               (unsigned long long *)d_hist, n_bins, cnt, d_retry);
+        // checked variant: the degraded jobs plus whatever the fast launch queued
+        hipLaunchKernelGGL(p1_histogram_kernel<true>, dim3((unsigned)(n_checked + n_fast)),
+                           dim3(kP1Threads), 0, ctx->stream, m.pos, m.tiles, d_jobs, n_fast, n_checked,
+                           d_e2, rp, (unsigned long long *)d_hist, n_bins, cnt, d_retry);
+        DBX_HIP(hipGetLastError());
+        DBX_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
     }
-
-    const size_t jb = sizeof(P1Job) * sorted.size();
-    const size_t eb = sizeof(float) * edge2.size();
-    char *buf = (char *)scratch(ctx, kSlotPass1, jb + eb + 256);
-    P1Job *d_jobs = (P1Job *)buf;
-    float *d_e2 = (float *)(buf + ((jb + 255) & ~(size_t)255));
-    DBX_HIP(hipMemcpyAsync(d_jobs, sorted.data(), jb, hipMemcpyHostToDevice, ctx->stream));
-    DBX_HIP(hipMemcpyAsync(d_e2, edge2.data(), eb, hipMemcpyHostToDevice, ctx->stream));
-    DBX_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
-    hipLaunchKernelGGL(p1_histogram_kernel, dim3((unsigned)sorted.size()), dim3(kP1Threads), 0,
-                       ctx->stream, m.pos, m.tiles, d_jobs, d_e2, rp,
-                       (unsigned long long *)d_hist, n_bins, (unsigned long long *)ctx->counters);
-    DBX_HIP(hipGetLastError());
-    DBX_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
     if (sync_stats) {
-        uint64_t c[2];
+        uint64_t c[3];
         DBX_HIP(hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
         DBX_HIP(hipStreamSynchronize(ctx->stream));
         st.out_of_range = c[0];
         st.window_misses = c[1];
+        st.retried = c[2];
         ctx->last_p1_misses = c[1];
-        DBX_HIP(hipEventElapsedTime(&ctx->last_p1_kernel_ms, ctx->ev[0], ctx->ev[1]));
+        ctx->last_p1_retried = c[2];
+        if (!sorted.empty())
+            DBX_HIP(hipEventElapsedTime(&ctx->last_p1_kernel_ms, ctx->ev[0], ctx->ev[1]));
     }
     ctx->last_p1_jobs = st.n_jobs;
     return st;

@@ -19,7 +19,7 @@ int dbx_debug_model_tiles(const dbx_model *m, void *out, int cap);
  * copies when cap >= count. */
 int64_t dbx_debug_model_positions(dbx_ctx *ctx, const dbx_model *m, float *out4, int64_t cap);
 /* Pass-1 plan for (m, rstep): work items of 12 int32 each (pair, a0, a1, b0, b1, same, shell,
- * k_lo, e2_off, pad x3) in launch order, and the shell-edge table (float, squared A^2).
+ * k_lo, e2_off, degraded, pad x2) in launch order, and the shell-edge table (float, squared A^2).
  * Returns DBX_OK and the counts; copies when the capacities suffice. */
 int dbx_debug_pass1_plan(const dbx_model *m, double rstep, void *jobs, int cap_jobs, float *edge2,
                          int cap_edge2, int *n_jobs, int *n_edge2);

@@ -1,0 +1,102 @@
+"""Pass-1 work plan on the host (no device): the planner's jobs cover every pair of every
+subgroup pair exactly once, and every job the fast kernel runs (exact window) holds all of
+its pairs' bins inside its LDS window.  The kernel's shell test and bin rule are emulated in
+NumPy float32 (IEEE, unfused), so these checks see the same per-pair values as the device."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import cudebye_amd as cd
+from cudebye_amd import synth
+
+W = 36864  # LDS window bins (kP1Bins)
+
+
+def host_plan(m):
+    L = cd.lib()
+    fn = L.dbx_debug_plan_host
+    fn.restype = C.c_int
+    x, y, z = (np.ascontiguousarray(v, np.float32) for v in (m.x, m.y, m.z))
+    gn = np.ascontiguousarray(m.group_n, np.int64)
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    nt, nj, ne = C.c_int(), C.c_int(), C.c_int()
+    rc = fn(p(x), p(y), p(z), p(gn), len(gn), C.c_double(m.rstep), None, 0, None, C.c_int64(0),
+            None, 0, None, 0, C.byref(nt), C.byref(nj), C.byref(ne))
+    assert rc == 0
+    tiles = np.zeros((nt.value, 8), np.int32)
+    pos = np.zeros((m.n_atoms, 4), np.float32)
+    jobs = np.zeros((nj.value, 12), np.int32)
+    e2 = np.zeros(ne.value, np.float32)
+    rc = fn(p(x), p(y), p(z), p(gn), len(gn), C.c_double(m.rstep), p(tiles), nt.value, p(pos),
+            C.c_int64(m.n_atoms), p(jobs), nj.value, p(e2), ne.value, C.byref(nt), C.byref(nj),
+            C.byref(ne))
+    assert rc == 0
+    return tiles, pos, jobs, e2
+
+
+def _bins(A, B, rp):
+    d = (B[None, :, :] - A[:, None, :]).astype(np.float32)
+    s = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+    v = np.sqrt(s) * np.float32(rp)
+    return v.astype(np.int64) + ((v - np.floor(v)) >= 0.5)
+
+
+def check_plan(m):
+    tiles, pos, jobs, e2 = host_plan(m)
+    tf = tiles.view(np.float32)
+    cen, rad, start, cnt, grp = tf[:, :3], tf[:, 3], tiles[:, 4], tiles[:, 5], tiles[:, 6]
+    rp = np.float32(1.0 / m.rstep)
+    S = len(m.group_n)
+    assert np.all(cnt >= 1) and np.all(cnt <= 64)
+    # tiles partition every subgroup
+    gstart = np.concatenate([[0], np.cumsum(m.group_n)])
+    for g in range(S):
+        sel = grp == g
+        assert cnt[sel].sum() == m.group_n[g]
+    pairs = [(a, b) for a in range(S) for b in range(a, S)]
+    seen = {}
+    counted = np.zeros(len(pairs), np.int64)
+    for J in jobs:
+        pair, a0, a1, b0, b1, same, shell, klo, eoff, degraded = J[:10]
+        ga, gb = pairs[pair]
+        lo, hi = e2[eoff + shell], e2[eoff + shell + 1]
+        for t in range(a0, a1):
+            assert grp[t] == ga
+            A = pos[start[t]:start[t] + cnt[t], :3]
+            for j in range(max(b0, t) if same else b0, b1):
+                assert grp[j] == gb
+                key = (pair, t, j, int(shell))
+                assert key not in seen, key  # each (A tile, B tile, shell) once
+                seen[key] = True
+                B = pos[start[j]:start[j] + cnt[j], :3]
+                e = (B - cen[t]).astype(np.float32)
+                d2c = (e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]) + e[:, 2] * e[:, 2]
+                sel = (d2c >= lo) & (d2c < hi)
+                if not np.any(sel):
+                    continue
+                k = _bins(A, B[sel], rp)
+                if same and j == t:
+                    ia = np.arange(cnt[t])[:, None]
+                    ib = np.arange(cnt[j])[sel][None, :]
+                    valid = ia <= ib
+                else:
+                    valid = np.ones(k.shape, bool)
+                counted[pair] += int(valid.sum())
+                if not degraded:
+                    inside = (k >= klo) & (k < klo + W)
+                    assert np.all(inside | ~valid), (pair, t, j, shell)
+    n = m.group_n
+    want = [n[a] * (n[a] + 1) // 2 if a == b else n[a] * n[b] for a, b in pairs]
+    assert list(counted) == want
+    return jobs
+
+
+@pytest.mark.parametrize("maker", ["c1", "rand3", "batch3", "batch0"])
+def test_plan_covers_every_pair_once_and_windows_hold(maker):
+    m = {"c1": synth.fcc_sphere,
+         "rand3": lambda: synth.small_random(3000, 3, 41, spread=20.0),
+         "batch3": lambda: synth.batch_model(3),
+         "batch0": lambda: synth.batch_model(0)}[maker]()
+    jobs = check_plan(m)
+    assert len(jobs) > 0
