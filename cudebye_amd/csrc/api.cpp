@@ -64,9 +64,10 @@ struct Pt {
 };
 
 // Recursive median bisection of pts[lo, hi) along the longest bounding-box axis into
-// tiles of at most kTileAtoms atoms; the left part always receives a whole number of
-// full tiles, so only the last tile of a subgroup can be short.
-void kd_split(std::vector<Pt> &pts, int64_t lo, int64_t hi, std::vector<std::pair<int64_t, int64_t>> &out)
+// leaves of at most `unit` atoms; the left part always receives a whole number of full
+// leaves, so only the last leaf of a range can be short.
+void kd_split(std::vector<Pt> &pts, int64_t lo, int64_t hi, int64_t unit,
+              std::vector<std::pair<int64_t, int64_t>> &out)
 {
     struct Range { int64_t lo, hi; };
     std::vector<Range> stack{{lo, hi}};
@@ -75,7 +76,7 @@ void kd_split(std::vector<Pt> &pts, int64_t lo, int64_t hi, std::vector<std::pai
         Range r = stack.back();
         stack.pop_back();
         const int64_t n = r.hi - r.lo;
-        if (n <= kTileAtoms) {
+        if (n <= unit) {
             leaves.push_back(r);
             continue;
         }
@@ -90,8 +91,8 @@ void kd_split(std::vector<Pt> &pts, int64_t lo, int64_t hi, std::vector<std::pai
         int ax = 0;
         for (int a = 1; a < 3; a++)
             if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
-        const int64_t nt = (n + kTileAtoms - 1) / kTileAtoms;
-        const int64_t nl = (nt / 2) * kTileAtoms;
+        const int64_t nt = (n + unit - 1) / unit;
+        const int64_t nl = (nt / 2) * unit;
         auto key = [ax](const Pt &p) { return ax == 0 ? p.x : (ax == 1 ? p.y : p.z); };
         std::nth_element(pts.begin() + r.lo, pts.begin() + r.lo + nl, pts.begin() + r.hi,
                          [&](const Pt &a, const Pt &b) {
@@ -107,8 +108,40 @@ void kd_split(std::vector<Pt> &pts, int64_t lo, int64_t hi, std::vector<std::pai
 
 }  // namespace
 
-// host part of a model: validation, tiling, tile metadata and the tile-ordered positions.
-// Within a subgroup, tiles are ordered by the diagonal of their bounding box.
+// metadata of the tile pts[lo, hi): bounding-box centre and a radius bounding every atom
+static Tile make_tile(const std::vector<Pt> &pts, int64_t lo, int64_t hi, int g, int64_t pos_start,
+                      int32_t parent)
+{
+    double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = lo; i < hi; i++) {
+        const double v[3] = {pts[i].x, pts[i].y, pts[i].z};
+        for (int a = 0; a < 3; a++) {
+            mn[a] = std::min(mn[a], v[a]);
+            mx[a] = std::max(mx[a], v[a]);
+        }
+    }
+    Tile t;
+    t.cx = (float)(0.5 * (mn[0] + mx[0]));
+    t.cy = (float)(0.5 * (mn[1] + mx[1]));
+    t.cz = (float)(0.5 * (mn[2] + mx[2]));
+    double r2 = 0;
+    for (int64_t i = lo; i < hi; i++) {
+        const double dx = pts[i].x - (double)t.cx, dy = pts[i].y - (double)t.cy, dz = pts[i].z - (double)t.cz;
+        r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
+    }
+    t.rad = (float)(std::sqrt(r2) * (1 + 1e-6) + 1e-5);
+    t.start = (int32_t)pos_start;
+    t.count = (int32_t)(hi - lo);
+    t.group = g;
+    t.parent = parent;
+    return t;
+}
+
+// Host part of a model: validation, tiling, tile metadata and the tile-ordered positions.
+// Each subgroup is cut into blocks of 64 atoms (level 0), each block into halves of 32
+// (level 1) and quarters of 16 (level 2), all by median bisection, so every sub-tile is
+// a contiguous, compact run inside its block.  Blocks are stored in ascending size of
+// their bounding box; tiles_h holds all level-0 tiles, then all level-1, then all level-2.
 static void tile_model_host(const float *x, const float *y, const float *z, const int64_t *group_n,
                             int S, ModelDev &m, std::vector<float4> &pos_h)
 {
@@ -125,12 +158,15 @@ static void tile_model_host(const float *x, const float *y, const float *z, cons
     m.n_groups = S;
     m.n_atoms = N;
     m.group_n.assign(group_n, group_n + S);
-    m.group_tile0.resize(S);
-    m.group_tile1.resize(S);
     m.bbox.assign(6 * S, 0.f);
     pos_h.clear();
     pos_h.reserve(N);
-    m.tiles_h.clear();
+    std::vector<Tile> lv[3];
+    std::vector<int32_t> c1a, c1b, c2a, c2b;  // children ranges (level 1 of each block, level 2 of each half)
+    for (int L = 0; L < 3; L++) {
+        m.lvl_t0[L].assign(S, 0);
+        m.lvl_t1[L].assign(S, 0);
+    }
     int64_t start = 0;
     for (int g = 0; g < S; g++) {
         const int64_t n = group_n[g];
@@ -148,59 +184,63 @@ static void tile_model_host(const float *x, const float *y, const float *z, cons
             m.bbox[6 * g + a] = n ? mn[a] : 0.f;
             m.bbox[6 * g + 3 + a] = n ? mx[a] : 0.f;
         }
-        std::vector<std::pair<int64_t, int64_t>> leaves;
-        if (n) kd_split(pts, 0, n, leaves);
-        m.group_tile0[g] = (int32_t)m.tiles_h.size();
-        // leaf radii, then leaves in ascending radius (pass 1 sizes its shells per run of
-        // consecutive tiles by their largest radius)
-        std::vector<std::pair<double, size_t>> lr(leaves.size());
-        for (size_t li = 0; li < leaves.size(); li++) {
-            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            for (int64_t i = leaves[li].first; i < leaves[li].second; i++) {
-                const double v[3] = {pts[i].x, pts[i].y, pts[i].z};
-                for (int a = 0; a < 3; a++) {
-                    lo[a] = std::min(lo[a], v[a]);
-                    hi[a] = std::max(hi[a], v[a]);
-                }
-            }
-            double e2 = 0;
-            for (int a = 0; a < 3; a++) e2 += (hi[a] - lo[a]) * (hi[a] - lo[a]);
-            lr[li] = {e2, li};
+        std::vector<std::pair<int64_t, int64_t>> blocks;
+        if (n) kd_split(pts, 0, n, kTileAtoms, blocks);
+        // blocks in ascending bounding-box diagonal
+        std::vector<std::pair<double, size_t>> key(blocks.size());
+        for (size_t b = 0; b < blocks.size(); b++) {
+            Tile t = make_tile(pts, blocks[b].first, blocks[b].second, g, 0, 0);
+            key[b] = {(double)t.rad, b};
         }
-        std::stable_sort(lr.begin(), lr.end());
-        std::vector<std::pair<int64_t, int64_t>> sorted_leaves(leaves.size());
-        for (size_t li = 0; li < leaves.size(); li++) sorted_leaves[li] = leaves[lr[li].second];
-        leaves.swap(sorted_leaves);
-        for (auto &lf : leaves) {
-            double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-            for (int64_t i = lf.first; i < lf.second; i++) {
-                const double v[3] = {pts[i].x, pts[i].y, pts[i].z};
-                for (int a = 0; a < 3; a++) {
-                    lo[a] = std::min(lo[a], v[a]);
-                    hi[a] = std::max(hi[a], v[a]);
-                }
+        std::stable_sort(key.begin(), key.end());
+        for (int L = 0; L < 3; L++) m.lvl_t0[L][g] = (int32_t)lv[L].size();
+        for (auto &kb : key) {
+            const int64_t lo = blocks[kb.second].first, hi = blocks[kb.second].second;
+            const int64_t shift = (int64_t)pos_h.size() - lo;  // pts index -> position index
+            const int32_t parent = (int32_t)lv[0].size();
+            lv[0].push_back(make_tile(pts, lo, hi, g, lo + shift, parent));
+            std::vector<std::pair<int64_t, int64_t>> halves;
+            kd_split(pts, lo, hi, 32, halves);
+            c1a.push_back((int32_t)lv[1].size());
+            for (auto &hf : halves) {
+                lv[1].push_back(make_tile(pts, hf.first, hf.second, g, hf.first + shift, parent));
+                std::vector<std::pair<int64_t, int64_t>> quarters;
+                kd_split(pts, hf.first, hf.second, 16, quarters);
+                c2a.push_back((int32_t)lv[2].size());
+                for (auto &qt : quarters)
+                    lv[2].push_back(make_tile(pts, qt.first, qt.second, g, qt.first + shift, parent));
+                c2b.push_back((int32_t)lv[2].size());
             }
-            Tile t;
-            t.cx = (float)(0.5 * (lo[0] + hi[0]));
-            t.cy = (float)(0.5 * (lo[1] + hi[1]));
-            t.cz = (float)(0.5 * (lo[2] + hi[2]));
-            double r2 = 0;
-            for (int64_t i = lf.first; i < lf.second; i++) {
-                const double dx = pts[i].x - (double)t.cx, dy = pts[i].y - (double)t.cy,
-                             dz = pts[i].z - (double)t.cz;
-                r2 = std::max(r2, dx * dx + dy * dy + dz * dz);
-            }
-            t.rad = (float)(std::sqrt(r2) * (1 + 1e-6) + 1e-5);
-            t.start = (int32_t)pos_h.size();
-            t.count = (int32_t)(lf.second - lf.first);
-            t.group = g;
-            t.pad = 0;
-            for (int64_t i = lf.first; i < lf.second; i++)
-                pos_h.push_back(make_float4(pts[i].x, pts[i].y, pts[i].z, 0.f));
-            m.tiles_h.push_back(t);
+            c1b.push_back((int32_t)lv[1].size());
+            for (int64_t i = lo; i < hi; i++) pos_h.push_back(make_float4(pts[i].x, pts[i].y, pts[i].z, 0.f));
         }
-        m.group_tile1[g] = (int32_t)m.tiles_h.size();
+        for (int L = 0; L < 3; L++) m.lvl_t1[L][g] = (int32_t)lv[L].size();
         start += n;
+    }
+    // one array: level 0, level 1, level 2 (parents are level-0 indices, unchanged)
+    const int32_t off1 = (int32_t)lv[0].size(), off2 = off1 + (int32_t)lv[1].size();
+    m.tiles_h = lv[0];
+    m.tiles_h.insert(m.tiles_h.end(), lv[1].begin(), lv[1].end());
+    m.tiles_h.insert(m.tiles_h.end(), lv[2].begin(), lv[2].end());
+    for (int g = 0; g < S; g++) {
+        m.lvl_t0[1][g] += off1;
+        m.lvl_t1[1][g] += off1;
+        m.lvl_t0[2][g] += off2;
+        m.lvl_t1[2][g] += off2;
+    }
+    m.group_tile0 = m.lvl_t0[0];
+    m.group_tile1 = m.lvl_t1[0];
+    m.child1_0.resize(c1a.size());
+    m.child1_1.resize(c1b.size());
+    for (size_t i = 0; i < c1a.size(); i++) {
+        m.child1_0[i] = c1a[i] + off1;
+        m.child1_1[i] = c1b[i] + off1;
+    }
+    m.child2_0.resize(c2a.size());
+    m.child2_1.resize(c2b.size());
+    for (size_t i = 0; i < c2a.size(); i++) {
+        m.child2_0[i] = c2a[i] + off2;
+        m.child2_1[i] = c2b[i] + off2;
     }
 }
 
@@ -224,10 +264,15 @@ static void build_model(dbx_ctx *ctx, const float *x, const float *y, const floa
 
 static void free_model(ModelDev &m)
 {
-    if (m.pos) (void)hipFree(m.pos);
-    if (m.tiles) (void)hipFree(m.tiles);
+    for (void *p : {(void *)m.pos, (void *)m.tiles, (void *)m.d_jobs, (void *)m.d_edge2, (void *)m.d_retry,
+                    (void *)m.d_alist})
+        if (p) (void)hipFree(p);
     m.pos = nullptr;
     m.tiles = nullptr;
+    m.d_jobs = nullptr;
+    m.d_edge2 = nullptr;
+    m.d_retry = nullptr;
+    m.d_alist = nullptr;
 }
 
 }  // namespace dbx
@@ -254,7 +299,8 @@ int dbx_open(int device, size_t hbm_budget, dbx_ctx **out)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&c->counters, 64) != hipSuccess) {
+        hipMalloc(&c->counters, 64) != hipSuccess ||
+        hipHostMalloc((void **)&c->counters_h, 64, hipHostMallocDefault) != hipSuccess) {
         delete c;
         return DBX_ERR_HIP;
     }
@@ -275,6 +321,7 @@ int dbx_close(dbx_ctx *ctx)
     for (int s = 0; s < 8; s++)
         if (ctx->slot_ptr[s]) (void)hipFree(ctx->slot_ptr[s]);
     if (ctx->counters) (void)hipFree(ctx->counters);
+    if (ctx->counters_h) (void)hipHostFree(ctx->counters_h);
     for (int i = 0; i < 4; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     (void)hipStreamDestroy(ctx->stream);
@@ -373,7 +420,7 @@ int dbx_model_info(const dbx_model *m, int64_t *n_atoms, int *n_groups, int *n_p
     return DBX_OK;
 }
 
-int dbx_histogram(dbx_ctx *ctx, const dbx_model *m, double rstep, int64_t n_bins, uint64_t *out,
+int dbx_histogram(dbx_ctx *ctx, dbx_model *m, double rstep, int64_t n_bins, uint64_t *out,
                   int out_is_device, uint64_t *n_out_of_range)
 {
     if (!m || !out) return DBX_ERR_ARG;
@@ -440,7 +487,7 @@ int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int
     });
 }
 
-int dbx_intensity(dbx_ctx *ctx, const dbx_model *m, const dbx_group_params *params, double rstep,
+int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params, double rstep,
                   int64_t n_bins, int first_n, const double *q, int nq, double *I_out,
                   uint64_t *hist_out)
 {
@@ -451,12 +498,7 @@ int dbx_intensity(dbx_ctx *ctx, const dbx_model *m, const dbx_group_params *para
         const int P = S * (S + 1) / 2;
         const size_t hb = sizeof(uint64_t) * (size_t)P * n_bins;
         uint64_t *dh = (uint64_t *)scratch(ctx, kSlotHist, hb);
-        P1Stats st = pass1_histograms(ctx, m->d, rstep, n_bins, dh);
-        if (st.out_of_range) {
-            rc = DBX_ERR_RANGE;
-            ctx->err = std::to_string(st.out_of_range) +
-                       " pair distances fall at or beyond the last bin (rmax too small)";
-        }
+        P1Stats st = pass1_histograms(ctx, m->d, rstep, n_bins, dh, false);
         const size_t nqd = (size_t)nq;
         double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 2)));
         double *dq = d, *dff = d + nqd, *dS = dff + P * nqd, *dI = dS + P * nqd;
@@ -469,6 +511,12 @@ int dbx_intensity(dbx_ctx *ctx, const dbx_model *m, const dbx_group_params *para
         DBX_HIP(hipMemcpyAsync(I_out, dI, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
         if (hist_out) DBX_HIP(hipMemcpyAsync(hist_out, dh, hb, hipMemcpyDeviceToHost, ctx->stream));
         DBX_HIP(hipStreamSynchronize(ctx->stream));
+        pass1_read_stats(ctx, st);
+        if (st.out_of_range) {
+            rc = DBX_ERR_RANGE;
+            ctx->err = std::to_string(st.out_of_range) +
+                       " pair distances fall at or beyond the last bin (rmax too small)";
+        }
     });
     if (grc != DBX_OK) return grc;
     return rc;
@@ -511,7 +559,8 @@ int dbx_debug_pass1_plan(const dbx_model *m, double rstep, void *jobs, int cap_j
     try {
         std::vector<P1Job> j;
         std::vector<float> e;
-        plan_pass1(m->d, rstep, j, e);
+        std::vector<int32_t> al;
+        plan_pass1(m->d, rstep, j, e, al);
         if (n_jobs) *n_jobs = (int)j.size();
         if (n_edge2) *n_edge2 = (int)e.size();
         if (jobs && cap_jobs >= (int)j.size()) std::memcpy(jobs, j.data(), sizeof(P1Job) * j.size());
@@ -526,7 +575,8 @@ int dbx_debug_pass1_plan(const dbx_model *m, double rstep, void *jobs, int cap_j
 int dbx_debug_plan_host(const float *x, const float *y, const float *z, const int64_t *group_n,
                         int n_groups, double rstep, void *tiles, int cap_tiles, float *pos4,
                         int64_t cap_pos, void *jobs, int cap_jobs, float *edge2, int cap_edge2,
-                        int *n_tiles, int *n_jobs, int *n_edge2)
+                        int32_t *alist, int cap_alist, int *n_tiles, int *n_jobs, int *n_edge2,
+                        int *n_alist)
 {
     if (!x || !group_n) return DBX_ERR_ARG;
     try {
@@ -535,10 +585,13 @@ int dbx_debug_plan_host(const float *x, const float *y, const float *z, const in
         tile_model_host(x, y, z, group_n, n_groups, m, pos_h);
         std::vector<P1Job> j;
         std::vector<float> e;
-        plan_pass1(m, rstep, j, e);
+        std::vector<int32_t> al;
+        plan_pass1(m, rstep, j, e, al);
         if (n_tiles) *n_tiles = (int)m.tiles_h.size();
         if (n_jobs) *n_jobs = (int)j.size();
         if (n_edge2) *n_edge2 = (int)e.size();
+        if (n_alist) *n_alist = (int)al.size();
+        if (alist && cap_alist >= (int)al.size()) std::memcpy(alist, al.data(), sizeof(int32_t) * al.size());
         if (tiles && cap_tiles >= (int)m.tiles_h.size())
             std::memcpy(tiles, m.tiles_h.data(), sizeof(Tile) * m.tiles_h.size());
         if (pos4 && cap_pos >= (int64_t)pos_h.size())

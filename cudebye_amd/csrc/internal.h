@@ -49,39 +49,52 @@ struct Tile {          // 32 bytes
     int32_t start;     // first atom (index into the float4 position array)
     int32_t count;     // atoms in the tile (1..64)
     int32_t group;
-    int32_t pad;
+    int32_t parent;    // level-0 tile (64-atom block) containing this tile
 };
 
-// One pass-1 work item: the pairs of A tiles [a0, a1) (subgroup row) with the B atoms
-// (subgroup col, tiles [b0, b1)) whose distance from the A tile's centre lies in shell
-// `shell` of pair `pair`.  Every pair lands in one LDS window of kP1Bins bins starting
+// One pass-1 work item: the pairs of a list of A tiles (subgroup row; 64-, 32- or 16-atom
+// tiles) with the B atoms (subgroup col, 64-atom tiles [b0, b1)) whose distance from the
+// A tile's centre lies in shell `shell` of pair `pair`.  Every pair lands in one LDS window of kP1Bins bins starting
 // at bin k_lo (pairs outside it, which only occur when the window is narrower than the
 // geometry needs, go straight to the global histogram).
 struct P1Job {
     int32_t pair;
-    int32_t a0, a1;
+    int32_t a_off, a_cnt;  // A tiles: alist[a_off .. a_off + a_cnt)
     int32_t b0, b1;
     int32_t same;   // 1: one subgroup, count b >= a (upper triangle incl. self)
     int32_t shell;  // index into the pair's shell-edge table
     int32_t k_lo;   // first bin of the LDS window
     int32_t e2_off; // offset of the pair's shell-edge table in the edge array
     int32_t degraded;  // 1: the window cannot hold the job's geometry (checked kernel)
-    int32_t pad[2];
+    int32_t R;         // B atoms per 64-lane step: 1, 2, 4 for A tiles of 64, 32, 16 atoms
+    int32_t pad;
 };
 
 constexpr int kP1Threads = 1024;   // 16 waves of 64
-constexpr int kP1Bins = 36864;     // LDS window: 144 KiB of uint32 counters (+1 sentinel)
-constexpr int kP1Cand = 3072;      // LDS candidate list (A tile, B tile), 12 KiB
+constexpr int kP1Bins = 35328;     // LDS window: 138 KiB of uint32 counters (+1 sentinel)
+constexpr int kP1Cand = 1024;      // LDS candidate list (A tile, B tile), 4 KiB
+// + a 1 KiB B-atom stage per wave (16 KiB) for the 32- and 16-atom A tiles
 
 struct ModelDev {
     int n_groups = 0;
     int64_t n_atoms = 0;
     std::vector<int64_t> group_n;
-    std::vector<int32_t> group_tile0, group_tile1;  // tile range per subgroup
+    std::vector<int32_t> group_tile0, group_tile1;  // level-0 tile range per subgroup
+    std::vector<int32_t> lvl_t0[3], lvl_t1[3];      // tile range per level and subgroup
+    std::vector<int32_t> child1_0, child1_1;        // level-1 children of each level-0 tile
+    std::vector<int32_t> child2_0, child2_1;        // level-2 children of each level-1 tile
     std::vector<Tile> tiles_h;                      // host copy (job planning)
     std::vector<float> bbox;                        // per subgroup: min xyz, max xyz
     float4 *pos = nullptr;                          // device, spatially ordered
     Tile *tiles = nullptr;                          // device
+    // pass-1 work plan for one bin width, built on first use and kept with the model
+    double plan_rstep = -1.0;
+    int plan_cu = 0;
+    int plan_jobs = 0, plan_fast = 0;
+    P1Job *d_jobs = nullptr;
+    float *d_edge2 = nullptr;
+    int *d_retry = nullptr;
+    int32_t *d_alist = nullptr;
 };
 
 }  // namespace dbx
@@ -102,6 +115,8 @@ struct dbx_ctx {
     // [0] out-of-range pairs, [1] pairs outside the LDS window (checked kernel),
     // [2] fast jobs queued for the checked kernel; then the retry list (int32)
     uint64_t *counters = nullptr;
+    uint64_t *counters_h = nullptr;  // pinned host copy of counters[0..3]
+    bool p1_timed = false;
     int cu_count = 0;
     int64_t last_p1_jobs = 0;
     uint64_t last_p1_misses = 0;
@@ -127,11 +142,13 @@ struct P1Stats {
     int n_jobs = 0;
     int n_checked = 0;
 };
-P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t n_bins,
+P1Stats pass1_histograms(dbx_ctx *ctx, ModelDev &m, double rstep, int64_t n_bins,
                          uint64_t *d_hist, bool sync_stats = true);
+// reads the pass-1 counters of the last launch (after the stream is synchronised)
+void pass1_read_stats(dbx_ctx *ctx, P1Stats &st);
 // host planning of pass 1: work items (largest first) and the shell-edge tables
 void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &jobs, std::vector<float> &edge2,
-                int n_cu = 256);
+                std::vector<int32_t> &alist, int n_cu = 256);
 
 // pass 2 (pass2.hip)
 void prefactor_dev(dbx_ctx *ctx, const dbx_group_params &ga, const dbx_group_params &gb,

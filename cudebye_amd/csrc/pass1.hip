@@ -9,13 +9,16 @@
 // (cuDebye-v15.cu:137), two subgroups -> the full n_row x n_col rectangle.
 //
 // MI355X design.  A workgroup (16 waves) owns a window of kP1Bins uint32 counters in LDS.
-// Each wave holds 64 atoms of an A tile in its lanes and streams B atoms as wave-uniform
-// values (v_readlane), so every B atom is one 64-lane step of ~25 VALU operations and one
-// LDS atomic per lane.  To keep all bins of a workgroup inside its LDS window, B atoms are
+// A wave holds the atoms of an A tile in its lanes and streams B atoms past them: every
+// 64-lane step is one pair per lane, ~25 VALU operations and one LDS atomic.  With 64-atom
+// A tiles the B atom is a wave-uniform value (v_readlane); with 32- or 16-atom A tiles the
+// lanes hold 2 or 4 copies of the tile and take 2 or 4 B atoms per step from a small
+// per-wave LDS stage.  To keep all bins of a workgroup inside its LDS window, B atoms are
 // taken shell by shell: a job handles the B atoms whose distance from the A tile's centre
 // lies in [s*h, (s+1)*h); every pair of such an atom with an A atom then lies within
-// [s*h - rad_A, (s+1)*h + rad_A].  At the end of a job the nonzero LDS counters are added
-// to the uint64 histogram in HBM.
+// [s*h - rad_A, (s+1)*h + rad_A], and the planner picks h (and the A tile size) so that
+// this range fits the window.  At the end of a job the nonzero LDS counters are added to
+// the uint64 histogram in HBM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -35,11 +38,32 @@ __device__ __forceinline__ float rd_lane(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+template <bool CHECKED>
+__device__ __forceinline__ void count_pair(uint32_t *h, int k, int k_lo, int64_t n_bins,
+                                           unsigned long long *hrow, uint32_t &miss, uint32_t &oor)
+{
+    if (!CHECKED) {
+        const unsigned kk = min((unsigned)(k - k_lo), (unsigned)kP1Bins);
+        atomicAdd(&h[kk], 1u);
+    } else {
+        const unsigned kk = (unsigned)(k - k_lo);
+        if (kk < (unsigned)kP1Bins) {
+            atomicAdd(&h[kk], 1u);
+        } else {
+            ++miss;
+            if (k >= 0 && (int64_t)k < n_bins)
+                atomicAdd(&hrow[k], 1ull);
+            else
+                ++oor;
+        }
+    }
+}
+
 // One workgroup runs one job: it zeroes its LDS window, culls the job's (A tile, B tile)
 // pairs into an LDS candidate list (B tiles with an atom possibly in the shell), then its
 // 16 waves take candidates one at a time from that list, so work spreads evenly over the
 // waves whatever the shape of the job.  For each candidate the wave loads the B tile into
-// its lanes, keeps the atoms inside the shell (ballot) and streams them past its 64 A atoms.
+// its lanes, keeps the atoms inside the shell (ballot) and streams them past its A atoms.
 //
 // CHECKED = false: every pair's bin is assumed to fall in the window (the planner sized the
 // shells for that).  An out-of-window bin is clamped into a sentinel counter h[kP1Bins]; a
@@ -49,12 +73,13 @@ __device__ __forceinline__ float rd_lane(float v, int lane)
 template <bool CHECKED>
 __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const float4 *__restrict__ pos, const Tile *__restrict__ tiles, const P1Job *__restrict__ jobs,
-    int n_fast, int n_checked, const float *__restrict__ edge2, float rp,
-    unsigned long long *__restrict__ hist, int64_t n_bins, unsigned long long *__restrict__ counters,
-    int *__restrict__ retry)
+    const int32_t *__restrict__ alist, int n_fast, int n_checked, const float *__restrict__ edge2,
+    float rp, unsigned long long *__restrict__ hist, int64_t n_bins,
+    unsigned long long *__restrict__ counters, int *__restrict__ retry)
 {
     __shared__ uint32_t h[kP1Bins + 1];
     __shared__ uint32_t cand[kP1Cand];
+    __shared__ float4 stage[kP1Threads / 64][64];
     __shared__ int s_ncand, s_next;
 
     int job_index;
@@ -75,6 +100,10 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int R = job.R;
+    const int TA = 64 / R;
+    const int slot = lane / TA;
+    const int ai = lane - slot * TA;
     const float e_lo = edge2[job.e2_off + job.shell];
     const float e_hi = edge2[job.e2_off + job.shell + 1];
     const float r_lo = e_lo > 0.f ? sqrtf(e_lo) : 0.f;
@@ -82,12 +111,13 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const int k_lo = job.k_lo;
     unsigned long long *hrow = hist + (int64_t)job.pair * n_bins;
     const int nblk = (job.b1 - job.b0 + 63) >> 6;
-    const int units = (job.a1 - job.a0) * nblk;
+    const int units = job.a_cnt * nblk;
     uint32_t miss = 0, oor = 0;
 
     int cur_ia = -1;
     Tile ta = {};
     bool a_ok = false;
+    int idx_a = 0;
     float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
 
     for (int u0 = 0; u0 < units; u0 += kP1Cand / 64) {
@@ -100,11 +130,10 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
         const int u1 = min(units, u0 + kP1Cand / 64);
         for (int u = u0 + wave; u < u1; u += kP1Threads / 64) {
             const int ial = u / nblk;
-            const int ia = job.a0 + ial;
+            const Tile A = tiles[alist[job.a_off + ial]];
             const int j = job.b0 + (u - ial * nblk) * 64 + lane;
-            const Tile A = tiles[ia];
             bool c = false;
-            if (j < job.b1 && (!job.same || j >= ia)) {
+            if (j < job.b1 && (!job.same || j >= A.parent)) {
                 const Tile tb = tiles[j];
                 const float dx = tb.cx - A.cx, dy = tb.cy - A.cy, dz = tb.cz - A.cz;
                 const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
@@ -129,43 +158,53 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             ci = __builtin_amdgcn_readfirstlane(ci);
             if (ci >= nc) break;
             const uint32_t e = cand[ci];
-            const int ia = job.a0 + (int)(e >> 24);
+            const int ia = alist[job.a_off + (int)(e >> 24)];
             const int j2 = job.b0 + (int)(e & 0xFFFFFFu);
             if (ia != cur_ia) {
                 cur_ia = ia;
                 ta = tiles[ia];
-                a_ok = lane < ta.count;
-                pa = pos[ta.start + (a_ok ? lane : 0)];
+                a_ok = ai < ta.count;
+                idx_a = ta.start + ai;
+                pa = pos[a_ok ? idx_a : ta.start];
             }
             const Tile tb = tiles[j2];
             const bool b_ok = lane < tb.count;
             const float4 pb = pos[tb.start + (b_ok ? lane : 0)];
             const float ex = pb.x - ta.cx, ey = pb.y - ta.cy, ez = pb.z - ta.cz;
             const float d2c = (ex * ex + ey * ey) + ez * ez;
-            uint64_t bm = __ballot(b_ok && d2c >= e_lo && d2c < e_hi);
-            const bool diag = job.same && (j2 == ia);
-            while (bm) {
-                const int m = __builtin_ctzll(bm);
-                bm &= bm - 1;
-                const float bx = rd_lane(pb.x, m), by = rd_lane(pb.y, m), bz = rd_lane(pb.z, m);
-                if (a_ok && (!diag || lane <= m)) {
-                    const int k = pair_bin(pa.x, pa.y, pa.z, bx, by, bz, rp);
-                    if (!CHECKED) {
-                        const unsigned kk = min((unsigned)(k - k_lo), (unsigned)kP1Bins);
-                        atomicAdd(&h[kk], 1u);
-                    } else {
-                        const unsigned kk = (unsigned)(k - k_lo);
-                        if (kk < (unsigned)kP1Bins) {
-                            atomicAdd(&h[kk], 1u);
-                        } else {
-                            ++miss;
-                            if (k >= 0 && (int64_t)k < n_bins)
-                                atomicAdd(&hrow[k], 1ull);
-                            else
-                                ++oor;
-                        }
-                    }
+            const bool in_shell = b_ok && d2c >= e_lo && d2c < e_hi;
+            uint64_t bm = __ballot(in_shell);
+            const bool diag = job.same && (j2 == ta.parent);  // B tile holds the A tile's atoms
+            if (R == 1) {
+                while (bm) {
+                    const int m = __builtin_ctzll(bm);
+                    bm &= bm - 1;
+                    const float bx = rd_lane(pb.x, m), by = rd_lane(pb.y, m), bz = rd_lane(pb.z, m);
+                    if (a_ok && (!diag || idx_a <= tb.start + m))
+                        count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, bx, by, bz, rp), k_lo, n_bins,
+                                            hrow, miss, oor);
                 }
+            } else if (bm) {
+                // stage the shell's B atoms (with their atom index) and take R per step
+                const int nb = __popcll(bm);
+                if (in_shell) {
+                    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0));
+                    stage[wave][below] = make_float4(pb.x, pb.y, pb.z, __int_as_float(tb.start + lane));
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                for (int t = 0; t < nb; t += R) {
+                    const int src = t + slot;
+                    const float4 b = stage[wave][min(src, nb - 1)];
+                    if (a_ok && src < nb && (!diag || idx_a <= __float_as_int(b.w)))
+                        count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
+                                            hrow, miss, oor);
+                }
+                // the stage is rewritten by the next candidate only after every lane's reads
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
             }
         }
         __syncthreads();
@@ -230,8 +269,16 @@ double point_box_max(const Tile &t, const float *bb)
 
 }  // namespace
 
+// The plan of one model and bin width.  Per subgroup pair (row ga, col gb):
+//   - A tiles: each 64-atom block of ga whose radius lets a shell of useful width fit the
+//     window, else its 32-atom halves that fit, else its 16-atom quarters (which run in the
+//     checked kernel if even they do not fit: fine bins with wide tiles);
+//   - per level, A tiles sorted by radius and cut into chunks; each chunk gets its own shell
+//     width h = usable - 2 rad_max - margins and one job per shell it can reach;
+//   - jobs far above a CU's mean share are split along their B-tile range;
+//   - order: exact-window jobs, largest first, then the checked ones.
 void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
-                std::vector<float> &edge2, int n_cu)
+                std::vector<float> &edge2, std::vector<int32_t> &alist, int n_cu)
 {
     DBX_REQUIRE(rstep > 0 && std::isfinite(rstep), DBX_ERR_ARG, "rstep must be positive");
     const int S = m.n_groups;
@@ -240,6 +287,8 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
     std::vector<P1Job> jobs;
     std::vector<double> cost;
     edge2.clear();
+    alist.clear();
+    const int off1 = m.lvl_t0[1].empty() ? 0 : m.lvl_t0[1][0];
     int p = 0;
     for (int ga = 0; ga < S; ga++) {
         for (int gb = ga; gb < S; gb++, p++) {
@@ -259,79 +308,100 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
             const double D = std::sqrt(D2) + 1e-3;  // bound on |b - c_A|
             const double mg = 2e-3 + 4e-6 * (D + radA);
             const double h_min = std::max(0.5, 0.02 * usable);
-            // A chunks (tiles of a subgroup are stored in ascending radius, so a chunk's
-            // largest radius is close to its typical one); CA sized for enough pairs per job
-            // to amortise the LDS window flush
-            const double rad_typ = m.tiles_h[a0 + (a1 - a0) / 2].rad;
-            const double h_typ = std::max(h_min, usable - 2 * rad_typ - 2 * mg);
-            const double nB = (double)m.group_n[gb];
-            const double shell_frac = std::min(1.0, h_typ / std::max(D, h_typ));
-            const double per_tile = 64.0 * std::max(1.0, nB * shell_frac);
-            int CA = (int)std::ceil(6.0e6 / per_tile);
-            CA = std::max(1, std::min(CA, 32));
-            const int nbt = b1 - b0;
-            const int nchunks = (a1 - a0 + CA - 1) / CA;
-            const int stride = (int)std::max<int64_t>(1, ((int64_t)nchunks * nbt) / 2000000);
-            for (int c0 = a0; c0 < a1; c0 += CA) {
-                const int c1 = std::min(a1, c0 + CA);
-                double dmin = 1e300, dmax = 0, rmaxc = 0, na = 0;
-                for (int t = c0; t < c1; t++) {
-                    const Tile &T = m.tiles_h[t];
-                    dmin = std::min(dmin, point_box_min(T, bbB));
-                    dmax = std::max(dmax, point_box_max(T, bbB));
-                    rmaxc = std::max(rmaxc, (double)T.rad);
-                    na += T.count;
+            const double rad_fit = 0.5 * (usable - h_min - 2 * mg);
+            // A tiles per level
+            std::vector<int32_t> sel[3];
+            for (int t = a0; t < a1; t++) {
+                if (m.tiles_h[t].rad <= rad_fit) {
+                    sel[0].push_back(t);
+                    continue;
                 }
-                // shells of this chunk: width h such that [s h - rad, (s+1) h + rad] fits the
-                // LDS window; narrower windows than that (fine bins, wide tiles) fall back to
-                // a window centred on the shell, with the rest of the pairs sent to HBM
-                double h = usable - 2 * rmaxc - 2 * mg;
-                const bool degraded = h < h_min;
-                if (degraded) h = h_min;
-                const int ns = (int)(D / h) + 1;
-                const int e2_off = (int)edge2.size();
-                edge2.push_back(-1.0f);
-                for (int sh = 1; sh < ns; sh++) edge2.push_back((float)((sh * h) * (sh * h)));
-                edge2.push_back(INFINITY);
-                // cost estimate: B tiles by centre distance from each A tile (sampled)
-                std::vector<double> shell_cost(ns, 0.0);
-                const int jfirst = (gb == ga) ? c0 : b0;
-                for (int t = c0; t < c1; t += std::max(1, (c1 - c0) / 4)) {
-                    const Tile &A = m.tiles_h[t];
-                    for (int j = std::max(jfirst, gb == ga ? t : b0); j < b1; j += stride) {
-                        const Tile &T = m.tiles_h[j];
-                        const double dx = T.cx - A.cx, dy = T.cy - A.cy, dz = T.cz - A.cz;
-                        int sh = (int)(std::sqrt(dx * dx + dy * dy + dz * dz) / h);
-                        if (sh >= ns) sh = ns - 1;
-                        shell_cost[sh] += (double)T.count * stride;
+                for (int c = m.child1_0[t]; c < m.child1_1[t]; c++) {
+                    if (m.tiles_h[c].rad <= rad_fit) {
+                        sel[1].push_back(c);
+                        continue;
                     }
+                    for (int q = m.child2_0[c - off1]; q < m.child2_1[c - off1]; q++) sel[2].push_back(q);
                 }
-                for (int sh = 0; sh < ns; sh++) {
-                    if (sh * h > dmax + mg + 1e-3 || (sh + 1) * h < dmin - mg - 1e-3) continue;
-                    P1Job J;
-                    std::memset(&J, 0, sizeof(J));
-                    J.pair = p;
-                    J.a0 = c0;
-                    J.a1 = c1;
-                    J.b0 = b0;
-                    J.b1 = b1;
-                    J.same = (ga == gb);
-                    J.shell = sh;
-                    J.e2_off = e2_off;
-                    double klo;
-                    if (!degraded)
-                        klo = std::floor((sh * h - rmaxc - mg) / rstep) - 2;
-                    else
-                        klo = std::floor(((sh + 0.5) * h) / rstep) - kP1Bins / 2;
-                    if (klo < 0) klo = 0;
-                    J.k_lo = (int32_t)std::min<double>(klo, (double)INT32_MAX - kP1Bins);
-                    J.degraded = degraded ? 1 : 0;
-                    jobs.push_back(J);
-                    // neighbouring shells share straddling tiles: spread the sampled cost
-                    double c = shell_cost[sh];
-                    if (sh > 0) c += 0.25 * shell_cost[sh - 1];
-                    if (sh + 1 < ns) c += 0.25 * shell_cost[sh + 1];
-                    cost.push_back(na * (c + 1.0));
+            }
+            const double nB = (double)m.group_n[gb];
+            const int nbt = b1 - b0;
+            for (int L = 0; L < 3; L++) {
+                std::vector<int32_t> &A = sel[L];
+                if (A.empty()) continue;
+                std::stable_sort(A.begin(), A.end(), [&](int x, int y) { return m.tiles_h[x].rad < m.tiles_h[y].rad; });
+                const int R = 1 << L;
+                const double ta_atoms = 64.0 / R;
+                const double rad_typ = m.tiles_h[A[A.size() / 2]].rad;
+                const double h_typ = std::max(h_min, usable - 2 * rad_typ - 2 * mg);
+                const double shell_frac = std::min(1.0, h_typ / std::max(D, h_typ));
+                const double per_tile = ta_atoms * std::max(1.0, nB * shell_frac);
+                int CA = (int)std::ceil(6.0e6 / per_tile);
+                CA = std::max(1, std::min(CA, 64));
+                const int nchunks = ((int)A.size() + CA - 1) / CA;
+                const int stride = (int)std::max<int64_t>(1, ((int64_t)nchunks * nbt) / 2000000);
+                for (int c0 = 0; c0 < (int)A.size(); c0 += CA) {
+                    const int c1 = std::min((int)A.size(), c0 + CA);
+                    const int a_off = (int)alist.size();
+                    double dmin = 1e300, dmax = 0, rmaxc = 0, na = 0;
+                    for (int i = c0; i < c1; i++) {
+                        const Tile &T = m.tiles_h[A[i]];
+                        alist.push_back(A[i]);
+                        dmin = std::min(dmin, point_box_min(T, bbB));
+                        dmax = std::max(dmax, point_box_max(T, bbB));
+                        rmaxc = std::max(rmaxc, (double)T.rad);
+                        na += T.count;
+                    }
+                    double h = usable - 2 * rmaxc - 2 * mg;
+                    const bool degraded = h < h_min;
+                    if (degraded) h = h_min;
+                    const int ns = (int)(D / h) + 1;
+                    const int e2_off = (int)edge2.size();
+                    edge2.push_back(-1.0f);
+                    for (int sh = 1; sh < ns; sh++) edge2.push_back((float)((sh * h) * (sh * h)));
+                    edge2.push_back(INFINITY);
+                    // cost estimate: B tiles by centre distance from a few A tiles (sampled)
+                    std::vector<double> shell_cost(ns, 0.0);
+                    const int astep = std::max(1, (c1 - c0) / 4);
+                    for (int i = c0; i < c1; i += astep) {
+                        const Tile &T0 = m.tiles_h[A[i]];
+                        const int jf = (gb == ga) ? std::max(b0, T0.parent) : b0;
+                        for (int j = jf; j < b1; j += stride) {
+                            const Tile &T = m.tiles_h[j];
+                            const double dx = T.cx - T0.cx, dy = T.cy - T0.cy, dz = T.cz - T0.cz;
+                            int sh = (int)(std::sqrt(dx * dx + dy * dy + dz * dz) / h);
+                            if (sh >= ns) sh = ns - 1;
+                            shell_cost[sh] += (double)T.count * stride * astep;
+                        }
+                    }
+                    for (int sh = 0; sh < ns; sh++) {
+                        if (sh * h > dmax + mg + 1e-3 || (sh + 1) * h < dmin - mg - 1e-3) continue;
+                        P1Job J;
+                        std::memset(&J, 0, sizeof(J));
+                        J.pair = p;
+                        J.a_off = a_off;
+                        J.a_cnt = c1 - c0;
+                        J.b0 = b0;
+                        J.b1 = b1;
+                        J.same = (ga == gb);
+                        J.shell = sh;
+                        J.e2_off = e2_off;
+                        J.R = R;
+                        double klo;
+                        if (!degraded)
+                            klo = std::floor((sh * h - rmaxc - mg) / rstep) - 2;
+                        else
+                            klo = std::floor(((sh + 0.5) * h) / rstep) - kP1Bins / 2;
+                        if (klo < 0) klo = 0;
+                        J.k_lo = (int32_t)std::min<double>(klo, (double)INT32_MAX - kP1Bins);
+                        J.degraded = degraded ? 1 : 0;
+                        jobs.push_back(J);
+                        // neighbouring shells share straddling tiles: spread the sampled cost
+                        double cst = shell_cost[sh];
+                        if (sh > 0) cst += 0.25 * shell_cost[sh - 1];
+                        if (sh + 1 < ns) cst += 0.25 * shell_cost[sh + 1];
+                        cost.push_back(na / (c1 - c0) * (cst + 1.0) * (1.0 + 0.15 * (R - 1)));
+                    }
                 }
             }
         }
@@ -369,63 +439,92 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
     for (size_t i = 0; i < order.size(); i++) sorted[i] = split_jobs[order[i]];
 }
 
-P1Stats pass1_histograms(dbx_ctx *ctx, const ModelDev &m, double rstep, int64_t n_bins,
+// builds (or reuses) the model's device copy of the pass-1 plan for rstep
+static void ensure_plan(dbx_ctx *ctx, ModelDev &m, double rstep)
+{
+    if (m.plan_rstep == rstep && m.plan_cu == ctx->cu_count && m.d_jobs) return;
+    std::vector<P1Job> sorted;
+    std::vector<float> edge2;
+    std::vector<int32_t> alist;
+    plan_pass1(m, rstep, sorted, edge2, alist, ctx->cu_count);
+    int n_fast = 0;
+    while (n_fast < (int)sorted.size() && !sorted[n_fast].degraded) n_fast++;
+    for (void *q : {(void *)m.d_jobs, (void *)m.d_edge2, (void *)m.d_retry, (void *)m.d_alist})
+        if (q) DBX_HIP(hipFree(q));
+    m.d_jobs = nullptr;
+    m.d_edge2 = nullptr;
+    m.d_retry = nullptr;
+    m.d_alist = nullptr;
+    m.plan_rstep = -1.0;
+    DBX_HIP(hipMalloc(&m.d_jobs, sizeof(P1Job) * std::max<size_t>(1, sorted.size())));
+    DBX_HIP(hipMalloc(&m.d_edge2, sizeof(float) * std::max<size_t>(1, edge2.size())));
+    DBX_HIP(hipMalloc(&m.d_retry, sizeof(int) * (size_t)(n_fast + 1)));
+    DBX_HIP(hipMalloc(&m.d_alist, sizeof(int32_t) * std::max<size_t>(1, alist.size())));
+    if (!sorted.empty())
+        DBX_HIP(hipMemcpy(m.d_jobs, sorted.data(), sizeof(P1Job) * sorted.size(), hipMemcpyHostToDevice));
+    if (!edge2.empty())
+        DBX_HIP(hipMemcpy(m.d_edge2, edge2.data(), sizeof(float) * edge2.size(), hipMemcpyHostToDevice));
+    if (!alist.empty())
+        DBX_HIP(hipMemcpy(m.d_alist, alist.data(), sizeof(int32_t) * alist.size(), hipMemcpyHostToDevice));
+    m.plan_jobs = (int)sorted.size();
+    m.plan_fast = n_fast;
+    m.plan_cu = ctx->cu_count;
+    m.plan_rstep = rstep;
+}
+
+P1Stats pass1_histograms(dbx_ctx *ctx, ModelDev &m, double rstep, int64_t n_bins,
                          uint64_t *d_hist, bool sync_stats)
 {
     DBX_REQUIRE(n_bins > 0 && n_bins < (int64_t)1 << 40, DBX_ERR_ARG, "bad n_bins");
+    DBX_REQUIRE(rstep > 0 && std::isfinite(rstep), DBX_ERR_ARG, "rstep must be positive");
     const int S = m.n_groups;
     const int P = S * (S + 1) / 2;
     const float rp = (float)(1 / rstep);  // cuDebye-v15.cu:739
-    std::vector<P1Job> sorted;
-    std::vector<float> edge2;
-    plan_pass1(m, rstep, sorted, edge2, ctx->cu_count);
-    int n_fast = 0;
-    while (n_fast < (int)sorted.size() && !sorted[n_fast].degraded) n_fast++;
-    const int n_checked = (int)sorted.size() - n_fast;
+    ensure_plan(ctx, m, rstep);
+    const int n_fast = m.plan_fast;
+    const int n_checked = m.plan_jobs - n_fast;
 
     P1Stats st;
-    st.n_jobs = (int)sorted.size();
+    st.n_jobs = m.plan_jobs;
     st.n_checked = n_checked;
     DBX_HIP(hipMemsetAsync(d_hist, 0, sizeof(uint64_t) * (size_t)P * n_bins, ctx->stream));
     DBX_HIP(hipMemsetAsync(ctx->counters, 0, 4 * sizeof(uint64_t), ctx->stream));
-    if (!sorted.empty()) {
-        const size_t jb = sizeof(P1Job) * sorted.size();
-        const size_t eb = sizeof(float) * edge2.size();
-        const size_t rb = sizeof(int) * (size_t)(n_fast + 1);
-        auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
-        char *buf = (char *)scratch(ctx, kSlotPass1, up(jb) + up(eb) + up(rb));
-        P1Job *d_jobs = (P1Job *)buf;
-        float *d_e2 = (float *)(buf + up(jb));
-        int *d_retry = (int *)(buf + up(jb) + up(eb));
-        DBX_HIP(hipMemcpyAsync(d_jobs, sorted.data(), jb, hipMemcpyHostToDevice, ctx->stream));
-        DBX_HIP(hipMemcpyAsync(d_e2, edge2.data(), eb, hipMemcpyHostToDevice, ctx->stream));
+    ctx->p1_timed = false;
+    if (m.plan_jobs > 0) {
         unsigned long long *cnt = (unsigned long long *)ctx->counters;
         DBX_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
         if (n_fast > 0)
             hipLaunchKernelGGL(p1_histogram_kernel<false>, dim3((unsigned)n_fast), dim3(kP1Threads), 0,
-                               ctx->stream, m.pos, m.tiles, d_jobs, n_fast, n_checked, d_e2, rp,
-                               (unsigned long long *)d_hist, n_bins, cnt, d_retry);
+                               ctx->stream, m.pos, m.tiles, m.d_jobs, m.d_alist, n_fast, n_checked,
+                               m.d_edge2, rp, (unsigned long long *)d_hist, n_bins, cnt, m.d_retry);
         // checked variant: the degraded jobs plus whatever the fast launch queued
         hipLaunchKernelGGL(p1_histogram_kernel<true>, dim3((unsigned)(n_checked + n_fast)),
-                           dim3(kP1Threads), 0, ctx->stream, m.pos, m.tiles, d_jobs, n_fast, n_checked,
-                           d_e2, rp, (unsigned long long *)d_hist, n_bins, cnt, d_retry);
+                           dim3(kP1Threads), 0, ctx->stream, m.pos, m.tiles, m.d_jobs, m.d_alist, n_fast,
+                           n_checked, m.d_edge2, rp, (unsigned long long *)d_hist, n_bins, cnt,
+                           m.d_retry);
         DBX_HIP(hipGetLastError());
         DBX_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+        ctx->p1_timed = true;
     }
-    if (sync_stats) {
-        uint64_t c[3];
-        DBX_HIP(hipMemcpyAsync(c, ctx->counters, sizeof(c), hipMemcpyDeviceToHost, ctx->stream));
-        DBX_HIP(hipStreamSynchronize(ctx->stream));
-        st.out_of_range = c[0];
-        st.window_misses = c[1];
-        st.retried = c[2];
-        ctx->last_p1_misses = c[1];
-        ctx->last_p1_retried = c[2];
-        if (!sorted.empty())
-            DBX_HIP(hipEventElapsedTime(&ctx->last_p1_kernel_ms, ctx->ev[0], ctx->ev[1]));
-    }
+    DBX_HIP(hipMemcpyAsync(ctx->counters_h, ctx->counters, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           ctx->stream));
     ctx->last_p1_jobs = st.n_jobs;
+    if (sync_stats) {
+        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        pass1_read_stats(ctx, st);
+    }
     return st;
+}
+
+void pass1_read_stats(dbx_ctx *ctx, P1Stats &st)
+{
+    const uint64_t *c = ctx->counters_h;
+    st.out_of_range = c[0];
+    st.window_misses = c[1];
+    st.retried = c[2];
+    ctx->last_p1_misses = c[1];
+    ctx->last_p1_retried = c[2];
+    if (ctx->p1_timed) DBX_HIP(hipEventElapsedTime(&ctx->last_p1_kernel_ms, ctx->ev[0], ctx->ev[1]));
 }
 
 }  // namespace dbx

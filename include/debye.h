@@ -91,8 +91,10 @@ int dbx_model_info(const dbx_model *m, int64_t *n_atoms, int *n_groups, int *n_p
  * Same subgroup: pairs col <= row including self pairs (bin 0 += n); cross: n_a*n_b.
  * out: uint64[P][n_bins] (host memory, or device memory when out_is_device != 0).
  * Returns DBX_ERR_RANGE (histogram still written, offending pairs not counted) if any
- * pair falls at k >= n_bins; *n_out_of_range (optional) receives their number. */
-int dbx_histogram(dbx_ctx *ctx, const dbx_model *m, double rstep, int64_t n_bins, uint64_t *out,
+ * pair falls at k >= n_bins; *n_out_of_range (optional) receives their number.
+ * The model keeps its pass-1 work plan for the last rstep (host planning runs once per
+ * model and bin width). */
+int dbx_histogram(dbx_ctx *ctx, dbx_model *m, double rstep, int64_t n_bins, uint64_t *out,
                   int out_is_device, uint64_t *n_out_of_range);
 
 /* ---- prefactor ----------------------------------------------------------------------
@@ -119,7 +121,7 @@ int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int
  * One model end to end on the device: pass 1, compaction, prefactors, pass 2.
  * params[n_groups]; I_out[nq]; hist_out ([P][n_bins], optional, host) receives the
  * histograms.  Equivalent to the reference's subgroup-pair loop (cuDebye-v15.cu:941-975). */
-int dbx_intensity(dbx_ctx *ctx, const dbx_model *m, const dbx_group_params *params,
+int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params,
                   double rstep, int64_t n_bins, int first_n, const double *q, int nq,
                   double *I_out, uint64_t *hist_out);
 

@@ -13,24 +13,26 @@ extern "C" {
 #endif
 
 /* Tiles of a model: 8 x 32-bit words each (cx, cy, cz, rad as float; start, count, group,
- * pad as int32).  Returns the tile count; fills `out` when cap >= count. */
+ * parent as int32).  Returns the tile count; fills `out` when cap >= count. */
 int dbx_debug_model_tiles(const dbx_model *m, void *out, int cap);
 /* Atom positions in device (tile) order as float4 x, y, z, 0.  Returns the atom count;
  * copies when cap >= count. */
 int64_t dbx_debug_model_positions(dbx_ctx *ctx, const dbx_model *m, float *out4, int64_t cap);
-/* Pass-1 plan for (m, rstep): work items of 12 int32 each (pair, a0, a1, b0, b1, same, shell,
- * k_lo, e2_off, degraded, pad x2) in launch order, and the shell-edge table (float, squared A^2).
+/* Pass-1 plan for (m, rstep): work items of 12 int32 each (pair, a_off, a_cnt, b0, b1, same,
+ * shell, k_lo, e2_off, degraded, R, pad) in launch order, and the shell-edge table (float, squared A^2).
  * Returns DBX_OK and the counts; copies when the capacities suffice. */
 int dbx_debug_pass1_plan(const dbx_model *m, double rstep, void *jobs, int cap_jobs, float *edge2,
                          int cap_edge2, int *n_jobs, int *n_edge2);
 
-/* The same tiling and plan computed on the host from arrays (no device call): tiles, the
- * tile-ordered positions (float4), work items and shell edges, each copied when its
- * capacity suffices; the counts are always returned. */
+/* The same tiling and plan computed on the host from arrays (no device call): tiles (all
+ * three levels: 64-, 32- and 16-atom tiles, in that order), the tile-ordered positions
+ * (float4), work items, shell edges and the A-tile index list the work items point into,
+ * each copied when its capacity suffices; the counts are always returned. */
 int dbx_debug_plan_host(const float *x, const float *y, const float *z, const int64_t *group_n,
                         int n_groups, double rstep, void *tiles, int cap_tiles, float *pos4,
                         int64_t cap_pos, void *jobs, int cap_jobs, float *edge2, int cap_edge2,
-                        int *n_tiles, int *n_jobs, int *n_edge2);
+                        int32_t *alist, int cap_alist, int *n_tiles, int *n_jobs, int *n_edge2,
+                        int *n_alist);
 
 #ifdef __cplusplus
 }

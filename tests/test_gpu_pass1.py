@@ -159,3 +159,23 @@ def test_repeatable_and_permutation_invariant(ctx):
     mod2 = ctx.model(m.x[perm], m.y[perm], m.z[perm], m.group_n)
     H3 = ctx.histogram(mod2, m.rstep, m.n_bins)
     assert np.array_equal(H1, H2) and np.array_equal(H1, H3)
+
+
+@pytest.mark.parametrize("maker", ["c3_small", "c4_model5", "c5_req7", "sparse"])
+def test_mixed_tile_sizes_bit_exact(ctx, maker):
+    """Workloads whose sparse subgroups need 32- and 16-atom A tiles (two or four B atoms per
+    step) and fine bins that run checked jobs: a 10 nm sphere of the rocksalt-like three-species
+    lattice at 0.0005 A bins, two request models of the batched/serving streams, a sparse cloud."""
+    m = {"c3_small": lambda: synth.rocksalt_sphere(diameter=100.0, nq=64),
+         "c4_model5": lambda: synth.batch_model(5),
+         "c5_req7": lambda: synth.serve_request(7),
+         "sparse": lambda: synth.small_random(1500, 2, 43, spread=45.0)}[maker]()
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    H = ctx.histogram(mod, m.rstep, m.n_bins)
+    n = m.group_n
+    S = len(n)
+    want = [n[a] * (n[a] + 1) // 2 if a == b else n[a] * n[b] for a in range(S) for b in range(a, S)]
+    assert [int(h.sum()) for h in H] == want
+    ref, ov = _oracle_hist(m)
+    assert ov == 0
+    assert np.array_equal(H, ref)

@@ -10,7 +10,7 @@ import pytest
 import cudebye_amd as cd
 from cudebye_amd import synth
 
-W = 36864  # LDS window bins (kP1Bins)
+W = 35328  # LDS window bins (kP1Bins)
 
 
 def host_plan(m):
@@ -20,19 +20,20 @@ def host_plan(m):
     x, y, z = (np.ascontiguousarray(v, np.float32) for v in (m.x, m.y, m.z))
     gn = np.ascontiguousarray(m.group_n, np.int64)
     p = lambda a: a.ctypes.data_as(C.c_void_p)
-    nt, nj, ne = C.c_int(), C.c_int(), C.c_int()
+    nt, nj, ne, na = C.c_int(), C.c_int(), C.c_int(), C.c_int()
     rc = fn(p(x), p(y), p(z), p(gn), len(gn), C.c_double(m.rstep), None, 0, None, C.c_int64(0),
-            None, 0, None, 0, C.byref(nt), C.byref(nj), C.byref(ne))
+            None, 0, None, 0, None, 0, C.byref(nt), C.byref(nj), C.byref(ne), C.byref(na))
     assert rc == 0
     tiles = np.zeros((nt.value, 8), np.int32)
     pos = np.zeros((m.n_atoms, 4), np.float32)
     jobs = np.zeros((nj.value, 12), np.int32)
     e2 = np.zeros(ne.value, np.float32)
+    al = np.zeros(max(1, na.value), np.int32)
     rc = fn(p(x), p(y), p(z), p(gn), len(gn), C.c_double(m.rstep), p(tiles), nt.value, p(pos),
-            C.c_int64(m.n_atoms), p(jobs), nj.value, p(e2), ne.value, C.byref(nt), C.byref(nj),
-            C.byref(ne))
+            C.c_int64(m.n_atoms), p(jobs), nj.value, p(e2), ne.value, p(al), na.value,
+            C.byref(nt), C.byref(nj), C.byref(ne), C.byref(na))
     assert rc == 0
-    return tiles, pos, jobs, e2
+    return tiles, pos, jobs, e2, al[:na.value]
 
 
 def _bins(A, B, rp):
@@ -43,29 +44,27 @@ def _bins(A, B, rp):
 
 
 def check_plan(m):
-    tiles, pos, jobs, e2 = host_plan(m)
+    tiles, pos, jobs, e2, alist = host_plan(m)
     tf = tiles.view(np.float32)
-    cen, rad, start, cnt, grp = tf[:, :3], tf[:, 3], tiles[:, 4], tiles[:, 5], tiles[:, 6]
+    cen, rad = tf[:, :3], tf[:, 3]
+    start, cnt, grp, parent = tiles[:, 4], tiles[:, 5], tiles[:, 6], tiles[:, 7]
     rp = np.float32(1.0 / m.rstep)
     S = len(m.group_n)
     assert np.all(cnt >= 1) and np.all(cnt <= 64)
-    # tiles partition every subgroup
-    gstart = np.concatenate([[0], np.cumsum(m.group_n)])
-    for g in range(S):
-        sel = grp == g
-        assert cnt[sel].sum() == m.group_n[g]
     pairs = [(a, b) for a in range(S) for b in range(a, S)]
     seen = {}
     counted = np.zeros(len(pairs), np.int64)
     for J in jobs:
-        pair, a0, a1, b0, b1, same, shell, klo, eoff, degraded = J[:10]
+        pair, a_off, a_cnt, b0, b1, same, shell, klo, eoff, degraded, R = J[:11]
         ga, gb = pairs[pair]
+        assert R in (1, 2, 4)
         lo, hi = e2[eoff + shell], e2[eoff + shell + 1]
-        for t in range(a0, a1):
-            assert grp[t] == ga
+        for t in alist[a_off:a_off + a_cnt]:
+            assert grp[t] == ga and cnt[t] <= 64 // R
             A = pos[start[t]:start[t] + cnt[t], :3]
-            for j in range(max(b0, t) if same else b0, b1):
-                assert grp[j] == gb
+            ia = np.arange(start[t], start[t] + cnt[t])
+            for j in range(max(b0, parent[t]) if same else b0, b1):
+                assert grp[j] == gb and cnt[j] <= 64
                 key = (pair, t, j, int(shell))
                 assert key not in seen, key  # each (A tile, B tile, shell) once
                 seen[key] = True
@@ -76,10 +75,9 @@ def check_plan(m):
                 if not np.any(sel):
                     continue
                 k = _bins(A, B[sel], rp)
-                if same and j == t:
-                    ia = np.arange(cnt[t])[:, None]
-                    ib = np.arange(cnt[j])[sel][None, :]
-                    valid = ia <= ib
+                if same and j == parent[t]:
+                    ib = np.arange(start[j], start[j] + cnt[j])[sel]
+                    valid = ia[:, None] <= ib[None, :]
                 else:
                     valid = np.ones(k.shape, bool)
                 counted[pair] += int(valid.sum())
@@ -92,11 +90,22 @@ def check_plan(m):
     return jobs
 
 
-@pytest.mark.parametrize("maker", ["c1", "rand3", "batch3", "batch0"])
+@pytest.mark.parametrize("maker", ["c1", "rand3", "batch3", "batch0", "sparse", "fine"])
 def test_plan_covers_every_pair_once_and_windows_hold(maker):
     m = {"c1": synth.fcc_sphere,
          "rand3": lambda: synth.small_random(3000, 3, 41, spread=20.0),
          "batch3": lambda: synth.batch_model(3),
-         "batch0": lambda: synth.batch_model(0)}[maker]()
+         "batch0": lambda: synth.batch_model(0),
+         # sparse species: a few atoms spread over a large volume (32/16-atom A tiles)
+         "sparse": lambda: synth.small_random(1500, 2, 43, spread=45.0),
+         # fine bins: narrow window (16-atom tiles, checked jobs)
+         "fine": lambda: synth.small_random(1200, 1, 44, spread=12.0, rstep=1e-4)}[maker]()
     jobs = check_plan(m)
     assert len(jobs) > 0
+
+
+def test_plan_uses_smaller_tiles_where_needed():
+    _, _, jobs, _, _ = host_plan(synth.small_random(1500, 2, 43, spread=45.0))
+    assert set(jobs[:, 10]) >= {2} or set(jobs[:, 10]) >= {4}
+    _, _, jobs, _, _ = host_plan(synth.spinel_sphere())
+    assert jobs[:, 9].sum() == 0  # no checked jobs on the bench workload
