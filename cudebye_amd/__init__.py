@@ -29,6 +29,8 @@ ERRORS = {-1: "DBX_ERR_ARG", -2: "DBX_ERR_HIP", -3: "DBX_ERR_IO", -4: "DBX_ERR_P
 DBX_ERR_RANGE = -5
 DBX_RUN_CRLF = 1
 DBX_RUN_QUIET = 2
+DBX_PASS2_MFMA = 0
+DBX_PASS2_FP64 = 1
 
 _lib = None
 
@@ -84,6 +86,8 @@ def lib():
             "dbx_selftest_bins": (i32, [vp, vp, i32, f32, vp]),
             "dbx_last_pass1_stats": (i32, [vp, P(i64), P(u64)]),
             "dbx_last_kernel_times": (i32, [vp, P(f64), P(f64)]),
+            "dbx_set_pass2_mode": (i32, [vp, i32]),
+            "dbx_selftest_sin": (i32, [vp, i32, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -97,7 +101,7 @@ EXPORTS = ("dbx_open", "dbx_close", "dbx_last_error", "dbx_stream", "dbx_q_from_
            "dbx_channels", "dbx_model_create", "dbx_model_destroy", "dbx_model_info",
            "dbx_histogram", "dbx_prefactor", "dbx_debye", "dbx_intensity", "dbx_run_deb",
            "dbx_selftest_sqrt", "dbx_selftest_bins", "dbx_last_pass1_stats",
-           "dbx_last_kernel_times")
+           "dbx_last_kernel_times", "dbx_set_pass2_mode", "dbx_selftest_sin")
 
 
 def _p(a):
@@ -122,9 +126,9 @@ def channels(rmax, rstep) -> int:
     return n
 
 
-def run_deb(path, crlf=False, quiet=True) -> int:
+def run_deb(path, crlf=False, quiet=True, mfma=False) -> int:
     """Runs a .deb file like `cuDebye.exe path` (writes path+'I' and/or path+'x<r>x<c>')."""
-    flags = (DBX_RUN_CRLF if crlf else 0) | (DBX_RUN_QUIET if quiet else 0)
+    flags = (DBX_RUN_CRLF if crlf else 0) | (DBX_RUN_QUIET if quiet else 0) | (4 if mfma else 0)
     return int(lib().dbx_run_deb(os.fspath(path).encode(), flags))
 
 
@@ -144,6 +148,15 @@ def selftest_bins(a, b, rp):
     rc = lib().dbx_selftest_bins(_p(a), _p(b), len(a), float(np.float32(rp)), _p(out))
     if rc != DBX_OK:
         raise DebyeError(rc, "selftest_bins")
+    return out
+
+
+def selftest_sin(x):
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.zeros_like(x)
+    rc = lib().dbx_selftest_sin(_p(x), len(x), _p(out))
+    if rc != DBX_OK:
+        raise DebyeError(rc, "selftest_sin")
     return out
 
 
@@ -218,6 +231,10 @@ class Context:
         nj, miss = C.c_int64(0), C.c_uint64(0)
         lib().dbx_last_pass1_stats(self.handle, C.byref(nj), C.byref(miss))
         return int(nj.value), int(miss.value)
+
+    def set_pass2_mode(self, mode):
+        """DBX_PASS2_MFMA (split-BF16 matrix cores, default) or DBX_PASS2_FP64."""
+        self._check(lib().dbx_set_pass2_mode(self.handle, mode), "set_pass2_mode")
 
     def kernel_times(self):
         """(pass-1 histogram kernel ms, pass-2 structure ms) of the last calls (HIP events)."""

@@ -8,6 +8,7 @@
 #include <numeric>
 #include <vector>
 
+#include "fastmath.h"
 #include "internal.h"
 
 namespace dbx {
@@ -387,6 +388,14 @@ int dbx_model_destroy(dbx_ctx *ctx, dbx_model *m)
     });
 }
 
+int dbx_set_pass2_mode(dbx_ctx *ctx, int mode)
+{
+    if (!ctx || (mode != DBX_PASS2_MFMA && mode != DBX_PASS2_FP64)) return DBX_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->pass2_mode = mode;
+    return DBX_OK;
+}
+
 int dbx_last_pass1_stats(const dbx_ctx *ctx, int64_t *n_jobs, uint64_t *window_misses)
 {
     if (!ctx) return DBX_ERR_ARG;
@@ -601,6 +610,14 @@ int dbx_debug_plan_host(const float *x, const float *y, const float *z, const in
     } catch (const Error &e) {
         return e.code;
     }
+    return DBX_OK;
+}
+
+// the pass-2 sine evaluated on the host (same source as the device code)
+int dbx_debug_sin_host(const double *x, int n, double *out)
+{
+    if (!x || !out || n < 0) return DBX_ERR_ARG;
+    for (int i = 0; i < n; i++) out[i] = sin_cw(x[i]);
     return DBX_OK;
 }
 

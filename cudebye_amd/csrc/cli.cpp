@@ -14,11 +14,13 @@ int main(int argc, char **argv)
             flags |= DBX_RUN_CRLF;
         else if (!std::strcmp(argv[i], "--quiet"))
             flags |= DBX_RUN_QUIET;
+        else if (!std::strcmp(argv[i], "--mfma"))
+            flags |= DBX_RUN_MFMA;
         else
             path = argv[i];
     }
     if (!path) {
-        std::fprintf(stderr, "usage: %s file.deb [--crlf] [--quiet]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s file.deb [--crlf] [--quiet] [--mfma]\n", argv[0]);
         return 2;
     }
     const int rc = dbx_run_deb(path, flags);

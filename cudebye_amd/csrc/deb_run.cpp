@@ -186,6 +186,9 @@ extern "C" int dbx_run_deb(const char *path, int flags)
         std::fprintf(stderr, "ERROR: cannot open device %d\n", d.device);
         return rc;
     }
+    // the reference evaluates the Debye sum in FP64; the driver does too unless asked for the
+    // matrix-core path (which rounds sinc to 2^-32 and may move a printed 6th digit)
+    dbx_set_pass2_mode(ctx, (flags & DBX_RUN_MFMA) ? DBX_PASS2_MFMA : DBX_PASS2_FP64);
     dbx_model *m = nullptr;
     double t1 = now_s();
     rc = dbx_model_create(ctx, d.x.data(), d.y.data(), d.z.data(), d.n.data(), d.S, &m);

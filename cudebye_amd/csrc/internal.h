@@ -125,11 +125,14 @@ struct dbx_ctx {
     float last_p2_ms = 0.f;         // hipEvent time of the last pass-2 structure computation
     hipEvent_t ev[4] = {};
     bool p2_timed = false;
+    int pass2_mode = DBX_PASS2_MFMA;  // DBX_PASS2_MFMA or DBX_PASS2_FP64
+    int last_p2_bins = 0;             // compacted bins of the last pass 2
 };
 
 namespace dbx {
 
-enum { kSlotPass1 = 0, kSlotPass2 = 1, kSlotPass2b = 2, kSlotHist = 3, kSlotQ = 4 };
+enum { kSlotPass1 = 0, kSlotPass2 = 1, kSlotPass2b = 2, kSlotHist = 3, kSlotQ = 4, kSlotDigits = 5,
+       kSlotRtab = 6 };
 // device buffer of at least `bytes` for `slot`; contents are not preserved when it grows
 void *scratch(dbx_ctx *ctx, int slot, size_t bytes);
 

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "binrule.h"
+#include "fastmath.h"
 #include "internal.h"
 
 #pragma clang fp contract(off)
@@ -39,9 +40,29 @@ __global__ void st_bins_kernel(const float *__restrict__ a, const float *__restr
     if (i < n) out[i] = pair_bin(a[3 * i], a[3 * i + 1], a[3 * i + 2], b[3 * i], b[3 * i + 1], b[3 * i + 2], rp);
 }
 
+__global__ void st_sin_kernel(const double *__restrict__ x, int n, double *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = sin_cw(x[i]);
+}
+
 }  // namespace dbx
 
 using namespace dbx;
+
+// out[i] = the device sine of the Debye sum (sin_cw) at x[i], |x| < 2^20.
+extern "C" int dbx_selftest_sin(const double *x, int n, double *out)
+{
+    if (!x || !out || n < 0) return DBX_ERR_ARG;
+    if (n == 0) return DBX_OK;
+    double *d = nullptr;
+    if (hipMalloc(&d, sizeof(double) * 2 * (size_t)n) != hipSuccess) return DBX_ERR_HIP;
+    (void)hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice);
+    hipLaunchKernelGGL(st_sin_kernel, dim3((n + 255) / 256), dim3(256), 0, 0, d, n, d + n);
+    hipError_t e = hipMemcpy(out, d + n, sizeof(double) * n, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    return e == hipSuccess ? DBX_OK : DBX_ERR_HIP;
+}
 
 // Checks sqrt_cr for every float bit pattern in [lo, hi) with 2^-96 <= s <= FLT_MAX
 // (plus s == 0 when lo == 0).  n_bad[0]: results that are not correctly rounded;

@@ -117,6 +117,13 @@ int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int
               double rstep, int first_n, const double *q, int nq, const double *ffoobb,
               double *I_out, double *S_out);
 
+/* Pass-2 arithmetic of a context (default DBX_PASS2_MFMA):
+ *   DBX_PASS2_MFMA: split-BF16 matrix-core contraction with exact FP32 accumulation; sinc is
+ *     evaluated in FP64 and rounded to a multiple of 2^-40 (|error| <= 2^-41 per bin);
+ *   DBX_PASS2_FP64: FP64 vector arithmetic throughout (the reference's debyeRowSum). */
+enum { DBX_PASS2_MFMA = 0, DBX_PASS2_FP64 = 1 };
+int dbx_set_pass2_mode(dbx_ctx *ctx, int mode);
+
 /* ---- whole model ----------------------------------------------------------------------
  * One model end to end on the device: pass 1, compaction, prefactors, pass 2.
  * params[n_groups]; I_out[nq]; hist_out ([P][n_bins], optional, host) receives the
@@ -129,8 +136,9 @@ int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params,
  * The reference program: `cuDebye.exe file.deb` (cuDebye-v15.cu:599-1003).  Parses the
  * .deb, runs every subgroup pair, writes <path>I (hist 0 or 2) and <path>x<r>x<c>
  * (hist 1 or 2) with the reference's formats.  flags: DBX_RUN_CRLF writes "\r\n" line
- * ends like the reference's Windows build; DBX_RUN_QUIET suppresses the log on stdout. */
-enum { DBX_RUN_CRLF = 1, DBX_RUN_QUIET = 2 };
+ * ends like the reference's Windows build; DBX_RUN_QUIET suppresses the log on stdout;
+ * DBX_RUN_MFMA evaluates pass 2 on the matrix cores (default: FP64, as the reference). */
+enum { DBX_RUN_CRLF = 1, DBX_RUN_QUIET = 2, DBX_RUN_MFMA = 4 };
 int dbx_run_deb(const char *path, int flags);
 
 /* ---- diagnostics --------------------------------------------------------------------
@@ -142,6 +150,8 @@ int dbx_run_deb(const char *path, int flags);
  * dbx_selftest_bins: out[i] = device bin of the pair (a[3i..], b[3i..]) with rp. */
 int dbx_selftest_sqrt(uint32_t lo, uint32_t hi, uint64_t *n_bad);
 int dbx_selftest_bins(const float *a, const float *b, int n, float rp, int32_t *out);
+/* dbx_selftest_sin: out[i] = the device FP64 sine used by the pass-2 sinc table, |x| < 2^20. */
+int dbx_selftest_sin(const double *x, int n, double *out);
 /* Statistics of the last pass-1 run on ctx: work items launched and pairs whose bin fell
  * outside their workgroup's LDS window (added to HBM directly; a performance figure). */
 int dbx_last_pass1_stats(const dbx_ctx *ctx, int64_t *n_jobs, uint64_t *window_misses);

@@ -34,6 +34,9 @@ int dbx_debug_plan_host(const float *x, const float *y, const float *z, const in
                         int32_t *alist, int cap_alist, int *n_tiles, int *n_jobs, int *n_edge2,
                         int *n_alist);
 
+/* The pass-2 FP64 sine (dbx_selftest_sin's device function) evaluated on the host. */
+int dbx_debug_sin_host(const double *x, int n, double *out);
+
 #ifdef __cplusplus
 }
 #endif

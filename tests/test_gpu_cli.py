@@ -50,3 +50,19 @@ def test_cli_errors(tmp_path):
     bad.write_text("0 8 1024\n1.5418 10 140 0.02\n50 0.001 0\n1\n3 X 1 0 1 1 1 1 1 1 1 1 1 1 0\n0 0 0\n1 1 1\n")
     r = subprocess.run([cd.CLI_PATH, str(bad)], capture_output=True, text=True)
     assert r.returncode != 0 and "match" in r.stderr
+
+
+def test_gamma_cli_mfma_pass2_within_bound(golden_dir, tmp_path):
+    """--mfma: pass 2 on the matrix cores; every printed I(Q) within one unit of the 6th
+    significant digit of the reference's printed value (both files round to 6 digits)."""
+    deb = tmp_path / "GammaAl2O3_10nm.deb"
+    shutil.copy(golden_dir / "GammaAl2O3_10nm.deb", deb)
+    r = subprocess.run([cd.CLI_PATH, str(deb), "--mfma", "--quiet"], capture_output=True, text=True,
+                       timeout=600)
+    assert r.returncode == 0, r.stderr
+    got = np.array([float(l.split("\t")[1]) for l in read_debi(f"{deb}I")])
+    ref = np.array([float(l.split("\t")[1]) for l in read_debi(str(golden_dir / "GammaAl2O3_10nm.debI.ref"))])
+    assert len(got) == len(ref) == 6501
+    rel = np.abs(got - ref) / np.abs(ref)
+    worst = np.argsort(-rel)[:5]
+    assert rel.max() <= 1.0001e-5, [(int(i), got[i], ref[i]) for i in worst]

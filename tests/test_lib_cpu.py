@@ -13,8 +13,8 @@ from oracle import oracle as O
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
-    with open(os.path.join(ROOT, "include", "debye.h")) as f:
+def _declared(header):
+    with open(os.path.join(ROOT, "include", header)) as f:
         text = f.read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(dbx_[a-z0-9_]+)\s*\(", text)))
@@ -22,11 +22,13 @@ def _declared():
 
 def test_library_exports_every_declared_symbol():
     L = cd.lib()
-    names = _declared()
+    names = _declared("debye.h")
     assert len(names) >= 17
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(cd.EXPORTS)
+    for n in _declared("debye_debug.h"):
+        assert hasattr(L, n), n
 
 
 def test_cli_built():
