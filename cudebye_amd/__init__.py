@@ -41,6 +41,12 @@ class DebyeError(RuntimeError):
         self.code = code
 
 
+class BatchItem(C.Structure):
+    """dbx_batch_item: one model of a batch."""
+    _fields_ = [("model", C.c_void_p), ("params", C.c_void_p), ("n_bins", C.c_int64),
+                ("first_n", C.c_int)]
+
+
 class GroupParams(C.Structure):
     """dbx_group_params: occ, Beq, a1..a5, b1..b5, c of one subgroup."""
     _fields_ = [("occ", C.c_double), ("beq", C.c_double), ("a", C.c_double * 5),
@@ -88,6 +94,7 @@ def lib():
             "dbx_last_kernel_times": (i32, [vp, P(f64), P(f64)]),
             "dbx_set_pass2_mode": (i32, [vp, i32]),
             "dbx_selftest_sin": (i32, [vp, i32, vp]),
+            "dbx_batch_intensity": (i32, [vp, vp, i32, f64, vp, i32, vp, vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -101,7 +108,8 @@ EXPORTS = ("dbx_open", "dbx_close", "dbx_last_error", "dbx_stream", "dbx_q_from_
            "dbx_channels", "dbx_model_create", "dbx_model_destroy", "dbx_model_info",
            "dbx_histogram", "dbx_prefactor", "dbx_debye", "dbx_intensity", "dbx_run_deb",
            "dbx_selftest_sqrt", "dbx_selftest_bins", "dbx_last_pass1_stats",
-           "dbx_last_kernel_times", "dbx_set_pass2_mode", "dbx_selftest_sin")
+           "dbx_last_kernel_times", "dbx_set_pass2_mode", "dbx_selftest_sin",
+           "dbx_batch_intensity")
 
 
 def _p(a):
@@ -268,3 +276,34 @@ class Context:
         self._check(lib().dbx_intensity(self.handle, m.handle, arr, rstep, n_bins, first_n, _p(q),
                                         len(q), _p(I), None if H is None else _p(H)), "intensity")
         return (I, H) if with_hist else I
+
+    def batch_intensity(self, items, rstep, q, with_hist=False, return_oor=False):
+        """Many models in one pass-1 launch and one pass-2 contraction (dbx_batch_intensity).
+        items: sequence of (Model, params, n_bins, first_n).  Returns I[n_items, nq]
+        (+ list of histograms, + out-of-range counts per item)."""
+        n = len(items)
+        q = np.ascontiguousarray(q, np.float64)
+        keep = []
+        arr = (BatchItem * n)()
+        hists = []
+        for i, (m, params, nb, first_n) in enumerate(items):
+            pa = (GroupParams * m.n_groups)(*params)
+            keep.append(pa)
+            arr[i].model = m.handle.value
+            arr[i].params = C.cast(pa, C.c_void_p).value
+            arr[i].n_bins = int(nb)
+            arr[i].first_n = int(first_n)
+            if with_hist:
+                hists.append(np.zeros((m.n_pairs, int(nb)), np.uint64))
+        I = np.zeros((n, len(q)))
+        oor = np.zeros(n, np.uint64)
+        hp = (C.c_void_p * n)(*[h.ctypes.data for h in hists]) if with_hist else None
+        self._check(lib().dbx_batch_intensity(self.handle, arr, n, rstep, _p(q), len(q), _p(I), _p(oor),
+                                              hp), "batch_intensity",
+                    allow=(DBX_ERR_RANGE,) if return_oor else ())
+        out = (I,)
+        if with_hist:
+            out += (hists,)
+        if return_oor:
+            out += ([int(v) for v in oor],)
+        return out[0] if len(out) == 1 else out

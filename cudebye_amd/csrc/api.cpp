@@ -32,6 +32,25 @@ void *scratch(dbx_ctx *ctx, int slot, size_t bytes)
     return ctx->slot_ptr[slot];
 }
 
+void *pinned(dbx_ctx *ctx, int slot, size_t bytes)
+{
+    if (bytes == 0) bytes = 64;
+    if (ctx->pin_bytes[slot] < bytes) {
+        if (ctx->pin_ptr[slot]) {
+            DBX_HIP(hipStreamSynchronize(ctx->stream));
+            DBX_HIP(hipHostFree(ctx->pin_ptr[slot]));
+            ctx->pin_ptr[slot] = nullptr;
+            ctx->pin_bytes[slot] = 0;
+        }
+        const size_t want = bytes + bytes / 2;
+        void *p = nullptr;
+        DBX_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+        ctx->pin_ptr[slot] = p;
+        ctx->pin_bytes[slot] = want;
+    }
+    return ctx->pin_ptr[slot];
+}
+
 template <class F>
 static int guarded(dbx_ctx *ctx, F &&f)
 {
@@ -265,14 +284,14 @@ static void build_model(dbx_ctx *ctx, const float *x, const float *y, const floa
 
 static void free_model(ModelDev &m)
 {
-    for (void *p : {(void *)m.pos, (void *)m.tiles, (void *)m.d_jobs, (void *)m.d_edge2, (void *)m.d_retry,
+    for (void *p : {(void *)m.pos, (void *)m.tiles, (void *)m.d_jobs, (void *)m.d_edge2, (void *)m.d_list,
                     (void *)m.d_alist})
         if (p) (void)hipFree(p);
     m.pos = nullptr;
     m.tiles = nullptr;
     m.d_jobs = nullptr;
     m.d_edge2 = nullptr;
-    m.d_retry = nullptr;
+    m.d_list = nullptr;
     m.d_alist = nullptr;
 }
 
@@ -319,8 +338,10 @@ int dbx_close(dbx_ctx *ctx)
     if (!ctx) return DBX_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (int s = 0; s < 8; s++)
+    for (int s = 0; s < 12; s++)
         if (ctx->slot_ptr[s]) (void)hipFree(ctx->slot_ptr[s]);
+    for (int s = 0; s < 8; s++)
+        if (ctx->pin_ptr[s]) (void)hipHostFree(ctx->pin_ptr[s]);
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->counters_h) (void)hipHostFree(ctx->counters_h);
     for (int i = 0; i < 4; i++)
@@ -496,6 +517,87 @@ int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int
     });
 }
 
+int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, double rstep,
+                        const double *q, int nq, double *I_out, uint64_t *n_out_of_range,
+                        uint64_t *const *hist_out)
+{
+    if (!items || n_items <= 0 || !q || nq <= 0 || !I_out) return DBX_ERR_ARG;
+    for (int i = 0; i < n_items; i++)
+        if (!items[i].model || !items[i].params || items[i].n_bins <= 0 || items[i].first_n < 0)
+            return DBX_ERR_ARG;
+    int rc = DBX_OK;
+    int grc = guarded(ctx, [&] {
+        // models longest first: pass 2 wants its rows ordered by length
+        std::vector<int> order(n_items);
+        for (int i = 0; i < n_items; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(),
+                         [&](int a, int b) { return items[a].n_bins > items[b].n_bins; });
+        std::vector<int64_t> hoff(n_items);
+        int64_t total = 0;
+        int n_rows = 0;
+        for (int i : order) {
+            const int S = items[i].model->d.n_groups;
+            hoff[i] = total;
+            total += (int64_t)(S * (S + 1) / 2) * items[i].n_bins;
+            n_rows += S * (S + 1) / 2;
+        }
+        uint64_t *dh = (uint64_t *)scratch(ctx, kSlotBatchHist, sizeof(uint64_t) * (size_t)total);
+        uint64_t *door = (uint64_t *)scratch(ctx, kSlotOor, sizeof(uint64_t) * (size_t)n_items);
+        const size_t nqd = (size_t)nq;
+        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * nqd * (1 + 2 * (size_t)n_rows + n_items));
+        double *dq = d, *dff = d + nqd, *dS = dff + n_rows * nqd, *dI = dS + n_rows * nqd;
+        DBX_HIP(hipMemsetAsync(door, 0, sizeof(uint64_t) * (size_t)n_items, ctx->stream));
+        DBX_HIP(hipMemcpyAsync(dq, q, sizeof(double) * nqd, hipMemcpyHostToDevice, ctx->stream));
+        std::vector<P1Item> p1;
+        std::vector<RowDesc> rows;
+        std::vector<GroupPair> pairs;
+        std::vector<int4> mrows;
+        for (int i : order) {
+            const dbx_batch_item &it = items[i];
+            const int S = it.model->d.n_groups;
+            P1Item pi{&it.model->d, it.n_bins, dh + hoff[i]};
+            pi.d_oor = door + i;
+            p1.push_back(pi);
+            mrows.push_back(make_int4((int)rows.size(), S * (S + 1) / 2, i, 0));
+            int p = 0;
+            for (int a = 0; a < S; a++)
+                for (int b = a; b < S; b++, p++) {
+                    rows.push_back(RowDesc{hoff[i] + (int64_t)p * it.n_bins, it.n_bins, it.first_n, 0});
+                    pairs.push_back(make_group_pair(it.params[a], it.params[b]));
+                }
+        }
+        pass1_batch(ctx, p1, rstep, false);
+        prefactor_rows(ctx, pairs, dq, nq, dff);
+        debye_structure_rows(ctx, dh, rows, rstep, dq, nq, dS);
+        combine_models(ctx, dff, dS, mrows, nq, dI);
+        DBX_HIP(hipMemcpyAsync(I_out, dI, sizeof(double) * nqd * n_items, hipMemcpyDeviceToHost, ctx->stream));
+        if (hist_out)
+            for (int i = 0; i < n_items; i++)
+                if (hist_out[i]) {
+                    const int S = items[i].model->d.n_groups;
+                    DBX_HIP(hipMemcpyAsync(hist_out[i], dh + hoff[i],
+                                           sizeof(uint64_t) * (size_t)(S * (S + 1) / 2) * items[i].n_bins,
+                                           hipMemcpyDeviceToHost, ctx->stream));
+                }
+        uint64_t *hoor = (uint64_t *)pinned(ctx, kPinOor, sizeof(uint64_t) * (size_t)n_items);
+        DBX_HIP(hipMemcpyAsync(hoor, door, sizeof(uint64_t) * (size_t)n_items, hipMemcpyDeviceToHost, ctx->stream));
+        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        P1Stats st;
+        pass1_read_stats(ctx, st);
+        int bad = 0;
+        for (int i = 0; i < n_items; i++) {
+            if (n_out_of_range) n_out_of_range[i] = hoor[i];
+            bad += hoor[i] != 0;
+        }
+        if (bad) {
+            rc = DBX_ERR_RANGE;
+            ctx->err = std::to_string(bad) + " model(s) have pair distances at or beyond their last bin";
+        }
+    });
+    if (grc != DBX_OK) return grc;
+    return rc;
+}
+
 int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params, double rstep,
                   int64_t n_bins, int first_n, const double *q, int nq, double *I_out,
                   uint64_t *hist_out)
@@ -512,9 +614,10 @@ int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params, do
         double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 2)));
         double *dq = d, *dff = d + nqd, *dS = dff + P * nqd, *dI = dS + P * nqd;
         DBX_HIP(hipMemcpyAsync(dq, q, sizeof(double) * nqd, hipMemcpyHostToDevice, ctx->stream));
-        int p = 0;
+        std::vector<GroupPair> pairs;
         for (int a = 0; a < S; a++)
-            for (int b = a; b < S; b++, p++) prefactor_dev(ctx, params[a], params[b], dq, nq, dff + p * nqd);
+            for (int b = a; b < S; b++) pairs.push_back(make_group_pair(params[a], params[b]));
+        prefactor_rows(ctx, pairs, dq, nq, dff);
         debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS);
         combine_intensity(ctx, dff, dS, P, nq, dI);
         DBX_HIP(hipMemcpyAsync(I_out, dI, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));

@@ -93,8 +93,25 @@ struct ModelDev {
     int plan_jobs = 0, plan_fast = 0;
     P1Job *d_jobs = nullptr;
     float *d_edge2 = nullptr;
-    int *d_retry = nullptr;
     int32_t *d_alist = nullptr;
+    int4 *d_list = nullptr;          // launch list of this model alone: (0, job, piece, pieces)
+    int plan_list_n = 0, plan_list_fast = 0;
+    std::vector<double> plan_cost;   // estimated cost of each job (launch order)
+    std::vector<P1Job> plan_h;       // host copy of the jobs
+};
+
+// One model of a pass-1 launch, as the kernel sees it (64 bytes).
+struct P1Ref {
+    const float4 *pos;
+    const Tile *tiles;
+    const P1Job *jobs;
+    const int32_t *alist;
+    const float *edge2;
+    unsigned long long *hist;  // this model's [P][n_bins] block
+    int64_t n_bins;
+    float rp;
+    int32_t pad;
+    unsigned long long *oor;   // this model's count of pairs at or beyond n_bins
 };
 
 }  // namespace dbx
@@ -110,8 +127,10 @@ struct dbx_ctx {
     std::string err;
     std::recursive_mutex mu;
     // grow-only device scratch, one buffer per slot (dbx::kSlot*)
-    void *slot_ptr[8] = {};
-    size_t slot_bytes[8] = {};
+    void *slot_ptr[12] = {};
+    size_t slot_bytes[12] = {};
+    void *pin_ptr[8] = {};
+    size_t pin_bytes[8] = {};
     // [0] out-of-range pairs, [1] pairs outside the LDS window (checked kernel),
     // [2] fast jobs queued for the checked kernel; then the retry list (int32)
     uint64_t *counters = nullptr;
@@ -132,9 +151,14 @@ struct dbx_ctx {
 namespace dbx {
 
 enum { kSlotPass1 = 0, kSlotPass2 = 1, kSlotPass2b = 2, kSlotHist = 3, kSlotQ = 4, kSlotDigits = 5,
-       kSlotRtab = 6 };
+       kSlotRtab = 6, kSlotBatch = 7, kSlotRows = 8, kSlotPairs = 9, kSlotBatchHist = 10,
+       kSlotOor = 11 };
 // device buffer of at least `bytes` for `slot`; contents are not preserved when it grows
 void *scratch(dbx_ctx *ctx, int slot, size_t bytes);
+// pinned host buffer of at least `bytes` for `slot` (staging of per-call uploads; every call
+// synchronises its stream before returning, so a slot is free again at the next call)
+enum { kPinRefs = 0, kPinList = 1, kPinRows = 2, kPinPairs = 3, kPinMisc = 4, kPinModels = 5, kPinOor = 6 };
+void *pinned(dbx_ctx *ctx, int slot, size_t bytes);
 
 // pass 1 (pass1.hip): histograms of all subgroup pairs into d_hist[P][n_bins]
 // (zeroed here).  Returns the number of pairs with bin >= n_bins.
@@ -147,15 +171,45 @@ struct P1Stats {
 };
 P1Stats pass1_histograms(dbx_ctx *ctx, ModelDev &m, double rstep, int64_t n_bins,
                          uint64_t *d_hist, bool sync_stats = true);
+// several models in one launch pair (fast + checked), all with bin width rstep
+struct P1Item {
+    ModelDev *m;
+    int64_t n_bins;
+    uint64_t *d_hist;  // [P][n_bins] of this model (zeroed by the call)
+    uint64_t *d_oor = nullptr;  // its out-of-range counter (caller-zeroed); null: the context's
+};
+P1Stats pass1_batch(dbx_ctx *ctx, const std::vector<P1Item> &items, double rstep, bool sync_stats);
 // reads the pass-1 counters of the last launch (after the stream is synchronised)
 void pass1_read_stats(dbx_ctx *ctx, P1Stats &st);
 // host planning of pass 1: work items (largest first) and the shell-edge tables
 void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &jobs, std::vector<float> &edge2,
-                std::vector<int32_t> &alist, int n_cu = 256);
+                std::vector<int32_t> &alist, int n_cu = 256, std::vector<double> *cost = nullptr);
 
 // pass 2 (pass2.hip)
+// one row of pass 2: a partial histogram at hist_base + h_off with n_bins bins
+struct RowDesc {
+    int64_t h_off;
+    int64_t n_bins;
+    int32_t first_n;  // weight rule: the first first_n nonzero bins at r <= 0.0005 count once
+    int32_t pad;
+};
+// the scattering parameters of one subgroup pair (row) for the prefactor
+struct GroupPair {
+    double occ_a, occ_b, beq_a, beq_b, ca, cb;
+    double aa[5], ba[5], ab[5], bb[5];
+};
+GroupPair make_group_pair(const dbx_group_params &ga, const dbx_group_params &gb);
 void prefactor_dev(dbx_ctx *ctx, const dbx_group_params &ga, const dbx_group_params &gb,
                    const double *d_q, int nq, double *d_ff);
+// ff[r][q] for every row r
+void prefactor_rows(dbx_ctx *ctx, const std::vector<GroupPair> &rows, const double *d_q, int nq,
+                    double *d_ff);
+// S[r][q] for every row r (rows of any models sharing rstep, first_n and the q grid)
+void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::vector<RowDesc> &rows,
+                          double rstep, const double *d_q, int nq, double *d_S);
+// I[mrows[m].z][q] = sum over rows [mrows[m].x, mrows[m].x + mrows[m].y) of ff S
+void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const std::vector<int4> &mrows,
+                    int nq, double *d_I);
 // S_p(q) for all p into d_S[P][nq] from device histograms d_hist[P][n_bins].
 void debye_structure(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
                      int first_n, const double *d_q, int nq, double *d_S);

@@ -72,30 +72,42 @@ __device__ __forceinline__ void count_pair(uint32_t *h, int k, int k_lo, int64_t
 // window is narrower than their geometry, and the queued ones).
 template <bool CHECKED>
 __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
-    const float4 *__restrict__ pos, const Tile *__restrict__ tiles, const P1Job *__restrict__ jobs,
-    const int32_t *__restrict__ alist, int n_fast, int n_checked, const float *__restrict__ edge2,
-    float rp, unsigned long long *__restrict__ hist, int64_t n_bins,
-    unsigned long long *__restrict__ counters, int *__restrict__ retry)
+    const P1Ref *__restrict__ refs, const int4 *__restrict__ list, int n_fast, int n_checked,
+    unsigned long long *__restrict__ counters, int4 *__restrict__ retry)
 {
     __shared__ uint32_t h[kP1Bins + 1];
     __shared__ uint32_t cand[kP1Cand];
     __shared__ float4 stage[kP1Threads / 64][64];
     __shared__ int s_ncand, s_next;
 
-    int job_index;
+    // list entry = (model, job, piece, pieces); the checked launch runs the planned checked
+    // jobs, then the fast jobs the fast launch queued for a re-run
+    int4 entry;
     if (!CHECKED) {
-        job_index = blockIdx.x;
+        entry = list[blockIdx.x];
     } else {
-        // checked launch: the planned checked jobs, then the fast jobs queued for a re-run
         if ((int)blockIdx.x < n_checked) {
-            job_index = n_fast + blockIdx.x;
+            entry = list[n_fast + blockIdx.x];
         } else {
             const int r = blockIdx.x - n_checked;
             if (r >= (int)counters[2]) return;
-            job_index = retry[r];
+            entry = retry[r];
         }
     }
-    const P1Job job = jobs[job_index];
+    const P1Ref M = refs[entry.x];
+    P1Job job = M.jobs[entry.y];
+    if (entry.w > 1) {
+        // piece z of w of the job's B-tile range
+        const int b0 = job.b0, nb = job.b1 - job.b0;
+        job.b0 = b0 + (int)((int64_t)nb * entry.z / entry.w);
+        job.b1 = b0 + (int)((int64_t)nb * (entry.z + 1) / entry.w);
+    }
+    const float4 *__restrict__ pos = M.pos;
+    const Tile *__restrict__ tiles = M.tiles;
+    const int32_t *__restrict__ alist = M.alist;
+    const float *__restrict__ edge2 = M.edge2;
+    const float rp = M.rp;
+    const int64_t n_bins = M.n_bins;
     for (int i = threadIdx.x; i <= kP1Bins; i += kP1Threads) h[i] = 0u;
 
     const int lane = threadIdx.x & 63;
@@ -109,7 +121,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const float r_lo = e_lo > 0.f ? sqrtf(e_lo) : 0.f;
     const float r_hi = sqrtf(e_hi);  // +inf for the last shell
     const int k_lo = job.k_lo;
-    unsigned long long *hrow = hist + (int64_t)job.pair * n_bins;
+    unsigned long long *hrow = M.hist + (int64_t)job.pair * n_bins;
     const int nblk = (job.b1 - job.b0 + 63) >> 6;
     const int units = job.a_cnt * nblk;
     uint32_t miss = 0, oor = 0;
@@ -217,13 +229,13 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
         }
         if (lane == 0) {
             if (miss) atomicAdd(&counters[1], (unsigned long long)miss);
-            if (oor) atomicAdd(&counters[0], (unsigned long long)oor);
+            if (oor) atomicAdd(M.oor, (unsigned long long)oor);
         }
     } else if (h[kP1Bins] != 0u) {
         // a bin fell outside the window: leave this job to the checked variant
         if (threadIdx.x == 0) {
             const unsigned long long r = atomicAdd(&counters[2], 1ull);
-            retry[r] = job_index;
+            retry[r] = entry;
         }
         return;
     }
@@ -234,7 +246,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             if (k < n_bins)
                 atomicAdd(&hrow[k], (unsigned long long)v);
             else
-                atomicAdd(&counters[0], (unsigned long long)v);
+                atomicAdd(M.oor, (unsigned long long)v);
         }
     }
 }
@@ -275,10 +287,12 @@ double point_box_max(const Tile &t, const float *bb)
 //     checked kernel if even they do not fit: fine bins with wide tiles);
 //   - per level, A tiles sorted by radius and cut into chunks; each chunk gets its own shell
 //     width h = usable - 2 rad_max - margins and one job per shell it can reach;
-//   - jobs far above a CU's mean share are split along their B-tile range;
+//   - with n_cu > 0, jobs far above a CU's mean share are split along their B-tile range
+//     (n_cu = 0 keeps jobs whole: the launch lists cut them into pieces instead);
 //   - order: exact-window jobs, largest first, then the checked ones.
 void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
-                std::vector<float> &edge2, std::vector<int32_t> &alist, int n_cu)
+                std::vector<float> &edge2, std::vector<int32_t> &alist, int n_cu,
+                std::vector<double> *sorted_cost)
 {
     DBX_REQUIRE(rstep > 0 && std::isfinite(rstep), DBX_ERR_ARG, "rstep must be positive");
     const int S = m.n_groups;
@@ -411,7 +425,7 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
     // a shell are spread over the whole range: tiles are ordered by size, not by position)
     double total = 0;
     for (double c : cost) total += c;
-    const double target = total / (double)(std::max(1, n_cu) * 4);
+    const double target = n_cu > 0 ? total / (double)(n_cu * 4) : 1e300;
     std::vector<P1Job> split_jobs;
     std::vector<double> split_cost;
     for (size_t i = 0; i < jobs.size(); i++) {
@@ -437,71 +451,171 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
     });
     sorted.resize(split_jobs.size());
     for (size_t i = 0; i < order.size(); i++) sorted[i] = split_jobs[order[i]];
+    if (sorted_cost) {
+        sorted_cost->resize(split_jobs.size());
+        for (size_t i = 0; i < order.size(); i++) (*sorted_cost)[i] = split_cost[order[i]];
+    }
 }
 
-// builds (or reuses) the model's device copy of the pass-1 plan for rstep
+// launch entries of model `mi`: every job cut into pieces of about `target` cost along its
+// B-tile range (at most 64 pieces, at most one per B tile), split into exact-window and
+// checked entries with their costs
+typedef std::vector<std::pair<double, int4>> EntryList;
+static void add_entries(const ModelDev &m, int mi, double target, EntryList &fast, EntryList &checked)
+{
+    for (int j = 0; j < m.plan_jobs; j++) {
+        const P1Job &J = m.plan_h[j];
+        const int nb = J.b1 - J.b0;
+        int pieces = (int)std::min<double>(std::ceil(m.plan_cost[j] / std::max(target, 1.0)), (double)nb);
+        pieces = std::max(1, std::min(pieces, 64));
+        for (int k = 0; k < pieces; k++) {
+            if ((int64_t)nb * (k + 1) / pieces <= (int64_t)nb * k / pieces) continue;
+            (J.degraded ? checked : fast).push_back({m.plan_cost[j] / pieces, make_int4(mi, j, k, pieces)});
+        }
+    }
+}
+
+static double plan_total(const ModelDev &m)
+{
+    double t = 0;
+    for (double c : m.plan_cost) t += c;
+    return t;
+}
+
+// exact-window entries then checked ones, each part largest first
+static void order_entries(EntryList &fast, EntryList &checked, int4 *out)
+{
+    auto by_cost = [](const std::pair<double, int4> &a, const std::pair<double, int4> &b) {
+        return a.first > b.first;
+    };
+    std::stable_sort(fast.begin(), fast.end(), by_cost);
+    std::stable_sort(checked.begin(), checked.end(), by_cost);
+    size_t n = 0;
+    for (auto &e : fast) out[n++] = e.second;
+    for (auto &e : checked) out[n++] = e.second;
+}
+
+// builds (or reuses) the model's device copy of the pass-1 plan for rstep; the jobs are kept
+// whole and cut into pieces per launch list (alone: about 4 pieces' worth per CU)
 static void ensure_plan(dbx_ctx *ctx, ModelDev &m, double rstep)
 {
     if (m.plan_rstep == rstep && m.plan_cu == ctx->cu_count && m.d_jobs) return;
     std::vector<P1Job> sorted;
     std::vector<float> edge2;
     std::vector<int32_t> alist;
-    plan_pass1(m, rstep, sorted, edge2, alist, ctx->cu_count);
+    std::vector<double> cost;
+    plan_pass1(m, rstep, sorted, edge2, alist, 0, &cost);
     int n_fast = 0;
     while (n_fast < (int)sorted.size() && !sorted[n_fast].degraded) n_fast++;
-    for (void *q : {(void *)m.d_jobs, (void *)m.d_edge2, (void *)m.d_retry, (void *)m.d_alist})
+    for (void *q : {(void *)m.d_jobs, (void *)m.d_edge2, (void *)m.d_list, (void *)m.d_alist})
         if (q) DBX_HIP(hipFree(q));
     m.d_jobs = nullptr;
     m.d_edge2 = nullptr;
-    m.d_retry = nullptr;
+    m.d_list = nullptr;
     m.d_alist = nullptr;
     m.plan_rstep = -1.0;
+    m.plan_h = sorted;
+    m.plan_cost = cost;
+    m.plan_jobs = (int)sorted.size();
+    m.plan_fast = n_fast;
+    EntryList fast, checked;
+    add_entries(m, 0, plan_total(m) / (double)(std::max(1, ctx->cu_count) * 4), fast, checked);
+    std::vector<int4> list(fast.size() + checked.size());
+    order_entries(fast, checked, list.data());
     DBX_HIP(hipMalloc(&m.d_jobs, sizeof(P1Job) * std::max<size_t>(1, sorted.size())));
     DBX_HIP(hipMalloc(&m.d_edge2, sizeof(float) * std::max<size_t>(1, edge2.size())));
-    DBX_HIP(hipMalloc(&m.d_retry, sizeof(int) * (size_t)(n_fast + 1)));
+    DBX_HIP(hipMalloc(&m.d_list, sizeof(int4) * std::max<size_t>(1, list.size())));
     DBX_HIP(hipMalloc(&m.d_alist, sizeof(int32_t) * std::max<size_t>(1, alist.size())));
-    if (!sorted.empty())
+    if (!sorted.empty()) {
         DBX_HIP(hipMemcpy(m.d_jobs, sorted.data(), sizeof(P1Job) * sorted.size(), hipMemcpyHostToDevice));
+        DBX_HIP(hipMemcpy(m.d_list, list.data(), sizeof(int4) * list.size(), hipMemcpyHostToDevice));
+    }
     if (!edge2.empty())
         DBX_HIP(hipMemcpy(m.d_edge2, edge2.data(), sizeof(float) * edge2.size(), hipMemcpyHostToDevice));
     if (!alist.empty())
         DBX_HIP(hipMemcpy(m.d_alist, alist.data(), sizeof(int32_t) * alist.size(), hipMemcpyHostToDevice));
-    m.plan_jobs = (int)sorted.size();
-    m.plan_fast = n_fast;
+    m.plan_list_n = (int)list.size();
+    m.plan_list_fast = (int)fast.size();
     m.plan_cu = ctx->cu_count;
     m.plan_rstep = rstep;
 }
 
-P1Stats pass1_histograms(dbx_ctx *ctx, ModelDev &m, double rstep, int64_t n_bins,
-                         uint64_t *d_hist, bool sync_stats)
+static P1Ref make_ref(const ModelDev &m, int64_t n_bins, uint64_t *d_hist, double rstep)
 {
-    DBX_REQUIRE(n_bins > 0 && n_bins < (int64_t)1 << 40, DBX_ERR_ARG, "bad n_bins");
-    DBX_REQUIRE(rstep > 0 && std::isfinite(rstep), DBX_ERR_ARG, "rstep must be positive");
-    const int S = m.n_groups;
-    const int P = S * (S + 1) / 2;
-    const float rp = (float)(1 / rstep);  // cuDebye-v15.cu:739
-    ensure_plan(ctx, m, rstep);
-    const int n_fast = m.plan_fast;
-    const int n_checked = m.plan_jobs - n_fast;
+    P1Ref r;
+    r.pos = m.pos;
+    r.tiles = m.tiles;
+    r.jobs = m.d_jobs;
+    r.alist = m.d_alist;
+    r.edge2 = m.d_edge2;
+    r.hist = (unsigned long long *)d_hist;
+    r.n_bins = n_bins;
+    r.rp = (float)(1 / rstep);  // cuDebye-v15.cu:739
+    r.pad = 0;
+    r.oor = nullptr;
+    return r;
+}
 
+P1Stats pass1_batch(dbx_ctx *ctx, const std::vector<P1Item> &items, double rstep, bool sync_stats)
+{
+    DBX_REQUIRE(rstep > 0 && std::isfinite(rstep), DBX_ERR_ARG, "rstep must be positive");
     P1Stats st;
-    st.n_jobs = m.plan_jobs;
+    std::vector<P1Ref> refs(items.size());
+    int n_fast = 0, n_checked = 0;
+    for (size_t i = 0; i < items.size(); i++) {
+        const P1Item &it = items[i];
+        DBX_REQUIRE(it.n_bins > 0 && it.n_bins < (int64_t)1 << 40, DBX_ERR_ARG, "bad n_bins");
+        ensure_plan(ctx, *it.m, rstep);
+        refs[i] = make_ref(*it.m, it.n_bins, it.d_hist, rstep);
+        refs[i].oor = it.d_oor ? (unsigned long long *)it.d_oor : (unsigned long long *)ctx->counters;
+        const int P = it.m->n_groups * (it.m->n_groups + 1) / 2;
+        DBX_HIP(hipMemsetAsync(it.d_hist, 0, sizeof(uint64_t) * (size_t)P * it.n_bins, ctx->stream));
+    }
+    // launch list: one model uses its own list; several get one merged list whose pieces are
+    // sized for the whole batch, exact-window entries then checked ones, largest first
+    const int4 *d_list = nullptr;
+    EntryList fast, checked;
+    if (items.size() == 1) {
+        d_list = items[0].m->d_list;
+        n_fast = items[0].m->plan_list_fast;
+        n_checked = items[0].m->plan_list_n - n_fast;
+    } else {
+        double total = 0;
+        for (const P1Item &it : items) total += plan_total(*it.m);
+        const double target = total / (double)(std::max(1, ctx->cu_count) * 4);
+        for (size_t i = 0; i < items.size(); i++) add_entries(*items[i].m, (int)i, target, fast, checked);
+        n_fast = (int)fast.size();
+        n_checked = (int)checked.size();
+    }
+    st.n_jobs = n_fast + n_checked;
     st.n_checked = n_checked;
-    DBX_HIP(hipMemsetAsync(d_hist, 0, sizeof(uint64_t) * (size_t)P * n_bins, ctx->stream));
+    const size_t ref_bytes = (sizeof(P1Ref) * refs.size() + 255) & ~(size_t)255;
+    const size_t list_bytes = items.size() == 1 ? 0 : (sizeof(int4) * st.n_jobs + 255) & ~(size_t)255;
+    const size_t retry_bytes = sizeof(int4) * (size_t)(n_fast + 1);
+    char *buf = (char *)scratch(ctx, kSlotBatch, ref_bytes + list_bytes + retry_bytes);
+    P1Ref *d_refs = (P1Ref *)buf;
+    int4 *d_retry = (int4 *)(buf + ref_bytes + list_bytes);
+    if (items.size() != 1) {
+        int4 *hl = (int4 *)pinned(ctx, kPinList, sizeof(int4) * std::max(1, st.n_jobs));
+        order_entries(fast, checked, hl);
+        d_list = (int4 *)(buf + ref_bytes);
+        DBX_HIP(hipMemcpyAsync((void *)d_list, hl, sizeof(int4) * st.n_jobs, hipMemcpyHostToDevice,
+                               ctx->stream));
+    }
+    P1Ref *hr = (P1Ref *)pinned(ctx, kPinRefs, sizeof(P1Ref) * refs.size());
+    std::memcpy(hr, refs.data(), sizeof(P1Ref) * refs.size());
+    DBX_HIP(hipMemcpyAsync(d_refs, hr, sizeof(P1Ref) * refs.size(), hipMemcpyHostToDevice, ctx->stream));
     DBX_HIP(hipMemsetAsync(ctx->counters, 0, 4 * sizeof(uint64_t), ctx->stream));
     ctx->p1_timed = false;
-    if (m.plan_jobs > 0) {
+    if (st.n_jobs > 0) {
         unsigned long long *cnt = (unsigned long long *)ctx->counters;
         DBX_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
         if (n_fast > 0)
             hipLaunchKernelGGL(p1_histogram_kernel<false>, dim3((unsigned)n_fast), dim3(kP1Threads), 0,
-                               ctx->stream, m.pos, m.tiles, m.d_jobs, m.d_alist, n_fast, n_checked,
-                               m.d_edge2, rp, (unsigned long long *)d_hist, n_bins, cnt, m.d_retry);
+                               ctx->stream, d_refs, d_list, n_fast, n_checked, cnt, d_retry);
         // checked variant: the degraded jobs plus whatever the fast launch queued
         hipLaunchKernelGGL(p1_histogram_kernel<true>, dim3((unsigned)(n_checked + n_fast)),
-                           dim3(kP1Threads), 0, ctx->stream, m.pos, m.tiles, m.d_jobs, m.d_alist, n_fast,
-                           n_checked, m.d_edge2, rp, (unsigned long long *)d_hist, n_bins, cnt,
-                           m.d_retry);
+                           dim3(kP1Threads), 0, ctx->stream, d_refs, d_list, n_fast, n_checked, cnt, d_retry);
         DBX_HIP(hipGetLastError());
         DBX_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
         ctx->p1_timed = true;
@@ -514,6 +628,12 @@ P1Stats pass1_histograms(dbx_ctx *ctx, ModelDev &m, double rstep, int64_t n_bins
         pass1_read_stats(ctx, st);
     }
     return st;
+}
+
+P1Stats pass1_histograms(dbx_ctx *ctx, ModelDev &m, double rstep, int64_t n_bins,
+                         uint64_t *d_hist, bool sync_stats)
+{
+    return pass1_batch(ctx, {P1Item{&m, n_bins, d_hist}}, rstep, sync_stats);
 }
 
 void pass1_read_stats(dbx_ctx *ctx, P1Stats &st)

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -28,18 +30,10 @@ namespace dbx {
 // ---------------------------------------------------------------------------------
 // prefactor, cuDebye-v15.cu:447-467 (FP64)
 // ---------------------------------------------------------------------------------
-struct GroupPair {
-    double occ_a, occ_b, beq_a, beq_b, ca, cb;
-    double aa[5], ba[5], ab[5], bb[5];
-};
-
-__global__ void p2_prefactor_kernel(GroupPair g, const double *__restrict__ q, int nq,
-                                    double *__restrict__ ff)
+__device__ double prefactor_one(const GroupPair &g, double Q)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nq) return;
     const double rp16pi2 = -0.006332573977646;  // -1/(16 pi^2), cuDebye-v15.cu:452
-    const double nh = q[i] * q[i] * rp16pi2;
+    const double nh = Q * Q * rp16pi2;
     double v = g.occ_a * g.occ_b;
     v = v * exp(nh * (g.beq_a + g.beq_b));
     double f1 = g.ca, f2 = g.cb;
@@ -47,11 +41,19 @@ __global__ void p2_prefactor_kernel(GroupPair g, const double *__restrict__ q, i
         f1 += g.aa[t] * exp(g.ba[t] * nh);
         f2 += g.ab[t] * exp(g.bb[t] * nh);
     }
-    ff[i] = v * f1 * f2;
+    return v * f1 * f2;
 }
 
-void prefactor_dev(dbx_ctx *ctx, const dbx_group_params &ga, const dbx_group_params &gb,
-                   const double *d_q, int nq, double *d_ff)
+// ff[r][q], grid.y over rows
+__global__ void p2_prefactor_kernel(const GroupPair *__restrict__ g, const double *__restrict__ q, int nq,
+                                    double *__restrict__ ff)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    ff[(int64_t)blockIdx.y * nq + i] = prefactor_one(g[blockIdx.y], q[i]);
+}
+
+GroupPair make_group_pair(const dbx_group_params &ga, const dbx_group_params &gb)
 {
     GroupPair g;
     g.occ_a = ga.occ; g.occ_b = gb.occ; g.beq_a = ga.beq; g.beq_b = gb.beq;
@@ -59,27 +61,56 @@ void prefactor_dev(dbx_ctx *ctx, const dbx_group_params &ga, const dbx_group_par
     for (int t = 0; t < 5; t++) {
         g.aa[t] = ga.a[t]; g.ba[t] = ga.b[t]; g.ab[t] = gb.a[t]; g.bb[t] = gb.b[t];
     }
-    hipLaunchKernelGGL(p2_prefactor_kernel, dim3((nq + 255) / 256), dim3(256), 0, ctx->stream, g,
-                       d_q, nq, d_ff);
+    return g;
+}
+
+void prefactor_rows(dbx_ctx *ctx, const std::vector<GroupPair> &rows, const double *d_q, int nq, double *d_ff)
+{
+    const int n = (int)rows.size();
+    if (n == 0) return;
+    GroupPair *hg = (GroupPair *)pinned(ctx, kPinPairs, sizeof(GroupPair) * n);
+    std::copy(rows.begin(), rows.end(), hg);
+    GroupPair *dg = (GroupPair *)scratch(ctx, kSlotPairs, sizeof(GroupPair) * n);
+    DBX_HIP(hipMemcpyAsync(dg, hg, sizeof(GroupPair) * n, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(p2_prefactor_kernel, dim3((nq + 255) / 256, n), dim3(256), 0, ctx->stream, dg, d_q, nq,
+                       d_ff);
     DBX_HIP(hipGetLastError());
 }
 
+void prefactor_dev(dbx_ctx *ctx, const dbx_group_params &ga, const dbx_group_params &gb,
+                   const double *d_q, int nq, double *d_ff)
+{
+    prefactor_rows(ctx, std::vector<GroupPair>{make_group_pair(ga, gb)}, d_q, nq, d_ff);
+}
+
 // ---------------------------------------------------------------------------------
-// compaction: union of nonzero bins of P partials, ascending
+// Rows.  Pass 2 works on a list of rows, each a partial histogram of its own length at
+// hist_base + h_off; the rows of one model are its P subgroup pairs, the rows of a batch are
+// the rows of all its models.  Rows are ordered by n_bins, longest first, so the rows that
+// hold bin k are a prefix of the list and a block of 16 rows needs the bins below the
+// n_bins of its first row only.
+//
+// Compaction: union of the nonzero bins of all rows, ascending (for one row, the list
+// thrust::copy_if keeps, cuDebye-v15.cu:315-329).
 // ---------------------------------------------------------------------------------
 constexpr int kScanBlock = 1024;
 
-__global__ void p2_count_kernel(const unsigned long long *__restrict__ hist, int P, int64_t n_bins,
-                                int k_self_max, int *__restrict__ block_counts,
+__device__ __forceinline__ bool row_holds(const RowDesc *rows, int r, int64_t k)
+{
+    return k < rows[r].n_bins;
+}
+
+__global__ void p2_count_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
+                                int n_rows, int64_t nb_max, int k_self_max, int *__restrict__ block_counts,
                                 unsigned long long *__restrict__ max_count)
 {
     __shared__ int wsum[kScanBlock / 64];
     const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
     bool nz = false;
     unsigned long long mx = 0;
-    if (k < n_bins && k > k_self_max)
-        for (int p = 0; p < P; p++) {
-            const unsigned long long v = hist[(int64_t)p * n_bins + k];
+    if (k < nb_max && k > k_self_max)
+        for (int r = 0; r < n_rows && row_holds(rows, r, k); r++) {
+            const unsigned long long v = hist[rows[r].h_off + k];
             nz |= v != 0ull;
             mx = v > mx ? v : mx;
         }
@@ -98,29 +129,44 @@ __global__ void p2_count_kernel(const unsigned long long *__restrict__ hist, int
     }
 }
 
-// exclusive scan of the block counts (one block; n_blocks small)
+// exclusive scan of the block counts, one workgroup
 __global__ void p2_scan_kernel(int *__restrict__ block_counts, int n_blocks, int *__restrict__ total)
 {
-    if (threadIdx.x == 0) {
-        int s = 0;
-        for (int i = 0; i < n_blocks; i++) {
-            int c = block_counts[i];
-            block_counts[i] = s;
-            s += c;
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n_blocks + 1023) / 1024;
+    const int b0 = min(n_blocks, t * per), b1 = min(n_blocks, b0 + per);
+    int s = 0;
+    for (int i = b0; i < b1; i++) s += block_counts[i];
+    part[t] = s;
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int i = 0; i < 1024; i++) {
+            const int c = part[i];
+            part[i] = acc;
+            acc += c;
         }
-        *total = s;
+        *total = acc;
+    }
+    __syncthreads();
+    int acc = part[t];
+    for (int i = b0; i < b1; i++) {
+        const int c = block_counts[i];
+        block_counts[i] = acc;
+        acc += c;
     }
 }
 
-__global__ void p2_scatter_kernel(const unsigned long long *__restrict__ hist, int P, int64_t n_bins,
-                                  int k_self_max, const int *__restrict__ block_offsets,
-                                  int *__restrict__ kz)
+__global__ void p2_scatter_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
+                                  int n_rows, int64_t nb_max, int k_self_max,
+                                  const int *__restrict__ block_offsets, int *__restrict__ kz)
 {
     __shared__ int woff[kScanBlock / 64];
     const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
     bool nz = false;
-    if (k < n_bins && k > k_self_max)
-        for (int p = 0; p < P; p++) nz |= hist[(int64_t)p * n_bins + k] != 0ull;
+    if (k < nb_max && k > k_self_max)
+        for (int r = 0; r < n_rows && row_holds(rows, r, k); r++) nz |= hist[rows[r].h_off + k] != 0ull;
     const uint64_t b = __ballot(nz);
     const int lane = threadIdx.x & 63;
     if (lane == 0) woff[threadIdx.x >> 6] = __popcll(b);
@@ -140,53 +186,79 @@ __global__ void p2_scatter_kernel(const unsigned long long *__restrict__ hist, i
     }
 }
 
-// weights of the compacted bins, FP64: A[p][i] = 2 N_p,kz[i]
-__global__ void p2_weights_kernel(const unsigned long long *__restrict__ hist, int P, int64_t n_bins,
-                                  const int *__restrict__ kz, int Kc, double *__restrict__ A)
+// kext[blk] = number of compacted bins below the n_bins of the block's first row
+__global__ void p2_kext_kernel(const int *__restrict__ kz, int Kc, const RowDesc *__restrict__ rows, int n_rows,
+                               int rows_per_block, int n_blocks, int *__restrict__ kext)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= n_blocks) return;
+    const int r = b * rows_per_block;
+    if (r >= n_rows) {
+        kext[b] = 0;
+        return;
+    }
+    const int64_t nb = rows[r].n_bins;
+    int lo = 0, hi = Kc;  // first index with kz >= nb
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (kz[mid] < nb)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    kext[b] = lo;
+}
+
+// weights of the compacted bins, FP64: A[r][i] = 2 N_r,kz[i]
+__global__ void p2_weights_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
+                                  int r0, int nr, const int *__restrict__ kz, int Kc, double *__restrict__ A)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Kc) return;
     const int64_t k = kz[i];
-    for (int p = 0; p < P; p++) A[(int64_t)p * Kc + i] = 2.0 * (double)hist[(int64_t)p * n_bins + k];
+    for (int t = 0; t < nr; t++) {
+        const RowDesc rd = rows[r0 + t];
+        A[(int64_t)t * Kc + i] = k < rd.n_bins ? 2.0 * (double)hist[rd.h_off + k] : 0.0;
+    }
 }
 
 // weight-1 bins (cuDebye-v15.cu:508-514) and the weight-2 bins at r <= 0.0005 that the
-// index condition excludes; one thread per partial.  self[p] = sum of weight-1 counts,
-// low[p * (k_self_max+1) + k] = 2 N for the other low bins (added to S with sinc(q r_k)).
-__global__ void p2_self_kernel(const unsigned long long *__restrict__ hist, int P, int64_t n_bins,
-                               int k_self_max, int first_n, double *__restrict__ self,
-                               double *__restrict__ low)
+// index condition excludes; one thread per row.  self[r] = sum of weight-1 counts,
+// low[r * (k_self_max+1) + k] = 2 N for the other low bins (added to S with sinc(q r_k)).
+__global__ void p2_self_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
+                               int n_rows, int k_self_max, double *__restrict__ self, double *__restrict__ low)
 {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    const RowDesc rd = rows[r];
     double s = 0;
     int rank = 0;
     for (int k = 0; k <= k_self_max; k++) {
-        const unsigned long long n = k < n_bins ? hist[(int64_t)p * n_bins + k] : 0ull;
+        const unsigned long long n = k < rd.n_bins ? hist[rd.h_off + k] : 0ull;
         double lw = 0;
         if (n) {
-            if (rank < first_n)
+            if (rank < rd.first_n)
                 s += (double)n;
             else
                 lw = 2.0 * (double)n;
             rank++;
         }
-        low[p * (k_self_max + 1) + k] = lw;
+        low[(int64_t)r * (k_self_max + 1) + k] = lw;
     }
-    self[p] = s;
+    self[r] = s;
 }
 
 // ---------------------------------------------------------------------------------
 // Debye sum, FP64: one thread per q, bins split over blockIdx.y and staged through LDS,
-// one sinc per (bin, q) shared by all partials; split partials reduced in fixed order.
+// one sinc per (bin, q) shared by all rows of the pass; split partials reduced in fixed order.
 // ---------------------------------------------------------------------------------
 constexpr int kDsThreads = 256;
-constexpr int kDsPmax = 10;  // partials per pass (S <= 4 subgroups in one pass)
+constexpr int kDsPmax = 10;  // rows per pass
 constexpr int kDsChunk = 256;
 
 __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
-    const int *__restrict__ kz, int Kc, int k_per_split, const double *__restrict__ A, int p0,
-    int np, double rstep, const double *__restrict__ q, int nq, double *__restrict__ Spart)
+    const int *__restrict__ kz, int Kc, int k_per_split, const double *__restrict__ A, int np, double rstep,
+    const double *__restrict__ q, int nq, double *__restrict__ Spart)
 {
     __shared__ double sr[kDsChunk];
     __shared__ double sa[kDsPmax][kDsChunk];
@@ -202,7 +274,7 @@ __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += kDsThreads) {
             sr[i] = (double)kz[c0 + i] * rstep;
-            for (int t = 0; t < np; t++) sa[t][i] = A[(int64_t)(p0 + t) * Kc + c0 + i];
+            for (int t = 0; t < np; t++) sa[t][i] = A[(int64_t)t * Kc + c0 + i];
         }
         __syncthreads();
         for (int i = 0; i < n; i++) {
@@ -221,68 +293,81 @@ __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
     }
 }
 
-__global__ void p2_reduce_splits_kernel(const double *__restrict__ Spart, int nsplit, int p0, int np,
-                                        int nq, double *__restrict__ S)
+// S[r0 + i] = sum over splits of Spart[s][i], i over nr rows x nq, fixed order
+__global__ void p2_reduce_splits_kernel(const double *__restrict__ Spart, int nsplit, int r0, int nr, int nq,
+                                        double *__restrict__ S)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np * nq) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t n = (int64_t)nr * nq;
+    if (i >= n) return;
     double acc = 0.0;
-    for (int s = 0; s < nsplit; s++) acc += Spart[(int64_t)s * np * nq + i];
-    S[(int64_t)p0 * nq + i] = acc;
+    for (int s = 0; s < nsplit; s++) acc += Spart[(int64_t)s * n + i];
+    S[(int64_t)r0 * nq + i] = acc;
 }
 
 // ---------------------------------------------------------------------------------
 // Debye sum on the matrix cores: S[rows x q] = A[rows x bins] . T[bins x q],
 // A = 2 N (integer), T = sinc(q r).  Split-BF16 limbs, exact FP32 accumulation:
-//   A = sum_i e_i 256^i,              e_i in [0, 255]           (count digits, I of them)
+//   A = sum_i e_i 256^i,   e_i in [-128, 128]   (balanced count digits, I of them: 2N < 2^(8I-1))
 //   T ~ V / 2^40,  V = rint(T 2^40) = sum_j d_j 256^j,  d_j in [-128, 128], d_4 in [-56, 256]
-// Every digit is an integer that BF16 holds exactly and every product e_i d_j is an integer
-// below 2^16 in magnitude, so an FP32 accumulator that collects the products of one scale
-// class s = i + j (at most 5 digit pairs) over 32 bins stays below 2^24 and is exact,
-// whatever order the matrix core adds in.  After every 32 bins the classes are folded into
-// FP64: S += sum_s 2^(8s - 40) acc_s.  The only approximation is the rounding of T to 2^-40
-// (|error| <= 2^-41 per bin, from an FP64 sin with full argument reduction).
+// Every digit is an integer that BF16 holds exactly.  A scale class s = i + j holds at most
+// min(I, 5) digit pairs, at most one of them with j = 4, so it gains at most 2^14 (min(I,5) + 1)
+// per bin; an FP32 accumulator that collects one class over 32 F(I) bins, F(I) = 31 / (I + 1),
+// stays below 2^24 and is exact, whatever order the matrix core adds in.  Every F(I) steps of
+// 32 bins the classes are folded into FP64: S += sum_s 2^(8s - 40) acc_s.  The only
+// approximation is the rounding of T to 2^-40 (|error| <= 2^-41 per bin, from an FP64 sin with
+// full argument reduction).
 // ---------------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kMfWaves = 4;   // waves per workgroup (one 16-value Q block each)
-constexpr int kMfImax = 5;    // count digits (counts below 2^40)
-constexpr int kMfJ = 5;       // sinc digits
-constexpr int kMfClasses = kMfImax + kMfJ - 1;
+constexpr int kMfImax = 5;  // count digits (2N below 2^39)
+constexpr int kMfJ = 5;     // sinc digits
 
-// count digits: cdig[i][row][kk] = byte i of 2 N_row,kz[kk] (0 past Kc or past P)
-__global__ void p2_count_digits_kernel(const unsigned long long *__restrict__ hist, int P,
-                                       int64_t n_bins, const int *__restrict__ kz, int Kc, int Kc_pad,
-                                       int rows_pad, int I, __bf16 *__restrict__ cdig)
+// balanced count digits: cdig[i][row][kk] = digit i of 2 N_row,kz[kk] (0 past Kc, past the row's
+// bins or past n_rows)
+__global__ void p2_count_digits_kernel(const unsigned long long *__restrict__ hist,
+                                       const RowDesc *__restrict__ rows, int n_rows, const int *__restrict__ kz,
+                                       int Kc, int Kc_pad, int rows_pad, int I, __bf16 *__restrict__ cdig)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (int64_t)rows_pad * Kc_pad) return;
     const int row = (int)(t / Kc_pad);
     const int kk = (int)(t - (int64_t)row * Kc_pad);
-    unsigned long long c = 0;
-    if (row < P && kk < Kc) c = 2ull * hist[(int64_t)row * n_bins + kz[kk]];
-    for (int i = 0; i < I; i++)
-        cdig[((int64_t)i * rows_pad + row) * Kc_pad + kk] = (__bf16)(float)((c >> (8 * i)) & 0xFFull);
+    long long v = 0;
+    if (row < n_rows && kk < Kc) {
+        const RowDesc rd = rows[row];
+        const int64_t k = kz[kk];
+        if (k < rd.n_bins) v = (long long)(2ull * hist[rd.h_off + k]);
+    }
+    for (int i = 0; i < I; i++) {
+        long long e = v;
+        if (i + 1 < I) {
+            e = ((v + 128) & 255) - 128;
+            v = (v - e) >> 8;
+        }
+        cdig[((int64_t)i * rows_pad + row) * Kc_pad + kk] = (__bf16)(float)e;
+    }
 }
 
-// T rounded to a multiple of 2^-40, as 5 signed base-256 digits (exact in bf16)
-__device__ __forceinline__ void fixed_digits(double T, float d[kMfJ])
+// The sinc digits: V = rint(2^40 sin(Q r) / (Q r)) as five balanced base-256 digits d_0..d_4
+// (d_j in [-128, 127] for j < 4, d_4 in [-56, 256]), each an integer that bf16 holds exactly.
+// iQ40 = 2^40 / Q.  V is formed in the low bits of an FP64 sum with 1.5 2^52 (one rounding,
+// to nearest even); W = V + 128 (1 + 256 + ... + 256^4) has the unsigned digits d_j + 128.
+__device__ __forceinline__ void sinc_digits(double Q, double iQ40, double r, double ir, float d[kMfJ])
 {
-    double r = rint(T * 1099511627776.0);  // 2^40
-    const double t4 = rint(r * 0x1p-32);
-    r = fma(-t4, 4294967296.0, r);
-    const double t3 = rint(r * 0x1p-24);
-    r = fma(-t3, 16777216.0, r);
-    const double t2 = rint(r * 0x1p-16);
-    r = fma(-t2, 65536.0, r);
-    const double t1 = rint(r * 0x1p-8);
-    const double t0 = fma(-t1, 256.0, r);
-    d[0] = (float)t0;
-    d[1] = (float)t1;
-    d[2] = (float)t2;
-    d[3] = (float)t3;
-    d[4] = (float)t4;
+    const double v = Q != 0.0 ? fma(sin_cw(Q * r) * iQ40, ir, 0x1.8p52) : 0x1.8p52 + 0x1p40;
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const unsigned lo = (unsigned)u;
+    const int hi = (int)((unsigned)(u >> 32) & 0xFFFFFu) - 0x80000;
+    const unsigned w = lo + 0x80808080u;
+    d[0] = (float)(w & 0xFFu) - 128.f;
+    d[1] = (float)((w >> 8) & 0xFFu) - 128.f;
+    d[2] = (float)((w >> 16) & 0xFFu) - 128.f;
+    d[3] = (float)(w >> 24) - 128.f;
+    d[4] = (float)(hi + (w < lo ? 1 : 0));
 }
 
 // bin distances and their reciprocals: rr[kk] = kz[kk] * rstep, ir[kk] = 1 / rr[kk]
@@ -297,113 +382,261 @@ __global__ void p2_rtab_kernel(const int *__restrict__ kz, int Kc, int Kc_pad, d
     ir[i] = 1.0 / r;
 }
 
-// One wave: 16 q values x RB blocks of 16 rows over this split's bins.  Operand layout of
+// One workgroup: WQ x WR waves.  Wave (wq, wr) owns 16 q values (block wq of the workgroup's
+// 16 WQ) and RB blocks of 16 rows (block wr of its WR RB 16).  Operand layout of
 // v_mfma_f32_16x16x32_bf16 (lane l): A[row l%16][k 8(l/16)..+7], B[k 8(l/16)..+7][col l%16],
-// C[row 4(l/16)+v][col l%16].  Each lane generates the 8 sinc values of its B fragment.
-template <int RB>
-__global__ __launch_bounds__(kMfWaves * 64) void p2_debye_mfma_kernel(
+// C[row 4(l/16)+v][col l%16].
+//   - Count digits: the I x WR x RB fragments (1 KB each) of a 32-bin step are loaded by the
+//     workgroup's waves together, two steps ahead, into a ring of three LDS slots (lane l of a
+//     fragment holds exactly the 16 bytes MFMA lane l reads); one barrier per step.
+//   - Sinc digits: the 16 q x 32 bin values of a step are generated by the WR waves of a q
+//     block together (8 / WR values per thread), through LDS (double-buffered) when WR > 1.
+//     The step's r and 1/r come through the same LDS ring.
+// The workgroup's bins end at kext of its first row block: rows are ordered by length,
+// longest first.
+template <int RB, int WQ, int WR, int I>
+__global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     const double *__restrict__ rr, const double *__restrict__ ir, int Kc_pad, int k_per_split,
-    const __bf16 *__restrict__ cdig, int I, int rows_pad, const double *__restrict__ q, int nq,
-    int nrows, double *__restrict__ Spart)
+    const int *__restrict__ kext, const __bf16 *__restrict__ cdig, int rows_pad, const double *__restrict__ q,
+    int nq, int nrows, double *__restrict__ Spart)
 {
+    constexpr int NW = WQ * WR;
+    constexpr int NF = I * WR * RB;
+    constexpr int LF = (NF + NW - 1) / NW;
+    constexpr int C = I + kMfJ - 1;
+    constexpr int V = 8 / WR;
+    constexpr int F = 31 / (I + 1);
+    __shared__ bf16x8 sa[3][NF][64];
+    __shared__ double srt[3][64];
+    __shared__ bf16x8 sd[WR > 1 ? 2 : 1][WR > 1 ? WQ : 1][kMfJ][64];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int q0 = (blockIdx.x * kMfWaves + wave) * 16;
-    if (q0 >= nq) return;
+    const int wq = wave % WQ, wr = wave / WQ;
+    const int q0 = blockIdx.x * (16 * WQ) + wq * 16;
+    const int rowt = blockIdx.z * (WR * RB * 16);
     const int col = lane & 15, kq = lane >> 4;
-    const double Q = q[min(q0 + col, nq - 1)];
-    const double iQ = Q != 0.0 ? 1.0 / Q : 0.0;
+    // generator role: thread g of the q block's WR waves -> fragment lane gl, values gt..gt+V-1
+    const int g = wr * 64 + lane;
+    const int gl = g / WR, gt = (g % WR) * V;
+    const double Q = q[min(q0 + (gl & 15), nq - 1)];
+    const double iQ40 = Q != 0.0 ? 0x1p40 / Q : 0.0;
+    const int gk = (gl >> 4) * 8 + gt;
+    const int kx = (kext[blockIdx.z * (WR * RB)] + 31) & ~31;
     const int kb = blockIdx.y * k_per_split;
-    const int ke = min(Kc_pad, kb + k_per_split);
-    f32x4 acc[RB][kMfClasses];
+    const int ke = min(min(Kc_pad, kx), kb + k_per_split);
+    const int nsteps = ke > kb ? (ke - kb) >> 5 : 0;
+    f32x4 acc[RB][C];
     double acc64[RB][4];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
-        for (int c = 0; c < kMfClasses; c++) acc[rb][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < C; c++) acc[rb][c] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int v = 0; v < 4; v++) acc64[rb][v] = 0.0;
     }
-    for (int k0 = kb; k0 < ke; k0 += 32) {
-        const int kk0 = k0 + kq * 8;
-        bf16x8 B[kMfJ];
-#pragma unroll
-        for (int t = 0; t < 8; t++) {
-            // sinc(Q r) = sin(Q r) (1/Q)(1/r): FP64 reduction of the argument, no division
-            const double T = Q != 0.0 ? sin_cw(Q * rr[kk0 + t]) * iQ * ir[kk0 + t] : 1.0;
-            float d[kMfJ];
-            fixed_digits(T, d);
-#pragma unroll
-            for (int j = 0; j < kMfJ; j++) B[j][t] = (__bf16)d[j];
-        }
+    auto fold = [&]() {
 #pragma unroll
         for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
-            for (int i = 0; i < kMfImax; i++) {
-                if (i < I) {
-                    const bf16x8 A = *reinterpret_cast<const bf16x8 *>(
-                        cdig + ((int64_t)i * rows_pad + rb * 16 + col) * Kc_pad + kk0);
+            for (int c = 0; c < C; c++) {
+                const double scale = __builtin_ldexp(1.0, 8 * c - 40);
 #pragma unroll
-                    for (int j = 0; j < kMfJ; j++)
-                        acc[rb][i + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[j], acc[rb][i + j], 0, 0, 0);
-                }
+                for (int v = 0; v < 4; v++) acc64[rb][v] += scale * (double)acc[rb][c][v];
+                acc[rb][c] = f32x4{0.f, 0.f, 0.f, 0.f};
             }
         }
-        // fold the exact FP32 class sums into FP64 (every 32 bins: class sums stay < 2^24)
+    };
+    // count-digit fragment f of the step at k0: digit f / (WR RB), row block f % (WR RB)
+    bf16x8 pf[LF];
+    double prt = 0.0;
+    auto load = [&](int k0) {
 #pragma unroll
-        for (int rb = 0; rb < RB; rb++) {
-#pragma unroll
-            for (int c = 0; c < kMfClasses; c++) {
-                if (c < I + kMfJ - 1) {
-                    const double scale = __builtin_ldexp(1.0, 8 * c - 40);
-#pragma unroll
-                    for (int v = 0; v < 4; v++) acc64[rb][v] += scale * (double)acc[rb][c][v];
-                    acc[rb][c] = f32x4{0.f, 0.f, 0.f, 0.f};
-                }
+        for (int m = 0; m < LF; m++) {
+            const int f = wave + m * NW;
+            if (f < NF) {
+                const int i = f / (WR * RB), blk = f - i * (WR * RB);
+                pf[m] = *reinterpret_cast<const bf16x8 *>(
+                    cdig + ((int64_t)i * rows_pad + rowt + blk * 16 + col) * Kc_pad + k0 + kq * 8);
             }
         }
+        if (wave == 0) prt = lane < 32 ? rr[k0 + lane] : ir[k0 + lane - 32];
+    };
+    auto store = [&](int slot) {
+#pragma unroll
+        for (int m = 0; m < LF; m++) {
+            const int f = wave + m * NW;
+            if (f < NF) sa[slot][f][lane] = pf[m];
+        }
+        if (wave == 0) srt[slot][lane] = prt;
+    };
+    if (nsteps > 0) {
+        load(kb);
+        store(0);
+        if (nsteps > 1) load(kb + 32);
     }
+    __syncthreads();
+    int slot = 0, nfold = 0;
+    for (int s = 0; s < nsteps; s++) {
+        bf16x8 B[kMfJ];
+        if constexpr (WR == 1) {
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                float d[kMfJ];
+                sinc_digits(Q, iQ40, srt[slot][gk + t], srt[slot][32 + gk + t], d);
+#pragma unroll
+                for (int j = 0; j < kMfJ; j++) B[j][t] = (__bf16)d[j];
+            }
+        } else {
+            __bf16 vals[kMfJ][V];
+#pragma unroll
+            for (int t = 0; t < V; t++) {
+                float d[kMfJ];
+                sinc_digits(Q, iQ40, srt[slot][gk + t], srt[slot][32 + gk + t], d);
+#pragma unroll
+                for (int j = 0; j < kMfJ; j++) vals[j][t] = (__bf16)d[j];
+            }
+#pragma unroll
+            for (int j = 0; j < kMfJ; j++) {
+                __bf16 *dst = reinterpret_cast<__bf16 *>(&sd[s & 1][wq][j][gl]) + gt;
+                if constexpr (V == 4) {
+                    bf16x4 w;
+#pragma unroll
+                    for (int t = 0; t < 4; t++) w[t] = vals[j][t];
+                    *reinterpret_cast<bf16x4 *>(dst) = w;
+                } else {
+                    bf16x2 w;
+#pragma unroll
+                    for (int t = 0; t < 2; t++) w[t] = vals[j][t];
+                    *reinterpret_cast<bf16x2 *>(dst) = w;
+                }
+            }
+        }
+        const int nslot = slot == 2 ? 0 : slot + 1;
+        if (s + 1 < nsteps) store(nslot);
+        if (s + 2 < nsteps) load(kb + (s + 2) * 32);
+        __syncthreads();
+        if constexpr (WR > 1) {
+#pragma unroll
+            for (int j = 0; j < kMfJ; j++) B[j] = sd[s & 1][wq][j][lane];
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+#pragma unroll
+            for (int i = 0; i < I; i++) {
+                const bf16x8 A = sa[slot][i * (WR * RB) + wr * RB + rb][lane];
+#pragma unroll
+                for (int j = 0; j < kMfJ; j++)
+                    acc[rb][i + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[j], acc[rb][i + j], 0, 0, 0);
+            }
+        }
+        if (++nfold == F) {
+            fold();
+            nfold = 0;
+        }
+        slot = nslot;
+    }
+    fold();
     const int qq = q0 + col;
     if (qq < nq) {
 #pragma unroll
         for (int rb = 0; rb < RB; rb++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int row = rb * 16 + kq * 4 + v;
+                const int row = rowt + (wr * RB + rb) * 16 + kq * 4 + v;
                 if (row < nrows) Spart[((int64_t)blockIdx.y * nrows + row) * nq + qq] = acc64[rb][v];
             }
     }
 }
 
-// S[p][q] += self[p] + sum_k low[p][k] sinc(q r_k)
-__global__ void p2_add_low_kernel(const double *__restrict__ self, const double *__restrict__ low,
-                                  int nlow, int P, double rstep, const double *__restrict__ q, int nq,
-                                  double *__restrict__ S)
+// launch shapes of the matrix-core Debye sum: (RB, WQ, WR) by row count
+struct MfShape {
+    int RB, WQ, WR;
+    int rows() const { return RB * WR * 16; }
+    int qs() const { return WQ * 16; }
+    int threads() const { return WQ * WR * 64; }
+};
+
+static MfShape mf_shape(int n_rows, int I)
 {
-    const int iq = blockIdx.x * blockDim.x + threadIdx.x;
-    if (iq >= nq) return;
-    for (int p = 0; p < P; p++) {
-        double s = self[p];
-        for (int k = 0; k < nlow; k++) {
-            const double w = low[p * nlow + k];
-            if (w != 0.0) {
-                const double r = (double)k * rstep;
-                const double x = q[iq] * r;
-                s += r == 0.0 ? w : w * sin(x) / x;
-            }
-        }
-        S[(int64_t)p * nq + iq] += s;
+    if (const char *e = getenv("DBX_TUNE_MF")) {  // TEMPORARY tuning knob
+        int a = 0, b = 0, c = 0;
+        if (sscanf(e, "%d,%d,%d", &a, &b, &c) == 3 && n_rows > 32 && !(c == 2 && I >= 5)) return {a, b, c};
     }
+    if (n_rows <= 16) return {1, 4, 1};
+    if (n_rows <= 32) return {2, 4, 1};
+    if (n_rows <= 64 || I >= 5) return {4, 4, 1};  // (4, 4, 2) with 5 digits overfills LDS
+    return {4, 4, 2};
 }
 
-__global__ void p2_combine_kernel(const double *__restrict__ ff, const double *__restrict__ S, int P,
-                                  int nq, double *__restrict__ I)
+template <int RB, int WQ, int WR>
+static void mf_launch_i(int I, dim3 grid, hipStream_t st, const double *rr, const double *ir, int Kc_pad,
+                        int kper, const int *kext, const __bf16 *cdig, int rows_pad, const double *q, int nq,
+                        int nrows, double *Spart)
+{
+    const dim3 blk(WQ * WR * 64);
+#define DBX_MF(II)                                                                                              \
+    hipLaunchKernelGGL((p2_debye_mfma_kernel<RB, WQ, WR, II>), grid, blk, 0, st, rr, ir, Kc_pad, kper, kext, \
+                       cdig, rows_pad, q, nq, nrows, Spart)
+    switch (I) {
+    case 1: DBX_MF(1); break;
+    case 2: DBX_MF(2); break;
+    case 3: DBX_MF(3); break;
+    case 4: DBX_MF(4); break;
+    default:
+        if constexpr (WR == 1)
+            DBX_MF(5);
+        else
+            throw Error(DBX_ERR_ARG, "five count digits need a one-row-wave shape");
+        break;
+    }
+#undef DBX_MF
+}
+
+static void mf_launch(const MfShape &sh, int I, dim3 grid, hipStream_t st, const double *rr, const double *ir,
+                      int Kc_pad, int kper, const int *kext, const __bf16 *cdig, int rows_pad, const double *q,
+                      int nq, int nrows, double *Spart)
+{
+    if (sh.RB == 1)
+        mf_launch_i<1, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, Spart);
+    else if (sh.RB == 2)
+        mf_launch_i<2, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, Spart);
+    else if (sh.WR == 1)
+        mf_launch_i<4, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, Spart);
+    else
+        mf_launch_i<4, 4, 2>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, Spart);
+}
+
+// S[r][q] += self[r] + sum_k low[r][k] sinc(q r_k); grid.y over rows
+__global__ void p2_add_low_kernel(const double *__restrict__ self, const double *__restrict__ low, int nlow,
+                                  double rstep, const double *__restrict__ q, int nq, double *__restrict__ S)
 {
     const int iq = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
     if (iq >= nq) return;
+    double s = self[r];
+    for (int k = 0; k < nlow; k++) {
+        const double w = low[(int64_t)r * nlow + k];
+        if (w != 0.0) {
+            const double rk = (double)k * rstep;
+            const double x = q[iq] * rk;
+            s += rk == 0.0 ? w : w * sin(x) / x;
+        }
+    }
+    S[(int64_t)r * nq + iq] += s;
+}
+
+// I[dest][q] = sum over the model's rows of ff S, FP64, rows in order;
+// mrows[m] = (first row, row count, dest, -)
+__global__ void p2_combine_kernel(const double *__restrict__ ff, const double *__restrict__ S,
+                                  const int4 *__restrict__ mrows, int nq, double *__restrict__ I)
+{
+    const int iq = blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.y;
+    if (iq >= nq) return;
+    const int4 rr = mrows[m];
     double acc = 0.0;
-    for (int p = 0; p < P; p++) acc += S[(int64_t)p * nq + iq] * ff[(int64_t)p * nq + iq];
-    I[iq] = acc;
+    for (int r = rr.x; r < rr.x + rr.y; r++) acc += S[(int64_t)r * nq + iq] * ff[(int64_t)r * nq + iq];
+    I[(int64_t)rr.z * nq + iq] = acc;
 }
 
 // largest k with double(k)*rstep <= 0.0005 (the reference's test r[k] <= 0.0005)
@@ -415,112 +648,142 @@ static int self_bin_max(double rstep)
     return k;
 }
 
-void debye_structure(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
-                     int first_n, const double *d_q, int nq, double *d_S)
+void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::vector<RowDesc> &rows,
+                          double rstep, const double *d_q, int nq, double *d_S)
 {
-    DBX_REQUIRE(P > 0 && nq > 0 && n_bins > 0, DBX_ERR_ARG, "empty pass-2 problem");
+    const int n_rows = (int)rows.size();
+    DBX_REQUIRE(n_rows > 0 && nq > 0, DBX_ERR_ARG, "empty pass-2 problem");
+    for (int r = 0; r < n_rows; r++) {
+        DBX_REQUIRE(rows[r].n_bins > 0, DBX_ERR_ARG, "pass-2 row without bins");
+        DBX_REQUIRE(r == 0 || rows[r].n_bins <= rows[r - 1].n_bins, DBX_ERR_ARG,
+                    "pass-2 rows must be ordered by length, longest first");
+    }
+    DBX_REQUIRE(rows[0].n_bins < (int64_t)INT32_MAX, DBX_ERR_ARG, "too many bins");
+    const int64_t nb_max = rows[0].n_bins;
     const int ksm = self_bin_max(rstep);
     const int nlow = ksm + 1;
-    const int64_t nblk = (n_bins + kScanBlock - 1) / kScanBlock;
-    // scratch: block counts, total + max count, kz, self, low, weights / digits
+    const int64_t nblk = (nb_max + kScanBlock - 1) / kScanBlock;
+    // scratch: rows, block counts, total + max count, kz, self, low, kext
+    const int n_rb16 = (n_rows + 15) / 16;
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off = (off + b + 255) & ~(size_t)255; return o; };
+    const size_t o_rows = take(sizeof(RowDesc) * n_rows);
     const size_t o_bc = take(sizeof(int) * (nblk + 1));
     const size_t o_tot = take(sizeof(unsigned long long) * 2);
-    const size_t o_kz = take(sizeof(int) * n_bins);
-    const size_t o_self = take(sizeof(double) * P);
-    const size_t o_low = take(sizeof(double) * P * nlow);
+    const size_t o_kz = take(sizeof(int) * nb_max);
+    const size_t o_self = take(sizeof(double) * n_rows);
+    const size_t o_low = take(sizeof(double) * (size_t)n_rows * nlow);
+    const size_t o_kext = take(sizeof(int) * (n_rb16 + 8));
     char *base = (char *)scratch(ctx, kSlotPass2, off);
+    RowDesc *d_rows = (RowDesc *)(base + o_rows);
     int *bc = (int *)(base + o_bc);
     unsigned long long *tot = (unsigned long long *)(base + o_tot);  // [0] Kc, [1] max count
     int *kz = (int *)(base + o_kz);
     double *self = (double *)(base + o_self);
     double *low = (double *)(base + o_low);
-    const unsigned long long *h = (const unsigned long long *)d_hist;
+    int *kext = (int *)(base + o_kext);
+    const unsigned long long *h = (const unsigned long long *)d_hist_base;
 
+    RowDesc *hrows = (RowDesc *)pinned(ctx, kPinRows, sizeof(RowDesc) * n_rows);
+    std::copy(rows.begin(), rows.end(), hrows);
     DBX_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    DBX_HIP(hipMemcpyAsync(d_rows, hrows, sizeof(RowDesc) * n_rows, hipMemcpyHostToDevice, ctx->stream));
     DBX_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, ctx->stream));
-    hipLaunchKernelGGL(p2_count_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, h, P,
-                       n_bins, ksm, bc, tot + 1);
-    hipLaunchKernelGGL(p2_scan_kernel, dim3(1), dim3(64), 0, ctx->stream, bc, (int)nblk, (int *)tot);
-    hipLaunchKernelGGL(p2_scatter_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, h,
-                       P, n_bins, ksm, bc, kz);
-    hipLaunchKernelGGL(p2_self_kernel, dim3((P + 63) / 64), dim3(64), 0, ctx->stream, h, P, n_bins,
-                       ksm, first_n, self, low);
-    unsigned long long hv[2];
-    DBX_HIP(hipMemcpyAsync(hv, tot, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream));
+    hipLaunchKernelGGL(p2_count_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, h, d_rows,
+                       n_rows, nb_max, ksm, bc, tot + 1);
+    hipLaunchKernelGGL(p2_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, bc, (int)nblk, (int *)tot);
+    hipLaunchKernelGGL(p2_scatter_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, h, d_rows,
+                       n_rows, nb_max, ksm, bc, kz);
+    hipLaunchKernelGGL(p2_self_kernel, dim3((n_rows + 63) / 64), dim3(64), 0, ctx->stream, h, d_rows, n_rows,
+                       ksm, self, low);
+    unsigned long long *hv = (unsigned long long *)pinned(ctx, kPinMisc, sizeof(unsigned long long) * 2);
+    DBX_HIP(hipMemcpyAsync(hv, tot, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, ctx->stream));
     DBX_HIP(hipStreamSynchronize(ctx->stream));
     const int Kc = (int)(hv[0] & 0xFFFFFFFFull);
     const unsigned long long max_count = hv[1];
     ctx->last_p2_bins = Kc;
     if (Kc > 0 && ctx->pass2_mode == DBX_PASS2_FP64) {
-        double *A = (double *)scratch(ctx, kSlotDigits, sizeof(double) * (size_t)P * Kc);
-        hipLaunchKernelGGL(p2_weights_kernel, dim3((Kc + 255) / 256), dim3(256), 0, ctx->stream, h, P,
-                           n_bins, kz, Kc, A);
         const int qblocks = (nq + kDsThreads - 1) / kDsThreads;
         int nsplit = std::max(1, std::min((2048 + qblocks - 1) / qblocks, (Kc + 511) / 512));
         const int kper = (Kc + nsplit - 1) / nsplit;
         nsplit = (Kc + kper - 1) / kper;
-        double *Spart = (double *)scratch(ctx, kSlotPass2b, sizeof(double) * (size_t)nsplit *
-                                                                std::min(P, kDsPmax) * nq);
-        for (int p0 = 0; p0 < P; p0 += kDsPmax) {
-            const int np = std::min(kDsPmax, P - p0);
-            hipLaunchKernelGGL(p2_debye_fp64_kernel, dim3(qblocks, nsplit), dim3(kDsThreads), 0,
-                               ctx->stream, kz, Kc, kper, A, p0, np, rstep, d_q, nq, Spart);
-            hipLaunchKernelGGL(p2_reduce_splits_kernel, dim3((np * nq + 255) / 256), dim3(256), 0,
-                               ctx->stream, Spart, nsplit, p0, np, nq, d_S);
+        const int npass = std::min(n_rows, kDsPmax);
+        double *A = (double *)scratch(ctx, kSlotDigits, sizeof(double) * (size_t)npass * Kc);
+        double *Spart = (double *)scratch(ctx, kSlotPass2b, sizeof(double) * (size_t)nsplit * npass * nq);
+        for (int r0 = 0; r0 < n_rows; r0 += kDsPmax) {
+            const int np = std::min(kDsPmax, n_rows - r0);
+            hipLaunchKernelGGL(p2_weights_kernel, dim3((Kc + 255) / 256), dim3(256), 0, ctx->stream, h, d_rows,
+                               r0, np, kz, Kc, A);
+            hipLaunchKernelGGL(p2_debye_fp64_kernel, dim3(qblocks, nsplit), dim3(kDsThreads), 0, ctx->stream,
+                               kz, Kc, kper, A, np, rstep, d_q, nq, Spart);
+            hipLaunchKernelGGL(p2_reduce_splits_kernel, dim3((unsigned)(((int64_t)np * nq + 255) / 256)),
+                               dim3(256), 0, ctx->stream, Spart, nsplit, r0, np, nq, d_S);
         }
     } else if (Kc > 0) {
-        // count digits: 2 N < 2^(8 I)
+        // balanced count digits: 2 N < 2^(8 I - 1)
         int I = 1;
-        while (I < kMfImax && (2.0 * (double)max_count) >= std::ldexp(1.0, 8 * I)) I++;
-        DBX_REQUIRE(2.0 * (double)max_count < std::ldexp(1.0, 8 * kMfImax), DBX_ERR_RANGE,
+        while (I < kMfImax && (2.0 * (double)max_count) >= std::ldexp(1.0, 8 * I - 1)) I++;
+        DBX_REQUIRE(2.0 * (double)max_count < std::ldexp(1.0, 8 * kMfImax - 1), DBX_ERR_RANGE,
                     "bin count too large for the split-BF16 Debye sum");
         const int Kc_pad = (Kc + 31) & ~31;
-        const int qgroups = (nq + 16 * kMfWaves - 1) / (16 * kMfWaves);
         const int ksteps = Kc_pad / 32;
-        int nsplit = std::max(1, std::min((4096 + qgroups - 1) / qgroups, ksteps));
-        const int kper = ((ksteps + nsplit - 1) / nsplit) * 32;
-        nsplit = (Kc_pad + kper - 1) / kper;
         double *rtab = (double *)scratch(ctx, kSlotRtab, sizeof(double) * 2 * (size_t)Kc_pad);
         hipLaunchKernelGGL(p2_rtab_kernel, dim3((Kc_pad + 255) / 256), dim3(256), 0, ctx->stream, kz, Kc,
                            Kc_pad, rstep, rtab, rtab + Kc_pad);
-        for (int p0 = 0; p0 < P; p0 += 32) {
-            const int np = std::min(32, P - p0);
-            const int RB = (np + 15) / 16;
-            const int rows_pad = RB * 16;
-            __bf16 *cdig = (__bf16 *)scratch(ctx, kSlotDigits, sizeof(__bf16) * (size_t)I * rows_pad * Kc_pad);
-            double *Spart = (double *)scratch(ctx, kSlotPass2b, sizeof(double) * (size_t)nsplit * np * nq);
-            const int64_t nd = (int64_t)rows_pad * Kc_pad;
-            hipLaunchKernelGGL(p2_count_digits_kernel, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0,
-                               ctx->stream, h + (int64_t)p0 * n_bins, np, n_bins, kz, Kc, Kc_pad, rows_pad, I,
-                               cdig);
-            const dim3 grid((unsigned)qgroups, (unsigned)nsplit);
-            if (RB == 1)
-                hipLaunchKernelGGL(p2_debye_mfma_kernel<1>, grid, dim3(kMfWaves * 64), 0, ctx->stream, rtab,
-                                   rtab + Kc_pad, Kc_pad, kper, cdig, I, rows_pad, d_q, nq, np, Spart);
-            else
-                hipLaunchKernelGGL(p2_debye_mfma_kernel<2>, grid, dim3(kMfWaves * 64), 0, ctx->stream, rtab,
-                                   rtab + Kc_pad, Kc_pad, kper, cdig, I, rows_pad, d_q, nq, np, Spart);
-            hipLaunchKernelGGL(p2_reduce_splits_kernel, dim3((np * nq + 255) / 256), dim3(256), 0,
-                               ctx->stream, Spart, nsplit, p0, np, nq, d_S);
-        }
+        const MfShape sh = mf_shape(n_rows, I);
+        const int rows_pad = (n_rows + sh.rows() - 1) / sh.rows() * sh.rows();
+        const int qgroups = (nq + sh.qs() - 1) / sh.qs();
+        const int rgroups = rows_pad / sh.rows();
+        const int target = 2048 * 4 / (sh.WQ * sh.WR);  // about 8 waves per SIMD over the grid
+        int nsplit = std::max(1, std::min((target + qgroups * rgroups - 1) / (qgroups * rgroups), ksteps));
+        const int kper = ((ksteps + nsplit - 1) / nsplit) * 32;
+        nsplit = (Kc_pad + kper - 1) / kper;
+        __bf16 *cdig = (__bf16 *)scratch(ctx, kSlotDigits, sizeof(__bf16) * (size_t)I * rows_pad * Kc_pad);
+        double *Spart = (double *)scratch(ctx, kSlotPass2b, sizeof(double) * (size_t)nsplit * n_rows * nq);
+        const int64_t nd = (int64_t)rows_pad * Kc_pad;
+        hipLaunchKernelGGL(p2_count_digits_kernel, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, ctx->stream,
+                           h, d_rows, n_rows, kz, Kc, Kc_pad, rows_pad, I, cdig);
+        const int nblocks = rows_pad / 16;
+        hipLaunchKernelGGL(p2_kext_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, kz, Kc, d_rows,
+                           n_rows, 16, nblocks, kext);
+        mf_launch(sh, I, dim3((unsigned)qgroups, (unsigned)nsplit, (unsigned)rgroups), ctx->stream, rtab,
+                  rtab + Kc_pad, Kc_pad, kper, kext, cdig, rows_pad, d_q, nq, n_rows, Spart);
+        hipLaunchKernelGGL(p2_reduce_splits_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
+                           dim3(256), 0, ctx->stream, Spart, nsplit, 0, n_rows, nq, d_S);
     } else {
-        DBX_HIP(hipMemsetAsync(d_S, 0, sizeof(double) * (size_t)P * nq, ctx->stream));
+        DBX_HIP(hipMemsetAsync(d_S, 0, sizeof(double) * (size_t)n_rows * nq, ctx->stream));
     }
-    hipLaunchKernelGGL(p2_add_low_kernel, dim3((nq + 255) / 256), dim3(256), 0, ctx->stream, self, low,
-                       nlow, P, rstep, d_q, nq, d_S);
+    hipLaunchKernelGGL(p2_add_low_kernel, dim3((nq + 255) / 256, n_rows), dim3(256), 0, ctx->stream, self, low,
+                       nlow, rstep, d_q, nq, d_S);
     DBX_HIP(hipGetLastError());
     DBX_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     ctx->p2_timed = true;
 }
 
-void combine_intensity(dbx_ctx *ctx, const double *d_ff, const double *d_S, int P, int nq,
-                       double *d_I)
+void debye_structure(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
+                     int first_n, const double *d_q, int nq, double *d_S)
 {
-    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256), dim3(256), 0, ctx->stream, d_ff,
-                       d_S, P, nq, d_I);
+    std::vector<RowDesc> rows(P);
+    for (int p = 0; p < P; p++) rows[p] = RowDesc{(int64_t)p * n_bins, n_bins, first_n, 0};
+    debye_structure_rows(ctx, d_hist, rows, rstep, d_q, nq, d_S);
+}
+
+void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const std::vector<int4> &mrows,
+                    int nq, double *d_I)
+{
+    const int nm = (int)mrows.size();
+    int4 *hm = (int4 *)pinned(ctx, kPinModels, sizeof(int4) * nm);
+    std::copy(mrows.begin(), mrows.end(), hm);
+    int4 *dm = (int4 *)scratch(ctx, kSlotRows, sizeof(int4) * nm);
+    DBX_HIP(hipMemcpyAsync(dm, hm, sizeof(int4) * nm, hipMemcpyHostToDevice, ctx->stream));
+    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, ctx->stream, d_ff, d_S, dm,
+                       nq, d_I);
     DBX_HIP(hipGetLastError());
+}
+
+void combine_intensity(dbx_ctx *ctx, const double *d_ff, const double *d_S, int P, int nq, double *d_I)
+{
+    combine_models(ctx, d_ff, d_S, std::vector<int4>{make_int4(0, P, 0, 0)}, nq, d_I);
 }
 
 }  // namespace dbx

@@ -132,6 +132,26 @@ int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params,
                   double rstep, int64_t n_bins, int first_n, const double *q, int nq,
                   double *I_out, uint64_t *hist_out);
 
+/* ---- batch of models ------------------------------------------------------------------
+ * Many models end to end in one pass-1 launch (a merged work list over all models) and one
+ * pass-2 contraction over the rows of all models, sharing the sinc table.  No reference
+ * counterpart in-process: it replaces running `cuDebye.exe` once per model
+ * (Matlab/multisim.m:42-58).  Every item's intensity equals dbx_intensity on that item alone
+ * (histograms bit-exact; I within the pass-2 mode's error bound).
+ * items[n_items]: model, its params[n_groups], n_bins and first_n (all items share rstep and
+ * the q grid).  I_out[n_items][nq].  n_out_of_range[n_items] (optional) receives each
+ * item's count of pairs at or beyond its last bin; DBX_ERR_RANGE when any is nonzero.
+ * hist_out (optional): hist_out[i], when not NULL, receives item i's [P_i][n_bins_i]. */
+typedef struct dbx_batch_item {
+    dbx_model *model;
+    const dbx_group_params *params;
+    int64_t n_bins;
+    int first_n;
+} dbx_batch_item;
+int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, double rstep,
+                        const double *q, int nq, double *I_out, uint64_t *n_out_of_range,
+                        uint64_t *const *hist_out);
+
 /* ---- .deb driver ------------------------------------------------------------------------
  * The reference program: `cuDebye.exe file.deb` (cuDebye-v15.cu:599-1003).  Parses the
  * .deb, runs every subgroup pair, writes <path>I (hist 0 or 2) and <path>x<r>x<c>

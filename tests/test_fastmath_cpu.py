@@ -1,7 +1,8 @@
-"""The FP64 sine of the pass-2 sinc table (Cody-Waite reduction by pi/2 + Taylor polynomials,
+"""The FP64 sine of the pass-2 sinc table (Cody-Waite reduction by pi + a Taylor polynomial,
 cudebye_amd/csrc/fastmath.h), host build, against a 50-digit mpmath reference: absolute error
 at the level of a few ulp over the argument range of the Debye sum (q r up to ~1e4) and next
-to multiples of pi/2 where the reduction cancels."""
+to multiples of pi/2 (where the reduction cancels, and where the polynomial's argument is
+largest)."""
 import ctypes as C
 
 import mpmath

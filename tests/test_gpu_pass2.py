@@ -12,7 +12,7 @@ pytestmark = pytest.mark.gpu
 
 # FP64 mode: only summation order and the device sin differ from the oracle's compensated
 # FP64 sum.  MFMA mode (split-BF16 digits, exact FP32 accumulation): the sinc table is
-# rounded to 2^-32, held to the north-star bound of 1e-6 relative on every I(Q).
+# rounded to a multiple of 2^-40, held to 1e-6 relative on every I(Q).
 TOL = {cd.DBX_PASS2_FP64: 1e-9, cd.DBX_PASS2_MFMA: 1e-6}
 MODES = [cd.DBX_PASS2_MFMA, cd.DBX_PASS2_FP64]
 
