@@ -17,8 +17,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstdio>
-#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -100,16 +98,22 @@ __device__ __forceinline__ bool row_holds(const RowDesc *rows, int r, int64_t k)
     return k < rows[r].n_bins;
 }
 
-__global__ void p2_count_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
-                                int n_rows, int64_t nb_max, int k_self_max, int *__restrict__ block_counts,
+constexpr int kFlagRows = 32;  // rows per thread of the nonzero-flag pass
+
+// flags[k] = 1 where some row holds a nonzero count at bin k > k_self_max; grid.y over chunks
+// of kFlagRows rows (a bin written by several chunks gets the same value from each).  Also the
+// largest count of any row (atomicMax).
+__global__ void p2_flags_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
+                                int n_rows, int64_t nb_max, int k_self_max, unsigned char *__restrict__ flags,
                                 unsigned long long *__restrict__ max_count)
 {
-    __shared__ int wsum[kScanBlock / 64];
     const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    const int r0 = blockIdx.y * kFlagRows;
+    const int r1 = min(n_rows, r0 + kFlagRows);
     bool nz = false;
     unsigned long long mx = 0;
     if (k < nb_max && k > k_self_max)
-        for (int r = 0; r < n_rows && row_holds(rows, r, k); r++) {
+        for (int r = r0; r < r1 && row_holds(rows, r, k); r++) {
             const unsigned long long v = hist[rows[r].h_off + k];
             nz |= v != 0ull;
             mx = v > mx ? v : mx;
@@ -119,6 +123,20 @@ __global__ void p2_count_kernel(const unsigned long long *__restrict__ hist, con
         mx = o > mx ? o : mx;
     }
     if ((threadIdx.x & 63) == 0 && mx) atomicMax(max_count, mx);
+    if (nz) flags[k] = 1;
+}
+
+__global__ void p2_zero_kernel(unsigned int *__restrict__ p, int64_t n_words)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_words) p[i] = 0u;
+}
+
+__global__ void p2_count_kernel(const unsigned char *__restrict__ flags, int64_t nb_max, int *__restrict__ block_counts)
+{
+    __shared__ int wsum[kScanBlock / 64];
+    const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    const bool nz = k < nb_max && flags[k];
     const uint64_t b = __ballot(nz);
     if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = __popcll(b);
     __syncthreads();
@@ -158,15 +176,12 @@ __global__ void p2_scan_kernel(int *__restrict__ block_counts, int n_blocks, int
     }
 }
 
-__global__ void p2_scatter_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
-                                  int n_rows, int64_t nb_max, int k_self_max,
+__global__ void p2_scatter_kernel(const unsigned char *__restrict__ flags, int64_t nb_max,
                                   const int *__restrict__ block_offsets, int *__restrict__ kz)
 {
     __shared__ int woff[kScanBlock / 64];
     const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
-    bool nz = false;
-    if (k < nb_max && k > k_self_max)
-        for (int r = 0; r < n_rows && row_holds(rows, r, k); r++) nz |= hist[rows[r].h_off + k] != 0ull;
+    const bool nz = k < nb_max && flags[k];
     const uint64_t b = __ballot(nz);
     const int lane = threadIdx.x & 63;
     if (lane == 0) woff[threadIdx.x >> 6] = __popcll(b);
@@ -354,11 +369,12 @@ __global__ void p2_count_digits_kernel(const unsigned long long *__restrict__ hi
 
 // The sinc digits: V = rint(2^40 sin(Q r) / (Q r)) as five balanced base-256 digits d_0..d_4
 // (d_j in [-128, 127] for j < 4, d_4 in [-56, 256]), each an integer that bf16 holds exactly.
-// iQ40 = 2^40 / Q.  V is formed in the low bits of an FP64 sum with 1.5 2^52 (one rounding,
-// to nearest even); W = V + 128 (1 + 256 + ... + 256^4) has the unsigned digits d_j + 128.
+// iQ40 = 2^40 / Q; Q = 0 is passed as 1e-30 (sinc rounds to exactly 1 there), so there is no
+// branch.  V is formed in the low bits of an FP64 sum with 1.5 2^52 (one rounding, to nearest
+// even); W = V + 128 (1 + 256 + ... + 256^4) has the unsigned digits d_j + 128.
 __device__ __forceinline__ void sinc_digits(double Q, double iQ40, double r, double ir, float d[kMfJ])
 {
-    const double v = Q != 0.0 ? fma(sin_cw(Q * r) * iQ40, ir, 0x1.8p52) : 0x1.8p52 + 0x1p40;
+    const double v = fma(sin_cw(Q * r) * iQ40, ir, 0x1.8p52);
     const unsigned long long u = (unsigned long long)__double_as_longlong(v);
     const unsigned lo = (unsigned)u;
     const int hi = (int)((unsigned)(u >> 32) & 0xFFFFFu) - 0x80000;
@@ -387,11 +403,12 @@ __global__ void p2_rtab_kernel(const int *__restrict__ kz, int Kc, int Kc_pad, d
 // v_mfma_f32_16x16x32_bf16 (lane l): A[row l%16][k 8(l/16)..+7], B[k 8(l/16)..+7][col l%16],
 // C[row 4(l/16)+v][col l%16].
 //   - Count digits: the I x WR x RB fragments (1 KB each) of a 32-bin step are loaded by the
-//     workgroup's waves together, two steps ahead, into a ring of three LDS slots (lane l of a
+//     workgroup's waves together, ahead of use, into a ring of three LDS slots (lane l of a
 //     fragment holds exactly the 16 bytes MFMA lane l reads); one barrier per step.
 //   - Sinc digits: the 16 q x 32 bin values of a step are generated by the WR waves of a q
-//     block together (8 / WR values per thread), through LDS (double-buffered) when WR > 1.
-//     The step's r and 1/r come through the same LDS ring.
+//     block together (8 / WR values per thread), through LDS (double-buffered) when WR > 1,
+//     one step ahead of the matrix cores so that both run at once.  The step's r and 1/r come
+//     through the same LDS ring.
 // The workgroup's bins end at kext of its first row block: rows are ordered by length,
 // longest first.
 template <int RB, int WQ, int WR, int I>
@@ -418,8 +435,9 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     // generator role: thread g of the q block's WR waves -> fragment lane gl, values gt..gt+V-1
     const int g = wr * 64 + lane;
     const int gl = g / WR, gt = (g % WR) * V;
-    const double Q = q[min(q0 + (gl & 15), nq - 1)];
-    const double iQ40 = Q != 0.0 ? 0x1p40 / Q : 0.0;
+    const double Q0 = q[min(q0 + (gl & 15), nq - 1)];
+    const double Q = Q0 != 0.0 ? Q0 : 1e-30;
+    const double iQ40 = 0x1p40 / Q;
     const int gk = (gl >> 4) * 8 + gt;
     const int kx = (kext[blockIdx.z * (WR * RB)] + 31) & ~31;
     const int kb = blockIdx.y * k_per_split;
@@ -446,45 +464,37 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
             }
         }
     };
-    // count-digit fragment f of the step at k0: digit f / (WR RB), row block f % (WR RB)
+    // count-digit fragment f of the step at k0: digit f / (WR RB), row block f % (WR RB).  Wave w
+    // moves fragments w, w + NW, ... (clamped to the last one: a duplicate writes the same
+    // bytes), and every wave moves the step's r table (same bytes again); no branches, so the
+    // scheduler can interleave all of it with the matrix instructions.
     bf16x8 pf[LF];
-    double prt = 0.0;
-    auto load = [&](int k0) {
+    double prt;
+    auto load_a = [&](int k0) {
 #pragma unroll
         for (int m = 0; m < LF; m++) {
-            const int f = wave + m * NW;
-            if (f < NF) {
-                const int i = f / (WR * RB), blk = f - i * (WR * RB);
-                pf[m] = *reinterpret_cast<const bf16x8 *>(
-                    cdig + ((int64_t)i * rows_pad + rowt + blk * 16 + col) * Kc_pad + k0 + kq * 8);
-            }
+            const int f = min(wave + m * NW, NF - 1);
+            const int i = f / (WR * RB), blk = f - i * (WR * RB);
+            pf[m] = *reinterpret_cast<const bf16x8 *>(
+                cdig + ((int64_t)i * rows_pad + rowt + blk * 16 + col) * Kc_pad + k0 + kq * 8);
         }
-        if (wave == 0) prt = lane < 32 ? rr[k0 + lane] : ir[k0 + lane - 32];
     };
-    auto store = [&](int slot) {
+    auto store_a = [&](int slot) {
 #pragma unroll
-        for (int m = 0; m < LF; m++) {
-            const int f = wave + m * NW;
-            if (f < NF) sa[slot][f][lane] = pf[m];
-        }
-        if (wave == 0) srt[slot][lane] = prt;
+        for (int m = 0; m < LF; m++) sa[slot][min(wave + m * NW, NF - 1)][lane] = pf[m];
     };
-    if (nsteps > 0) {
-        load(kb);
-        store(0);
-        if (nsteps > 1) load(kb + 32);
-    }
-    __syncthreads();
-    int slot = 0, nfold = 0;
-    for (int s = 0; s < nsteps; s++) {
-        bf16x8 B[kMfJ];
+    auto load_r = [&](int k0) { prt = (lane < 32 ? rr : ir - 32)[k0 + lane]; };
+    auto store_r = [&](int slot) { srt[slot][lane] = prt; };
+    // sinc digits of the step whose r table is in srt[slot]: B fragments (WR == 1) or sd[buf]
+    bf16x8 Bn[kMfJ];
+    auto gen = [&](int slot, int buf) {
         if constexpr (WR == 1) {
 #pragma unroll
             for (int t = 0; t < 8; t++) {
                 float d[kMfJ];
                 sinc_digits(Q, iQ40, srt[slot][gk + t], srt[slot][32 + gk + t], d);
 #pragma unroll
-                for (int j = 0; j < kMfJ; j++) B[j][t] = (__bf16)d[j];
+                for (int j = 0; j < kMfJ; j++) Bn[j][t] = (__bf16)d[j];
             }
         } else {
             __bf16 vals[kMfJ][V];
@@ -497,7 +507,7 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
             }
 #pragma unroll
             for (int j = 0; j < kMfJ; j++) {
-                __bf16 *dst = reinterpret_cast<__bf16 *>(&sd[s & 1][wq][j][gl]) + gt;
+                __bf16 *dst = reinterpret_cast<__bf16 *>(&sd[buf][wq][j][gl]) + gt;
                 if constexpr (V == 4) {
                     bf16x4 w;
 #pragma unroll
@@ -511,11 +521,38 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
                 }
             }
         }
-        const int nslot = slot == 2 ? 0 : slot + 1;
-        if (s + 1 < nsteps) store(nslot);
-        if (s + 2 < nsteps) load(kb + (s + 2) * 32);
-        __syncthreads();
-        if constexpr (WR > 1) {
+    };
+    // Pipeline: at step s the matrix cores take A(s) (count digits, LDS ring slot s%3) and the
+    // sinc digits of step s, while the vector units generate the sinc digits of step s+1 from
+    // the r table of step s+1 (ring slot (s+1)%3).  A(s+1) and the r table of step s+2 are
+    // written to the ring during step s; A(s+2) and the r table of step s+3 are fetched into
+    // registers.  A slot is rewritten two barriers after its last read.
+    if (nsteps > 0) {
+        load_a(kb);
+        load_r(kb);
+        store_a(0);
+        store_r(0);
+        if (nsteps > 1) {
+            load_r(kb + 32);
+            store_r(1);
+        }
+    }
+    __syncthreads();
+    if (nsteps > 0) {
+        gen(0, 0);
+        if (nsteps > 1) load_a(kb + 32);
+        if (nsteps > 2) load_r(kb + 64);
+    }
+    __syncthreads();
+    int slot = 0, nfold = 0;
+    for (int s = 0; s < nsteps; s++) {
+        const int s1 = slot == 2 ? 0 : slot + 1;
+        const int s2 = s1 == 2 ? 0 : s1 + 1;
+        bf16x8 B[kMfJ];
+        if constexpr (WR == 1) {
+#pragma unroll
+            for (int j = 0; j < kMfJ; j++) B[j] = Bn[j];
+        } else {
 #pragma unroll
             for (int j = 0; j < kMfJ; j++) B[j] = sd[s & 1][wq][j][lane];
         }
@@ -529,11 +566,25 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
                     acc[rb][i + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[j], acc[rb][i + j], 0, 0, 0);
             }
         }
+        // Branch-free: past the last step the generator and the ring take stale or clamped
+        // (in-bounds) data that no later step reads.
+        gen(s1, (s + 1) & 1);
+        store_a(s1);
+        store_r(s2);
+        load_a(min(kb + (s + 2) * 32, ke - 32));
+        load_r(min(kb + (s + 3) * 32, ke - 32));
+        // interleave: one matrix instruction, then three vector instructions of the generator
+#pragma unroll
+        for (int x = 0; x < RB * I * kMfJ; x++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
+        }
         if (++nfold == F) {
             fold();
             nfold = 0;
         }
-        slot = nslot;
+        __syncthreads();
+        slot = s1;
     }
     fold();
     const int qq = q0 + col;
@@ -558,10 +609,6 @@ struct MfShape {
 
 static MfShape mf_shape(int n_rows, int I)
 {
-    if (const char *e = getenv("DBX_TUNE_MF")) {  // TEMPORARY tuning knob
-        int a = 0, b = 0, c = 0;
-        if (sscanf(e, "%d,%d,%d", &a, &b, &c) == 3 && n_rows > 32 && !(c == 2 && I >= 5)) return {a, b, c};
-    }
     if (n_rows <= 16) return {1, 4, 1};
     if (n_rows <= 32) return {2, 4, 1};
     if (n_rows <= 64 || I >= 5) return {4, 4, 1};  // (4, 4, 2) with 5 digits overfills LDS
@@ -674,6 +721,7 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     const size_t o_self = take(sizeof(double) * n_rows);
     const size_t o_low = take(sizeof(double) * (size_t)n_rows * nlow);
     const size_t o_kext = take(sizeof(int) * (n_rb16 + 8));
+    const size_t o_flags = take((size_t)(nblk * kScanBlock));
     char *base = (char *)scratch(ctx, kSlotPass2, off);
     RowDesc *d_rows = (RowDesc *)(base + o_rows);
     int *bc = (int *)(base + o_bc);
@@ -682,18 +730,23 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     double *self = (double *)(base + o_self);
     double *low = (double *)(base + o_low);
     int *kext = (int *)(base + o_kext);
+    unsigned char *flags = (unsigned char *)(base + o_flags);
     const unsigned long long *h = (const unsigned long long *)d_hist_base;
 
     RowDesc *hrows = (RowDesc *)pinned(ctx, kPinRows, sizeof(RowDesc) * n_rows);
     std::copy(rows.begin(), rows.end(), hrows);
     DBX_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     DBX_HIP(hipMemcpyAsync(d_rows, hrows, sizeof(RowDesc) * n_rows, hipMemcpyHostToDevice, ctx->stream));
-    DBX_HIP(hipMemsetAsync(tot, 0, sizeof(unsigned long long) * 2, ctx->stream));
-    hipLaunchKernelGGL(p2_count_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, h, d_rows,
-                       n_rows, nb_max, ksm, bc, tot + 1);
+    hipLaunchKernelGGL(p2_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, (unsigned int *)tot, (int64_t)4);
+    const int64_t fwords = nblk * kScanBlock / 4;
+    hipLaunchKernelGGL(p2_zero_kernel, dim3((unsigned)((fwords + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (unsigned int *)flags, fwords);
+    hipLaunchKernelGGL(p2_flags_kernel, dim3((unsigned)nblk, (unsigned)((n_rows + kFlagRows - 1) / kFlagRows)),
+                       dim3(kScanBlock), 0, ctx->stream, h, d_rows, n_rows, nb_max, ksm, flags, tot + 1);
+    hipLaunchKernelGGL(p2_count_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, flags, nb_max, bc);
     hipLaunchKernelGGL(p2_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, bc, (int)nblk, (int *)tot);
-    hipLaunchKernelGGL(p2_scatter_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, h, d_rows,
-                       n_rows, nb_max, ksm, bc, kz);
+    hipLaunchKernelGGL(p2_scatter_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, flags, nb_max,
+                       bc, kz);
     hipLaunchKernelGGL(p2_self_kernel, dim3((n_rows + 63) / 64), dim3(64), 0, ctx->stream, h, d_rows, n_rows,
                        ksm, self, low);
     unsigned long long *hv = (unsigned long long *)pinned(ctx, kPinMisc, sizeof(unsigned long long) * 2);
