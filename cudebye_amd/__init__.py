@@ -47,6 +47,33 @@ class BatchItem(C.Structure):
                 ("first_n", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    """dbx_server_config."""
+    _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", C.c_int), ("max_step_pairs", C.c_double),
+                ("prep_threads", C.c_int), ("pass2_mode", C.c_int)]
+
+
+class RequestDesc(C.Structure):
+    """dbx_request."""
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("group_n", C.c_void_p),
+                ("n_groups", C.c_int), ("params", C.c_void_p), ("rstep", C.c_double),
+                ("n_bins", C.c_int64), ("first_n", C.c_int), ("q", C.c_void_p), ("nq", C.c_int),
+                ("flags", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    """dbx_server_stats."""
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("steps", C.c_uint64), ("pairs", C.c_double), ("resident_bytes", C.c_uint64),
+                ("budget_bytes", C.c_uint64), ("max_step_requests", C.c_double)]
+
+
+DBX_REQ_KEEP = 1
+DBX_REQ_QUEUED, DBX_REQ_RUNNING, DBX_REQ_DONE, DBX_REQ_FAILED = 0, 1, 2, 3
+DBX_ERR_BUDGET = -7
+DBX_ERR_STATE = -8
+
+
 class GroupParams(C.Structure):
     """dbx_group_params: occ, Beq, a1..a5, b1..b5, c of one subgroup."""
     _fields_ = [("occ", C.c_double), ("beq", C.c_double), ("a", C.c_double * 5),
@@ -95,6 +122,16 @@ def lib():
             "dbx_set_pass2_mode": (i32, [vp, i32]),
             "dbx_selftest_sin": (i32, [vp, i32, vp]),
             "dbx_batch_intensity": (i32, [vp, vp, i32, f64, vp, i32, vp, vp, vp]),
+            "dbx_server_open": (i32, [i32, vp, P(vp)]),
+            "dbx_server_close": (i32, [vp]),
+            "dbx_server_last_error": (C.c_char_p, [vp]),
+            "dbx_submit": (i32, [vp, vp, P(u64)]),
+            "dbx_poll": (i32, [vp, u64, P(i32)]),
+            "dbx_wait": (i32, [vp, u64, vp, P(f64)]),
+            "dbx_reweight": (i32, [vp, u64, vp, vp]),
+            "dbx_request_histogram": (i32, [vp, u64, vp]),
+            "dbx_release": (i32, [vp, u64]),
+            "dbx_server_stats_get": (i32, [vp, vp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -109,7 +146,9 @@ EXPORTS = ("dbx_open", "dbx_close", "dbx_last_error", "dbx_stream", "dbx_q_from_
            "dbx_histogram", "dbx_prefactor", "dbx_debye", "dbx_intensity", "dbx_run_deb",
            "dbx_selftest_sqrt", "dbx_selftest_bins", "dbx_last_pass1_stats",
            "dbx_last_kernel_times", "dbx_set_pass2_mode", "dbx_selftest_sin",
-           "dbx_batch_intensity")
+           "dbx_batch_intensity", "dbx_server_open", "dbx_server_close", "dbx_server_last_error",
+           "dbx_submit", "dbx_poll", "dbx_wait", "dbx_reweight", "dbx_request_histogram",
+           "dbx_release", "dbx_server_stats_get")
 
 
 def _p(a):
@@ -307,3 +346,92 @@ class Context:
         if return_oor:
             out += ([int(v) for v in oor],)
         return out[0] if len(out) == 1 else out
+
+
+class Server:
+    """Continuous-batching server (dbx_server_open): submit requests from any thread, wait for
+    their intensities, re-weight resident ones with new form factors."""
+
+    def __init__(self, device=0, hbm_budget=0, max_batch=0, max_step_pairs=0.0, prep_threads=0,
+                 pass2_mode=DBX_PASS2_MFMA):
+        cfg = ServerConfig(hbm_budget, max_batch, max_step_pairs, prep_threads, pass2_mode)
+        h = C.c_void_p()
+        rc = lib().dbx_server_open(device, C.byref(cfg), C.byref(h))
+        if rc != DBX_OK:
+            raise DebyeError(rc, "cannot open server")
+        self.handle = h
+        self._nq = {}
+
+    def _err(self, rc, what):
+        raise DebyeError(rc, f"{what}: {lib().dbx_server_last_error(self.handle).decode()}")
+
+    def close(self):
+        if self.handle:
+            lib().dbx_server_close(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def submit(self, x, y, z, group_n, params, rstep, n_bins, q, first_n=16, keep=False):
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        z = np.ascontiguousarray(z, np.float32)
+        gn = np.ascontiguousarray(group_n, np.int64)
+        q = np.ascontiguousarray(q, np.float64)
+        pa = (GroupParams * len(gn))(*params)
+        req = RequestDesc(x.ctypes.data, y.ctypes.data, z.ctypes.data, gn.ctypes.data, len(gn),
+                          C.cast(pa, C.c_void_p).value, rstep, int(n_bins), first_n, q.ctypes.data, len(q),
+                          DBX_REQ_KEEP if keep else 0)
+        t = C.c_uint64()
+        rc = lib().dbx_submit(self.handle, C.byref(req), C.byref(t))
+        if rc != DBX_OK:
+            self._err(rc, "submit")
+        self._nq[t.value] = (len(q), len(gn), int(n_bins))
+        return t.value
+
+    def poll(self, ticket):
+        st = C.c_int()
+        rc = lib().dbx_poll(self.handle, ticket, C.byref(st))
+        if rc != DBX_OK:
+            self._err(rc, "poll")
+        return st.value
+
+    def wait(self, ticket, allow=()):
+        """(I, latency_ms, status)"""
+        I = np.zeros(self._nq[ticket][0])
+        lat = C.c_double()
+        rc = lib().dbx_wait(self.handle, ticket, _p(I), C.byref(lat))
+        if rc != DBX_OK and rc not in allow:
+            self._err(rc, f"request {ticket}")
+        return I, lat.value, rc
+
+    def reweight(self, ticket, params):
+        nq, S, _ = self._nq[ticket]
+        pa = (GroupParams * S)(*params)
+        I = np.zeros(nq)
+        rc = lib().dbx_reweight(self.handle, ticket, pa, _p(I))
+        if rc != DBX_OK:
+            self._err(rc, "reweight")
+        return I
+
+    def histogram(self, ticket):
+        nq, S, nb = self._nq[ticket]
+        H = np.zeros((S * (S + 1) // 2, nb), np.uint64)
+        rc = lib().dbx_request_histogram(self.handle, ticket, _p(H))
+        if rc != DBX_OK:
+            self._err(rc, "histogram")
+        return H
+
+    def release(self, ticket):
+        rc = lib().dbx_release(self.handle, ticket)
+        if rc != DBX_OK:
+            self._err(rc, "release")
+
+    def stats(self):
+        st = ServerStats()
+        lib().dbx_server_stats_get(self.handle, C.byref(st))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}

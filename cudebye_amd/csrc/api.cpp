@@ -162,7 +162,7 @@ static Tile make_tile(const std::vector<Pt> &pts, int64_t lo, int64_t hi, int g,
 // (level 1) and quarters of 16 (level 2), all by median bisection, so every sub-tile is
 // a contiguous, compact run inside its block.  Blocks are stored in ascending size of
 // their bounding box; tiles_h holds all level-0 tiles, then all level-1, then all level-2.
-static void tile_model_host(const float *x, const float *y, const float *z, const int64_t *group_n,
+void tile_model_host(const float *x, const float *y, const float *z, const int64_t *group_n,
                             int S, ModelDev &m, std::vector<float4> &pos_h)
 {
     DBX_REQUIRE(S > 0 && S <= 64, DBX_ERR_ARG, "n_groups must be in [1, 64]");
@@ -264,15 +264,89 @@ static void tile_model_host(const float *x, const float *y, const float *z, cons
     }
 }
 
-static void build_model(dbx_ctx *ctx, const float *x, const float *y, const float *z,
-                        const int64_t *group_n, int S, ModelDev &m)
+DevArena::DevArena(char *base, size_t bytes) : base_(base), cap_(bytes & ~(size_t)255)
 {
-    std::vector<float4> pos_h;
-    tile_model_host(x, y, z, group_n, S, m, pos_h);
-    const size_t pb = sizeof(float4) * std::max<size_t>(1, pos_h.size());
-    const size_t tb = sizeof(Tile) * std::max<size_t>(1, m.tiles_h.size());
-    DBX_HIP(hipMalloc(&m.pos, pb));
-    DBX_HIP(hipMalloc(&m.tiles, tb));
+    if (cap_) free_[0] = cap_;
+}
+
+void *DevArena::alloc(size_t bytes)
+{
+    bytes = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        const size_t off = it->first, sz = it->second;
+        free_.erase(it);
+        if (sz > bytes) free_[off + bytes] = sz - bytes;
+        live_[off] = bytes;
+        used_ += bytes;
+        return base_ + off;
+    }
+    return nullptr;
+}
+
+void DevArena::free(void *p)
+{
+    if (!p) return;
+    const size_t off = (size_t)((char *)p - base_);
+    auto lv = live_.find(off);
+    if (lv == live_.end()) return;
+    size_t o = off, sz = lv->second;
+    live_.erase(lv);
+    used_ -= sz;
+    auto nx = free_.lower_bound(o);
+    if (nx != free_.end() && nx->first == o + sz) {
+        sz += nx->second;
+        nx = free_.erase(nx);
+    }
+    if (nx != free_.begin()) {
+        auto pv = std::prev(nx);
+        if (pv->first + pv->second == o) {
+            o = pv->first;
+            sz += pv->second;
+            free_.erase(pv);
+        }
+    }
+    free_[o] = sz;
+}
+
+size_t DevArena::largest_free() const
+{
+    size_t b = 0;
+    for (auto &f : free_) b = std::max(b, f.second);
+    return b;
+}
+
+void *model_alloc(ModelDev &m, size_t bytes)
+{
+    if (m.arena) {
+        void *p = m.arena->alloc(bytes);
+        DBX_REQUIRE(p != nullptr, DBX_ERR_BUDGET, "serving memory budget exhausted");
+        return p;
+    }
+    void *p = nullptr;
+    DBX_HIP(hipMalloc(&p, bytes));
+    return p;
+}
+
+void model_free(ModelDev &m, void *p)
+{
+    if (!p) return;
+    if (m.arena)
+        m.arena->free(p);
+    else
+        (void)hipFree(p);
+}
+
+size_t model_device_bytes(const ModelDev &m, size_t n_pos)
+{
+    auto al = [](size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; };
+    return al(sizeof(float4) * n_pos) + al(sizeof(Tile) * m.tiles_h.size());
+}
+
+void upload_model(dbx_ctx *ctx, ModelDev &m, const std::vector<float4> &pos_h)
+{
+    m.pos = (float4 *)model_alloc(m, sizeof(float4) * std::max<size_t>(1, pos_h.size()));
+    m.tiles = (Tile *)model_alloc(m, sizeof(Tile) * std::max<size_t>(1, m.tiles_h.size()));
     if (!pos_h.empty())
         DBX_HIP(hipMemcpyAsync(m.pos, pos_h.data(), sizeof(float4) * pos_h.size(),
                                hipMemcpyHostToDevice, ctx->stream));
@@ -282,11 +356,19 @@ static void build_model(dbx_ctx *ctx, const float *x, const float *y, const floa
     DBX_HIP(hipStreamSynchronize(ctx->stream));
 }
 
-static void free_model(ModelDev &m)
+static void build_model(dbx_ctx *ctx, const float *x, const float *y, const float *z,
+                        const int64_t *group_n, int S, ModelDev &m)
+{
+    std::vector<float4> pos_h;
+    tile_model_host(x, y, z, group_n, S, m, pos_h);
+    upload_model(ctx, m, pos_h);
+}
+
+void free_model(ModelDev &m)
 {
     for (void *p : {(void *)m.pos, (void *)m.tiles, (void *)m.d_jobs, (void *)m.d_edge2, (void *)m.d_list,
                     (void *)m.d_alist})
-        if (p) (void)hipFree(p);
+        model_free(m, p);
     m.pos = nullptr;
     m.tiles = nullptr;
     m.d_jobs = nullptr;

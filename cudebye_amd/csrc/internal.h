@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -75,7 +76,27 @@ constexpr int kP1Bins = 35328;     // LDS window: 138 KiB of uint32 counters (+1
 constexpr int kP1Cand = 1024;      // LDS candidate list (A tile, B tile), 4 KiB
 // + a 1 KiB B-atom stage per wave (16 KiB) for the 32- and 16-atom A tiles
 
+// Host-managed suballocator over one device allocation (the serving budget): allocation and
+// release are host bookkeeping only, so releasing memory never synchronises the device.
+// First fit over a free list ordered by offset, 256-byte granules, neighbours coalesced.
+class DevArena {
+public:
+    DevArena(char *base, size_t bytes);
+    void *alloc(size_t bytes);  // nullptr when no free range is large enough
+    void free(void *p);
+    size_t used() const { return used_; }
+    size_t capacity() const { return cap_; }
+    size_t largest_free() const;
+
+private:
+    char *base_;
+    size_t cap_, used_ = 0;
+    std::map<size_t, size_t> free_;  // offset -> size
+    std::map<size_t, size_t> live_;  // offset -> size
+};
+
 struct ModelDev {
+    DevArena *arena = nullptr;  // device memory from this arena (else hipMalloc)
     int n_groups = 0;
     int64_t n_atoms = 0;
     std::vector<int64_t> group_n;
@@ -113,6 +134,11 @@ struct P1Ref {
     int32_t pad;
     unsigned long long *oor;   // this model's count of pairs at or beyond n_bins
 };
+
+// device memory of a model: from its arena when it has one (throws DBX_ERR_BUDGET when the
+// arena is full), else hipMalloc
+void *model_alloc(ModelDev &m, size_t bytes);
+void model_free(ModelDev &m, void *p);
 
 }  // namespace dbx
 
@@ -157,7 +183,8 @@ enum { kSlotPass1 = 0, kSlotPass2 = 1, kSlotPass2b = 2, kSlotHist = 3, kSlotQ = 
 void *scratch(dbx_ctx *ctx, int slot, size_t bytes);
 // pinned host buffer of at least `bytes` for `slot` (staging of per-call uploads; every call
 // synchronises its stream before returning, so a slot is free again at the next call)
-enum { kPinRefs = 0, kPinList = 1, kPinRows = 2, kPinPairs = 3, kPinMisc = 4, kPinModels = 5, kPinOor = 6 };
+enum { kPinRefs = 0, kPinList = 1, kPinRows = 2, kPinPairs = 3, kPinMisc = 4, kPinModels = 5, kPinOor = 6,
+       kPinStepI = 7 };
 void *pinned(dbx_ctx *ctx, int slot, size_t bytes);
 
 // pass 1 (pass1.hip): histograms of all subgroup pairs into d_hist[P][n_bins]
@@ -184,6 +211,28 @@ void pass1_read_stats(dbx_ctx *ctx, P1Stats &st);
 // host planning of pass 1: work items (largest first) and the shell-edge tables
 void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &jobs, std::vector<float> &edge2,
                 std::vector<int32_t> &alist, int n_cu = 256, std::vector<double> *cost = nullptr);
+
+// the pass-1 plan on the host (make_plan_host, no device) and its installation on a model
+struct PlanHost {
+    double rstep = 0;
+    int cu = 0;
+    std::vector<P1Job> jobs;
+    std::vector<float> edge2;
+    std::vector<int32_t> alist;
+    std::vector<double> cost;
+    std::vector<int4> list;
+    int n_fast = 0, list_fast = 0;
+};
+PlanHost make_plan_host(const ModelDev &m, double rstep, int n_cu);
+size_t plan_device_bytes(const PlanHost &ph);
+void install_plan(dbx_ctx *ctx, ModelDev &m, PlanHost &&ph);
+
+// models (api.cpp): host tiling, device upload, release
+void tile_model_host(const float *x, const float *y, const float *z, const int64_t *group_n, int S,
+                     ModelDev &m, std::vector<float4> &pos_h);
+size_t model_device_bytes(const ModelDev &m, size_t n_pos);
+void upload_model(dbx_ctx *ctx, ModelDev &m, const std::vector<float4> &pos_h);
+void free_model(ModelDev &m);
 
 // pass 2 (pass2.hip)
 // one row of pass 2: a partial histogram at hist_base + h_off with n_bins bins
@@ -214,6 +263,9 @@ void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const s
 void debye_structure(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
                      int first_n, const double *d_q, int nq, double *d_S);
 // I[q] = sum_p ff[p][q] S[p][q]
+void launch_prefactor(hipStream_t st, const GroupPair *d_pairs, int n, const double *d_q, int nq, double *d_ff);
+void launch_combine(hipStream_t st, const double *d_ff, const double *d_S, const int4 *d_mrows, int nm, int nq,
+                    double *d_I);
 void combine_intensity(dbx_ctx *ctx, const double *d_ff, const double *d_S, int P, int nq,
                        double *d_I);
 

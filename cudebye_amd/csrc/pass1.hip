@@ -495,49 +495,79 @@ static void order_entries(EntryList &fast, EntryList &checked, int4 *out)
     for (auto &e : checked) out[n++] = e.second;
 }
 
-// builds (or reuses) the model's device copy of the pass-1 plan for rstep; the jobs are kept
-// whole and cut into pieces per launch list (alone: about 4 pieces' worth per CU)
-static void ensure_plan(dbx_ctx *ctx, ModelDev &m, double rstep)
+// The pass-1 plan of a model for one bin width, built on the host (no device needed): the
+// jobs are kept whole and cut into pieces per launch list (alone: about 4 pieces' worth per CU).
+PlanHost make_plan_host(const ModelDev &m, double rstep, int n_cu)
 {
-    if (m.plan_rstep == rstep && m.plan_cu == ctx->cu_count && m.d_jobs) return;
-    std::vector<P1Job> sorted;
-    std::vector<float> edge2;
-    std::vector<int32_t> alist;
-    std::vector<double> cost;
-    plan_pass1(m, rstep, sorted, edge2, alist, 0, &cost);
-    int n_fast = 0;
-    while (n_fast < (int)sorted.size() && !sorted[n_fast].degraded) n_fast++;
+    PlanHost ph;
+    ph.rstep = rstep;
+    ph.cu = n_cu;
+    plan_pass1(m, rstep, ph.jobs, ph.edge2, ph.alist, 0, &ph.cost);
+    ph.n_fast = 0;
+    while (ph.n_fast < (int)ph.jobs.size() && !ph.jobs[ph.n_fast].degraded) ph.n_fast++;
+    // the launch list of the model alone
+    ModelDev tmp;
+    tmp.plan_h = ph.jobs;
+    tmp.plan_cost = ph.cost;
+    tmp.plan_jobs = (int)ph.jobs.size();
+    EntryList fast, checked;
+    add_entries(tmp, 0, plan_total(tmp) / (double)(std::max(1, n_cu) * 4), fast, checked);
+    ph.list.resize(fast.size() + checked.size());
+    order_entries(fast, checked, ph.list.data());
+    ph.list_fast = (int)fast.size();
+    return ph;
+}
+
+size_t plan_device_bytes(const PlanHost &ph)
+{
+    auto al = [](size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; };
+    return al(sizeof(P1Job) * ph.jobs.size()) + al(sizeof(float) * ph.edge2.size()) +
+           al(sizeof(int4) * ph.list.size()) + al(sizeof(int32_t) * ph.alist.size());
+}
+
+// replaces the model's plan by ph (device arrays from the model's allocator, copies on the
+// context's stream, synchronised)
+void install_plan(dbx_ctx *ctx, ModelDev &m, PlanHost &&ph)
+{
     for (void *q : {(void *)m.d_jobs, (void *)m.d_edge2, (void *)m.d_list, (void *)m.d_alist})
-        if (q) DBX_HIP(hipFree(q));
+        if (q) model_free(m, q);
     m.d_jobs = nullptr;
     m.d_edge2 = nullptr;
     m.d_list = nullptr;
     m.d_alist = nullptr;
     m.plan_rstep = -1.0;
-    m.plan_h = sorted;
-    m.plan_cost = cost;
-    m.plan_jobs = (int)sorted.size();
-    m.plan_fast = n_fast;
-    EntryList fast, checked;
-    add_entries(m, 0, plan_total(m) / (double)(std::max(1, ctx->cu_count) * 4), fast, checked);
-    std::vector<int4> list(fast.size() + checked.size());
-    order_entries(fast, checked, list.data());
-    DBX_HIP(hipMalloc(&m.d_jobs, sizeof(P1Job) * std::max<size_t>(1, sorted.size())));
-    DBX_HIP(hipMalloc(&m.d_edge2, sizeof(float) * std::max<size_t>(1, edge2.size())));
-    DBX_HIP(hipMalloc(&m.d_list, sizeof(int4) * std::max<size_t>(1, list.size())));
-    DBX_HIP(hipMalloc(&m.d_alist, sizeof(int32_t) * std::max<size_t>(1, alist.size())));
-    if (!sorted.empty()) {
-        DBX_HIP(hipMemcpy(m.d_jobs, sorted.data(), sizeof(P1Job) * sorted.size(), hipMemcpyHostToDevice));
-        DBX_HIP(hipMemcpy(m.d_list, list.data(), sizeof(int4) * list.size(), hipMemcpyHostToDevice));
+    m.d_jobs = (P1Job *)model_alloc(m, sizeof(P1Job) * std::max<size_t>(1, ph.jobs.size()));
+    m.d_edge2 = (float *)model_alloc(m, sizeof(float) * std::max<size_t>(1, ph.edge2.size()));
+    m.d_list = (int4 *)model_alloc(m, sizeof(int4) * std::max<size_t>(1, ph.list.size()));
+    m.d_alist = (int32_t *)model_alloc(m, sizeof(int32_t) * std::max<size_t>(1, ph.alist.size()));
+    if (!ph.jobs.empty()) {
+        DBX_HIP(hipMemcpyAsync(m.d_jobs, ph.jobs.data(), sizeof(P1Job) * ph.jobs.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        DBX_HIP(hipMemcpyAsync(m.d_list, ph.list.data(), sizeof(int4) * ph.list.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
     }
-    if (!edge2.empty())
-        DBX_HIP(hipMemcpy(m.d_edge2, edge2.data(), sizeof(float) * edge2.size(), hipMemcpyHostToDevice));
-    if (!alist.empty())
-        DBX_HIP(hipMemcpy(m.d_alist, alist.data(), sizeof(int32_t) * alist.size(), hipMemcpyHostToDevice));
-    m.plan_list_n = (int)list.size();
-    m.plan_list_fast = (int)fast.size();
-    m.plan_cu = ctx->cu_count;
-    m.plan_rstep = rstep;
+    if (!ph.edge2.empty())
+        DBX_HIP(hipMemcpyAsync(m.d_edge2, ph.edge2.data(), sizeof(float) * ph.edge2.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    if (!ph.alist.empty())
+        DBX_HIP(hipMemcpyAsync(m.d_alist, ph.alist.data(), sizeof(int32_t) * ph.alist.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+    DBX_HIP(hipStreamSynchronize(ctx->stream));
+    m.plan_h = std::move(ph.jobs);
+    m.plan_cost = std::move(ph.cost);
+    m.plan_jobs = (int)m.plan_h.size();
+    m.plan_fast = ph.n_fast;
+    m.plan_list_n = (int)ph.list.size();
+    m.plan_list_fast = ph.list_fast;
+    m.plan_cu = ph.cu;
+    m.plan_rstep = ph.rstep;
+}
+
+// builds (or reuses) the model's device copy of the pass-1 plan for rstep
+static void ensure_plan(dbx_ctx *ctx, ModelDev &m, double rstep)
+{
+    if (m.plan_rstep == rstep && m.plan_cu == ctx->cu_count && m.d_jobs) return;
+    install_plan(ctx, m, make_plan_host(m, rstep, ctx->cu_count));
 }
 
 static P1Ref make_ref(const ModelDev &m, int64_t n_bins, uint64_t *d_hist, double rstep)

@@ -839,4 +839,18 @@ void combine_intensity(dbx_ctx *ctx, const double *d_ff, const double *d_S, int 
     combine_models(ctx, d_ff, d_S, std::vector<int4>{make_int4(0, P, 0, 0)}, nq, d_I);
 }
 
+// stream-explicit launches for callers that record them into a graph (no allocation, no copy)
+void launch_prefactor(hipStream_t st, const GroupPair *d_pairs, int n, const double *d_q, int nq, double *d_ff)
+{
+    hipLaunchKernelGGL(p2_prefactor_kernel, dim3((nq + 255) / 256, n), dim3(256), 0, st, d_pairs, d_q, nq, d_ff);
+    DBX_HIP(hipGetLastError());
+}
+
+void launch_combine(hipStream_t st, const double *d_ff, const double *d_S, const int4 *d_mrows, int nm, int nq,
+                    double *d_I)
+{
+    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, st, d_ff, d_S, d_mrows, nq, d_I);
+    DBX_HIP(hipGetLastError());
+}
+
 }  // namespace dbx

@@ -1,0 +1,661 @@
+// Serving: a stream of structure models of uneven size through continuous batching.
+//
+// No reference counterpart in-process: the reference runs one process per model
+// (Matlab/multisim.m:42-58 launching cuDebye.exe, cuDebye-v15.cu:599-1003).
+//
+//   submit -> [prep threads: tiling + pass-1 plan on the host] -> ready queue (FIFO)
+//          -> admission: every device buffer of the request from the budget arena
+//          -> [lane threads, one stream each: one step = the admitted requests that fit the
+//              step caps; pass 1 over all of them in one launch pair, then per q grid one
+//              prefactor + compaction + Debye contraction + combine] -> done
+// A request's device memory comes from one arena sized by the HBM budget; a request waits
+// in the ready queue until its buffers fit.  Memory is returned at completion (or at
+// dbx_release for DBX_REQ_KEEP requests, whose histograms and S_p(q) stay resident).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "internal.h"
+
+namespace dbx {
+
+enum { kQueued = 0, kReady = 1, kAdmitted = 2, kRunning = 3, kDone = 4, kFailed = 5 };
+
+static double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    uint64_t id = 0;
+    std::vector<float> x, y, z;
+    std::vector<int64_t> group_n;
+    std::vector<dbx_group_params> params;
+    std::vector<double> q;
+    double rstep = 0;
+    int64_t n_bins = 0;
+    int first_n = 0;
+    int flags = 0;
+    int P = 0;
+    // host preparation
+    ModelDev m;
+    std::vector<float4> pos_h;
+    PlanHost plan;
+    // device buffers (arena)
+    std::vector<void *> dev;
+    uint64_t *d_hist = nullptr;
+    uint64_t *d_oor = nullptr;
+    double *d_S = nullptr;  // [P][nq]
+    double *d_q = nullptr;
+    double *d_ff = nullptr;  // [P][nq], reweight
+    double *d_I = nullptr;   // [nq], reweight
+    GroupPair *d_pairs = nullptr;
+    int4 *d_mrows = nullptr;
+    size_t bytes = 0;
+    // results
+    std::vector<double> I;
+    uint64_t oor = 0;
+    int state = kQueued;
+    int err = DBX_OK;
+    std::string msg;
+    double t_submit = 0, t_done = 0;
+    // reweight: captured graph, pinned staging
+    hipGraph_t rw_graph = nullptr;
+    hipGraphExec_t rw_exec = nullptr;
+    GroupPair *h_pairs = nullptr;
+    double *h_I = nullptr;
+};
+
+}  // namespace dbx
+
+struct dbx_server {
+    int device = 0;
+    dbx_server_config cfg{};
+    void *arena_mem = nullptr;
+    std::unique_ptr<dbx::DevArena> arena;
+    std::mutex mu;
+    std::condition_variable cv_prep, cv_sched, cv_done;
+    std::deque<dbx::Request *> prep_q, ready_q, admitted_q;
+    std::unordered_map<uint64_t, std::unique_ptr<dbx::Request>> reqs;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::vector<std::thread> threads;
+    dbx_ctx *lane[2] = {nullptr, nullptr};
+    dbx_ctx *rw_ctx = nullptr;
+    std::mutex rw_mu;
+    std::string err;
+    dbx_server_stats st{};
+};
+
+namespace dbx {
+
+static size_t al256(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; }
+
+// device bytes of a prepared request while resident
+static size_t request_bytes(const Request &r)
+{
+    const size_t nq = r.q.size(), P = (size_t)r.P;
+    return model_device_bytes(r.m, r.pos_h.size()) + plan_device_bytes(r.plan) +
+           al256(sizeof(uint64_t) * P * (size_t)r.n_bins) + al256(8) + 2 * al256(sizeof(double) * P * nq) +
+           2 * al256(sizeof(double) * nq) + al256(sizeof(GroupPair) * P) + al256(sizeof(int4));
+}
+
+// every device buffer of the request from the arena; false (nothing kept) when one does not fit
+static bool alloc_request(DevArena &A, Request &r)
+{
+    const size_t nq = r.q.size(), P = (size_t)r.P;
+    auto take = [&](size_t b) -> void * {
+        void *p = A.alloc(b);
+        if (p) r.dev.push_back(p);
+        return p;
+    };
+    bool ok = true;
+    auto need = [&](void *p) { ok = ok && p != nullptr; return p; };
+    r.m.pos = (float4 *)need(take(sizeof(float4) * std::max<size_t>(1, r.pos_h.size())));
+    r.m.tiles = (Tile *)need(take(sizeof(Tile) * std::max<size_t>(1, r.m.tiles_h.size())));
+    r.m.d_jobs = (P1Job *)need(take(sizeof(P1Job) * std::max<size_t>(1, r.plan.jobs.size())));
+    r.m.d_edge2 = (float *)need(take(sizeof(float) * std::max<size_t>(1, r.plan.edge2.size())));
+    r.m.d_list = (int4 *)need(take(sizeof(int4) * std::max<size_t>(1, r.plan.list.size())));
+    r.m.d_alist = (int32_t *)need(take(sizeof(int32_t) * std::max<size_t>(1, r.plan.alist.size())));
+    r.d_hist = (uint64_t *)need(take(sizeof(uint64_t) * P * (size_t)r.n_bins));
+    r.d_oor = (uint64_t *)need(take(8));
+    r.d_S = (double *)need(take(sizeof(double) * P * nq));
+    r.d_ff = (double *)need(take(sizeof(double) * P * nq));
+    r.d_q = (double *)need(take(sizeof(double) * nq));
+    r.d_I = (double *)need(take(sizeof(double) * nq));
+    r.d_pairs = (GroupPair *)need(take(sizeof(GroupPair) * P));
+    r.d_mrows = (int4 *)need(take(sizeof(int4)));
+    if (!ok) {
+        for (void *p : r.dev) A.free(p);
+        r.dev.clear();
+        r.m.pos = nullptr;
+        r.m.tiles = nullptr;
+        r.m.d_jobs = nullptr;
+        r.m.d_edge2 = nullptr;
+        r.m.d_list = nullptr;
+        r.m.d_alist = nullptr;
+    }
+    return ok;
+}
+
+static void free_request_device(DevArena &A, Request &r)
+{
+    for (void *p : r.dev) A.free(p);
+    r.dev.clear();
+    r.m.pos = nullptr;
+    r.m.tiles = nullptr;
+    r.m.d_jobs = nullptr;
+    r.m.d_edge2 = nullptr;
+    r.m.d_list = nullptr;
+    r.m.d_alist = nullptr;
+    r.d_hist = nullptr;
+    r.d_S = nullptr;
+    if (r.rw_exec) (void)hipGraphExecDestroy(r.rw_exec);
+    if (r.rw_graph) (void)hipGraphDestroy(r.rw_graph);
+    r.rw_exec = nullptr;
+    r.rw_graph = nullptr;
+}
+
+static void fail(Request &r, int code, const std::string &msg)
+{
+    r.state = kFailed;
+    r.err = code;
+    r.msg = msg;
+}
+
+static void prep_loop(dbx_server *s)
+{
+    (void)hipSetDevice(s->device);
+    for (;;) {
+        Request *r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_prep.wait(lk, [&] { return s->stop || !s->prep_q.empty(); });
+            if (s->stop) return;
+            r = s->prep_q.front();
+            s->prep_q.pop_front();
+        }
+        int code = DBX_OK;
+        std::string msg;
+        try {
+            tile_model_host(r->x.data(), r->y.data(), r->z.data(), r->group_n.data(), (int)r->group_n.size(), r->m,
+                            r->pos_h);
+            r->plan = make_plan_host(r->m, r->rstep, s->lane[0]->cu_count);
+            r->bytes = request_bytes(*r);
+        } catch (const Error &e) {
+            code = e.code;
+            msg = e.what();
+        } catch (const std::exception &e) {
+            code = DBX_ERR_NOMEM;
+            msg = e.what();
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (code == DBX_OK && r->bytes > s->arena->capacity()) {
+            code = DBX_ERR_BUDGET;
+            msg = "request needs " + std::to_string(r->bytes) + " device bytes, more than the serving budget";
+        }
+        if (code != DBX_OK) {
+            fail(*r, code, msg);
+            r->t_done = now_ms();
+            s->st.failed++;
+            s->cv_done.notify_all();
+            continue;
+        }
+        r->state = kReady;
+        s->ready_q.push_back(r);
+        s->cv_sched.notify_all();
+    }
+}
+
+// admission (caller holds s->mu): ready requests in arrival order while their buffers fit
+static void admit(dbx_server *s)
+{
+    while (!s->ready_q.empty()) {
+        Request *r = s->ready_q.front();
+        if (!alloc_request(*s->arena, *r)) break;
+        s->ready_q.pop_front();
+        r->state = kAdmitted;
+        s->admitted_q.push_back(r);
+    }
+    s->st.resident_bytes = s->arena->used();
+}
+
+// one step on a lane: uploads, pass 1 over all requests, pass 2 per q grid, results
+static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
+{
+    const double rstep = batch[0]->rstep;
+    for (Request *r : batch) {
+        // the model's device arrays and its plan (already in the arena)
+        ModelDev &m = r->m;
+        if (!r->pos_h.empty())
+            DBX_HIP(hipMemcpyAsync(m.pos, r->pos_h.data(), sizeof(float4) * r->pos_h.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        if (!m.tiles_h.empty())
+            DBX_HIP(hipMemcpyAsync(m.tiles, m.tiles_h.data(), sizeof(Tile) * m.tiles_h.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        PlanHost &ph = r->plan;
+        if (!ph.jobs.empty()) {
+            DBX_HIP(hipMemcpyAsync(m.d_jobs, ph.jobs.data(), sizeof(P1Job) * ph.jobs.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+            DBX_HIP(hipMemcpyAsync(m.d_list, ph.list.data(), sizeof(int4) * ph.list.size(), hipMemcpyHostToDevice,
+                                   ctx->stream));
+        }
+        if (!ph.edge2.empty())
+            DBX_HIP(hipMemcpyAsync(m.d_edge2, ph.edge2.data(), sizeof(float) * ph.edge2.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        if (!ph.alist.empty())
+            DBX_HIP(hipMemcpyAsync(m.d_alist, ph.alist.data(), sizeof(int32_t) * ph.alist.size(),
+                                   hipMemcpyHostToDevice, ctx->stream));
+        DBX_HIP(hipMemcpyAsync(r->d_q, r->q.data(), sizeof(double) * r->q.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        m.plan_h = ph.jobs;
+        m.plan_cost = ph.cost;
+        m.plan_jobs = (int)ph.jobs.size();
+        m.plan_fast = ph.n_fast;
+        m.plan_list_n = (int)ph.list.size();
+        m.plan_list_fast = ph.list_fast;
+        m.plan_cu = ph.cu;
+        m.plan_rstep = ph.rstep;
+        DBX_HIP(hipMemsetAsync(r->d_oor, 0, 8, ctx->stream));
+    }
+    // the host arrays above must outlive their copies
+    DBX_HIP(hipStreamSynchronize(ctx->stream));
+    std::vector<P1Item> items;
+    for (Request *r : batch) {
+        P1Item it{&r->m, r->n_bins, r->d_hist};
+        it.d_oor = r->d_oor;
+        items.push_back(it);
+    }
+    pass1_batch(ctx, items, rstep, false);
+    // pass 2: one contraction per q grid, rows longest first
+    std::vector<Request *> order(batch);
+    std::stable_sort(order.begin(), order.end(), [](Request *a, Request *b) {
+        if (a->q.size() != b->q.size()) return a->q.size() < b->q.size();
+        return a->n_bins > b->n_bins;
+    });
+    size_t g0 = 0;
+    while (g0 < order.size()) {
+        size_t g1 = g0 + 1;
+        while (g1 < order.size() && order[g1]->q == order[g0]->q) g1++;
+        // rows longest first within the group
+        std::vector<Request *> grp(order.begin() + g0, order.begin() + g1);
+        std::stable_sort(grp.begin(), grp.end(), [](Request *a, Request *b) { return a->n_bins > b->n_bins; });
+        const int nq = (int)grp[0]->q.size();
+        uint64_t *base = grp[0]->d_hist;
+        for (Request *r : grp) base = std::min(base, r->d_hist);
+        std::vector<RowDesc> rows;
+        std::vector<GroupPair> pairs;
+        std::vector<int4> mrows;
+        for (size_t i = 0; i < grp.size(); i++) {
+            Request *r = grp[i];
+            const int S = (int)r->group_n.size();
+            mrows.push_back(make_int4((int)rows.size(), r->P, (int)i, 0));
+            int p = 0;
+            for (int a = 0; a < S; a++)
+                for (int b = a; b < S; b++, p++) {
+                    rows.push_back(RowDesc{(int64_t)(r->d_hist - base) + (int64_t)p * r->n_bins, r->n_bins,
+                                           r->first_n, 0});
+                    pairs.push_back(make_group_pair(r->params[a], r->params[b]));
+                }
+        }
+        const size_t R = rows.size();
+        double *buf = (double *)scratch(ctx, kSlotQ, sizeof(double) * (size_t)nq * (2 * R + grp.size()));
+        double *dff = buf, *dS = buf + R * nq, *dI = dS + R * nq;
+        prefactor_rows(ctx, pairs, grp[0]->d_q, nq, dff);
+        debye_structure_rows(ctx, base, rows, rstep, grp[0]->d_q, nq, dS);
+        combine_models(ctx, dff, dS, mrows, nq, dI);
+        double *hI = (double *)pinned(ctx, kPinStepI, sizeof(double) * (size_t)nq * grp.size());
+        DBX_HIP(hipMemcpyAsync(hI, dI, sizeof(double) * (size_t)nq * grp.size(), hipMemcpyDeviceToHost, ctx->stream));
+        for (size_t i = 0; i < grp.size(); i++) {
+            Request *r = grp[i];
+            DBX_HIP(hipMemcpyAsync(r->d_S, dS + (size_t)mrows[i].x * nq, sizeof(double) * (size_t)r->P * nq,
+                                   hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        uint64_t *hoor = (uint64_t *)pinned(ctx, kPinOor, sizeof(uint64_t) * grp.size());
+        for (size_t i = 0; i < grp.size(); i++)
+            DBX_HIP(hipMemcpyAsync(hoor + i, grp[i]->d_oor, 8, hipMemcpyDeviceToHost, ctx->stream));
+        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        for (size_t i = 0; i < grp.size(); i++) {
+            Request *r = grp[i];
+            r->I.assign(hI + i * (size_t)nq, hI + (i + 1) * (size_t)nq);
+            r->oor = hoor[i];
+        }
+        g0 = g1;
+    }
+}
+
+static void lane_loop(dbx_server *s, int li)
+{
+    (void)hipSetDevice(s->device);
+    dbx_ctx *ctx = s->lane[li];
+    for (;;) {
+        std::vector<Request *> batch;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            for (;;) {
+                if (s->stop) return;
+                admit(s);
+                if (!s->admitted_q.empty()) break;
+                s->cv_sched.wait(lk);
+            }
+            // the step: admitted requests with the head's bin width, within the caps
+            const double rstep = s->admitted_q.front()->rstep;
+            double pairs = 0;
+            for (auto it = s->admitted_q.begin(); it != s->admitted_q.end();) {
+                Request *r = *it;
+                const double n = (double)r->x.size();
+                const double pr = 0.5 * n * (n + 1);
+                if (r->rstep != rstep || (!batch.empty() && (pairs + pr > s->cfg.max_step_pairs ||
+                                                             (int)batch.size() >= s->cfg.max_batch))) {
+                    ++it;
+                    continue;
+                }
+                batch.push_back(r);
+                pairs += pr;
+                r->state = kRunning;
+                it = s->admitted_q.erase(it);
+            }
+            s->st.steps++;
+            s->st.max_step_requests = std::max(s->st.max_step_requests, (double)batch.size());
+            s->st.pairs += pairs;
+        }
+        int code = DBX_OK;
+        std::string msg;
+        try {
+            run_step(s, ctx, batch);
+        } catch (const Error &e) {
+            code = e.code;
+            msg = e.what();
+        } catch (const std::exception &e) {
+            code = DBX_ERR_HIP;
+            msg = e.what();
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        const double t = now_ms();
+        for (Request *r : batch) {
+            r->t_done = t;
+            if (code != DBX_OK) {
+                fail(*r, code, msg);
+            } else if (r->oor) {
+                fail(*r, DBX_ERR_RANGE,
+                     std::to_string(r->oor) + " pair distances fall at or beyond the last bin (rmax too small)");
+            } else {
+                r->state = kDone;
+            }
+            if (r->state == kDone) s->st.completed++;
+            else s->st.failed++;
+            if (!(r->flags & DBX_REQ_KEEP) || r->state != kDone) free_request_device(*s->arena, *r);
+        }
+        s->st.resident_bytes = s->arena->used();
+        s->cv_done.notify_all();
+        s->cv_sched.notify_all();
+    }
+}
+
+}  // namespace dbx
+
+using namespace dbx;
+
+extern "C" {
+
+int dbx_server_open(int device, const dbx_server_config *cfg, dbx_server **out)
+{
+    if (!out) return DBX_ERR_ARG;
+    *out = nullptr;
+    dbx_server *s = new dbx_server;
+    s->device = device;
+    if (cfg) s->cfg = *cfg;
+    if (s->cfg.max_batch <= 0) s->cfg.max_batch = 256;
+    if (!(s->cfg.max_step_pairs > 0)) s->cfg.max_step_pairs = 2e10;
+    if (s->cfg.prep_threads <= 0) s->cfg.prep_threads = 4;
+    int rc = DBX_OK;
+    for (int i = 0; i < 2 && rc == DBX_OK; i++) rc = dbx_open(device, 0, &s->lane[i]);
+    if (rc == DBX_OK) rc = dbx_open(device, 0, &s->rw_ctx);
+    if (rc == DBX_OK) {
+        for (int i = 0; i < 2; i++) s->lane[i]->pass2_mode = s->cfg.pass2_mode;
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) rc = DBX_ERR_HIP;
+        size_t budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : (size_t)(0.75 * (double)fr);
+        budget &= ~(size_t)255;
+        if (rc == DBX_OK && hipMalloc(&s->arena_mem, budget) != hipSuccess) rc = DBX_ERR_NOMEM;
+        if (rc == DBX_OK) {
+            s->arena.reset(new DevArena((char *)s->arena_mem, budget));
+            s->st.budget_bytes = budget;
+        }
+    }
+    if (rc != DBX_OK) {
+        for (dbx_ctx *c : {s->lane[0], s->lane[1], s->rw_ctx})
+            if (c) dbx_close(c);
+        if (s->arena_mem) (void)hipFree(s->arena_mem);
+        delete s;
+        return rc;
+    }
+    for (int i = 0; i < s->cfg.prep_threads; i++) s->threads.emplace_back(prep_loop, s);
+    for (int i = 0; i < 2; i++) s->threads.emplace_back(lane_loop, s, i);
+    *out = s;
+    return DBX_OK;
+}
+
+int dbx_server_close(dbx_server *s)
+{
+    if (!s) return DBX_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_prep.notify_all();
+    s->cv_sched.notify_all();
+    s->cv_done.notify_all();
+    for (auto &t : s->threads) t.join();
+    (void)hipSetDevice(s->device);
+    for (auto &kv : s->reqs) {
+        Request &r = *kv.second;
+        free_request_device(*s->arena, r);
+        if (r.h_pairs) (void)hipHostFree(r.h_pairs);
+        if (r.h_I) (void)hipHostFree(r.h_I);
+    }
+    for (dbx_ctx *c : {s->lane[0], s->lane[1], s->rw_ctx})
+        if (c) dbx_close(c);
+    if (s->arena_mem) (void)hipFree(s->arena_mem);
+    delete s;
+    return DBX_OK;
+}
+
+const char *dbx_server_last_error(dbx_server *s) { return s ? s->err.c_str() : "null server"; }
+
+int dbx_submit(dbx_server *s, const dbx_request *q, uint64_t *ticket)
+{
+    if (!s || !q || !ticket) return DBX_ERR_ARG;
+    if (q->n_groups <= 0 || !q->group_n || !q->params || !q->q || q->nq <= 0 || q->n_bins <= 0 ||
+        !(q->rstep > 0) || q->first_n < 0)
+        return DBX_ERR_ARG;
+    int64_t n = 0;
+    for (int g = 0; g < q->n_groups; g++) {
+        if (q->group_n[g] < 0) return DBX_ERR_ARG;
+        n += q->group_n[g];
+    }
+    if (n > 0 && (!q->x || !q->y || !q->z)) return DBX_ERR_ARG;
+    for (int64_t i = 0; i < n; i++)
+        if (!std::isfinite(q->x[i]) || !std::isfinite(q->y[i]) || !std::isfinite(q->z[i])) return DBX_ERR_ARG;
+    std::unique_ptr<Request> r(new Request);
+    r->x.assign(q->x, q->x + n);
+    r->y.assign(q->y, q->y + n);
+    r->z.assign(q->z, q->z + n);
+    r->group_n.assign(q->group_n, q->group_n + q->n_groups);
+    r->params.assign(q->params, q->params + q->n_groups);
+    r->q.assign(q->q, q->q + q->nq);
+    r->rstep = q->rstep;
+    r->n_bins = q->n_bins;
+    r->first_n = q->first_n;
+    r->flags = q->flags;
+    r->P = q->n_groups * (q->n_groups + 1) / 2;
+    // histograms alone beyond the budget: refuse now
+    if (sizeof(uint64_t) * (size_t)r->P * (size_t)r->n_bins + 16 * (size_t)n > s->arena->capacity()) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = "request larger than the serving budget";
+        return DBX_ERR_BUDGET;
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->id = s->next_id++;
+    r->t_submit = now_ms();
+    *ticket = r->id;
+    s->prep_q.push_back(r.get());
+    s->reqs[r->id] = std::move(r);
+    s->st.submitted++;
+    s->cv_prep.notify_one();
+    return DBX_OK;
+}
+
+int dbx_poll(dbx_server *s, uint64_t ticket, int *state)
+{
+    if (!s || !state) return DBX_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return DBX_ERR_ARG;
+    const int st = it->second->state;
+    *state = st == kDone ? DBX_REQ_DONE : st == kFailed ? DBX_REQ_FAILED : st == kRunning ? DBX_REQ_RUNNING
+                                                                                         : DBX_REQ_QUEUED;
+    return DBX_OK;
+}
+
+int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms)
+{
+    if (!s) return DBX_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return DBX_ERR_ARG;
+    Request *r = it->second.get();
+    s->cv_done.wait(lk, [&] { return r->state == kDone || r->state == kFailed || s->stop; });
+    if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    if (r->state != kDone && r->state != kFailed) return DBX_ERR_STATE;
+    if (I_out && !r->I.empty()) std::memcpy(I_out, r->I.data(), sizeof(double) * r->I.size());
+    if (r->state == kFailed) {
+        s->err = r->msg;
+        return r->err;
+    }
+    return DBX_OK;
+}
+
+int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params, double *I_out)
+{
+    if (!s || !params || !I_out) return DBX_ERR_ARG;
+    Request *r = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) return DBX_ERR_ARG;
+        r = it->second.get();
+        if (r->state != kDone || !(r->flags & DBX_REQ_KEEP) || !r->d_S) {
+            s->err = "reweight needs a finished request submitted with DBX_REQ_KEEP";
+            return DBX_ERR_STATE;
+        }
+    }
+    std::lock_guard<std::mutex> rl(s->rw_mu);
+    try {
+        (void)hipSetDevice(s->device);
+        const int nq = (int)r->q.size();
+        hipStream_t st = s->rw_ctx->stream;
+        if (!r->h_pairs) {
+            DBX_HIP(hipHostMalloc((void **)&r->h_pairs, sizeof(GroupPair) * r->P, hipHostMallocDefault));
+            DBX_HIP(hipHostMalloc((void **)&r->h_I, sizeof(double) * nq, hipHostMallocDefault));
+        }
+        const int S = (int)r->group_n.size();
+        int p = 0;
+        for (int a = 0; a < S; a++)
+            for (int b = a; b < S; b++, p++) r->h_pairs[p] = make_group_pair(params[a], params[b]);
+        if (!r->rw_exec) {
+            // decode of a resident request: new form factors -> prefactors -> I = sum_p ff_p S_p,
+            // captured once and replayed for every reweight
+            const int4 mr = make_int4(0, r->P, 0, 0);
+            DBX_HIP(hipMemcpyAsync(r->d_mrows, &mr, sizeof(int4), hipMemcpyHostToDevice, st));
+            DBX_HIP(hipStreamSynchronize(st));
+            DBX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            DBX_HIP(hipMemcpyAsync(r->d_pairs, r->h_pairs, sizeof(GroupPair) * r->P, hipMemcpyHostToDevice, st));
+            launch_prefactor(st, r->d_pairs, r->P, r->d_q, nq, r->d_ff);
+            launch_combine(st, r->d_ff, r->d_S, r->d_mrows, 1, nq, r->d_I);
+            DBX_HIP(hipMemcpyAsync(r->h_I, r->d_I, sizeof(double) * nq, hipMemcpyDeviceToHost, st));
+            DBX_HIP(hipStreamEndCapture(st, &r->rw_graph));
+            DBX_HIP(hipGraphInstantiate(&r->rw_exec, r->rw_graph, nullptr, nullptr, 0));
+        }
+        DBX_HIP(hipGraphLaunch(r->rw_exec, st));
+        DBX_HIP(hipStreamSynchronize(st));
+        std::memcpy(I_out, r->h_I, sizeof(double) * nq);
+    } catch (const Error &e) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = e.what();
+        return e.code;
+    }
+    return DBX_OK;
+}
+
+int dbx_request_histogram(dbx_server *s, uint64_t ticket, uint64_t *hist_out)
+{
+    if (!s || !hist_out) return DBX_ERR_ARG;
+    Request *r = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) return DBX_ERR_ARG;
+        r = it->second.get();
+        if (r->state != kDone || !r->d_hist) {
+            s->err = "histograms are kept only for finished DBX_REQ_KEEP requests";
+            return DBX_ERR_STATE;
+        }
+    }
+    std::lock_guard<std::mutex> rl(s->rw_mu);
+    (void)hipSetDevice(s->device);
+    if (hipMemcpyAsync(hist_out, r->d_hist, sizeof(uint64_t) * (size_t)r->P * r->n_bins, hipMemcpyDeviceToHost,
+                       s->rw_ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(s->rw_ctx->stream) != hipSuccess)
+        return DBX_ERR_HIP;
+    return DBX_OK;
+}
+
+int dbx_release(dbx_server *s, uint64_t ticket)
+{
+    if (!s) return DBX_ERR_ARG;
+    std::unique_ptr<Request> r;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(ticket);
+        if (it == s->reqs.end()) return DBX_ERR_ARG;
+        if (it->second->state != kDone && it->second->state != kFailed) {
+            s->err = "release of a request that has not finished";
+            return DBX_ERR_STATE;
+        }
+        r = std::move(it->second);
+        s->reqs.erase(it);
+    }
+    {
+        // a reweight of this request may still hold the decode lane
+        std::lock_guard<std::mutex> rl(s->rw_mu);
+        std::lock_guard<std::mutex> lk(s->mu);
+        free_request_device(*s->arena, *r);
+        s->st.resident_bytes = s->arena->used();
+    }
+    if (r->h_pairs) (void)hipHostFree(r->h_pairs);
+    if (r->h_I) (void)hipHostFree(r->h_I);
+    s->cv_sched.notify_all();
+    return DBX_OK;
+}
+
+int dbx_server_stats_get(dbx_server *s, dbx_server_stats *out)
+{
+    if (!s || !out) return DBX_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->st;
+    return DBX_OK;
+}
+
+}  // extern "C"

@@ -152,6 +152,69 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
                         const double *q, int nq, double *I_out, uint64_t *n_out_of_range,
                         uint64_t *const *hist_out);
 
+/* ---- serving ---------------------------------------------------------------------------
+ * A stream of structure models of uneven size (refinement / multi-simulation style), served
+ * by continuous batching.  No reference counterpart: it replaces the one-process-per-model
+ * loop of Matlab/multisim.m:42-58 (cuDebye.exe per .deb).
+ *   - dbx_submit copies a request (host arrays) and returns at once.  Host threads tile and
+ *     plan it; it is admitted when its device memory fits the server's HBM budget (a request
+ *     larger than the whole budget is refused with DBX_ERR_BUDGET).
+ *   - Two worker lanes (each with its own stream) take the admitted requests in steps: one
+ *     pass-1 launch over all requests of the step, then one pass-2 contraction per q grid.
+ *     Requests that arrive while a step is on the GPU join the next step; the two lanes'
+ *     steps overlap on the device.
+ *   - Results equal dbx_intensity on each request alone (histograms bit-exact, I within the
+ *     pass-2 bound).  A request whose distances exceed its bins fails alone (DBX_ERR_RANGE).
+ *   - With DBX_REQ_KEEP the histograms and the per-pair structure sums S_p(q) stay resident
+ *     after completion: dbx_reweight recomputes I for new form-factor parameters by replaying
+ *     a captured hipGraph (prefactor, combine and their copies); dbx_release frees them.
+ *     Without it the request's device memory is freed at completion.
+ * Calls are thread-safe. */
+typedef struct dbx_server dbx_server;
+typedef struct dbx_server_config {
+    size_t hbm_budget;      /* device bytes for resident requests (0: 3/4 of free memory) */
+    int max_batch;          /* requests per step (0: 256) */
+    double max_step_pairs;  /* pair distances per step (0: 2e10) */
+    int prep_threads;       /* host threads for tiling and planning (0: 4) */
+    int pass2_mode;         /* DBX_PASS2_MFMA (default) or DBX_PASS2_FP64 */
+} dbx_server_config;
+typedef struct dbx_request {
+    const float *x, *y, *z;           /* positions in subgroup order (copied) */
+    const int64_t *group_n;           /* [n_groups] */
+    int n_groups;
+    const dbx_group_params *params;   /* [n_groups] (copied) */
+    double rstep;
+    int64_t n_bins;
+    int first_n;
+    const double *q;                  /* [nq] (copied) */
+    int nq;
+    int flags;                        /* DBX_REQ_KEEP */
+} dbx_request;
+enum { DBX_REQ_KEEP = 1 };
+enum { DBX_REQ_QUEUED = 0, DBX_REQ_RUNNING = 1, DBX_REQ_DONE = 2, DBX_REQ_FAILED = 3 };
+typedef struct dbx_server_stats {
+    uint64_t submitted, completed, failed, steps;
+    double pairs;                 /* pair distances histogrammed */
+    uint64_t resident_bytes, budget_bytes;
+    double max_step_requests;     /* most requests in one step */
+} dbx_server_stats;
+int dbx_server_open(int device, const dbx_server_config *cfg, dbx_server **out);
+int dbx_server_close(dbx_server *s);
+const char *dbx_server_last_error(dbx_server *s);
+int dbx_submit(dbx_server *s, const dbx_request *req, uint64_t *ticket);
+/* state: DBX_REQ_QUEUED (waiting for preparation or memory), _RUNNING, _DONE, _FAILED */
+int dbx_poll(dbx_server *s, uint64_t ticket, int *state);
+/* Blocks until the request ends; I_out[nq] (optional) receives I(q), latency_ms (optional)
+ * the time from submission to completion.  Returns the request's status. */
+int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms);
+/* I_out[nq] = intensity of a finished DBX_REQ_KEEP request for new params[n_groups]. */
+int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params, double *I_out);
+/* hist_out[P][n_bins] of a finished DBX_REQ_KEEP request (host). */
+int dbx_request_histogram(dbx_server *s, uint64_t ticket, uint64_t *hist_out);
+/* Frees a finished request (and its resident memory). */
+int dbx_release(dbx_server *s, uint64_t ticket);
+int dbx_server_stats_get(dbx_server *s, dbx_server_stats *out);
+
 /* ---- .deb driver ------------------------------------------------------------------------
  * The reference program: `cuDebye.exe file.deb` (cuDebye-v15.cu:599-1003).  Parses the
  * .deb, runs every subgroup pair, writes <path>I (hist 0 or 2) and <path>x<r>x<c>

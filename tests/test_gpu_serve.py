@@ -1,0 +1,205 @@
+"""Serving (dbx_server): continuous batching of a stream of models of uneven size.
+
+Every request's result must be the same alone and in the mix: histograms bit-exact against the
+same model run alone through dbx_intensity, I(q) within 1e-9 relative of that run (both use the
+matrix-core pass 2; only the FP64 order of the split sums differs) and within 1e-6 of the FP64
+oracle.  Re-weighting a resident request replays a captured graph; each replay is checked
+against the direct computation with the same form factors."""
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import cudebye_amd as cd
+from cudebye_amd import synth
+from oracle import oracle as O
+from tests.test_gpu_pass1 import _oracle_hist
+
+pytestmark = pytest.mark.gpu
+
+MIX_TOL = 1e-9
+ORACLE_TOL = 1e-6
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = cd.Context(0)
+    yield c
+    c.close()
+
+
+def _gp(p):
+    return cd.GroupParams.make(p["occ"], p["beq"], p["a"], p["b"], p["c"])
+
+
+def _oracle_I(m, H, params=None):
+    params = params or m.params
+
+    class Pm:
+        pass
+    Pm.group_n = m.group_n
+    Pm.occ = np.array([p["occ"] for p in params])
+    Pm.beq = np.array([p["beq"] for p in params])
+    Pm.a = np.array([p["a"] for p in params])
+    Pm.b = np.array([p["b"] for p in params])
+    Pm.c = np.array([p["c"] for p in params])
+    return O.intensity_from_histograms(H, Pm, m.q, m.rstep, first_n=m.first_n)
+
+
+def _models(n, seed, nqs=(64, 128, 96)):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        na = int(np.exp(rng.uniform(np.log(100), np.log(3000))))
+        S = int(rng.integers(1, 5))
+        out.append(synth.small_random(na, S, seed * 1000 + i, spread=float(rng.uniform(4.0, 15.0)),
+                                      nq=int(nqs[i % len(nqs)])))
+    return out
+
+
+def _alone(ctx, m):
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    I, H = ctx.intensity(mod, [_gp(p) for p in m.params], m.rstep, m.n_bins, m.q, first_n=m.first_n,
+                         with_hist=True)
+    mod.close()
+    return I, H
+
+
+def _submit(srv, m, keep=True, n_bins=None, params=None):
+    return srv.submit(m.x, m.y, m.z, m.group_n, [_gp(p) for p in (params or m.params)], m.rstep,
+                      n_bins or m.n_bins, m.q, first_n=m.first_n, keep=keep)
+
+
+def _rel(a, b):
+    return np.max(np.abs(a - b) / np.abs(b))
+
+
+def test_serve_results_same_alone_and_in_the_mix(ctx):
+    models = _models(16, 3)
+    alone = [_alone(ctx, m) for m in models]
+    with cd.Server(0) as srv:
+        tickets = [None] * len(models)
+
+        def late():
+            # arrivals while the first ones are on the GPU
+            for i in range(8, 16):
+                time.sleep(0.002)
+                tickets[i] = _submit(srv, models[i])
+
+        th = threading.Thread(target=late)
+        for i in range(8):
+            tickets[i] = _submit(srv, models[i])
+        th.start()
+        th.join()
+        worst = 0.0
+        for i, m in enumerate(models):
+            I, lat, rc = srv.wait(tickets[i])
+            assert rc == cd.DBX_OK and lat > 0
+            H = srv.histogram(tickets[i])
+            assert np.array_equal(H, alone[i][1]), f"request {i} histogram"
+            assert _rel(I, alone[i][0]) < MIX_TOL, f"request {i}"
+            if i % 4 == 0:
+                ref, _ = _oracle_hist(m)
+                assert np.array_equal(H, ref)
+                worst = max(worst, _rel(I, _oracle_I(m, ref)))
+            srv.release(tickets[i])
+        st = srv.stats()
+        print(f"steps {st['steps']} max step {st['max_step_requests']} worst vs oracle {worst:.2e}")
+        assert worst < ORACLE_TOL
+        assert st["completed"] == len(models) and st["resident_bytes"] == 0
+
+
+def test_serve_reweight_replays_match_direct(ctx):
+    m = _models(1, 7, nqs=(200,))[0]
+    rng = np.random.default_rng(11)
+    sets = [[synth.cromer_mann(rng) for _ in m.params] for _ in range(3)]
+    with cd.Server(0) as srv:
+        t = _submit(srv, m)
+        I0, _, _ = srv.wait(t)
+        H = srv.histogram(t)
+        mod = ctx.model(m.x, m.y, m.z, m.group_n)
+        for rep, k in enumerate([0, 1, 0, 2, 1]):
+            Ir = srv.reweight(t, [_gp(p) for p in sets[k]])
+            # eager work on the device between replays
+            Id = ctx.intensity(mod, [_gp(p) for p in sets[k]], m.rstep, m.n_bins, m.q, first_n=m.first_n)
+            assert _rel(Ir, Id) < MIX_TOL, f"replay {rep}"
+            assert _rel(Ir, _oracle_I(m, H, sets[k])) < ORACLE_TOL
+        assert _rel(srv.reweight(t, [_gp(p) for p in m.params]), I0) < 1e-12
+        srv.release(t)
+        with pytest.raises(cd.DebyeError):
+            srv.reweight(t, [_gp(p) for p in m.params])
+
+
+def test_serve_admission_waits_for_budget():
+    # five requests of nearly the same size; a budget that holds two of them resident
+    models = [synth.small_random(1500, 2, 500 + i, spread=10.0, nq=64) for i in range(5)]
+    sizes = []
+    with cd.Server(0) as srv:
+        for m in models:
+            t = _submit(srv, m)
+            srv.wait(t)
+            sizes.append(srv.stats()["resident_bytes"])
+            srv.release(t)
+    assert min(sizes) > 0 and max(sizes) < 1.2 * min(sizes)
+    budget = int(2.5 * max(sizes))
+    with cd.Server(0, hbm_budget=budget) as srv:
+        ts = [_submit(srv, m) for m in models]
+        srv.wait(ts[0])
+        srv.wait(ts[1])
+        time.sleep(0.3)
+        states = [srv.poll(t) for t in ts]
+        assert states[:2] == [cd.DBX_REQ_DONE] * 2
+        assert states[2:] == [cd.DBX_REQ_QUEUED] * 3, states
+        assert srv.stats()["resident_bytes"] <= budget
+        # each release admits the next waiting request
+        for k in range(2, 5):
+            srv.release(ts[k - 2])
+            _, _, rc = srv.wait(ts[k])
+            assert rc == cd.DBX_OK
+            assert srv.stats()["resident_bytes"] <= budget
+        assert srv.stats()["completed"] == len(models)
+
+
+def test_serve_request_over_budget_refused():
+    m = _models(1, 17)[0]
+    with cd.Server(0, hbm_budget=1 << 16) as srv:
+        with pytest.raises(cd.DebyeError) as e:
+            _submit(srv, m, n_bins=1 << 20)
+        assert e.value.code == cd.DBX_ERR_BUDGET
+
+
+def test_serve_bad_request_fails_alone(ctx):
+    models = _models(4, 19, nqs=(64,))
+    short = O.channels(5.0, models[1].rstep)
+    alone = [_alone(ctx, m) for m in models]
+    with cd.Server(0) as srv:
+        ts = [_submit(srv, m, n_bins=short if i == 1 else None) for i, m in enumerate(models)]
+        for i, t in enumerate(ts):
+            I, _, rc = srv.wait(t, allow=(cd.DBX_ERR_RANGE,))
+            if i == 1:
+                assert rc == cd.DBX_ERR_RANGE and srv.poll(t) == cd.DBX_REQ_FAILED
+            else:
+                assert rc == cd.DBX_OK
+                assert np.array_equal(srv.histogram(t), alone[i][1])
+                assert _rel(I, alone[i][0]) < MIX_TOL
+            srv.release(t)
+
+
+def test_serve_poisson_stream_small(ctx):
+    rng = np.random.default_rng(23)
+    models = _models(24, 29)
+    with cd.Server(0) as srv:
+        ts = []
+        for m in models:
+            time.sleep(rng.exponential(0.002))
+            ts.append(_submit(srv, m, keep=False))
+        lats = []
+        for t in ts:
+            I, lat, rc = srv.wait(t)
+            assert rc == cd.DBX_OK and np.all(np.isfinite(I))
+            lats.append(lat)
+            srv.release(t)
+        st = srv.stats()
+        print(f"p50 {np.percentile(lats, 50):.2f} ms p99 {np.percentile(lats, 99):.2f} ms steps {st['steps']}")
+        assert st["completed"] == len(models) and st["resident_bytes"] == 0
