@@ -132,6 +132,8 @@ def lib():
             "dbx_request_histogram": (i32, [vp, u64, vp]),
             "dbx_release": (i32, [vp, u64]),
             "dbx_server_stats_get": (i32, [vp, vp]),
+            "dbx_debug_set_profiling": (i32, [vp, i32]),
+            "dbx_debug_pass2_profile": (i32, [vp, P(f64), P(f64), P(f64), P(i32), P(i32)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -288,6 +290,19 @@ class Context:
         a, b = C.c_double(0), C.c_double(0)
         self._check(lib().dbx_last_kernel_times(self.handle, C.byref(a), C.byref(b)), "kernel_times")
         return a.value, b.value
+
+    def set_profiling(self, on=True):
+        """Events around the matrix-core contraction kernel and a count of its instructions."""
+        self._check(lib().dbx_debug_set_profiling(self.handle, int(on)), "set_profiling")
+
+    def pass2_profile(self):
+        """(kernel ms, issued BF16 matrix flops, useful flops, compacted bins, count digits) of
+        the last profiled matrix-core pass 2."""
+        ms, fl, us = C.c_double(), C.c_double(), C.c_double()
+        kc, dg = C.c_int(), C.c_int()
+        self._check(lib().dbx_debug_pass2_profile(self.handle, C.byref(ms), C.byref(fl), C.byref(us), C.byref(kc),
+                                                  C.byref(dg)), "pass2_profile")
+        return ms.value, fl.value, us.value, kc.value, dg.value
 
     def prefactor(self, ga: GroupParams, gb: GroupParams, q):
         q = np.ascontiguousarray(q, np.float64)

@@ -425,6 +425,8 @@ int dbx_close(dbx_ctx *ctx)
     for (int s = 0; s < 8; s++)
         if (ctx->pin_ptr[s]) (void)hipHostFree(ctx->pin_ptr[s]);
     if (ctx->counters) (void)hipFree(ctx->counters);
+    for (int i = 0; i < 2; i++)
+        if (ctx->ev_mf[i]) (void)hipEventDestroy(ctx->ev_mf[i]);
     if (ctx->counters_h) (void)hipHostFree(ctx->counters_h);
     for (int i = 0; i < 4; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
@@ -799,6 +801,35 @@ int dbx_debug_plan_host(const float *x, const float *y, const float *z, const in
 }
 
 // the pass-2 sine evaluated on the host (same source as the device code)
+int dbx_debug_set_profiling(dbx_ctx *ctx, int on)
+{
+    if (!ctx) return DBX_ERR_ARG;
+    return guarded(ctx, [&] {
+        if (on && !ctx->ev_mf[0]) {
+            DBX_HIP(hipEventCreate(&ctx->ev_mf[0]));
+            DBX_HIP(hipEventCreate(&ctx->ev_mf[1]));
+        }
+        ctx->profiling = on != 0;
+        ctx->mf_timed = false;
+    });
+}
+
+int dbx_debug_pass2_profile(dbx_ctx *ctx, double *kernel_ms, double *mfma_flops, double *useful_flops,
+                            int *n_bins_compacted, int *count_digits)
+{
+    if (!ctx) return DBX_ERR_ARG;
+    return guarded(ctx, [&] {
+        DBX_REQUIRE(ctx->mf_timed, DBX_ERR_STATE, "no profiled matrix-core pass 2 on this context");
+        float ms = 0.f;
+        DBX_HIP(hipEventElapsedTime(&ms, ctx->ev_mf[0], ctx->ev_mf[1]));
+        if (kernel_ms) *kernel_ms = ms;
+        if (mfma_flops) *mfma_flops = ctx->mf_flops;
+        if (useful_flops) *useful_flops = ctx->mf_useful;
+        if (n_bins_compacted) *n_bins_compacted = ctx->last_p2_bins;
+        if (count_digits) *count_digits = ctx->mf_digits;
+    });
+}
+
 int dbx_debug_sin_host(const double *x, int n, double *out)
 {
     if (!x || !out || n < 0) return DBX_ERR_ARG;

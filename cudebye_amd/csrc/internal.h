@@ -172,6 +172,12 @@ struct dbx_ctx {
     bool p2_timed = false;
     int pass2_mode = DBX_PASS2_MFMA;  // DBX_PASS2_MFMA or DBX_PASS2_FP64
     int last_p2_bins = 0;             // compacted bins of the last pass 2
+    // profiling of the matrix-core contraction (dbx_debug_set_profiling)
+    bool profiling = false;
+    hipEvent_t ev_mf[2] = {};
+    bool mf_timed = false;
+    double mf_flops = 0, mf_useful = 0;
+    int mf_digits = 0;
 };
 
 namespace dbx {

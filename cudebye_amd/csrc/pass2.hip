@@ -799,8 +799,27 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
         const int nblocks = rows_pad / 16;
         hipLaunchKernelGGL(p2_kext_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, kz, Kc, d_rows,
                            n_rows, 16, nblocks, kext);
+        if (ctx->profiling) DBX_HIP(hipEventRecord(ctx->ev_mf[0], ctx->stream));
         mf_launch(sh, I, dim3((unsigned)qgroups, (unsigned)nsplit, (unsigned)rgroups), ctx->stream, rtab,
                   rtab + Kc_pad, Kc_pad, kper, kext, cdig, rows_pad, d_q, nq, n_rows, Spart);
+        if (ctx->profiling) {
+            DBX_HIP(hipEventRecord(ctx->ev_mf[1], ctx->stream));
+            // issued matrix instructions: per workgroup, its steps x waves x RB x I x 5
+            std::vector<int> hk(nblocks);
+            DBX_HIP(hipMemcpyAsync(hk.data(), kext, sizeof(int) * nblocks, hipMemcpyDeviceToHost, ctx->stream));
+            DBX_HIP(hipStreamSynchronize(ctx->stream));
+            double steps = 0, useful = 0;
+            for (int z = 0; z < rgroups; z++) {
+                const int kx = std::min(Kc_pad, (hk[z * (sh.rows() / 16)] + 31) & ~31);
+                steps += (double)(kx / 32);
+            }
+            for (int r = 0; r < n_rows; r++) useful += (double)hk[r / 16];
+            const double waves = (double)qgroups * sh.WQ * sh.WR;
+            ctx->mf_flops = steps * waves * sh.RB * I * 5 * 16384.0;
+            ctx->mf_useful = useful * (double)nq * 2.0;
+            ctx->mf_digits = I;
+            ctx->mf_timed = true;
+        }
         hipLaunchKernelGGL(p2_reduce_splits_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
                            dim3(256), 0, ctx->stream, Spart, nsplit, 0, n_rows, nq, d_S);
     } else {
