@@ -133,6 +133,7 @@ def lib():
             "dbx_release": (i32, [vp, u64]),
             "dbx_server_stats_get": (i32, [vp, vp]),
             "dbx_debug_set_profiling": (i32, [vp, i32]),
+            "dbx_selftest_round": (i32, [C.c_uint32, C.c_uint32, P(u64)]),
             "dbx_debug_pass2_profile": (i32, [vp, P(f64), P(f64), P(f64), P(i32), P(i32)]),
         }
         for name, (res, args) in sig.items():
@@ -179,6 +180,15 @@ def run_deb(path, crlf=False, quiet=True, mfma=False) -> int:
     """Runs a .deb file like `cuDebye.exe path` (writes path+'I' and/or path+'x<r>x<c>')."""
     flags = (DBX_RUN_CRLF if crlf else 0) | (DBX_RUN_QUIET if quiet else 0) | (4 if mfma else 0)
     return int(lib().dbx_run_deb(os.fspath(path).encode(), flags))
+
+
+def selftest_round(lo=0, hi=0x4F000000):
+    """Count of floats in [lo, hi) where the hardware round-half-up convert differs from roundf."""
+    n = C.c_uint64(0)
+    rc = lib().dbx_selftest_round(lo, hi, C.byref(n))
+    if rc != DBX_OK:
+        raise DebyeError(rc, "selftest_round")
+    return int(n.value)
 
 
 def selftest_sqrt(lo, hi):

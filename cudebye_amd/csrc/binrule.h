@@ -32,13 +32,24 @@ __device__ __forceinline__ int round_pos(float v)
     return (int)v + (__builtin_amdgcn_fractf(v) >= 0.5f ? 1 : 0);
 }
 
-// bin of one pair: the reference's unfused FP32 chain
+// The same rounding by the hardware's convert-to-nearest-with-ties-up instruction
+// (v_cvt_rpi_i32_f32: floor(v + 0.5) as an integer, one instruction).  Used only where
+// dbx_selftest_round has shown it equal to round_pos for every float in the range.
+__device__ __forceinline__ int round_rpi(float v)
+{
+    int r;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+
+// bin of one pair: the reference's unfused FP32 chain; sqrt_cr(s) * rp is below 2^31 for
+// every pair the histograms count (larger values land past the last bin either way)
 __device__ __forceinline__ int pair_bin(float ax, float ay, float az, float bx, float by, float bz,
                                         float rp)
 {
     const float dx = bx - ax, dy = by - ay, dz = bz - az;
     const float s = (dx * dx + dy * dy) + dz * dz;
-    return round_pos(sqrt_cr(s) * rp);
+    return round_rpi(sqrt_cr(s) * rp);
 }
 
 }  // namespace dbx

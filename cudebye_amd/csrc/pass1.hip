@@ -33,11 +33,6 @@
 
 namespace dbx {
 
-__device__ __forceinline__ float rd_lane(float v, int lane)
-{
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
 template <bool CHECKED>
 __device__ __forceinline__ void count_pair(uint32_t *h, int k, int k_lo, int64_t n_bins,
                                            unsigned long long *hrow, uint32_t &miss, uint32_t &oor)
@@ -187,17 +182,9 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             const bool in_shell = b_ok && d2c >= e_lo && d2c < e_hi;
             uint64_t bm = __ballot(in_shell);
             const bool diag = job.same && (j2 == ta.parent);  // B tile holds the A tile's atoms
-            if (R == 1) {
-                while (bm) {
-                    const int m = __builtin_ctzll(bm);
-                    bm &= bm - 1;
-                    const float bx = rd_lane(pb.x, m), by = rd_lane(pb.y, m), bz = rd_lane(pb.z, m);
-                    if (a_ok && (!diag || idx_a <= tb.start + m))
-                        count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, bx, by, bz, rp), k_lo, n_bins,
-                                            hrow, miss, oor);
-                }
-            } else if (bm) {
-                // stage the shell's B atoms (with their atom index) and take R per step
+            if (bm) {
+                // stage the shell's B atoms (with their atom index) in the wave's LDS area; the
+                // lanes then read them back as broadcasts, R per step
                 const int nb = __popcll(bm);
                 if (in_shell) {
                     const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32),
@@ -207,12 +194,32 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                for (int t = 0; t < nb; t += R) {
-                    const int src = t + slot;
-                    const float4 b = stage[wave][min(src, nb - 1)];
-                    if (a_ok && src < nb && (!diag || idx_a <= __float_as_int(b.w)))
-                        count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
-                                            hrow, miss, oor);
+                if (R == 1 && !diag) {
+                    if (a_ok) {
+                        // two B atoms per iteration: two independent chains in flight
+                        int t = 0;
+                        for (; t + 1 < nb; t += 2) {
+                            const float4 b0 = stage[wave][t];
+                            const float4 b1 = stage[wave][t + 1];
+                            const int k0 = pair_bin(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
+                            const int k1 = pair_bin(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
+                            count_pair<CHECKED>(h, k0, k_lo, n_bins, hrow, miss, oor);
+                            count_pair<CHECKED>(h, k1, k_lo, n_bins, hrow, miss, oor);
+                        }
+                        if (t < nb) {
+                            const float4 b = stage[wave][t];
+                            count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
+                                                hrow, miss, oor);
+                        }
+                    }
+                } else {
+                    for (int t = 0; t < nb; t += R) {
+                        const int src = t + slot;
+                        const float4 b = stage[wave][min(src, nb - 1)];
+                        if (a_ok && src < nb && (!diag || idx_a <= __float_as_int(b.w)))
+                            count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
+                                                hrow, miss, oor);
+                    }
                 }
                 // the stage is rewritten by the next candidate only after every lane's reads
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -33,6 +33,19 @@ __global__ void st_sqrt_kernel(uint32_t lo, uint32_t hi, unsigned long long *bad
     if (nmis) atomicAdd(&bad[1], nmis);
 }
 
+// n_bad[0]: floats v in [lo, hi) (bit patterns, v >= 0) where round_rpi(v) != round_pos(v)
+__global__ void st_round_kernel(uint32_t lo, uint32_t hi, unsigned long long *bad)
+{
+    unsigned long long nbad = 0;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t u = (uint64_t)lo + blockIdx.x * blockDim.x + threadIdx.x; u < hi; u += stride) {
+        const float v = __uint_as_float((uint32_t)u);
+        if (!(v >= 0.f) || !(v < 2147483520.f)) continue;
+        if (round_rpi(v) != round_pos(v)) nbad++;
+    }
+    if (nbad) atomicAdd(&bad[0], nbad);
+}
+
 __global__ void st_bins_kernel(const float *__restrict__ a, const float *__restrict__ b, int n,
                                float rp, int *__restrict__ out)
 {
@@ -104,4 +117,21 @@ extern "C" int dbx_selftest_bins(const float *a, const float *b, int n, float rp
     (void)hipFree(db);
     (void)hipFree(dout);
     return e == hipSuccess ? DBX_OK : DBX_ERR_HIP;
+}
+
+// Counts the floats v in [lo, hi) (bit patterns; 0 <= v < 2^31) where the hardware's
+// round-half-up conversion differs from the reference's roundf (half away from zero).
+extern "C" int dbx_selftest_round(uint32_t lo, uint32_t hi, uint64_t *n_bad)
+{
+    if (!n_bad || hi < lo) return DBX_ERR_ARG;
+    unsigned long long *d = nullptr;
+    if (hipMalloc(&d, sizeof(*d)) != hipSuccess) return DBX_ERR_HIP;
+    (void)hipMemset(d, 0, sizeof(*d));
+    hipLaunchKernelGGL(st_round_kernel, dim3(8192), dim3(256), 0, 0, lo, hi, d);
+    unsigned long long h = 0;
+    hipError_t e = hipMemcpy(&h, d, sizeof(h), hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (e != hipSuccess) return DBX_ERR_HIP;
+    *n_bad = h;
+    return DBX_OK;
 }

@@ -34,6 +34,11 @@ int dbx_debug_plan_host(const float *x, const float *y, const float *z, const in
                         int32_t *alist, int cap_alist, int *n_tiles, int *n_jobs, int *n_edge2,
                         int *n_alist);
 
+/* Floats v in [lo, hi) (bit patterns, 0 <= v < 2^31) where the device's one-instruction
+ * round-half-up conversion (v_cvt_rpi_i32_f32) differs from roundf's half-away-from-zero
+ * rounding of the bin rule (cuDebye-v15.cu:104); n_bad[0] receives the count. */
+int dbx_selftest_round(uint32_t lo, uint32_t hi, uint64_t *n_bad);
+
 /* The pass-2 FP64 sine (dbx_selftest_sin's device function) evaluated on the host. */
 int dbx_debug_sin_host(const double *x, int n, double *out);
 
