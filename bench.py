@@ -11,16 +11,20 @@ sums.  `value` is G pair-distances/s, a pair-distance counted as the reference c
 (N(N+1)/2 per model, self pairs included).
 
 `roofline` is for the pass-1 histogram kernel (9 FP32 flops per pair against the FP32 vector
-peak), timed with HIP events on the library's stream; `pass2_roofline` for the matrix-core
-contraction (issued BF16 matrix flops against the dense BF16 peak).  `cpu_baseline` is the
-OpenMP oracle (a port of the reference path) on the host cores of the same machine.
+peak), timed with HIP events on the library's stream, beside its LDS-atomic rate (one per pair)
+against the device's measured LDS-atomic peak.  `pass2_roofline` is for the matrix-core
+contraction: algorithmic flops (2 nQ per nonzero bin of every row, the contraction counted once
+whatever the digit count) against the dense BF16 peak, with the issued matrix flops (MFMA
+utilisation) beside it.  `cpu_baseline` is the OpenMP oracle (a port of the reference path) on
+the host cores of the same machine.
 
 `batched`: the 256-model stream (FCC/HCP spheres and ellipsoids with stacking faults, 1k-60k
 atoms, 2 species) through the continuous-batching server, all requests submitted at once,
 against the plain path (one model per call: model creation + dbx_intensity, sequential).
 `serving`: the mixed stream (500-200k atoms, 1-4 species, 2048/4096/8192 Q) with Poisson
-arrivals at a fixed rate, server against the plain path fed the same arrival times: models/s
-and p50/p99 latency (submission to result).
+arrivals at a fixed rate (80% of the server's capacity measured on the same requests), server
+against the plain path fed the same arrival times: models/s and p50/p99 latency (submission to
+result).
 """
 import argparse
 import json
@@ -35,8 +39,9 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak
 BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 matrix peak
 FLOPS_PER_PAIR = 9         # 3 sub, 3 mul, 2 add, 1 scale-mul
-SERVE_RATE = 250.0         # Poisson arrivals per second of the mixed stream
-SERVE_N = 150              # requests in the mixed stream
+SERVE_LOAD = 0.8           # Poisson arrival rate of the mixed stream, as a fraction of the
+                           # server's measured capacity on the same requests (all at once)
+SERVE_N = 600              # requests in the mixed stream
 
 
 def _gp(cd, p):
@@ -66,12 +71,15 @@ def primary(args, cd, synth, np):
         mf_ms, mf_flops, mf_useful, kc, digits = ctx.pass2_profile()
         ctx.set_profiling(False)
         mod.close()
+    lds_peak = cd.lds_atomic_rate()
 
     ms_per_step = elapsed / args.steps * 1e3
     value = pairs * args.steps / elapsed / 1e9
     k_ms = float(np.mean(p1_ms))
     achieved = FLOPS_PER_PAIR * pairs / (k_ms * 1e-3) / 1e12
-    mf_ach = mf_flops / (mf_ms * 1e-3) / 1e12
+    mf_ach = mf_useful / (mf_ms * 1e-3) / 1e12
+    mf_issued = mf_flops / (mf_ms * 1e-3) / 1e12
+    p2_mean = float(np.mean(p2_ms))
     out = {
         "metric": "G pair-distances/s (end to end: pass 1 histograms + pass 2 I(Q)); "
                   "batched models/s + p50/p99 latency",
@@ -98,7 +106,12 @@ def primary(args, cd, synth, np):
         },
         "pass1_kernel_ms": round(k_ms, 4),
         "pass1_gpairs_per_s": round(pairs / (k_ms * 1e-3) / 1e9, 2),
-        "pass2_ms": round(float(np.mean(p2_ms)), 4),
+        "pass1_lds_atomics_per_s": round(pairs / (k_ms * 1e-3), 0),
+        "lds_atomic_peak_per_s": round(lds_peak, 0),
+        "pass1_lds_atomic_frac": round(pairs / (k_ms * 1e-3) / lds_peak, 4),
+        "pass2_ms": round(p2_mean, 4),
+        "pass2_gterms_per_s": round(mf_useful / 2 / (p2_mean * 1e-3) / 1e9, 2),
+        "sinc_gelements_per_s": round(kc * len(m.q) / (mf_ms * 1e-3) / 1e9, 2),
         "pass1_jobs": n_jobs,
         "pass1_window_misses": misses,
         "roofline": {
@@ -115,12 +128,16 @@ def primary(args, cd, synth, np):
             "bound": "mfma",
             "kernel": "p2_debye_mfma_kernel",
             "kernel_ms": round(mf_ms, 4),
-            "achieved": round(mf_ach, 2),
+            "achieved": round(mf_ach, 3),
             "peak": BF16_PEAK_TFLOPS,
             "unit": "TFLOP/s",
-            "frac": round(mf_ach / BF16_PEAK_TFLOPS, 4),
-            "unit_figure": f"issued v_mfma_f32_16x16x32_bf16 flops ({mf_flops:.4g}; useful FP64-equivalent "
-                           f"multiply-adds {mf_useful:.4g} flops), {kc} compacted bins, {digits} count digits",
+            "frac": round(mf_ach / BF16_PEAK_TFLOPS, 5),
+            "unit_figure": f"2 nQ flops per nonzero bin of every row ({mf_useful:.4g} per launch), "
+                           f"{kc} compacted bins, {digits} count digits",
+            "mfma_issued_tflops": round(mf_issued, 2),
+            "mfma_issued_frac": round(mf_issued / BF16_PEAK_TFLOPS, 4),
+            "note": "one model has 3 rows: the contraction is bound by generating the sinc table "
+                    "(FP64 VALU), not by the matrix cores; see the batched stream for many rows",
         },
     }
     return out, m, I, H, pairs
@@ -129,6 +146,30 @@ def primary(args, cd, synth, np):
 def stream_c4(cd, synth, np):
     models = synth.batch(256)
     params = [[_gp(cd, p) for p in m.params] for m in models]
+    # the batched contraction of all 256 models (768 rows) in one dbx_batch_intensity call
+    with cd.Context(0) as ctx:
+        items = [(ctx.model(m.x, m.y, m.z, m.group_n), pa, m.n_bins, m.first_n) for m, pa in zip(models, params)]
+        ctx.batch_intensity(items, models[0].rstep, models[0].q)
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        ctx.batch_intensity(items, models[0].rstep, models[0].q)
+        t_batch = time.perf_counter() - t0
+        p1_b, p2_b = ctx.kernel_times()
+        mf_ms, mf_flops, mf_useful, kc, digits = ctx.pass2_profile()
+        for it in items:
+            it[0].close()
+    batch_call = {
+        "models_per_s": round(len(models) / t_batch, 1),
+        "pass1_kernel_ms": round(p1_b, 3),
+        "pass2_ms": round(p2_b, 3),
+        "pass2_roofline": {"bound": "mfma", "kernel": "p2_debye_mfma_kernel", "kernel_ms": round(mf_ms, 3),
+                           "achieved": round(mf_useful / (mf_ms * 1e-3) / 1e12, 2), "peak": BF16_PEAK_TFLOPS,
+                           "unit": "TFLOP/s", "frac": round(mf_useful / (mf_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+                           "mfma_issued_tflops": round(mf_flops / (mf_ms * 1e-3) / 1e12, 1),
+                           "mfma_issued_frac": round(mf_flops / (mf_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+                           "unit_figure": f"2 nQ flops per nonzero bin of every row ({mf_useful:.4g}), "
+                                          f"768 rows, {kc} compacted bins, {digits} count digits"},
+    }
     # plain: one model per call (model creation + dbx_intensity), sequential
     with cd.Context(0) as ctx:
         m0 = models[0]
@@ -169,14 +210,32 @@ def stream_c4(cd, synth, np):
         "server_steps": int(st["steps"]),
         "max_rel_I_server_vs_plain": rel,
         "plain": "per model: dbx_model_create (host tiling + upload) + dbx_intensity, one at a time",
+        "batch_call": batch_call,
     }
+
+
+def _server_burst(cd, models, params):
+    with cd.Server(0) as srv:
+        m0 = models[0]
+        t = srv.submit(m0.x, m0.y, m0.z, m0.group_n, params[0], m0.rstep, m0.n_bins, m0.q, first_n=m0.first_n)
+        srv.wait(t)
+        srv.release(t)
+        t0 = time.perf_counter()
+        ts = [srv.submit(m.x, m.y, m.z, m.group_n, pa, m.rstep, m.n_bins, m.q, first_n=m.first_n)
+              for m, pa in zip(models, params)]
+        for t in ts:
+            srv.wait(t)
+            srv.release(t)
+        return len(models) / (time.perf_counter() - t0)
 
 
 def stream_c5(cd, synth, np):
     models = [synth.serve_request(i) for i in range(SERVE_N)]
     params = [[_gp(cd, p) for p in m.params] for m in models]
+    capacity = _server_burst(cd, models, params)
+    rate = SERVE_LOAD * capacity
     rng = np.random.default_rng(5150)
-    arrivals = np.cumsum(rng.exponential(1.0 / SERVE_RATE, len(models)))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(models)))
     # plain: a FIFO worker serving one model per call at the same arrival times
     with cd.Context(0) as ctx:
         m0 = models[0]
@@ -220,7 +279,8 @@ def stream_c5(cd, synth, np):
     return {
         "workload": f"{n} seeded requests, 500-200k atoms log-uniform, 1-4 species, FCC/HCP spheres, "
                     "2048/4096/8192 Q, bin 0.001 A",
-        "arrival_rate_per_s": SERVE_RATE,
+        "server_capacity_per_s": round(capacity, 1),
+        "arrival_rate_per_s": round(rate, 1),
         "pairs": float(sum(m.pair_count for m in models)),
         "server": {"models_per_s": round(n / t_srv, 1),
                    "p50_ms": round(float(np.percentile(lat_srv, 50)), 2),
@@ -238,15 +298,18 @@ def cpu_leg(out, m, I, H, pairs, np):
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     st = np.concatenate([[0], np.cumsum(m.group_n)[:-1]])
     rp = O.rp_of(m.rstep)
-    t = time.perf_counter()
-    Href = np.zeros_like(H)
-    p = 0
-    for r in range(len(m.group_n)):
-        for c in range(r, len(m.group_n)):
-            O.histogram(m.x, m.y, m.z, st[r], m.group_n[r], st[c], m.group_n[c], r == c, rp,
-                        m.n_bins, nthreads=nthreads, out=Href[p])
-            p += 1
-    t1 = time.perf_counter() - t
+    times = []
+    for rep in range(4):  # one warm-up, then the median of three
+        t = time.perf_counter()
+        Href = np.zeros_like(H)
+        p = 0
+        for r in range(len(m.group_n)):
+            for c in range(r, len(m.group_n)):
+                O.histogram(m.x, m.y, m.z, st[r], m.group_n[r], st[c], m.group_n[c], r == c, rp,
+                            m.n_bins, nthreads=nthreads, out=Href[p])
+                p += 1
+        times.append(time.perf_counter() - t)
+    t1 = float(np.median(times[1:]))
 
     class Pm:
         group_n = m.group_n
@@ -267,8 +330,8 @@ def cpu_leg(out, m, I, H, pairs, np):
         "unit": "G pair-distances/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": f"full pass 1 of the same model ({t1:.2f} s); pass 2 timed on every 8th of "
-                  f"the 4096 Q points and scaled x8 ({t2:.2f} s est.)",
+        "sample": f"full pass 1 of the same model (median of 3 after a warm-up: {t1:.2f} s); pass 2 "
+                  f"timed on every 8th of the 4096 Q points and scaled x8 ({t2:.2f} s est.)",
         "cpu": _cpu_model(),
     }
     out["parity"] = {

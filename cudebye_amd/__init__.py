@@ -29,6 +29,8 @@ ERRORS = {-1: "DBX_ERR_ARG", -2: "DBX_ERR_HIP", -3: "DBX_ERR_IO", -4: "DBX_ERR_P
 DBX_ERR_RANGE = -5
 DBX_RUN_CRLF = 1
 DBX_RUN_QUIET = 2
+DBX_RUN_MFMA = 4
+DBX_RUN_FROM_DEBX = 8
 DBX_PASS2_MFMA = 0
 DBX_PASS2_FP64 = 1
 
@@ -115,9 +117,11 @@ def lib():
             "dbx_debye": (i32, [vp, vp, i32, i32, i64, f64, i32, vp, i32, vp, vp, vp]),
             "dbx_intensity": (i32, [vp, vp, vp, f64, i64, i32, vp, i32, vp, vp]),
             "dbx_run_deb": (i32, [C.c_char_p, i32]),
+            "dbx_run_folder": (i32, [C.c_char_p, i32]),
+            "dbx_read_debx": (i32, [C.c_char_p, f64, i64, vp, P(i64)]),
             "dbx_selftest_sqrt": (i32, [C.c_uint32, C.c_uint32, P(u64)]),
             "dbx_selftest_bins": (i32, [vp, vp, i32, f32, vp]),
-            "dbx_last_pass1_stats": (i32, [vp, P(i64), P(u64)]),
+            "dbx_last_pass1_stats": (i32, [vp, P(i64), P(u64), P(u64)]),
             "dbx_last_kernel_times": (i32, [vp, P(f64), P(f64)]),
             "dbx_set_pass2_mode": (i32, [vp, i32]),
             "dbx_selftest_sin": (i32, [vp, i32, vp]),
@@ -134,6 +138,7 @@ def lib():
             "dbx_server_stats_get": (i32, [vp, vp]),
             "dbx_debug_set_profiling": (i32, [vp, i32]),
             "dbx_selftest_round": (i32, [C.c_uint32, C.c_uint32, P(u64)]),
+            "dbx_selftest_lds_atomic_rate": (i32, [P(f64)]),
             "dbx_debug_pass2_profile": (i32, [vp, P(f64), P(f64), P(f64), P(i32), P(i32)]),
         }
         for name, (res, args) in sig.items():
@@ -151,7 +156,7 @@ EXPORTS = ("dbx_open", "dbx_close", "dbx_last_error", "dbx_stream", "dbx_q_from_
            "dbx_last_kernel_times", "dbx_set_pass2_mode", "dbx_selftest_sin",
            "dbx_batch_intensity", "dbx_server_open", "dbx_server_close", "dbx_server_last_error",
            "dbx_submit", "dbx_poll", "dbx_wait", "dbx_reweight", "dbx_request_histogram",
-           "dbx_release", "dbx_server_stats_get")
+           "dbx_release", "dbx_server_stats_get", "dbx_read_debx", "dbx_run_folder")
 
 
 def _p(a):
@@ -176,10 +181,37 @@ def channels(rmax, rstep) -> int:
     return n
 
 
-def run_deb(path, crlf=False, quiet=True, mfma=False) -> int:
-    """Runs a .deb file like `cuDebye.exe path` (writes path+'I' and/or path+'x<r>x<c>')."""
-    flags = (DBX_RUN_CRLF if crlf else 0) | (DBX_RUN_QUIET if quiet else 0) | (4 if mfma else 0)
+def run_deb(path, crlf=False, quiet=True, mfma=False, from_debx=False) -> int:
+    """Runs a .deb file like `cuDebye.exe path` (writes path+'I' and/or path+'x<r>x<c>').
+    from_debx: pass 2 only, from the path+'x<r>x<c>' histograms a previous run wrote."""
+    flags = ((DBX_RUN_CRLF if crlf else 0) | (DBX_RUN_QUIET if quiet else 0) | (DBX_RUN_MFMA if mfma else 0)
+             | (DBX_RUN_FROM_DEBX if from_debx else 0))
     return int(lib().dbx_run_deb(os.fspath(path).encode(), flags))
+
+
+def run_folder(folder, crlf=False, quiet=True, mfma=False) -> int:
+    """Every .deb in `folder` without a .debI, through one server (Matlab/multisim.m)."""
+    flags = (DBX_RUN_CRLF if crlf else 0) | (DBX_RUN_QUIET if quiet else 0) | (DBX_RUN_MFMA if mfma else 0)
+    return int(lib().dbx_run_folder(os.fspath(folder).encode(), flags))
+
+
+def read_debx(path, rstep, n_bins):
+    """uint64[n_bins] histogram of one .debx file."""
+    h = np.zeros(int(n_bins), np.uint64)
+    nnz = C.c_int64(0)
+    rc = lib().dbx_read_debx(os.fspath(path).encode(), rstep, int(n_bins), _p(h), C.byref(nnz))
+    if rc != DBX_OK:
+        raise DebyeError(rc, f"cannot read {path}")
+    return h
+
+
+def lds_atomic_rate():
+    """Measured LDS atomic throughput of device 0 (ds_add_u32 per second, conflict-free)."""
+    v = C.c_double(0)
+    rc = lib().dbx_selftest_lds_atomic_rate(C.byref(v))
+    if rc != DBX_OK:
+        raise DebyeError(rc, "lds_atomic_rate")
+    return v.value
 
 
 def selftest_round(lo=0, hi=0x4F000000):
@@ -286,9 +318,13 @@ class Context:
                     allow=(DBX_ERR_RANGE,) if return_oor else ())
         return (out, int(oor.value)) if return_oor else out
 
-    def pass1_stats(self):
-        nj, miss = C.c_int64(0), C.c_uint64(0)
-        lib().dbx_last_pass1_stats(self.handle, C.byref(nj), C.byref(miss))
+    def pass1_stats(self, with_retried=False):
+        """(work items launched, pairs outside their LDS window[, exact-window items re-run by the
+        checked kernel]) of the last pass 1."""
+        nj, miss, rt = C.c_int64(0), C.c_uint64(0), C.c_uint64(0)
+        lib().dbx_last_pass1_stats(self.handle, C.byref(nj), C.byref(miss), C.byref(rt))
+        if with_retried:
+            return int(nj.value), int(miss.value), int(rt.value)
         return int(nj.value), int(miss.value)
 
     def set_pass2_mode(self, mode):

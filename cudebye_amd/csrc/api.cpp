@@ -501,11 +501,12 @@ int dbx_set_pass2_mode(dbx_ctx *ctx, int mode)
     return DBX_OK;
 }
 
-int dbx_last_pass1_stats(const dbx_ctx *ctx, int64_t *n_jobs, uint64_t *window_misses)
+int dbx_last_pass1_stats(const dbx_ctx *ctx, int64_t *n_jobs, uint64_t *window_misses, uint64_t *retried)
 {
     if (!ctx) return DBX_ERR_ARG;
     if (n_jobs) *n_jobs = ctx->last_p1_jobs;
     if (window_misses) *window_misses = ctx->last_p1_misses;
+    if (retried) *retried = ctx->last_p1_retried;
     return DBX_OK;
 }
 

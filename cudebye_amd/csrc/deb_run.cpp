@@ -1,9 +1,15 @@
 // The .deb driver: the reference program `cuDebye.exe file.deb` (cuDebye-v15.cu:599-1003)
 // on top of the C ABI.  Parses the .deb (cuDebye-v15.cu:624-833), runs every subgroup pair
 // on the device, prints the reference's sanity checks (cuDebye-v15.cu:360-410) and writes
-// <path>x<r>x<c> (cuDebye-v15.cu:413-430) and <path>I (cuDebye-v15.cu:973-998).
+// <path>x<r>x<c> (cuDebye-v15.cu:413-430) and <path>I (cuDebye-v15.cu:973-998).  Also the
+// reader of those .debx files (pass 2 again from written histograms) and the folder run of
+// Matlab/multisim.m on the server.
 #include <hip/hip_runtime.h>
 
+#include <dirent.h>
+#include <sys/stat.h>
+
+#include <algorithm>
 #include <chrono>
 #include <cstdarg>
 #include <cmath>
@@ -152,70 +158,12 @@ double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-}  // namespace
-
-extern "C" int dbx_run_deb(const char *path, int flags)
+// per-pair checks (cuDebye-v15.cu:360-410), .debx files (cuDebye-v15.cu:413-430, unless the
+// histograms were read from them) and the .debI file (cuDebye-v15.cu:973-998)
+int write_outputs(const char *path, const DebFile &d, const std::vector<double> &tt, const uint64_t *H,
+                  int64_t channels, const std::vector<double> &I, bool write_debx, const char *eol, bool quiet)
 {
-    if (!path) return DBX_ERR_ARG;
-    const bool quiet = flags & DBX_RUN_QUIET;
-    const char *eol = (flags & DBX_RUN_CRLF) ? "\r\n" : "\n";
-    double t0 = now_s();
-    DebFile d;
-    std::string err;
-    int rc = parse_deb(path, d, err);
-    if (rc != DBX_OK) {
-        std::fprintf(stderr, "ERROR: %s\n", err.c_str());
-        return rc;
-    }
-    const int nTT = dbx_q_from_2theta(d.lambda, d.tt_min, d.tt_max, d.tt_step, nullptr, nullptr, 0);
-    const int64_t channels = dbx_channels(d.rmax, d.rstep);
-    if (nTT < 1 || channels < 2) {
-        std::fprintf(stderr, "ERROR: invalid 2theta or distance grid in the header\n");
-        return DBX_ERR_ARG;
-    }
-    std::vector<double> tt(nTT), K(nTT);
-    dbx_q_from_2theta(d.lambda, d.tt_min, d.tt_max, d.tt_step, tt.data(), K.data(), nTT);
-    logf(quiet, "Loaded %s: %lld atoms, %d subgroups, %d 2theta values (K %g..%g), %lld channels, "
-        "hist=%d (%.3f s)\n",
-        path, (long long)d.x.size(), d.S, nTT, K[0], K[nTT - 1], (long long)channels, d.hist,
-        now_s() - t0);
-
-    dbx_ctx *ctx = nullptr;
-    rc = dbx_open(d.device, 0, &ctx);
-    if (rc != DBX_OK) {
-        std::fprintf(stderr, "ERROR: cannot open device %d\n", d.device);
-        return rc;
-    }
-    // the reference evaluates the Debye sum in FP64; the driver does too unless asked for the
-    // matrix-core path (which rounds sinc to 2^-32 and may move a printed 6th digit)
-    dbx_set_pass2_mode(ctx, (flags & DBX_RUN_MFMA) ? DBX_PASS2_MFMA : DBX_PASS2_FP64);
-    dbx_model *m = nullptr;
-    double t1 = now_s();
-    rc = dbx_model_create(ctx, d.x.data(), d.y.data(), d.z.data(), d.n.data(), d.S, &m);
-    if (rc != DBX_OK) {
-        std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
-        dbx_close(ctx);
-        return rc;
-    }
-    const int P = d.S * (d.S + 1) / 2;
-    std::vector<uint64_t> H((size_t)P * channels);
-    std::vector<double> I(nTT, 0.0);
-    const int first_n = d.block > 0 ? d.block : 1;
-    if (d.hist != 1)
-        rc = dbx_intensity(ctx, m, d.gp.data(), d.rstep, channels, first_n, K.data(), nTT, I.data(),
-                           H.data());
-    else
-        rc = dbx_histogram(ctx, m, d.rstep, channels, H.data(), 0, nullptr);
-    const double t_calc = now_s() - t1;
-    if (rc != DBX_OK) {
-        std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
-        if (rc != DBX_ERR_RANGE) {
-            dbx_model_destroy(ctx, m);
-            dbx_close(ctx);
-            return rc;
-        }
-    }
-    // per-pair checks (cuDebye-v15.cu:360-410) and .debx files (cuDebye-v15.cu:413-430)
+    int rc = DBX_OK;
     int p = 0;
     for (int row = 0; row < d.S; row++) {
         for (int col = row; col < d.S; col++, p++) {
@@ -233,10 +181,10 @@ extern "C" int dbx_run_deb(const char *path, int flags)
             const uint64_t na = d.n[row], nb = d.n[col];
             const uint64_t expect = row == col ? (na * na - na) / 2 + na : na * nb;
             logf(quiet, ">>>> Subgroups %d and %d: %lld unique distances, %llu observed, %llu expected\n", row,
-                col, (long long)nnz, (unsigned long long)sum, (unsigned long long)expect);
+                 col, (long long)nnz, (unsigned long long)sum, (unsigned long long)expect);
             if (sum != expect)
-                logf(quiet, "WARNING: %llu Distances Expected and %llu Observed!!!\n",
-                    (unsigned long long)expect, (unsigned long long)sum);
+                logf(quiet, "WARNING: %llu Distances Expected and %llu Observed!!!\n", (unsigned long long)expect,
+                     (unsigned long long)sum);
             if (row == col) {
                 if (h[0] != na)
                     logf(quiet,
@@ -244,15 +192,17 @@ extern "C" int dbx_run_deb(const char *path, int flags)
                          "Number of Atoms within a Tolerance of 0.0005 A is %llu\n",
                          (unsigned long long)zero);
                 if (near > na)
-                    logf(quiet, "WARNING: Possible Atom Collision!!! Found around %llu interatomic Distances "
-                        "lower than the Radius of an Hydrogen Atom.\n",
-                        (unsigned long long)(near - na));
+                    logf(quiet,
+                         "WARNING: Possible Atom Collision!!! Found around %llu interatomic Distances "
+                         "lower than the Radius of an Hydrogen Atom.\n",
+                         (unsigned long long)(near - na));
             } else if (near > 0) {
-                logf(quiet, "WARNING: Possible Atom Collision!!! Found around %llu interatomic Distances lower "
-                    "than the Radius of an Hydrogen Atom.\n",
-                    (unsigned long long)near);
+                logf(quiet,
+                     "WARNING: Possible Atom Collision!!! Found around %llu interatomic Distances lower "
+                     "than the Radius of an Hydrogen Atom.\n",
+                     (unsigned long long)near);
             }
-            if (d.hist > 0) {
+            if (d.hist > 0 && write_debx) {
                 const std::string fn = std::string(path) + "x" + std::to_string(row) + "x" + std::to_string(col);
                 FILE *f = std::fopen(fn.c_str(), "wb");
                 if (!f) {
@@ -261,13 +211,11 @@ extern "C" int dbx_run_deb(const char *path, int flags)
                     continue;
                 }
                 for (int64_t k = 0; k < channels; k++)
-                    if (h[k])
-                        std::fprintf(f, "%.7g\t%llu%s", (double)k * d.rstep, (unsigned long long)h[k], eol);
+                    if (h[k]) std::fprintf(f, "%.7g\t%llu%s", (double)k * d.rstep, (unsigned long long)h[k], eol);
                 std::fclose(f);
             }
         }
     }
-    logf(quiet, "Time for Calculation of Distances and Evaluation of the Debye Double Sum: %.3f s\n", t_calc);
     if (d.hist != 1) {
         const std::string fn = std::string(path) + "I";
         FILE *f = std::fopen(fn.c_str(), "wb");
@@ -275,11 +223,261 @@ extern "C" int dbx_run_deb(const char *path, int flags)
             std::fprintf(stderr, "ERROR: cannot write %s\n", fn.c_str());
             rc = DBX_ERR_IO;
         } else {
-            for (int i = 0; i < nTT; i++) std::fprintf(f, "%g\t%g%s", tt[i], I[i], eol);
+            for (size_t i = 0; i < tt.size(); i++) std::fprintf(f, "%g\t%g%s", tt[i], I[i], eol);
             std::fclose(f);
         }
     }
-    dbx_model_destroy(ctx, m);
+    return rc;
+}
+
+// the .deb's grids; false (message printed) when they are invalid
+bool grids(const DebFile &d, std::vector<double> &tt, std::vector<double> &K, int64_t &channels)
+{
+    const int nTT = dbx_q_from_2theta(d.lambda, d.tt_min, d.tt_max, d.tt_step, nullptr, nullptr, 0);
+    channels = dbx_channels(d.rmax, d.rstep);
+    if (nTT < 1 || channels < 2) {
+        std::fprintf(stderr, "ERROR: invalid 2theta or distance grid in the header\n");
+        return false;
+    }
+    tt.resize(nTT);
+    K.resize(nTT);
+    dbx_q_from_2theta(d.lambda, d.tt_min, d.tt_max, d.tt_step, tt.data(), K.data(), nTT);
+    return true;
+}
+
+}  // namespace
+
+// Reads one .debx file (lines "%.7g\t%llu": r_k and N_k of each nonzero bin, the format of
+// cuDebye-v15.cu:417-423) into hist[n_bins] (zeroed first).  k = rint(r / rstep), and r must
+// round-trip: |r - k rstep| <= 1e-6 |r| + 1e-12 (seven significant digits).
+extern "C" int dbx_read_debx(const char *path, double rstep, int64_t n_bins, uint64_t *hist, int64_t *n_nonzero)
+{
+    if (!path || !hist || n_bins <= 0 || !(rstep > 0)) return DBX_ERR_ARG;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) return DBX_ERR_IO;
+    std::memset(hist, 0, sizeof(uint64_t) * (size_t)n_bins);
+    char line[256];
+    int64_t nnz = 0;
+    int rc = DBX_OK;
+    while (std::fgets(line, sizeof(line), f)) {
+        if (blank(line) || line[0] == '\n') continue;
+        char *e = nullptr;
+        const double r = std::strtod(line, &e);
+        if (e == line) {
+            rc = DBX_ERR_PARSE;
+            break;
+        }
+        char *e2 = nullptr;
+        const unsigned long long n = std::strtoull(e, &e2, 10);
+        if (e2 == e || !std::isfinite(r) || r < 0) {
+            rc = DBX_ERR_PARSE;
+            break;
+        }
+        const double kd = std::rint(r / rstep);
+        if (std::fabs(r - kd * rstep) > 1e-6 * std::fabs(r) + 1e-12) {
+            rc = DBX_ERR_PARSE;
+            break;
+        }
+        if (kd >= (double)n_bins) {
+            rc = DBX_ERR_RANGE;
+            break;
+        }
+        const int64_t k = (int64_t)kd;
+        if (!hist[k]) nnz++;
+        hist[k] += n;
+    }
+    std::fclose(f);
+    if (n_nonzero) *n_nonzero = nnz;
+    return rc;
+}
+
+extern "C" int dbx_run_deb(const char *path, int flags)
+{
+    if (!path) return DBX_ERR_ARG;
+    const bool quiet = flags & DBX_RUN_QUIET;
+    const bool from_debx = flags & DBX_RUN_FROM_DEBX;
+    const char *eol = (flags & DBX_RUN_CRLF) ? "\r\n" : "\n";
+    double t0 = now_s();
+    DebFile d;
+    std::string err;
+    int rc = parse_deb(path, d, err);
+    if (rc != DBX_OK) {
+        std::fprintf(stderr, "ERROR: %s\n", err.c_str());
+        return rc;
+    }
+    std::vector<double> tt, K;
+    int64_t channels = 0;
+    if (!grids(d, tt, K, channels)) return DBX_ERR_ARG;
+    const int nTT = (int)tt.size();
+    logf(quiet, "Loaded %s: %lld atoms, %d subgroups, %d 2theta values (K %g..%g), %lld channels, "
+                "hist=%d (%.3f s)\n",
+         path, (long long)d.x.size(), d.S, nTT, K[0], K[nTT - 1], (long long)channels, d.hist, now_s() - t0);
+
+    dbx_ctx *ctx = nullptr;
+    rc = dbx_open(d.device, 0, &ctx);
+    if (rc != DBX_OK) {
+        std::fprintf(stderr, "ERROR: cannot open device %d\n", d.device);
+        return rc;
+    }
+    // the reference evaluates the Debye sum in FP64; the driver does too unless asked for the
+    // matrix-core path (which rounds sinc to a multiple of 2^-40 and may move a printed 6th digit)
+    dbx_set_pass2_mode(ctx, (flags & DBX_RUN_MFMA) ? DBX_PASS2_MFMA : DBX_PASS2_FP64);
+    const int P = d.S * (d.S + 1) / 2;
+    std::vector<uint64_t> H((size_t)P * channels);
+    std::vector<double> I(nTT, 0.0);
+    const int first_n = d.block > 0 ? d.block : 1;
+    double t1 = now_s();
+    dbx_model *m = nullptr;
+    if (from_debx) {
+        // pass 2 only, from the histograms a previous run wrote (<path>x<r>x<c>)
+        int p = 0;
+        for (int row = 0; row < d.S && rc == DBX_OK; row++)
+            for (int col = row; col < d.S && rc == DBX_OK; col++, p++) {
+                const std::string fn = std::string(path) + "x" + std::to_string(row) + "x" + std::to_string(col);
+                rc = dbx_read_debx(fn.c_str(), d.rstep, channels, &H[(size_t)p * channels], nullptr);
+                if (rc != DBX_OK) std::fprintf(stderr, "ERROR: cannot read histogram file %s\n", fn.c_str());
+            }
+        if (rc == DBX_OK && d.hist != 1) {
+            std::vector<double> ff((size_t)P * nTT);
+            p = 0;
+            for (int a = 0; a < d.S && rc == DBX_OK; a++)
+                for (int b = a; b < d.S && rc == DBX_OK; b++, p++)
+                    rc = dbx_prefactor(ctx, &d.gp[a], &d.gp[b], K.data(), nTT, &ff[(size_t)p * nTT]);
+            if (rc == DBX_OK)
+                rc = dbx_debye(ctx, H.data(), 0, P, channels, d.rstep, first_n, K.data(), nTT, ff.data(), I.data(),
+                               nullptr);
+            if (rc != DBX_OK) std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
+        }
+        if (rc != DBX_OK) {
+            dbx_close(ctx);
+            return rc;
+        }
+    } else {
+        rc = dbx_model_create(ctx, d.x.data(), d.y.data(), d.z.data(), d.n.data(), d.S, &m);
+        if (rc != DBX_OK) {
+            std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
+            dbx_close(ctx);
+            return rc;
+        }
+        if (d.hist != 1)
+            rc = dbx_intensity(ctx, m, d.gp.data(), d.rstep, channels, first_n, K.data(), nTT, I.data(), H.data());
+        else
+            rc = dbx_histogram(ctx, m, d.rstep, channels, H.data(), 0, nullptr);
+        if (rc != DBX_OK) {
+            std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
+            if (rc != DBX_ERR_RANGE) {
+                dbx_model_destroy(ctx, m);
+                dbx_close(ctx);
+                return rc;
+            }
+        }
+    }
+    const double t_calc = now_s() - t1;
+    const int wrc = write_outputs(path, d, tt, H.data(), channels, I, !from_debx, eol, quiet);
+    if (rc == DBX_OK) rc = wrc;
+    logf(quiet, "Time for Calculation of Distances and Evaluation of the Debye Double Sum: %.3f s\n", t_calc);
+    if (m) dbx_model_destroy(ctx, m);
     dbx_close(ctx);
     return rc;
+}
+
+// Every .deb in `dir` that has no .debI yet (the skip-if-output-exists rule of
+// Matlab/multisim.m:38-58), in name order, through one server: all files are submitted at
+// once, so small models share steps and large ones overlap.  Each file's outputs are those of
+// dbx_run_deb on it alone.  Returns DBX_OK, or the first failing file's status (the other
+// files still run).
+extern "C" int dbx_run_folder(const char *dir, int flags)
+{
+    if (!dir) return DBX_ERR_ARG;
+    const bool quiet = flags & DBX_RUN_QUIET;
+    const char *eol = (flags & DBX_RUN_CRLF) ? "\r\n" : "\n";
+    DIR *dp = opendir(dir);
+    if (!dp) {
+        std::fprintf(stderr, "ERROR: >>%s<< is not a folder\n", dir);
+        return DBX_ERR_IO;
+    }
+    std::vector<std::string> files;
+    while (dirent *e = readdir(dp)) {
+        const std::string name = e->d_name;
+        if (name.size() < 5 || name.compare(name.size() - 4, 4, ".deb") != 0) continue;
+        const std::string full = std::string(dir) + "/" + name;
+        struct stat sb;
+        if (stat((full + "I").c_str(), &sb) == 0) continue;  // already calculated
+        files.push_back(full);
+    }
+    closedir(dp);
+    std::sort(files.begin(), files.end());
+    logf(quiet, "%zu file(s) to calculate in %s\n", files.size(), dir);
+    if (files.empty()) return DBX_OK;
+
+    struct Job {
+        std::string path;
+        DebFile d;
+        std::vector<double> tt, K;
+        int64_t channels = 0;
+        uint64_t ticket = 0;
+        int rc = DBX_OK;
+    };
+    std::vector<Job> jobs(files.size());
+    int device = 0;
+    for (size_t i = 0; i < files.size(); i++) {
+        Job &j = jobs[i];
+        j.path = files[i];
+        std::string err;
+        j.rc = parse_deb(j.path.c_str(), j.d, err);
+        if (j.rc != DBX_OK)
+            std::fprintf(stderr, "ERROR: %s: %s\n", j.path.c_str(), err.c_str());
+        else if (!grids(j.d, j.tt, j.K, j.channels))
+            j.rc = DBX_ERR_ARG;
+        if (i == 0 || (jobs[0].rc != DBX_OK && j.rc == DBX_OK)) device = j.d.device;
+    }
+    dbx_server_config cfg{};
+    cfg.pass2_mode = (flags & DBX_RUN_MFMA) ? DBX_PASS2_MFMA : DBX_PASS2_FP64;
+    dbx_server *srv = nullptr;
+    int rc = dbx_server_open(device, &cfg, &srv);
+    if (rc != DBX_OK) {
+        std::fprintf(stderr, "ERROR: cannot open device %d\n", device);
+        return rc;
+    }
+    for (Job &j : jobs) {
+        if (j.rc != DBX_OK) continue;
+        dbx_request q{};
+        q.x = j.d.x.data();
+        q.y = j.d.y.data();
+        q.z = j.d.z.data();
+        q.group_n = j.d.n.data();
+        q.n_groups = j.d.S;
+        q.params = j.d.gp.data();
+        q.rstep = j.d.rstep;
+        q.n_bins = j.channels;
+        q.first_n = j.d.block > 0 ? j.d.block : 1;
+        q.q = j.K.data();
+        q.nq = (int)j.K.size();
+        q.flags = DBX_REQ_KEEP;
+        j.rc = dbx_submit(srv, &q, &j.ticket);
+        if (j.rc != DBX_OK) std::fprintf(stderr, "ERROR: %s: %s\n", j.path.c_str(), dbx_server_last_error(srv));
+    }
+    int first_bad = DBX_OK;
+    for (Job &j : jobs) {
+        if (j.rc == DBX_OK) {
+            std::vector<double> I(j.K.size());
+            double ms = 0;
+            j.rc = dbx_wait(srv, j.ticket, I.data(), &ms);
+            if (j.rc == DBX_OK) {
+                const int P = j.d.S * (j.d.S + 1) / 2;
+                std::vector<uint64_t> H((size_t)P * j.channels);
+                j.rc = dbx_request_histogram(srv, j.ticket, H.data());
+                if (j.rc == DBX_OK) {
+                    logf(quiet, "Calculated %s (%.3f s)\n", j.path.c_str(), ms * 1e-3);
+                    j.rc = write_outputs(j.path.c_str(), j.d, j.tt, H.data(), j.channels, I, true, eol, quiet);
+                }
+            } else {
+                std::fprintf(stderr, "ERROR: %s: %s\n", j.path.c_str(), dbx_server_last_error(srv));
+            }
+            dbx_release(srv, j.ticket);
+        }
+        if (j.rc != DBX_OK && first_bad == DBX_OK) first_bad = j.rc;
+    }
+    dbx_server_close(srv);
+    return first_bad;
 }

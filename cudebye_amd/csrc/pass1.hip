@@ -212,11 +212,37 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                                                 hrow, miss, oor);
                         }
                     }
+                } else if (!diag) {
+                    // R B atoms per step (lane slot takes atom t + slot): whole steps need no
+                    // per-lane test, two steps per iteration; then at most one partial step
+                    const int nfull = nb & ~(R - 1);
+                    if (a_ok) {
+                        int t = 0;
+                        for (; t + R < nfull; t += 2 * R) {
+                            const float4 b0 = stage[wave][t + slot];
+                            const float4 b1 = stage[wave][t + R + slot];
+                            const int k0 = pair_bin(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
+                            const int k1 = pair_bin(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
+                            count_pair<CHECKED>(h, k0, k_lo, n_bins, hrow, miss, oor);
+                            count_pair<CHECKED>(h, k1, k_lo, n_bins, hrow, miss, oor);
+                        }
+                        if (t < nfull) {
+                            const float4 b = stage[wave][t + slot];
+                            count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
+                                                hrow, miss, oor);
+                        }
+                        if (nfull + slot < nb) {
+                            const float4 b = stage[wave][nfull + slot];
+                            count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
+                                                hrow, miss, oor);
+                        }
+                    }
                 } else {
+                    // the B tile holds the A tile's own atoms: pairs with index b >= a only
                     for (int t = 0; t < nb; t += R) {
                         const int src = t + slot;
                         const float4 b = stage[wave][min(src, nb - 1)];
-                        if (a_ok && src < nb && (!diag || idx_a <= __float_as_int(b.w)))
+                        if (a_ok && src < nb && idx_a <= __float_as_int(b.w))
                             count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
                                                 hrow, miss, oor);
                     }

@@ -126,6 +126,20 @@ __global__ void p2_flags_kernel(const unsigned long long *__restrict__ hist, con
     if (nz) flags[k] = 1;
 }
 
+// profiling: total nonzero bins over all rows (the bins the reference's per-pair sum contracts)
+__global__ void p2_row_nnz_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
+                                  int n_rows, int64_t nb_max, unsigned long long *__restrict__ total)
+{
+    const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
+    const int r0 = blockIdx.y * kFlagRows;
+    const int r1 = min(n_rows, r0 + kFlagRows);
+    unsigned long long c = 0;
+    if (k < nb_max)
+        for (int r = r0; r < r1 && row_holds(rows, r, k); r++) c += hist[rows[r].h_off + k] != 0ull;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off);
+    if ((threadIdx.x & 63) == 0 && c) atomicAdd(total, c);
+}
+
 __global__ void p2_zero_kernel(unsigned int *__restrict__ p, int64_t n_words)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -808,15 +822,22 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
             std::vector<int> hk(nblocks);
             DBX_HIP(hipMemcpyAsync(hk.data(), kext, sizeof(int) * nblocks, hipMemcpyDeviceToHost, ctx->stream));
             DBX_HIP(hipStreamSynchronize(ctx->stream));
-            double steps = 0, useful = 0;
+            double steps = 0;
             for (int z = 0; z < rgroups; z++) {
                 const int kx = std::min(Kc_pad, (hk[z * (sh.rows() / 16)] + 31) & ~31);
                 steps += (double)(kx / 32);
             }
-            for (int r = 0; r < n_rows; r++) useful += (double)hk[r / 16];
+            // algorithmic work: 2 nQ flops per nonzero bin of every row
+            hipLaunchKernelGGL(p2_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, (unsigned int *)tot, (int64_t)2);
+            hipLaunchKernelGGL(p2_row_nnz_kernel,
+                               dim3((unsigned)nblk, (unsigned)((n_rows + kFlagRows - 1) / kFlagRows)),
+                               dim3(kScanBlock), 0, ctx->stream, h, d_rows, n_rows, nb_max, tot);
+            unsigned long long nnz = 0;
+            DBX_HIP(hipMemcpyAsync(&nnz, tot, sizeof(nnz), hipMemcpyDeviceToHost, ctx->stream));
+            DBX_HIP(hipStreamSynchronize(ctx->stream));
             const double waves = (double)qgroups * sh.WQ * sh.WR;
             ctx->mf_flops = steps * waves * sh.RB * I * 5 * 16384.0;
-            ctx->mf_useful = useful * (double)nq * 2.0;
+            ctx->mf_useful = (double)nnz * (double)nq * 2.0;
             ctx->mf_digits = I;
             ctx->mf_timed = true;
         }

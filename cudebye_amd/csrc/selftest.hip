@@ -46,6 +46,19 @@ __global__ void st_round_kernel(uint32_t lo, uint32_t hi, unsigned long long *ba
     if (nbad) atomicAdd(&bad[0], nbad);
 }
 
+// LDS atomic throughput: every lane adds 1 to a u32 of a 32768-word window, lanes of a wave on
+// consecutive words (no bank conflict), `iters` times at moving offsets
+__global__ __launch_bounds__(1024) void st_lds_atomic_kernel(int iters, unsigned int *out)
+{
+    __shared__ uint32_t h[32768];
+    for (int i = threadIdx.x; i < 32768; i += 1024) h[i] = 0u;
+    __syncthreads();
+    const unsigned base = (threadIdx.x >> 6) * 2048u + (threadIdx.x & 63);
+    for (int it = 0; it < iters; it++) atomicAdd(&h[(base + (unsigned)it * 64u) & 32767u], 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, h[blockIdx.x & 32767u]);
+}
+
 __global__ void st_bins_kernel(const float *__restrict__ a, const float *__restrict__ b, int n,
                                float rp, int *__restrict__ out)
 {
@@ -133,5 +146,34 @@ extern "C" int dbx_selftest_round(uint32_t lo, uint32_t hi, uint64_t *n_bad)
     (void)hipFree(d);
     if (e != hipSuccess) return DBX_ERR_HIP;
     *n_bad = h;
+    return DBX_OK;
+}
+
+// Measured LDS atomic rate of the device (ds_add_u32 per second over all CUs), from one launch of
+// 4 workgroups per CU, 1024 lanes each, 4096 adds per lane, timed with HIP events.
+extern "C" int dbx_selftest_lds_atomic_rate(double *atomics_per_s)
+{
+    if (!atomics_per_s) return DBX_ERR_ARG;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return DBX_ERR_HIP;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return DBX_ERR_HIP;
+    unsigned int *d = nullptr;
+    if (hipMalloc(&d, sizeof(unsigned int)) != hipSuccess) return DBX_ERR_HIP;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const int iters = 4096, blocks = 4 * cus;
+    hipLaunchKernelGGL(st_lds_atomic_kernel, dim3(blocks), dim3(1024), 0, 0, 64, d);  // warm-up
+    (void)hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(st_lds_atomic_kernel, dim3(blocks), dim3(1024), 0, 0, iters, d);
+    (void)hipEventRecord(e1, 0);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(d);
+    if (e != hipSuccess || ms <= 0.f) return DBX_ERR_HIP;
+    *atomics_per_s = (double)blocks * 1024.0 * iters / (ms * 1e-3);
     return DBX_OK;
 }

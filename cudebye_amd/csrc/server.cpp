@@ -19,6 +19,8 @@
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <thread>
@@ -48,10 +50,14 @@ struct Request {
     int first_n = 0;
     int flags = 0;
     int P = 0;
+    int64_t n_atoms = 0;
     // host preparation
     ModelDev m;
     std::vector<float4> pos_h;
     PlanHost plan;
+    // everything uploaded, in its device layout: one copy per request
+    std::vector<char> image;
+    size_t off[7] = {};  // pos, tiles, jobs, edge2, list, alist, q
     // device buffers (arena)
     std::vector<void *> dev;
     uint64_t *d_hist = nullptr;
@@ -86,7 +92,9 @@ struct dbx_server {
     std::unique_ptr<dbx::DevArena> arena;
     std::mutex mu;
     std::condition_variable cv_prep, cv_sched, cv_done;
-    std::deque<dbx::Request *> prep_q, ready_q, admitted_q;
+    std::deque<dbx::Request *> prep_q, admitted_q;
+    std::map<uint64_t, dbx::Request *> ready;  // prepared, by submission order
+    std::set<uint64_t> preparing;              // submitted, not yet prepared
     std::unordered_map<uint64_t, std::unique_ptr<dbx::Request>> reqs;
     uint64_t next_id = 1;
     bool stop = false;
@@ -102,13 +110,34 @@ namespace dbx {
 
 static size_t al256(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; }
 
+// the request's uploads in their device layout (positions, tiles, plan, q grid), 256-byte aligned
+static void build_image(Request &r)
+{
+    const void *src[7] = {r.pos_h.data(), r.m.tiles_h.data(), r.plan.jobs.data(), r.plan.edge2.data(),
+                          r.plan.list.data(), r.plan.alist.data(), r.q.data()};
+    const size_t len[7] = {sizeof(float4) * r.pos_h.size(), sizeof(Tile) * r.m.tiles_h.size(),
+                           sizeof(P1Job) * r.plan.jobs.size(), sizeof(float) * r.plan.edge2.size(),
+                           sizeof(int4) * r.plan.list.size(), sizeof(int32_t) * r.plan.alist.size(),
+                           sizeof(double) * r.q.size()};
+    size_t off = 0;
+    for (int i = 0; i < 7; i++) {
+        r.off[i] = off;
+        off += al256(len[i]);
+    }
+    r.image.assign(off, 0);
+    for (int i = 0; i < 7; i++)
+        if (len[i]) std::memcpy(r.image.data() + r.off[i], src[i], len[i]);
+    // the tile-ordered positions now live in the image only
+    std::vector<float4>().swap(r.pos_h);
+}
+
 // device bytes of a prepared request while resident
 static size_t request_bytes(const Request &r)
 {
     const size_t nq = r.q.size(), P = (size_t)r.P;
-    return model_device_bytes(r.m, r.pos_h.size()) + plan_device_bytes(r.plan) +
-           al256(sizeof(uint64_t) * P * (size_t)r.n_bins) + al256(8) + 2 * al256(sizeof(double) * P * nq) +
-           2 * al256(sizeof(double) * nq) + al256(sizeof(GroupPair) * P) + al256(sizeof(int4));
+    return al256(r.image.size()) + al256(sizeof(uint64_t) * P * (size_t)r.n_bins) + al256(8) +
+           2 * al256(sizeof(double) * P * nq) + al256(sizeof(double) * nq) + al256(sizeof(GroupPair) * P) +
+           al256(sizeof(int4));
 }
 
 // every device buffer of the request from the arena; false (nothing kept) when one does not fit
@@ -122,17 +151,20 @@ static bool alloc_request(DevArena &A, Request &r)
     };
     bool ok = true;
     auto need = [&](void *p) { ok = ok && p != nullptr; return p; };
-    r.m.pos = (float4 *)need(take(sizeof(float4) * std::max<size_t>(1, r.pos_h.size())));
-    r.m.tiles = (Tile *)need(take(sizeof(Tile) * std::max<size_t>(1, r.m.tiles_h.size())));
-    r.m.d_jobs = (P1Job *)need(take(sizeof(P1Job) * std::max<size_t>(1, r.plan.jobs.size())));
-    r.m.d_edge2 = (float *)need(take(sizeof(float) * std::max<size_t>(1, r.plan.edge2.size())));
-    r.m.d_list = (int4 *)need(take(sizeof(int4) * std::max<size_t>(1, r.plan.list.size())));
-    r.m.d_alist = (int32_t *)need(take(sizeof(int32_t) * std::max<size_t>(1, r.plan.alist.size())));
+    char *img = (char *)need(take(r.image.size()));
+    if (img) {
+        r.m.pos = (float4 *)(img + r.off[0]);
+        r.m.tiles = (Tile *)(img + r.off[1]);
+        r.m.d_jobs = (P1Job *)(img + r.off[2]);
+        r.m.d_edge2 = (float *)(img + r.off[3]);
+        r.m.d_list = (int4 *)(img + r.off[4]);
+        r.m.d_alist = (int32_t *)(img + r.off[5]);
+        r.d_q = (double *)(img + r.off[6]);
+    }
     r.d_hist = (uint64_t *)need(take(sizeof(uint64_t) * P * (size_t)r.n_bins));
     r.d_oor = (uint64_t *)need(take(8));
     r.d_S = (double *)need(take(sizeof(double) * P * nq));
     r.d_ff = (double *)need(take(sizeof(double) * P * nq));
-    r.d_q = (double *)need(take(sizeof(double) * nq));
     r.d_I = (double *)need(take(sizeof(double) * nq));
     r.d_pairs = (GroupPair *)need(take(sizeof(GroupPair) * P));
     r.d_mrows = (int4 *)need(take(sizeof(int4)));
@@ -192,6 +224,11 @@ static void prep_loop(dbx_server *s)
             tile_model_host(r->x.data(), r->y.data(), r->z.data(), r->group_n.data(), (int)r->group_n.size(), r->m,
                             r->pos_h);
             r->plan = make_plan_host(r->m, r->rstep, s->lane[0]->cu_count);
+            // the caller's coordinates are no longer needed
+            std::vector<float>().swap(r->x);
+            std::vector<float>().swap(r->y);
+            std::vector<float>().swap(r->z);
+            build_image(*r);
             r->bytes = request_bytes(*r);
         } catch (const Error &e) {
             code = e.code;
@@ -201,6 +238,7 @@ static void prep_loop(dbx_server *s)
             msg = e.what();
         }
         std::lock_guard<std::mutex> lk(s->mu);
+        s->preparing.erase(r->id);
         if (code == DBX_OK && r->bytes > s->arena->capacity()) {
             code = DBX_ERR_BUDGET;
             msg = "request needs " + std::to_string(r->bytes) + " device bytes, more than the serving budget";
@@ -210,21 +248,25 @@ static void prep_loop(dbx_server *s)
             r->t_done = now_ms();
             s->st.failed++;
             s->cv_done.notify_all();
+            s->cv_sched.notify_all();  // a later request may have been waiting for this one
             continue;
         }
         r->state = kReady;
-        s->ready_q.push_back(r);
+        s->ready[r->id] = r;
         s->cv_sched.notify_all();
     }
 }
 
-// admission (caller holds s->mu): ready requests in arrival order while their buffers fit
+// admission (caller holds s->mu): prepared requests in submission order while their buffers
+// fit; a request also waits while an earlier one is still being prepared
 static void admit(dbx_server *s)
 {
-    while (!s->ready_q.empty()) {
-        Request *r = s->ready_q.front();
+    while (!s->ready.empty()) {
+        auto it = s->ready.begin();
+        if (!s->preparing.empty() && *s->preparing.begin() < it->first) break;
+        Request *r = it->second;
         if (!alloc_request(*s->arena, *r)) break;
-        s->ready_q.pop_front();
+        s->ready.erase(it);
         r->state = kAdmitted;
         s->admitted_q.push_back(r);
     }
@@ -236,29 +278,10 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
 {
     const double rstep = batch[0]->rstep;
     for (Request *r : batch) {
-        // the model's device arrays and its plan (already in the arena)
+        // the model's device arrays, its plan and its q grid: one copy of the request's image
         ModelDev &m = r->m;
-        if (!r->pos_h.empty())
-            DBX_HIP(hipMemcpyAsync(m.pos, r->pos_h.data(), sizeof(float4) * r->pos_h.size(), hipMemcpyHostToDevice,
-                                   ctx->stream));
-        if (!m.tiles_h.empty())
-            DBX_HIP(hipMemcpyAsync(m.tiles, m.tiles_h.data(), sizeof(Tile) * m.tiles_h.size(),
-                                   hipMemcpyHostToDevice, ctx->stream));
         PlanHost &ph = r->plan;
-        if (!ph.jobs.empty()) {
-            DBX_HIP(hipMemcpyAsync(m.d_jobs, ph.jobs.data(), sizeof(P1Job) * ph.jobs.size(), hipMemcpyHostToDevice,
-                                   ctx->stream));
-            DBX_HIP(hipMemcpyAsync(m.d_list, ph.list.data(), sizeof(int4) * ph.list.size(), hipMemcpyHostToDevice,
-                                   ctx->stream));
-        }
-        if (!ph.edge2.empty())
-            DBX_HIP(hipMemcpyAsync(m.d_edge2, ph.edge2.data(), sizeof(float) * ph.edge2.size(),
-                                   hipMemcpyHostToDevice, ctx->stream));
-        if (!ph.alist.empty())
-            DBX_HIP(hipMemcpyAsync(m.d_alist, ph.alist.data(), sizeof(int32_t) * ph.alist.size(),
-                                   hipMemcpyHostToDevice, ctx->stream));
-        DBX_HIP(hipMemcpyAsync(r->d_q, r->q.data(), sizeof(double) * r->q.size(), hipMemcpyHostToDevice,
-                               ctx->stream));
+        DBX_HIP(hipMemcpyAsync(m.pos, r->image.data(), r->image.size(), hipMemcpyHostToDevice, ctx->stream));
         m.plan_h = ph.jobs;
         m.plan_cost = ph.cost;
         m.plan_jobs = (int)ph.jobs.size();
@@ -269,8 +292,9 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
         m.plan_rstep = ph.rstep;
         DBX_HIP(hipMemsetAsync(r->d_oor, 0, 8, ctx->stream));
     }
-    // the host arrays above must outlive their copies
+    // the images must outlive their copies
     DBX_HIP(hipStreamSynchronize(ctx->stream));
+    for (Request *r : batch) std::vector<char>().swap(r->image);
     std::vector<P1Item> items;
     for (Request *r : batch) {
         P1Item it{&r->m, r->n_bins, r->d_hist};
@@ -354,7 +378,7 @@ static void lane_loop(dbx_server *s, int li)
             double pairs = 0;
             for (auto it = s->admitted_q.begin(); it != s->admitted_q.end();) {
                 Request *r = *it;
-                const double n = (double)r->x.size();
+                const double n = (double)r->n_atoms;
                 const double pr = 0.5 * n * (n + 1);
                 if (r->rstep != rstep || (!batch.empty() && (pairs + pr > s->cfg.max_step_pairs ||
                                                              (int)batch.size() >= s->cfg.max_batch))) {
@@ -418,7 +442,7 @@ int dbx_server_open(int device, const dbx_server_config *cfg, dbx_server **out)
     if (cfg) s->cfg = *cfg;
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 256;
     if (!(s->cfg.max_step_pairs > 0)) s->cfg.max_step_pairs = 2e10;
-    if (s->cfg.prep_threads <= 0) s->cfg.prep_threads = 4;
+    if (s->cfg.prep_threads <= 0) s->cfg.prep_threads = 6;
     int rc = DBX_OK;
     for (int i = 0; i < 2 && rc == DBX_OK; i++) rc = dbx_open(device, 0, &s->lane[i]);
     if (rc == DBX_OK) rc = dbx_open(device, 0, &s->rw_ctx);
@@ -500,6 +524,7 @@ int dbx_submit(dbx_server *s, const dbx_request *q, uint64_t *ticket)
     r->first_n = q->first_n;
     r->flags = q->flags;
     r->P = q->n_groups * (q->n_groups + 1) / 2;
+    r->n_atoms = n;
     // histograms alone beyond the budget: refuse now
     if (sizeof(uint64_t) * (size_t)r->P * (size_t)r->n_bins + 16 * (size_t)n > s->arena->capacity()) {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -510,6 +535,7 @@ int dbx_submit(dbx_server *s, const dbx_request *q, uint64_t *ticket)
     r->id = s->next_id++;
     r->t_submit = now_ms();
     *ticket = r->id;
+    s->preparing.insert(r->id);
     s->prep_q.push_back(r.get());
     s->reqs[r->id] = std::move(r);
     s->st.submitted++;

@@ -175,7 +175,7 @@ typedef struct dbx_server_config {
     size_t hbm_budget;      /* device bytes for resident requests (0: 3/4 of free memory) */
     int max_batch;          /* requests per step (0: 256) */
     double max_step_pairs;  /* pair distances per step (0: 2e10) */
-    int prep_threads;       /* host threads for tiling and planning (0: 4) */
+    int prep_threads;       /* host threads for tiling and planning (0: 6) */
     int pass2_mode;         /* DBX_PASS2_MFMA (default) or DBX_PASS2_FP64 */
 } dbx_server_config;
 typedef struct dbx_request {
@@ -221,8 +221,20 @@ int dbx_server_stats_get(dbx_server *s, dbx_server_stats *out);
  * (hist 1 or 2) with the reference's formats.  flags: DBX_RUN_CRLF writes "\r\n" line
  * ends like the reference's Windows build; DBX_RUN_QUIET suppresses the log on stdout;
  * DBX_RUN_MFMA evaluates pass 2 on the matrix cores (default: FP64, as the reference). */
-enum { DBX_RUN_CRLF = 1, DBX_RUN_QUIET = 2, DBX_RUN_MFMA = 4 };
+enum { DBX_RUN_CRLF = 1, DBX_RUN_QUIET = 2, DBX_RUN_MFMA = 4, DBX_RUN_FROM_DEBX = 8 };
 int dbx_run_deb(const char *path, int flags);
+/* DBX_RUN_FROM_DEBX: pass 2 only, from the <path>x<r>x<c> histograms a previous run wrote
+ * (read with dbx_read_debx), for new header parameters; the .debx files are left as they are.
+ *
+ * dbx_read_debx: one .debx file (lines "%.7g\t%llu" of r_k and N_k, cuDebye-v15.cu:417-423)
+ * into hist[n_bins] (zeroed first), k = rint(r / rstep); DBX_ERR_PARSE for a line whose r is
+ * not a bin centre to seven digits, DBX_ERR_RANGE for a bin >= n_bins.  n_nonzero (optional)
+ * receives the number of nonzero bins. */
+int dbx_read_debx(const char *path, double rstep, int64_t n_bins, uint64_t *hist, int64_t *n_nonzero);
+/* Folder run (Matlab/multisim.m:38-58, one cuDebye.exe per .deb without a .debI): every such
+ * .deb in `dir`, in name order, through one server (dbx_server_open); each file's outputs are
+ * those of dbx_run_deb on it alone.  flags as for dbx_run_deb (DBX_RUN_FROM_DEBX excepted). */
+int dbx_run_folder(const char *dir, int flags);
 
 /* ---- diagnostics --------------------------------------------------------------------
  * Self tests of the device arithmetic behind the pass-1 bin rule (no reference
@@ -235,9 +247,10 @@ int dbx_selftest_sqrt(uint32_t lo, uint32_t hi, uint64_t *n_bad);
 int dbx_selftest_bins(const float *a, const float *b, int n, float rp, int32_t *out);
 /* dbx_selftest_sin: out[i] = the device FP64 sine used by the pass-2 sinc table, |x| < 2^20. */
 int dbx_selftest_sin(const double *x, int n, double *out);
-/* Statistics of the last pass-1 run on ctx: work items launched and pairs whose bin fell
- * outside their workgroup's LDS window (added to HBM directly; a performance figure). */
-int dbx_last_pass1_stats(const dbx_ctx *ctx, int64_t *n_jobs, uint64_t *window_misses);
+/* Statistics of the last pass-1 run on ctx: work items launched, pairs whose bin fell
+ * outside their workgroup's LDS window (added to HBM directly) and exact-window work items
+ * that met an out-of-window bin and were re-run by the checked kernel (performance figures). */
+int dbx_last_pass1_stats(const dbx_ctx *ctx, int64_t *n_jobs, uint64_t *window_misses, uint64_t *retried);
 /* Device times (HIP events on the context's stream) of the last pass-1 histogram kernel
  * and of the last pass-2 structure computation (compaction through the Debye sum), ms. */
 int dbx_last_kernel_times(dbx_ctx *ctx, double *pass1_ms, double *pass2_ms);

@@ -39,6 +39,10 @@ int dbx_debug_plan_host(const float *x, const float *y, const float *z, const in
  * rounding of the bin rule (cuDebye-v15.cu:104); n_bad[0] receives the count. */
 int dbx_selftest_round(uint32_t lo, uint32_t hi, uint64_t *n_bad);
 
+/* Measured LDS atomic throughput of the current device (ds_add_u32 on conflict-free addresses,
+ * all CUs), the peak against which pass 1's one LDS increment per pair is reported. */
+int dbx_selftest_lds_atomic_rate(double *atomics_per_s);
+
 /* The pass-2 FP64 sine (dbx_selftest_sin's device function) evaluated on the host. */
 int dbx_debug_sin_host(const double *x, int n, double *out);
 
@@ -46,8 +50,9 @@ int dbx_debug_sin_host(const double *x, int n, double *out);
  * records HIP events around the contraction kernel and reads back its per-row-block bin
  * extents to count the matrix instructions it issues.  dbx_debug_pass2_profile returns,
  * for the last pass 2: the contraction kernel's time (ms), the BF16 matrix flops it issued
- * (16 x 16 x 32 x 2 per v_mfma_f32_16x16x32_bf16), the useful multiply-adds (rows x their
- * compacted bins x q, counted as 2 flops each), the compacted bin count and the count digits. */
+ * (16 x 16 x 32 x 2 per v_mfma_f32_16x16x32_bf16), the algorithmic flops (2 nQ per nonzero bin
+ * of every row: the contraction counted once, whatever the digit count), the compacted bin
+ * count and the count digits. */
 int dbx_debug_set_profiling(dbx_ctx *ctx, int on);
 int dbx_debug_pass2_profile(dbx_ctx *ctx, double *kernel_ms, double *mfma_flops, double *useful_flops,
                             int *n_bins_compacted, int *count_digits);

@@ -42,6 +42,12 @@ def test_sqrt_correctly_rounded_exhaustive():
     assert bad == 0 and mism == 0
 
 
+def test_round_half_up_exhaustive():
+    """Every float v in [0, 2^31): the one-instruction round-half-up conversion the bin rule
+    uses (v_cvt_rpi_i32_f32) equals roundf's half-away-from-zero (trunc + (fract >= 0.5))."""
+    assert cd.selftest_round(0, 0x4F000000) == 0
+
+
 def test_bin_rule_matches_oracle_random_and_near_ties():
     rng = np.random.default_rng(11)
     n = 400000

@@ -75,6 +75,16 @@ def _rel(a, b):
     return np.max(np.abs(a - b) / np.abs(b))
 
 
+def _finish(srv, t, timeout=60.0, allow=()):
+    """srv.wait(t) after polling until the request has ended, so that a request that never
+    ends fails the test instead of blocking it."""
+    deadline = time.time() + timeout
+    while srv.poll(t) not in (cd.DBX_REQ_DONE, cd.DBX_REQ_FAILED):
+        assert time.time() < deadline, f"request {t} did not finish within {timeout} s"
+        time.sleep(0.005)
+    return srv.wait(t, allow=allow)
+
+
 def test_serve_results_same_alone_and_in_the_mix(ctx):
     models = _models(16, 3)
     alone = [_alone(ctx, m) for m in models]
@@ -94,7 +104,7 @@ def test_serve_results_same_alone_and_in_the_mix(ctx):
         th.join()
         worst = 0.0
         for i, m in enumerate(models):
-            I, lat, rc = srv.wait(tickets[i])
+            I, lat, rc = _finish(srv, tickets[i])
             assert rc == cd.DBX_OK and lat > 0
             H = srv.histogram(tickets[i])
             assert np.array_equal(H, alone[i][1]), f"request {i} histogram"
@@ -116,7 +126,7 @@ def test_serve_reweight_replays_match_direct(ctx):
     sets = [[synth.cromer_mann(rng) for _ in m.params] for _ in range(3)]
     with cd.Server(0) as srv:
         t = _submit(srv, m)
-        I0, _, _ = srv.wait(t)
+        I0, _, _ = _finish(srv, t)
         H = srv.histogram(t)
         mod = ctx.model(m.x, m.y, m.z, m.group_n)
         for rep, k in enumerate([0, 1, 0, 2, 1]):
@@ -138,15 +148,15 @@ def test_serve_admission_waits_for_budget():
     with cd.Server(0) as srv:
         for m in models:
             t = _submit(srv, m)
-            srv.wait(t)
+            _finish(srv, t)
             sizes.append(srv.stats()["resident_bytes"])
             srv.release(t)
     assert min(sizes) > 0 and max(sizes) < 1.2 * min(sizes)
     budget = int(2.5 * max(sizes))
     with cd.Server(0, hbm_budget=budget) as srv:
         ts = [_submit(srv, m) for m in models]
-        srv.wait(ts[0])
-        srv.wait(ts[1])
+        _finish(srv, ts[0])
+        _finish(srv, ts[1])
         time.sleep(0.3)
         states = [srv.poll(t) for t in ts]
         assert states[:2] == [cd.DBX_REQ_DONE] * 2
@@ -155,7 +165,7 @@ def test_serve_admission_waits_for_budget():
         # each release admits the next waiting request
         for k in range(2, 5):
             srv.release(ts[k - 2])
-            _, _, rc = srv.wait(ts[k])
+            _, _, rc = _finish(srv, ts[k])
             assert rc == cd.DBX_OK
             assert srv.stats()["resident_bytes"] <= budget
         assert srv.stats()["completed"] == len(models)
@@ -176,7 +186,7 @@ def test_serve_bad_request_fails_alone(ctx):
     with cd.Server(0) as srv:
         ts = [_submit(srv, m, n_bins=short if i == 1 else None) for i, m in enumerate(models)]
         for i, t in enumerate(ts):
-            I, _, rc = srv.wait(t, allow=(cd.DBX_ERR_RANGE,))
+            I, _, rc = _finish(srv, t, allow=(cd.DBX_ERR_RANGE,))
             if i == 1:
                 assert rc == cd.DBX_ERR_RANGE and srv.poll(t) == cd.DBX_REQ_FAILED
             else:
@@ -196,7 +206,7 @@ def test_serve_poisson_stream_small(ctx):
             ts.append(_submit(srv, m, keep=False))
         lats = []
         for t in ts:
-            I, lat, rc = srv.wait(t)
+            I, lat, rc = _finish(srv, t)
             assert rc == cd.DBX_OK and np.all(np.isfinite(I))
             lats.append(lat)
             srv.release(t)

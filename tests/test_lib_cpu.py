@@ -61,3 +61,32 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(cd, "_lib", None)
     with pytest.raises(ImportError):
         cd.lib()
+
+
+def test_read_debx_round_trips_reference_files(tmp_path, golden_dir, gamma_debx):
+    """The reader of .debx files (host only): the reference's Gamma histograms, written in its
+    exact format (the oracle writer reproduces the six files byte for byte), read back."""
+    g = O.parse_deb(str(golden_dir / "GammaAl2O3_10nm.deb"))
+    nb = O.channels(g.rmax, g.rstep)
+    for key, (bins, cnt) in gamma_debx.items():
+        path = tmp_path / f"g.debx{key}"
+        O.write_debx(str(path), bins, cnt, g.rstep, crlf=True)
+        h = cd.read_debx(path, g.rstep, nb)
+        ref = np.zeros(nb, np.uint64)
+        ref[bins] = cnt
+        assert np.array_equal(h, ref), key
+
+
+def test_read_debx_rejects_bad_files(tmp_path):
+    p = tmp_path / "bad.debx"
+    p.write_text("1.2345\t7\n")          # not a bin centre at 0.001 A... 1.2345 / 0.001 = 1234.5
+    with pytest.raises(cd.DebyeError) as e:
+        cd.read_debx(p, 1e-3, 5000)
+    assert e.value.code == -4
+    p.write_text("4.5\t3\n")
+    with pytest.raises(cd.DebyeError) as e:
+        cd.read_debx(p, 1e-3, 1000)         # bin 4500 >= 1000
+    assert e.value.code == cd.DBX_ERR_RANGE
+    p.write_text("0.001\t3\n0.002\t5\n\n")
+    h = cd.read_debx(p, 1e-3, 10)
+    assert h[1] == 3 and h[2] == 5 and h.sum() == 8
