@@ -223,12 +223,15 @@ def selftest_round(lo=0, hi=0x4F000000):
     return int(n.value)
 
 
-def selftest_sqrt(lo, hi):
-    """(not correctly rounded, differs from sqrtf) counts over float bit patterns [lo, hi)."""
-    n = (C.c_uint64 * 2)()
+def selftest_sqrt(lo, hi, with_raw=False):
+    """(not correctly rounded, differs from sqrtf) counts over float bit patterns [lo, hi);
+    with_raw adds the counts where the bare hardware square root lies below / above."""
+    n = (C.c_uint64 * 4)()
     rc = lib().dbx_selftest_sqrt(lo, hi, n)
     if rc != DBX_OK:
         raise DebyeError(rc, "selftest_sqrt")
+    if with_raw:
+        return int(n[0]), int(n[1]), int(n[2]), int(n[3])
     return int(n[0]), int(n[1])
 
 

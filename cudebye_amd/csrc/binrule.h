@@ -42,14 +42,21 @@ __device__ __forceinline__ int round_rpi(float v)
     return r;
 }
 
-// bin of one pair: the reference's unfused FP32 chain; sqrt_cr(s) * rp is below 2^31 for
-// every pair the histograms count (larger values land past the last bin either way)
-__device__ __forceinline__ int pair_bin(float ax, float ay, float az, float bx, float by, float bz,
-                                        float rp)
+// distance of one pair in bin units before rounding: the reference's unfused FP32 chain
+__device__ __forceinline__ float pair_scaled(float ax, float ay, float az, float bx, float by, float bz,
+                                             float rp)
 {
     const float dx = bx - ax, dy = by - ay, dz = bz - az;
     const float s = (dx * dx + dy * dy) + dz * dz;
-    return round_rpi(sqrt_cr(s) * rp);
+    return sqrt_cr(s) * rp;
+}
+
+// bin of one pair; the scaled distance is below 2^31 for every pair the histograms count
+// (larger values land past the last bin either way)
+__device__ __forceinline__ int pair_bin(float ax, float ay, float az, float bx, float by, float bz,
+                                        float rp)
+{
+    return round_rpi(pair_scaled(ax, ay, az, bx, by, bz, rp));
 }
 
 }  // namespace dbx

@@ -72,7 +72,7 @@ struct P1Job {
 };
 
 constexpr int kP1Threads = 1024;   // 16 waves of 64
-constexpr int kP1Bins = 35328;     // LDS window: 138 KiB of uint32 counters (+1 sentinel)
+constexpr int kP1Bins = 35328;     // LDS window: 138 KiB of uint32 counters (+ guard and sentinel)
 constexpr int kP1Cand = 1024;      // LDS candidate list (A tile, B tile), 4 KiB
 // + a 1 KiB B-atom stage per wave (16 KiB) for the 32- and 16-atom A tiles
 

@@ -33,17 +33,30 @@
 
 namespace dbx {
 
+// The LDS window: h[0] guard (bins below the window), h[1 + i] bin k_lo + i for
+// 0 <= i < kP1Bins, h[kP1Bins + 1] sentinel (bins above it).
+struct Window {
+    int k_lo;
+    int off;     // 1 - k_lo: h[k + off] holds bin k
+    float vlo;   // k_lo - 1 and k_lo + kP1Bins, exact in FP32 (the planner keeps fast jobs
+    float vhi;   // below 2^24 bins)
+};
+
+// v = the pair's distance in bin units (pair_scaled).  Fast variant: v clamped to the guard
+// and sentinel bins before the round-half-up conversion, so a bin outside the window lands in
+// h[0] or h[kP1Bins + 1] (three instructions: med3, convert, address).
 template <bool CHECKED>
-__device__ __forceinline__ void count_pair(uint32_t *h, int k, int k_lo, int64_t n_bins,
+__device__ __forceinline__ void count_pair(uint32_t *h, float v, const Window &w, int64_t n_bins,
                                            unsigned long long *hrow, uint32_t &miss, uint32_t &oor)
 {
     if (!CHECKED) {
-        const unsigned kk = min((unsigned)(k - k_lo), (unsigned)kP1Bins);
-        atomicAdd(&h[kk], 1u);
+        const int k = round_rpi(__builtin_amdgcn_fmed3f(v, w.vlo, w.vhi));
+        atomicAdd(&h[k + w.off], 1u);
     } else {
-        const unsigned kk = (unsigned)(k - k_lo);
+        const int k = round_rpi(v);
+        const unsigned kk = (unsigned)(k - w.k_lo);
         if (kk < (unsigned)kP1Bins) {
-            atomicAdd(&h[kk], 1u);
+            atomicAdd(&h[kk + 1], 1u);
         } else {
             ++miss;
             if (k >= 0 && (int64_t)k < n_bins)
@@ -61,7 +74,7 @@ __device__ __forceinline__ void count_pair(uint32_t *h, int k, int k_lo, int64_t
 // its lanes, keeps the atoms inside the shell (ballot) and streams them past its A atoms.
 //
 // CHECKED = false: every pair's bin is assumed to fall in the window (the planner sized the
-// shells for that).  An out-of-window bin is clamped into a sentinel counter h[kP1Bins]; a
+// shells for that).  An out-of-window bin is clamped into a guard or sentinel counter; a
 // job whose sentinel is nonzero flushes nothing and is queued for the checked variant.
 // CHECKED = true: out-of-window pairs go straight to the uint64 histogram in HBM (jobs whose
 // window is narrower than their geometry, and the queued ones).
@@ -70,7 +83,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const P1Ref *__restrict__ refs, const int4 *__restrict__ list, int n_fast, int n_checked,
     unsigned long long *__restrict__ counters, int4 *__restrict__ retry)
 {
-    __shared__ uint32_t h[kP1Bins + 1];
+    __shared__ uint32_t h[kP1Bins + 2];
     __shared__ uint32_t cand[kP1Cand];
     __shared__ float4 stage[kP1Threads / 64][64];
     __shared__ int s_ncand, s_next;
@@ -103,7 +116,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const float *__restrict__ edge2 = M.edge2;
     const float rp = M.rp;
     const int64_t n_bins = M.n_bins;
-    for (int i = threadIdx.x; i <= kP1Bins; i += kP1Threads) h[i] = 0u;
+    for (int i = threadIdx.x; i < kP1Bins + 2; i += kP1Threads) h[i] = 0u;
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -116,6 +129,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const float r_lo = e_lo > 0.f ? sqrtf(e_lo) : 0.f;
     const float r_hi = sqrtf(e_hi);  // +inf for the last shell
     const int k_lo = job.k_lo;
+    const Window win{k_lo, 1 - k_lo, (float)(k_lo - 1), (float)(k_lo + kP1Bins)};
     unsigned long long *hrow = M.hist + (int64_t)job.pair * n_bins;
     const int nblk = (job.b1 - job.b0 + 63) >> 6;
     const int units = job.a_cnt * nblk;
@@ -201,14 +215,14 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                         for (; t + 1 < nb; t += 2) {
                             const float4 b0 = stage[wave][t];
                             const float4 b1 = stage[wave][t + 1];
-                            const int k0 = pair_bin(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
-                            const int k1 = pair_bin(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
-                            count_pair<CHECKED>(h, k0, k_lo, n_bins, hrow, miss, oor);
-                            count_pair<CHECKED>(h, k1, k_lo, n_bins, hrow, miss, oor);
+                            const float v0 = pair_scaled(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
+                            const float v1 = pair_scaled(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
+                            count_pair<CHECKED>(h, v0, win, n_bins, hrow, miss, oor);
+                            count_pair<CHECKED>(h, v1, win, n_bins, hrow, miss, oor);
                         }
                         if (t < nb) {
                             const float4 b = stage[wave][t];
-                            count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
+                            count_pair<CHECKED>(h, pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), win, n_bins,
                                                 hrow, miss, oor);
                         }
                     }
@@ -221,19 +235,19 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                         for (; t + R < nfull; t += 2 * R) {
                             const float4 b0 = stage[wave][t + slot];
                             const float4 b1 = stage[wave][t + R + slot];
-                            const int k0 = pair_bin(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
-                            const int k1 = pair_bin(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
-                            count_pair<CHECKED>(h, k0, k_lo, n_bins, hrow, miss, oor);
-                            count_pair<CHECKED>(h, k1, k_lo, n_bins, hrow, miss, oor);
+                            const float v0 = pair_scaled(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
+                            const float v1 = pair_scaled(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
+                            count_pair<CHECKED>(h, v0, win, n_bins, hrow, miss, oor);
+                            count_pair<CHECKED>(h, v1, win, n_bins, hrow, miss, oor);
                         }
                         if (t < nfull) {
                             const float4 b = stage[wave][t + slot];
-                            count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
+                            count_pair<CHECKED>(h, pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), win, n_bins,
                                                 hrow, miss, oor);
                         }
                         if (nfull + slot < nb) {
                             const float4 b = stage[wave][nfull + slot];
-                            count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
+                            count_pair<CHECKED>(h, pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), win, n_bins,
                                                 hrow, miss, oor);
                         }
                     }
@@ -243,7 +257,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                         const int src = t + slot;
                         const float4 b = stage[wave][min(src, nb - 1)];
                         if (a_ok && src < nb && idx_a <= __float_as_int(b.w))
-                            count_pair<CHECKED>(h, pair_bin(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), k_lo, n_bins,
+                            count_pair<CHECKED>(h, pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), win, n_bins,
                                                 hrow, miss, oor);
                     }
                 }
@@ -264,7 +278,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             if (miss) atomicAdd(&counters[1], (unsigned long long)miss);
             if (oor) atomicAdd(M.oor, (unsigned long long)oor);
         }
-    } else if (h[kP1Bins] != 0u) {
+    } else if ((h[0] | h[kP1Bins + 1]) != 0u) {
         // a bin fell outside the window: leave this job to the checked variant
         if (threadIdx.x == 0) {
             const unsigned long long r = atomicAdd(&counters[2], 1ull);
@@ -273,7 +287,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
         return;
     }
     for (int i = threadIdx.x; i < kP1Bins; i += kP1Threads) {
-        const uint32_t v = h[i];
+        const uint32_t v = h[i + 1];
         if (v) {
             const int64_t k = (int64_t)k_lo + i;
             if (k < n_bins)
@@ -441,7 +455,8 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                             klo = std::floor(((sh + 0.5) * h) / rstep) - kP1Bins / 2;
                         if (klo < 0) klo = 0;
                         J.k_lo = (int32_t)std::min<double>(klo, (double)INT32_MAX - kP1Bins);
-                        J.degraded = degraded ? 1 : 0;
+                        // the fast variant's window bounds must be exact in FP32
+                        J.degraded = (degraded || J.k_lo + (double)kP1Bins >= 16777216.0) ? 1 : 0;
                         jobs.push_back(J);
                         // neighbouring shells share straddling tiles: spread the sampled cost
                         double cst = shell_cost[sh];

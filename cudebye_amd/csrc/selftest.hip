@@ -15,12 +15,15 @@ namespace dbx {
 
 __global__ void st_sqrt_kernel(uint32_t lo, uint32_t hi, unsigned long long *bad)
 {
-    unsigned long long nbad = 0, nmis = 0;
+    unsigned long long nbad = 0, nmis = 0, nlo = 0, nhi = 0;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t u = (uint64_t)lo + blockIdx.x * blockDim.x + threadIdx.x; u < hi; u += stride) {
         const float s = __uint_as_float((uint32_t)u);
         if (!(s >= 0x1p-96f) || !(s <= 3.4028235e38f)) continue;
         const float r = sqrt_cr(s);
+        const float raw = __builtin_amdgcn_sqrtf(s);
+        nlo += __float_as_uint(raw) < __float_as_uint(r);
+        nhi += __float_as_uint(raw) > __float_as_uint(r);
         const double R = r;
         const double Rd = __uint_as_float(__float_as_uint(r) - 1);
         const double Ru = __uint_as_float(__float_as_uint(r) + 1);
@@ -31,6 +34,8 @@ __global__ void st_sqrt_kernel(uint32_t lo, uint32_t hi, unsigned long long *bad
     }
     if (nbad) atomicAdd(&bad[0], nbad);
     if (nmis) atomicAdd(&bad[1], nmis);
+    if (nlo) atomicAdd(&bad[2], nlo);
+    if (nhi) atomicAdd(&bad[3], nhi);
 }
 
 // n_bad[0]: floats v in [lo, hi) (bit patterns, v >= 0) where round_rpi(v) != round_pos(v)
@@ -92,15 +97,16 @@ extern "C" int dbx_selftest_sin(const double *x, int n, double *out)
 
 // Checks sqrt_cr for every float bit pattern in [lo, hi) with 2^-96 <= s <= FLT_MAX
 // (plus s == 0 when lo == 0).  n_bad[0]: results that are not correctly rounded;
-// n_bad[1]: results that differ from the compiler's correctly rounded sqrtf.
+// n_bad[1]: results that differ from the compiler's correctly rounded sqrtf; n_bad[2] and
+// n_bad[3]: inputs where the bare v_sqrt_f32 result lies below / above the correctly rounded one.
 extern "C" int dbx_selftest_sqrt(uint32_t lo, uint32_t hi, uint64_t *n_bad)
 {
     if (!n_bad || hi < lo) return DBX_ERR_ARG;
     unsigned long long *d = nullptr;
-    if (hipMalloc(&d, 2 * sizeof(*d)) != hipSuccess) return DBX_ERR_HIP;
-    (void)hipMemset(d, 0, 2 * sizeof(*d));
+    if (hipMalloc(&d, 4 * sizeof(*d)) != hipSuccess) return DBX_ERR_HIP;
+    (void)hipMemset(d, 0, 4 * sizeof(*d));
     hipLaunchKernelGGL(st_sqrt_kernel, dim3(8192), dim3(256), 0, 0, lo, hi, d);
-    unsigned long long h[2] = {0, 0};
+    unsigned long long h[4] = {0, 0, 0, 0};
     hipError_t e = hipMemcpy(h, d, sizeof(h), hipMemcpyDeviceToHost);
     (void)hipFree(d);
     if (e != hipSuccess) return DBX_ERR_HIP;
@@ -109,6 +115,8 @@ extern "C" int dbx_selftest_sqrt(uint32_t lo, uint32_t hi, uint64_t *n_bad)
     }
     n_bad[0] = h[0];
     n_bad[1] = h[1];
+    n_bad[2] = h[2];
+    n_bad[3] = h[3];
     return DBX_OK;
 }
 

@@ -241,7 +241,9 @@ int dbx_run_folder(const char *dir, int flags);
  * counterpart: the reference relies on CUDA's sqrtf and roundf).
  * dbx_selftest_sqrt: for every float bit pattern in [lo, hi) with 2^-96 <= s <= FLT_MAX,
  *   n_bad[0] counts device square roots that are not correctly rounded, n_bad[1] those
- *   that differ from the compiler's correctly rounded sqrtf.
+ *   that differ from the compiler's correctly rounded sqrtf; n_bad[2] and n_bad[3] count the
+ *   inputs where the bare hardware square root (before the correction) lies one float below /
+ *   above the correctly rounded result.  n_bad has four entries.
  * dbx_selftest_bins: out[i] = device bin of the pair (a[3i..], b[3i..]) with rp. */
 int dbx_selftest_sqrt(uint32_t lo, uint32_t hi, uint64_t *n_bad);
 int dbx_selftest_bins(const float *a, const float *b, int n, float rp, int32_t *out);
