@@ -331,7 +331,7 @@ class Context:
         return int(nj.value), int(miss.value)
 
     def set_pass2_mode(self, mode):
-        """DBX_PASS2_MFMA (split-BF16 matrix cores, default) or DBX_PASS2_FP64."""
+        """DBX_PASS2_MFMA (8-bit integer digits on the matrix cores, default) or DBX_PASS2_FP64."""
         self._check(lib().dbx_set_pass2_mode(self.handle, mode), "set_pass2_mode")
 
     def kernel_times(self):
@@ -345,7 +345,7 @@ class Context:
         self._check(lib().dbx_debug_set_profiling(self.handle, int(on)), "set_profiling")
 
     def pass2_profile(self):
-        """(kernel ms, issued BF16 matrix flops, useful flops, compacted bins, count digits) of
+        """(kernel ms, issued int8 matrix ops, useful flops, compacted bins, count digits) of
         the last profiled matrix-core pass 2."""
         ms, fl, us = C.c_double(), C.c_double(), C.c_double()
         kc, dg = C.c_int(), C.c_int()

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "fastmath.h"
+#include "mfdigits.h"
 #include "internal.h"
 
 namespace dbx {
@@ -835,6 +836,30 @@ int dbx_debug_sin_host(const double *x, int n, double *out)
 {
     if (!x || !out || n < 0) return DBX_ERR_ARG;
     for (int i = 0; i < n; i++) out[i] = sin_cw(x[i]);
+    return DBX_OK;
+}
+
+int dbx_debug_sincos_host(const double *x, int n, double *s_out, double *c_out)
+{
+    if (!x || !s_out || !c_out || n < 0) return DBX_ERR_ARG;
+    for (int i = 0; i < n; i++) sincos_cw(x[i], s_out[i], c_out[i]);
+    return DBX_OK;
+}
+
+int dbx_debug_sinc_digits_host(const double *q, const double *r, int n, signed char *digits)
+{
+    if (!q || !r || !digits || n < 0 || n % 4) return DBX_ERR_ARG;
+    for (int i = 0; i < n; i += 4) {
+        uint32_t lo[4], hi[4];
+        for (int u = 0; u < 4; u++) {
+            const double Q = q[i + u] != 0.0 ? q[i + u] : 1e-30;
+            sinc_word(sin_cw(Q * r[i + u]), kMfC / Q, 1.0 / r[i + u], lo[u], hi[u]);
+        }
+        int dj[kMfJ];
+        pack4(lo, hi, dj);
+        for (int j = 0; j < kMfJ; j++)
+            for (int u = 0; u < 4; u++) digits[(size_t)(i + u) * kMfJ + j] = (signed char)(dj[j] >> (8 * u));
+    }
     return DBX_OK;
 }
 

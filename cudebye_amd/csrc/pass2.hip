@@ -17,10 +17,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <vector>
 
 #include "fastmath.h"
+#include "mfdigits.h"
 #include "internal.h"
 
 namespace dbx {
@@ -322,47 +324,47 @@ __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
     }
 }
 
-// S[r0 + i] = sum over splits of Spart[s][i], i over nr rows x nq, fixed order
+// S[r0 + i] = sum over splits of Spart[s][i], i over nr rows x nq, fixed order; plus
+// shift * rsum[row] when rsum is given (the q-independent part of the matrix-core sum)
 __global__ void p2_reduce_splits_kernel(const double *__restrict__ Spart, int nsplit, int r0, int nr, int nq,
+                                        const unsigned long long *__restrict__ rsum, double shift,
                                         double *__restrict__ S)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)nr * nq;
     if (i >= n) return;
-    double acc = 0.0;
+    double acc = rsum ? shift * (double)rsum[r0 + i / nq] : 0.0;
     for (int s = 0; s < nsplit; s++) acc += Spart[(int64_t)s * n + i];
     S[(int64_t)r0 * nq + i] = acc;
 }
 
 // ---------------------------------------------------------------------------------
-// Debye sum on the matrix cores: S[rows x q] = A[rows x bins] . T[bins x q],
-// A = 2 N (integer), T = sinc(q r).  Split-BF16 limbs, exact FP32 accumulation:
-//   A = sum_i e_i 256^i,   e_i in [-128, 128]   (balanced count digits, I of them: 2N < 2^(8I-1))
-//   T ~ V / 2^40,  V = rint(T 2^40) = sum_j d_j 256^j,  d_j in [-128, 128], d_4 in [-56, 256]
-// Every digit is an integer that BF16 holds exactly.  A scale class s = i + j holds at most
-// min(I, 5) digit pairs, at most one of them with j = 4, so it gains at most 2^14 (min(I,5) + 1)
-// per bin; an FP32 accumulator that collects one class over 32 F(I) bins, F(I) = 31 / (I + 1),
-// stays below 2^24 and is exact, whatever order the matrix core adds in.  Every F(I) steps of
-// 32 bins the classes are folded into FP64: S += sum_s 2^(8s - 40) acc_s.  The only
-// approximation is the rounding of T to 2^-40 (|error| <= 2^-41 per bin, from an FP64 sin with
-// full argument reduction).
+// Debye sum on the matrix cores: S[rows x q] = A[rows x bins] . T[bins x q], A = 2 N (integer),
+// T = sinc(q r), as signed 8-bit digits (the encoding in mfdigits.h) with exact 32-bit integer
+// accumulation (v_mfma_i32_16x16x64_i8).  A digit product is at most 2^14 in magnitude and a
+// scale class s = i + j collects at most min(I, 5) of them per bin, so a 32-bit accumulator
+// over 256 steps of 64 bins stays below 2^31 and is exact whatever order the matrix core adds
+// in.  Every 256 steps the classes are folded into FP64, S += sum_s 2^(8s) acc_s / c, and the
+// 3/8 part, (3/8) sum_k A_k, is added per row from the exact integer sum of its counts.  The
+// only approximation is the rounding of c (T - 3/8) to an integer (|err| <= 1/(2c) < 2^-40 per
+// bin), with T from an FP64 sine with full argument reduction.
 // ---------------------------------------------------------------------------------
-typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kMfImax = 5;  // count digits (2N below 2^39)
-constexpr int kMfJ = 5;     // sinc digits
+constexpr int kMfK = 64;       // bins per step: the K of one matrix instruction
+constexpr int kMfFold = 256;   // steps between folds of the 32-bit classes
 
-// balanced count digits: cdig[i][row][kk] = digit i of 2 N_row,kz[kk] (0 past Kc, past the row's
-// bins or past n_rows)
+// balanced count digits as bytes: cdig[i][row][kk] = digit i of 2 N_row,kz[kk] (0 past Kc, past
+// the row's bins or past n_rows), and rsum[row] += the row's sum of 2 N over the compacted bins.
+// Kc_pad is a multiple of 64, so the 64 entries of a wave share one row.
 __global__ void p2_count_digits_kernel(const unsigned long long *__restrict__ hist,
                                        const RowDesc *__restrict__ rows, int n_rows, const int *__restrict__ kz,
-                                       int Kc, int Kc_pad, int rows_pad, int I, __bf16 *__restrict__ cdig)
+                                       int Kc, int Kc_pad, int rows_pad, int I, signed char *__restrict__ cdig,
+                                       unsigned long long *__restrict__ rsum)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)rows_pad * Kc_pad) return;
+    if (t >= (int64_t)rows_pad * Kc_pad) return;  // whole waves: the total is a multiple of 64
     const int row = (int)(t / Kc_pad);
     const int kk = (int)(t - (int64_t)row * Kc_pad);
     long long v = 0;
@@ -371,75 +373,64 @@ __global__ void p2_count_digits_kernel(const unsigned long long *__restrict__ hi
         const int64_t k = kz[kk];
         if (k < rd.n_bins) v = (long long)(2ull * hist[rd.h_off + k]);
     }
+    unsigned long long sum = (unsigned long long)v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+    if ((threadIdx.x & 63) == 0 && row < n_rows && sum != 0ull) atomicAdd(&rsum[row], sum);
     for (int i = 0; i < I; i++) {
         long long e = v;
         if (i + 1 < I) {
             e = ((v + 128) & 255) - 128;
             v = (v - e) >> 8;
         }
-        cdig[((int64_t)i * rows_pad + row) * Kc_pad + kk] = (__bf16)(float)e;
+        cdig[((int64_t)i * rows_pad + row) * Kc_pad + kk] = (signed char)e;
     }
 }
 
-// The sinc digits: V = rint(2^40 sin(Q r) / (Q r)) as five balanced base-256 digits d_0..d_4
-// (d_j in [-128, 127] for j < 4, d_4 in [-56, 256]), each an integer that bf16 holds exactly.
-// iQ40 = 2^40 / Q; Q = 0 is passed as 1e-30 (sinc rounds to exactly 1 there), so there is no
-// branch.  V is formed in the low bits of an FP64 sum with 1.5 2^52 (one rounding, to nearest
-// even); W = V + 128 (1 + 256 + ... + 256^4) has the unsigned digits d_j + 128.
-__device__ __forceinline__ void sinc_digits(double Q, double iQ40, double r, double ir, float d[kMfJ])
-{
-    const double v = fma(sin_cw(Q * r) * iQ40, ir, 0x1.8p52);
-    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const unsigned lo = (unsigned)u;
-    const int hi = (int)((unsigned)(u >> 32) & 0xFFFFFu) - 0x80000;
-    const unsigned w = lo + 0x80808080u;
-    d[0] = (float)(w & 0xFFu) - 128.f;
-    d[1] = (float)((w >> 8) & 0xFFu) - 128.f;
-    d[2] = (float)((w >> 16) & 0xFFu) - 128.f;
-    d[3] = (float)(w >> 24) - 128.f;
-    d[4] = (float)(hi + (w < lo ? 1 : 0));
-}
-
-// bin distances and their reciprocals: rr[kk] = kz[kk] * rstep, ir[kk] = 1 / rr[kk]
-// (entries past Kc: 1, with zero counts)
+// bin distances and their reciprocals: rr[kk] = kz[kk] * rstep, ir[kk] = 1 / rr[kk].  Entries
+// past Kc (zero counts) continue the bins after the last one, so that the table stays strictly
+// increasing in whole bins and 64 entries spanning 63 bins are always 64 consecutive bins.
 __global__ void p2_rtab_kernel(const int *__restrict__ kz, int Kc, int Kc_pad, double rstep,
                                double *__restrict__ rr, double *__restrict__ ir)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Kc_pad) return;
-    const double r = i < Kc ? (double)kz[i] * rstep : 1.0;
+    const int k = i < Kc ? kz[i] : (Kc > 0 ? kz[Kc - 1] : 0) + (i - Kc + 1);
+    const double r = (double)k * rstep;
     rr[i] = r;
     ir[i] = 1.0 / r;
 }
 
 // One workgroup: WQ x WR waves.  Wave (wq, wr) owns 16 q values (block wq of the workgroup's
 // 16 WQ) and RB blocks of 16 rows (block wr of its WR RB 16).  Operand layout of
-// v_mfma_f32_16x16x32_bf16 (lane l): A[row l%16][k 8(l/16)..+7], B[k 8(l/16)..+7][col l%16],
-// C[row 4(l/16)+v][col l%16].
-//   - Count digits: the I x WR x RB fragments (1 KB each) of a 32-bin step are loaded by the
+// v_mfma_i32_16x16x64_i8 (lane l): A[row l%16][k 16(l/16)..+15], B[k 16(l/16)..+15][col l%16],
+// C[row 4(l/16)+v][col l%16].  A step is 64 compacted bins.
+//   - Count digits: the I x WR x RB fragments (1 KB each) of a step are loaded by the
 //     workgroup's waves together, ahead of use, into a ring of three LDS slots (lane l of a
-//     fragment holds exactly the 16 bytes MFMA lane l reads); one barrier per step.
-//   - Sinc digits: the 16 q x 32 bin values of a step are generated by the WR waves of a q
-//     block together (8 / WR values per thread), through LDS (double-buffered) when WR > 1,
+//     fragment holds exactly the 16 bytes matrix lane l reads); one barrier per step.
+//   - Sinc digits: the 16 q x 64 bin values of a step are generated by the WR waves of a q
+//     block together (16 / WR values per thread), through LDS (double-buffered) when WR > 1,
 //     one step ahead of the matrix cores so that both run at once.  The step's r and 1/r come
-//     through the same LDS ring.
+//     through the same LDS ring.  With ROTATE, a step whose 64 bins are consecutive takes one
+//     sine-cosine pair per thread and rotates it by q rstep for the following bins.
 // The workgroup's bins end at kext of its first row block: rows are ordered by length,
 // longest first.
 template <int RB, int WQ, int WR, int I>
 __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     const double *__restrict__ rr, const double *__restrict__ ir, int Kc_pad, int k_per_split,
-    const int *__restrict__ kext, const __bf16 *__restrict__ cdig, int rows_pad, const double *__restrict__ q,
-    int nq, int nrows, double *__restrict__ Spart)
+    const int *__restrict__ kext, const signed char *__restrict__ cdig, int rows_pad,
+    const double *__restrict__ q, int nq, int nrows, double rstep, double *__restrict__ Spart)
 {
     constexpr int NW = WQ * WR;
     constexpr int NF = I * WR * RB;
     constexpr int LF = (NF + NW - 1) / NW;
     constexpr int C = I + kMfJ - 1;
-    constexpr int V = 8 / WR;
-    constexpr int F = 31 / (I + 1);
-    __shared__ bf16x8 sa[3][NF][64];
-    __shared__ double srt[3][64];
-    __shared__ bf16x8 sd[WR > 1 ? 2 : 1][WR > 1 ? WQ : 1][kMfJ][64];
+    constexpr int V = 16 / WR;
+    constexpr int NG = V / 4;
+    constexpr bool ROTATE = WR == 1 && RB <= 2;
+    __shared__ i32x4 sa[3][NF][64];
+    __shared__ double srt[3][2 * kMfK];
+    __shared__ i32x4 sd[WR > 1 ? 2 : 1][WR > 1 ? WQ : 1][kMfJ][64];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wq = wave % WQ, wr = wave / WQ;
@@ -451,18 +442,26 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     const int gl = g / WR, gt = (g % WR) * V;
     const double Q0 = q[min(q0 + (gl & 15), nq - 1)];
     const double Q = Q0 != 0.0 ? Q0 : 1e-30;
-    const double iQ40 = 0x1p40 / Q;
-    const int gk = (gl >> 4) * 8 + gt;
-    const int kx = (kext[blockIdx.z * (WR * RB)] + 31) & ~31;
+    const double cQ = kMfC / Q;
+    const int gk = (gl >> 4) * 16 + gt;
+    // rotation by one bin: (sin, cos) of Q rstep; the wave's smallest Q, for the test that a
+    // rotated run starts at Q r >= 0.5
+    double sdQ = 0.0, cdQ = 1.0, qmin = Q;
+    if constexpr (ROTATE) {
+        sincos_cw(Q * rstep, sdQ, cdQ);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) qmin = fmin(qmin, __shfl_xor(qmin, off));
+    }
+    const int kx = (kext[blockIdx.z * (WR * RB)] + kMfK - 1) & ~(kMfK - 1);
     const int kb = blockIdx.y * k_per_split;
     const int ke = min(min(Kc_pad, kx), kb + k_per_split);
-    const int nsteps = ke > kb ? (ke - kb) >> 5 : 0;
-    f32x4 acc[RB][C];
+    const int nsteps = ke > kb ? (ke - kb) / kMfK : 0;
+    i32x4 acc[RB][C];
     double acc64[RB][4];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
-        for (int c = 0; c < C; c++) acc[rb][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < C; c++) acc[rb][c] = i32x4{0, 0, 0, 0};
 #pragma unroll
         for (int v = 0; v < 4; v++) acc64[rb][v] = 0.0;
     }
@@ -471,10 +470,9 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
         for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
             for (int c = 0; c < C; c++) {
-                const double scale = __builtin_ldexp(1.0, 8 * c - 40);
 #pragma unroll
-                for (int v = 0; v < 4; v++) acc64[rb][v] += scale * (double)acc[rb][c][v];
-                acc[rb][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+                for (int v = 0; v < 4; v++) acc64[rb][v] += __builtin_ldexp((double)acc[rb][c][v], 8 * c);
+                acc[rb][c] = i32x4{0, 0, 0, 0};
             }
         }
     };
@@ -482,58 +480,79 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     // moves fragments w, w + NW, ... (clamped to the last one: a duplicate writes the same
     // bytes), and every wave moves the step's r table (same bytes again); no branches, so the
     // scheduler can interleave all of it with the matrix instructions.
-    bf16x8 pf[LF];
-    double prt;
+    i32x4 pf[LF];
+    double prr, pir;
     auto load_a = [&](int k0) {
 #pragma unroll
         for (int m = 0; m < LF; m++) {
             const int f = min(wave + m * NW, NF - 1);
             const int i = f / (WR * RB), blk = f - i * (WR * RB);
-            pf[m] = *reinterpret_cast<const bf16x8 *>(
-                cdig + ((int64_t)i * rows_pad + rowt + blk * 16 + col) * Kc_pad + k0 + kq * 8);
+            pf[m] = *reinterpret_cast<const i32x4 *>(
+                cdig + ((int64_t)i * rows_pad + rowt + blk * 16 + col) * Kc_pad + k0 + kq * 16);
         }
     };
     auto store_a = [&](int slot) {
 #pragma unroll
         for (int m = 0; m < LF; m++) sa[slot][min(wave + m * NW, NF - 1)][lane] = pf[m];
     };
-    auto load_r = [&](int k0) { prt = (lane < 32 ? rr : ir - 32)[k0 + lane]; };
-    auto store_r = [&](int slot) { srt[slot][lane] = prt; };
+    auto load_r = [&](int k0) {
+        prr = rr[k0 + lane];
+        pir = ir[k0 + lane];
+    };
+    auto store_r = [&](int slot) {
+        srt[slot][lane] = prr;
+        srt[slot][kMfK + lane] = pir;
+    };
+    // The step in ring slot `slot` may use rotations when its 64 bins are consecutive (the table
+    // is strictly increasing in whole bins, so a span of 63 bins means that) and the wave's
+    // smallest Q r at its first bin is >= 0.5: fifteen rotations add below 4e-15 to sin there,
+    // and so below 8e-15 to sinc, under 1% of the 1/c grid of the sinc digits.  One answer per
+    // wave.
+    auto rotatable = [&](int slot) -> bool {
+        if constexpr (!ROTATE) {
+            return false;
+        } else {
+            const double span = srt[slot][kMfK - 1] - srt[slot][0];
+            const int ok = (span > 62.5 * rstep) && (span < 63.5 * rstep) && (qmin * srt[slot][0] >= 0.5);
+            return __builtin_amdgcn_readfirstlane(ok) != 0;
+        }
+    };
     // sinc digits of the step whose r table is in srt[slot]: B fragments (WR == 1) or sd[buf]
-    bf16x8 Bn[kMfJ];
-    auto gen = [&](int slot, int buf) {
+    i32x4 Bn[kMfJ];
+    auto gen = [&](auto rot_tag, int slot, int buf) {
+        constexpr bool ROT = decltype(rot_tag)::value;
+        double sv = 0.0, cv = 0.0;
+        if constexpr (ROT) sincos_cw(Q * srt[slot][gk], sv, cv);
+        int dw[NG][kMfJ];
+#pragma unroll
+        for (int g4 = 0; g4 < NG; g4++) {
+            uint32_t lo[4], hi[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int t = g4 * 4 + u;
+                double s;
+                if constexpr (ROT) {
+                    if (t > 0) {
+                        const double sn = fma(sv, cdQ, cv * sdQ);
+                        cv = fma(cv, cdQ, -(sv * sdQ));
+                        sv = sn;
+                    }
+                    s = sv;
+                } else {
+                    s = sin_cw(Q * srt[slot][gk + t]);
+                }
+                sinc_word(s, cQ, srt[slot][kMfK + gk + t], lo[u], hi[u]);
+            }
+            pack4(lo, hi, dw[g4]);
+        }
         if constexpr (WR == 1) {
 #pragma unroll
-            for (int t = 0; t < 8; t++) {
-                float d[kMfJ];
-                sinc_digits(Q, iQ40, srt[slot][gk + t], srt[slot][32 + gk + t], d);
-#pragma unroll
-                for (int j = 0; j < kMfJ; j++) Bn[j][t] = (__bf16)d[j];
-            }
+            for (int j = 0; j < kMfJ; j++) Bn[j] = i32x4{dw[0][j], dw[1][j], dw[2][j], dw[3][j]};
         } else {
-            __bf16 vals[kMfJ][V];
+            static_assert(NG == 2, "two waves per q block: eight bins per thread");
 #pragma unroll
-            for (int t = 0; t < V; t++) {
-                float d[kMfJ];
-                sinc_digits(Q, iQ40, srt[slot][gk + t], srt[slot][32 + gk + t], d);
-#pragma unroll
-                for (int j = 0; j < kMfJ; j++) vals[j][t] = (__bf16)d[j];
-            }
-#pragma unroll
-            for (int j = 0; j < kMfJ; j++) {
-                __bf16 *dst = reinterpret_cast<__bf16 *>(&sd[buf][wq][j][gl]) + gt;
-                if constexpr (V == 4) {
-                    bf16x4 w;
-#pragma unroll
-                    for (int t = 0; t < 4; t++) w[t] = vals[j][t];
-                    *reinterpret_cast<bf16x4 *>(dst) = w;
-                } else {
-                    bf16x2 w;
-#pragma unroll
-                    for (int t = 0; t < 2; t++) w[t] = vals[j][t];
-                    *reinterpret_cast<bf16x2 *>(dst) = w;
-                }
-            }
+            for (int j = 0; j < kMfJ; j++)
+                *(reinterpret_cast<i32x2 *>(&sd[buf][wq][j][gl]) + (gt >> 3)) = i32x2{dw[0][j], dw[1][j]};
         }
     };
     // Pipeline: at step s the matrix cores take A(s) (count digits, LDS ring slot s%3) and the
@@ -547,22 +566,25 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
         store_a(0);
         store_r(0);
         if (nsteps > 1) {
-            load_r(kb + 32);
+            load_r(kb + kMfK);
             store_r(1);
         }
     }
     __syncthreads();
     if (nsteps > 0) {
-        gen(0, 0);
-        if (nsteps > 1) load_a(kb + 32);
-        if (nsteps > 2) load_r(kb + 64);
+        if (rotatable(0))
+            gen(std::true_type{}, 0, 0);
+        else
+            gen(std::false_type{}, 0, 0);
+        if (nsteps > 1) load_a(kb + kMfK);
+        if (nsteps > 2) load_r(kb + 2 * kMfK);
     }
     __syncthreads();
-    int slot = 0, nfold = 0;
-    for (int s = 0; s < nsteps; s++) {
-        const int s1 = slot == 2 ? 0 : slot + 1;
-        const int s2 = s1 == 2 ? 0 : s1 + 1;
-        bf16x8 B[kMfJ];
+    // one step: the matrix instructions of step s interleaved with the generation of step s+1;
+    // branch-free inside (past the last step the generator and the ring take stale or clamped,
+    // in-bounds data that no later step reads)
+    auto step = [&](auto rot_tag, int s, int slot, int s1, int s2) {
+        i32x4 B[kMfJ];
         if constexpr (WR == 1) {
 #pragma unroll
             for (int j = 0; j < kMfJ; j++) B[j] = Bn[j];
@@ -574,26 +596,33 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
         for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
             for (int i = 0; i < I; i++) {
-                const bf16x8 A = sa[slot][i * (WR * RB) + wr * RB + rb][lane];
+                const i32x4 A = sa[slot][i * (WR * RB) + wr * RB + rb][lane];
 #pragma unroll
                 for (int j = 0; j < kMfJ; j++)
-                    acc[rb][i + j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[j], acc[rb][i + j], 0, 0, 0);
+                    acc[rb][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[j], acc[rb][i + j], 0, 0, 0);
             }
         }
-        // Branch-free: past the last step the generator and the ring take stale or clamped
-        // (in-bounds) data that no later step reads.
-        gen(s1, (s + 1) & 1);
+        gen(rot_tag, s1, (s + 1) & 1);
         store_a(s1);
         store_r(s2);
-        load_a(min(kb + (s + 2) * 32, ke - 32));
-        load_r(min(kb + (s + 3) * 32, ke - 32));
+        load_a(min(kb + (s + 2) * kMfK, ke - kMfK));
+        load_r(min(kb + (s + 3) * kMfK, ke - kMfK));
         // interleave: one matrix instruction, then three vector instructions of the generator
 #pragma unroll
         for (int x = 0; x < RB * I * kMfJ; x++) {
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
             __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
         }
-        if (++nfold == F) {
+    };
+    int slot = 0, nfold = 0;
+    for (int s = 0; s < nsteps; s++) {
+        const int s1 = slot == 2 ? 0 : slot + 1;
+        const int s2 = s1 == 2 ? 0 : s1 + 1;
+        if (rotatable(s1))
+            step(std::true_type{}, s, slot, s1, s2);
+        else
+            step(std::false_type{}, s, slot, s1, s2);
+        if (++nfold == kMfFold) {
             fold();
             nfold = 0;
         }
@@ -603,12 +632,13 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     fold();
     const int qq = q0 + col;
     if (qq < nq) {
+        constexpr double inv_c = 1.0 / kMfC;
 #pragma unroll
         for (int rb = 0; rb < RB; rb++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int row = rowt + (wr * RB + rb) * 16 + kq * 4 + v;
-                if (row < nrows) Spart[((int64_t)blockIdx.y * nrows + row) * nq + qq] = acc64[rb][v];
+                if (row < nrows) Spart[((int64_t)blockIdx.y * nrows + row) * nq + qq] = acc64[rb][v] * inv_c;
             }
     }
 }
@@ -631,13 +661,13 @@ static MfShape mf_shape(int n_rows, int I)
 
 template <int RB, int WQ, int WR>
 static void mf_launch_i(int I, dim3 grid, hipStream_t st, const double *rr, const double *ir, int Kc_pad,
-                        int kper, const int *kext, const __bf16 *cdig, int rows_pad, const double *q, int nq,
-                        int nrows, double *Spart)
+                        int kper, const int *kext, const signed char *cdig, int rows_pad, const double *q,
+                        int nq, int nrows, double rstep, double *Spart)
 {
     const dim3 blk(WQ * WR * 64);
 #define DBX_MF(II)                                                                                              \
     hipLaunchKernelGGL((p2_debye_mfma_kernel<RB, WQ, WR, II>), grid, blk, 0, st, rr, ir, Kc_pad, kper, kext, \
-                       cdig, rows_pad, q, nq, nrows, Spart)
+                       cdig, rows_pad, q, nq, nrows, rstep, Spart)
     switch (I) {
     case 1: DBX_MF(1); break;
     case 2: DBX_MF(2); break;
@@ -654,17 +684,17 @@ static void mf_launch_i(int I, dim3 grid, hipStream_t st, const double *rr, cons
 }
 
 static void mf_launch(const MfShape &sh, int I, dim3 grid, hipStream_t st, const double *rr, const double *ir,
-                      int Kc_pad, int kper, const int *kext, const __bf16 *cdig, int rows_pad, const double *q,
-                      int nq, int nrows, double *Spart)
+                      int Kc_pad, int kper, const int *kext, const signed char *cdig, int rows_pad,
+                      const double *q, int nq, int nrows, double rstep, double *Spart)
 {
     if (sh.RB == 1)
-        mf_launch_i<1, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, Spart);
+        mf_launch_i<1, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
     else if (sh.RB == 2)
-        mf_launch_i<2, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, Spart);
+        mf_launch_i<2, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
     else if (sh.WR == 1)
-        mf_launch_i<4, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, Spart);
+        mf_launch_i<4, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
     else
-        mf_launch_i<4, 4, 2>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, Spart);
+        mf_launch_i<4, 4, 2>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
 }
 
 // S[r][q] += self[r] + sum_k low[r][k] sinc(q r_k); grid.y over rows
@@ -735,6 +765,7 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     const size_t o_self = take(sizeof(double) * n_rows);
     const size_t o_low = take(sizeof(double) * (size_t)n_rows * nlow);
     const size_t o_kext = take(sizeof(int) * (n_rb16 + 8));
+    const size_t o_rsum = take(sizeof(unsigned long long) * n_rows);
     const size_t o_flags = take((size_t)(nblk * kScanBlock));
     char *base = (char *)scratch(ctx, kSlotPass2, off);
     RowDesc *d_rows = (RowDesc *)(base + o_rows);
@@ -744,6 +775,7 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     double *self = (double *)(base + o_self);
     double *low = (double *)(base + o_low);
     int *kext = (int *)(base + o_kext);
+    unsigned long long *rsum = (unsigned long long *)(base + o_rsum);
     unsigned char *flags = (unsigned char *)(base + o_flags);
     const unsigned long long *h = (const unsigned long long *)d_hist_base;
 
@@ -784,16 +816,16 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
             hipLaunchKernelGGL(p2_debye_fp64_kernel, dim3(qblocks, nsplit), dim3(kDsThreads), 0, ctx->stream,
                                kz, Kc, kper, A, np, rstep, d_q, nq, Spart);
             hipLaunchKernelGGL(p2_reduce_splits_kernel, dim3((unsigned)(((int64_t)np * nq + 255) / 256)),
-                               dim3(256), 0, ctx->stream, Spart, nsplit, r0, np, nq, d_S);
+                               dim3(256), 0, ctx->stream, Spart, nsplit, r0, np, nq, nullptr, 0.0, d_S);
         }
     } else if (Kc > 0) {
-        // balanced count digits: 2 N < 2^(8 I - 1)
+        // balanced count digits: 2 N < 127 256^(I - 1)
         int I = 1;
-        while (I < kMfImax && (2.0 * (double)max_count) >= std::ldexp(1.0, 8 * I - 1)) I++;
-        DBX_REQUIRE(2.0 * (double)max_count < std::ldexp(1.0, 8 * kMfImax - 1), DBX_ERR_RANGE,
-                    "bin count too large for the split-BF16 Debye sum");
-        const int Kc_pad = (Kc + 31) & ~31;
-        const int ksteps = Kc_pad / 32;
+        while (I < kMfImax && 2.0 * (double)max_count >= mf_count_limit(I)) I++;
+        DBX_REQUIRE(2.0 * (double)max_count < mf_count_limit(kMfImax), DBX_ERR_RANGE,
+                    "bin count too large for the matrix-core Debye sum");
+        const int Kc_pad = (Kc + kMfK - 1) & ~(kMfK - 1);
+        const int ksteps = Kc_pad / kMfK;
         double *rtab = (double *)scratch(ctx, kSlotRtab, sizeof(double) * 2 * (size_t)Kc_pad);
         hipLaunchKernelGGL(p2_rtab_kernel, dim3((Kc_pad + 255) / 256), dim3(256), 0, ctx->stream, kz, Kc,
                            Kc_pad, rstep, rtab, rtab + Kc_pad);
@@ -803,19 +835,21 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
         const int rgroups = rows_pad / sh.rows();
         const int target = 2048 * 4 / (sh.WQ * sh.WR);  // about 8 waves per SIMD over the grid
         int nsplit = std::max(1, std::min((target + qgroups * rgroups - 1) / (qgroups * rgroups), ksteps));
-        const int kper = ((ksteps + nsplit - 1) / nsplit) * 32;
+        const int kper = ((ksteps + nsplit - 1) / nsplit) * kMfK;
         nsplit = (Kc_pad + kper - 1) / kper;
-        __bf16 *cdig = (__bf16 *)scratch(ctx, kSlotDigits, sizeof(__bf16) * (size_t)I * rows_pad * Kc_pad);
+        signed char *cdig = (signed char *)scratch(ctx, kSlotDigits, (size_t)I * rows_pad * Kc_pad);
         double *Spart = (double *)scratch(ctx, kSlotPass2b, sizeof(double) * (size_t)nsplit * n_rows * nq);
         const int64_t nd = (int64_t)rows_pad * Kc_pad;
+        hipLaunchKernelGGL(p2_zero_kernel, dim3((unsigned)((2 * n_rows + 255) / 256)), dim3(256), 0, ctx->stream,
+                           (unsigned int *)rsum, (int64_t)2 * n_rows);
         hipLaunchKernelGGL(p2_count_digits_kernel, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, ctx->stream,
-                           h, d_rows, n_rows, kz, Kc, Kc_pad, rows_pad, I, cdig);
+                           h, d_rows, n_rows, kz, Kc, Kc_pad, rows_pad, I, cdig, rsum);
         const int nblocks = rows_pad / 16;
         hipLaunchKernelGGL(p2_kext_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, kz, Kc, d_rows,
                            n_rows, 16, nblocks, kext);
         if (ctx->profiling) DBX_HIP(hipEventRecord(ctx->ev_mf[0], ctx->stream));
         mf_launch(sh, I, dim3((unsigned)qgroups, (unsigned)nsplit, (unsigned)rgroups), ctx->stream, rtab,
-                  rtab + Kc_pad, Kc_pad, kper, kext, cdig, rows_pad, d_q, nq, n_rows, Spart);
+                  rtab + Kc_pad, Kc_pad, kper, kext, cdig, rows_pad, d_q, nq, n_rows, rstep, Spart);
         if (ctx->profiling) {
             DBX_HIP(hipEventRecord(ctx->ev_mf[1], ctx->stream));
             // issued matrix instructions: per workgroup, its steps x waves x RB x I x 5
@@ -824,8 +858,8 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
             DBX_HIP(hipStreamSynchronize(ctx->stream));
             double steps = 0;
             for (int z = 0; z < rgroups; z++) {
-                const int kx = std::min(Kc_pad, (hk[z * (sh.rows() / 16)] + 31) & ~31);
-                steps += (double)(kx / 32);
+                const int kx = std::min(Kc_pad, (hk[z * (sh.rows() / 16)] + kMfK - 1) & ~(kMfK - 1));
+                steps += (double)(kx / kMfK);
             }
             // algorithmic work: 2 nQ flops per nonzero bin of every row
             hipLaunchKernelGGL(p2_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, (unsigned int *)tot, (int64_t)2);
@@ -836,13 +870,13 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
             DBX_HIP(hipMemcpyAsync(&nnz, tot, sizeof(nnz), hipMemcpyDeviceToHost, ctx->stream));
             DBX_HIP(hipStreamSynchronize(ctx->stream));
             const double waves = (double)qgroups * sh.WQ * sh.WR;
-            ctx->mf_flops = steps * waves * sh.RB * I * 5 * 16384.0;
+            ctx->mf_flops = steps * waves * sh.RB * I * kMfJ * 32768.0;
             ctx->mf_useful = (double)nnz * (double)nq * 2.0;
             ctx->mf_digits = I;
             ctx->mf_timed = true;
         }
         hipLaunchKernelGGL(p2_reduce_splits_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
-                           dim3(256), 0, ctx->stream, Spart, nsplit, 0, n_rows, nq, d_S);
+                           dim3(256), 0, ctx->stream, Spart, nsplit, 0, n_rows, nq, rsum, kMfShift, d_S);
     } else {
         DBX_HIP(hipMemsetAsync(d_S, 0, sizeof(double) * (size_t)n_rows * nq, ctx->stream));
     }

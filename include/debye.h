@@ -118,8 +118,9 @@ int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int
               double *I_out, double *S_out);
 
 /* Pass-2 arithmetic of a context (default DBX_PASS2_MFMA):
- *   DBX_PASS2_MFMA: split-BF16 matrix-core contraction with exact FP32 accumulation; sinc is
- *     evaluated in FP64 and rounded to a multiple of 2^-40 (|error| <= 2^-41 per bin);
+ *   DBX_PASS2_MFMA: matrix-core contraction on signed 8-bit digits with exact 32-bit integer
+ *     accumulation; sinc is evaluated in FP64, shifted by 3/8 and rounded to a multiple of
+ *     1 / (3 2^38) (|error| < 2^-40 per bin);
  *   DBX_PASS2_FP64: FP64 vector arithmetic throughout (the reference's debyeRowSum). */
 enum { DBX_PASS2_MFMA = 0, DBX_PASS2_FP64 = 1 };
 int dbx_set_pass2_mode(dbx_ctx *ctx, int mode);

@@ -43,14 +43,21 @@ int dbx_selftest_round(uint32_t lo, uint32_t hi, uint64_t *n_bad);
  * all CUs), the peak against which pass 1's one LDS increment per pair is reported. */
 int dbx_selftest_lds_atomic_rate(double *atomics_per_s);
 
-/* The pass-2 FP64 sine (dbx_selftest_sin's device function) evaluated on the host. */
+/* The pass-2 FP64 sine (dbx_selftest_sin's device function) evaluated on the host, and the
+ * sine-cosine pair the sinc generator starts each run of consecutive bins from. */
 int dbx_debug_sin_host(const double *x, int n, double *out);
+int dbx_debug_sincos_host(const double *x, int n, double *s_out, double *c_out);
+
+/* The five signed sinc digits d_0..d_4 (digits[5 i + j]) that the matrix-core Debye sum forms
+ * for sinc(q[i] r[i]): sinc = 3/8 + sum_j d_j 256^j / (3 2^38) within 2^-40 (the device code,
+ * built for the host; q = 0 is taken as 1e-30).  n must be a multiple of 4. */
+int dbx_debug_sinc_digits_host(const double *q, const double *r, int n, signed char *digits);
 
 /* Profiling of the matrix-core Debye sum on ctx (off by default).  When on, each pass 2
  * records HIP events around the contraction kernel and reads back its per-row-block bin
  * extents to count the matrix instructions it issues.  dbx_debug_pass2_profile returns,
- * for the last pass 2: the contraction kernel's time (ms), the BF16 matrix flops it issued
- * (16 x 16 x 32 x 2 per v_mfma_f32_16x16x32_bf16), the algorithmic flops (2 nQ per nonzero bin
+ * for the last pass 2: the contraction kernel's time (ms), the 8-bit integer matrix operations
+ * it issued (16 x 16 x 64 x 2 per v_mfma_i32_16x16x64_i8), the algorithmic flops (2 nQ per nonzero bin
  * of every row: the contraction counted once, whatever the digit count), the compacted bin
  * count and the count digits. */
 int dbx_debug_set_profiling(dbx_ctx *ctx, int on);

@@ -11,8 +11,8 @@ from tests.conftest import read_debi
 pytestmark = pytest.mark.gpu
 
 # FP64 mode: only summation order and the device sin differ from the oracle's compensated
-# FP64 sum.  MFMA mode (split-BF16 digits, exact FP32 accumulation): the sinc table is
-# rounded to a multiple of 2^-40, held to 1e-6 relative on every I(Q).
+# FP64 sum.  MFMA mode (signed 8-bit digits, exact 32-bit integer accumulation): the sinc table
+# is rounded to a multiple of 1 / (3 2^38), held to 1e-6 relative on every I(Q).
 TOL = {cd.DBX_PASS2_FP64: 1e-9, cd.DBX_PASS2_MFMA: 1e-6}
 MODES = [cd.DBX_PASS2_MFMA, cd.DBX_PASS2_FP64]
 
