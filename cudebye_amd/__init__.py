@@ -26,7 +26,7 @@ CLI_PATH = os.path.join(_HERE, "bin", "debye")
 DBX_OK = 0
 ERRORS = {-1: "DBX_ERR_ARG", -2: "DBX_ERR_HIP", -3: "DBX_ERR_IO", -4: "DBX_ERR_PARSE",
           -5: "DBX_ERR_RANGE", -6: "DBX_ERR_NOMEM", -7: "DBX_ERR_BUDGET", -8: "DBX_ERR_STATE"}
-DBX_ERR_RANGE = -5
+DBX_ERR_ARG, DBX_ERR_HIP, DBX_ERR_IO, DBX_ERR_PARSE, DBX_ERR_RANGE, DBX_ERR_NOMEM = -1, -2, -3, -4, -5, -6
 DBX_RUN_CRLF = 1
 DBX_RUN_QUIET = 2
 DBX_RUN_MFMA = 4
@@ -156,7 +156,8 @@ EXPORTS = ("dbx_open", "dbx_close", "dbx_last_error", "dbx_stream", "dbx_q_from_
            "dbx_last_kernel_times", "dbx_set_pass2_mode", "dbx_selftest_sin",
            "dbx_batch_intensity", "dbx_server_open", "dbx_server_close", "dbx_server_last_error",
            "dbx_submit", "dbx_poll", "dbx_wait", "dbx_reweight", "dbx_request_histogram",
-           "dbx_release", "dbx_server_stats_get", "dbx_read_debx", "dbx_run_folder")
+           "dbx_release", "dbx_server_stats_get", "dbx_read_debx", "dbx_run_folder",
+           "dbx_deb_header_defaults", "dbx_xyz_to_deb", "dbx_average_debi")
 
 
 def _p(a):
@@ -203,6 +204,49 @@ def read_debx(path, rstep, n_bins):
     if rc != DBX_OK:
         raise DebyeError(rc, f"cannot read {path}")
     return h
+
+
+class DebHeader(C.Structure):
+    """dbx_deb_header: the .deb header values the converter writes."""
+    _fields_ = [("device", C.c_int), ("block", C.c_int), ("grid", C.c_int), ("wavelength", C.c_double),
+                ("tt_min", C.c_double), ("tt_max", C.c_double), ("tt_step", C.c_double), ("rstep", C.c_double),
+                ("hist", C.c_int)]
+
+
+DBX_AVG_NONE, DBX_AVG_INTEGRAL, DBX_AVG_MAX = 0, 1, 2
+
+
+def xyz_to_deb(xyz, deb=None, table=None, **header):
+    """Converts an .xyz file to .deb (Matlab/xyz2deb.m); header fields of DebHeader override the
+    converter's defaults.  Returns the .deb path."""
+    h = DebHeader()
+    lib().dbx_deb_header_defaults(C.byref(h))
+    for k, v in header.items():
+        setattr(h, k, v)
+    xyz = os.fspath(xyz)
+    enc = lambda v: None if v is None else os.fspath(v).encode()
+    rc = lib().dbx_xyz_to_deb(xyz.encode(), enc(deb), enc(table), C.byref(h))
+    if rc != DBX_OK:
+        raise DebyeError(rc, f"cannot convert {xyz}")
+    return os.fspath(deb) if deb is not None else xyz[:-3] + "deb"
+
+
+def average_debi(paths, norm="none", step=0.02):
+    """(2theta, I) average of several .debI patterns (Matlab/plotdebi.m); norm: "none" (mean),
+    "int" (divided by its trapezoidal integral) or "max" (by its maximum)."""
+    code = {"none": DBX_AVG_NONE, "int": DBX_AVG_INTEGRAL, "integral": DBX_AVG_INTEGRAL,
+            "max": DBX_AVG_MAX, "maximum": DBX_AVG_MAX}[norm]
+    arr = (C.c_char_p * len(paths))(*[os.fspath(p).encode() for p in paths])
+    n = C.c_int(0)
+    fn = lib().dbx_average_debi
+    rc = fn(arr, len(paths), code, C.c_double(step), None, None, 0, C.byref(n))
+    if rc != DBX_OK:
+        raise DebyeError(rc, "average_debi")
+    tt, I = np.zeros(n.value), np.zeros(n.value)
+    rc = fn(arr, len(paths), code, C.c_double(step), _p(tt), _p(I), n.value, C.byref(n))
+    if rc != DBX_OK:
+        raise DebyeError(rc, "average_debi")
+    return tt, I
 
 
 def lds_atomic_rate():

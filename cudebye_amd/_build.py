@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdebye.so")
 CLI = os.path.join(HERE, "bin", "debye")
-SOURCES = ["pass1.hip", "pass2.hip", "api.cpp", "server.cpp", "deb_run.cpp", "selftest.hip"]
+SOURCES = ["pass1.hip", "pass2.hip", "api.cpp", "server.cpp", "deb_run.cpp", "xyz2deb.cpp", "selftest.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -47,7 +47,7 @@ def build(verbose=False):
             if o.strip():
                 print(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs + ["-ldl"])
     os.replace(tmp, LIB)
     _run([HIPCC, "-O2", "-std=c++17", "-o", CLI + ".tmp", os.path.join(CSRC, "cli.cpp"),
           "-L" + HERE, "-ldebye", "-Wl,-rpath,$ORIGIN/.."])

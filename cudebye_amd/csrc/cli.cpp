@@ -1,17 +1,76 @@
 // `debye file.deb [--crlf] [--quiet] [--mfma] [--from-debx]`: the command-line program of
 // cuDebye (cuDebye.exe <file.deb>, cuDebye-v15.cu:599-614) on top of libdebye.
-// `debye --folder DIR [...]`: every .deb in DIR without a .debI (Matlab/multisim.m), served
-// together.
+// `debye --folder DIR [...]`: every .xyz in DIR without a .deb converted, then every .deb without
+// a .debI calculated (Matlab/multisim.m), served together.
+// `debye --xyz2deb FILE.xyz [--table PATH] [--rstep R]`: the converter (Matlab/xyz2deb.m).
+// `debye --average [--norm none|int|max] [--step S] -o OUT A.debI B.debI ...`: the averaged
+// pattern of several .debI files (Matlab/plotdebi.m), written as "%.10g\t%.10g" lines.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <string>
+#include <vector>
 
 #include "../../include/debye.h"
 
+static int usage(const char *prog)
+{
+    std::fprintf(stderr,
+                 "usage: %s file.deb [--crlf] [--quiet] [--mfma] [--from-debx]\n"
+                 "       %s --folder DIR [--crlf] [--quiet] [--mfma]\n"
+                 "       %s --xyz2deb FILE.xyz [--table PATH] [--rstep R]\n"
+                 "       %s --average [--norm none|int|max] [--step S] -o OUT A.debI B.debI ...\n",
+                 prog, prog, prog, prog);
+    return 2;
+}
+
+static int average(int argc, char **argv)
+{
+    int norm = DBX_AVG_NONE;
+    double step = 0.02;
+    const char *out = nullptr;
+    std::vector<const char *> files;
+    for (int i = 2; i < argc; i++) {
+        if (!std::strcmp(argv[i], "--norm") && i + 1 < argc) {
+            const char *v = argv[++i];
+            if (!std::strcmp(v, "int") || !std::strcmp(v, "integral"))
+                norm = DBX_AVG_INTEGRAL;
+            else if (!std::strcmp(v, "max") || !std::strcmp(v, "maximum"))
+                norm = DBX_AVG_MAX;
+            else if (std::strcmp(v, "none") != 0)
+                return usage(argv[0]);
+        } else if (!std::strcmp(argv[i], "--step") && i + 1 < argc) {
+            step = std::atof(argv[++i]);
+        } else if (!std::strcmp(argv[i], "-o") && i + 1 < argc) {
+            out = argv[++i];
+        } else {
+            files.push_back(argv[i]);
+        }
+    }
+    if (!out || files.empty()) return usage(argv[0]);
+    int n = 0;
+    int rc = dbx_average_debi(files.data(), (int)files.size(), norm, step, nullptr, nullptr, 0, &n);
+    if (rc != DBX_OK) return 1;
+    std::vector<double> tt(n), I(n);
+    rc = dbx_average_debi(files.data(), (int)files.size(), norm, step, tt.data(), I.data(), n, &n);
+    if (rc != DBX_OK) return 1;
+    FILE *f = std::fopen(out, "w");
+    if (!f) {
+        std::fprintf(stderr, "ERROR: cannot write %s\n", out);
+        return 1;
+    }
+    for (int k = 0; k < n; k++) std::fprintf(f, "%.10g\t%.10g\n", tt[k], I[k]);
+    return std::fclose(f) == 0 ? 0 : 1;
+}
+
 int main(int argc, char **argv)
 {
+    if (argc > 1 && !std::strcmp(argv[1], "--average")) return average(argc, argv);
     const char *path = nullptr;
+    const char *table = nullptr;
+    double rstep = 0;
     int flags = 0;
-    bool folder = false;
+    bool folder = false, convert = false;
     for (int i = 1; i < argc; i++) {
         if (!std::strcmp(argv[i], "--crlf"))
             flags |= DBX_RUN_CRLF;
@@ -23,13 +82,21 @@ int main(int argc, char **argv)
             flags |= DBX_RUN_FROM_DEBX;
         else if (!std::strcmp(argv[i], "--folder"))
             folder = true;
+        else if (!std::strcmp(argv[i], "--xyz2deb"))
+            convert = true;
+        else if (!std::strcmp(argv[i], "--table") && i + 1 < argc)
+            table = argv[++i];
+        else if (!std::strcmp(argv[i], "--rstep") && i + 1 < argc)
+            rstep = std::atof(argv[++i]);
         else
             path = argv[i];
     }
-    if (!path) {
-        std::fprintf(stderr, "usage: %s file.deb [--crlf] [--quiet] [--mfma] [--from-debx]\n"
-                             "       %s --folder DIR [--crlf] [--quiet] [--mfma]\n", argv[0], argv[0]);
-        return 2;
+    if (!path) return usage(argv[0]);
+    if (convert) {
+        dbx_deb_header h;
+        dbx_deb_header_defaults(&h);
+        if (rstep > 0) h.rstep = rstep;
+        return dbx_xyz_to_deb(path, nullptr, table, &h) == DBX_OK ? 0 : 1;
     }
     const int rc = folder ? dbx_run_folder(path, flags & ~DBX_RUN_FROM_DEBX) : dbx_run_deb(path, flags);
     return rc == DBX_OK ? 0 : 1;

@@ -396,6 +396,26 @@ extern "C" int dbx_run_folder(const char *dir, int flags)
         std::fprintf(stderr, "ERROR: >>%s<< is not a folder\n", dir);
         return DBX_ERR_IO;
     }
+    // first every .xyz without its .deb, converted with the default header (multisim.m:17-36)
+    std::vector<std::string> xyz;
+    while (dirent *e = readdir(dp)) {
+        const std::string name = e->d_name;
+        if (name.size() < 5 || name.compare(name.size() - 4, 4, ".xyz") != 0) continue;
+        const std::string full = std::string(dir) + "/" + name;
+        struct stat sb;
+        if (stat((full.substr(0, full.size() - 3) + "deb").c_str(), &sb) == 0) continue;
+        xyz.push_back(full);
+    }
+    std::sort(xyz.begin(), xyz.end());
+    for (size_t i = 0; i < xyz.size(); i++) {
+        logf(quiet, "Convert File %zu of %zu -- %s\n", i + 1, xyz.size(), xyz[i].c_str());
+        const int rc = dbx_xyz_to_deb(xyz[i].c_str(), nullptr, nullptr, nullptr);
+        if (rc != DBX_OK) {
+            closedir(dp);
+            return rc;
+        }
+    }
+    rewinddir(dp);
     std::vector<std::string> files;
     while (dirent *e = readdir(dp)) {
         const std::string name = e->d_name;

@@ -232,10 +232,40 @@ int dbx_run_deb(const char *path, int flags);
  * not a bin centre to seven digits, DBX_ERR_RANGE for a bin >= n_bins.  n_nonzero (optional)
  * receives the number of nonzero bins. */
 int dbx_read_debx(const char *path, double rstep, int64_t n_bins, uint64_t *hist, int64_t *n_nonzero);
-/* Folder run (Matlab/multisim.m:38-58, one cuDebye.exe per .deb without a .debI): every such
- * .deb in `dir`, in name order, through one server (dbx_server_open); each file's outputs are
- * those of dbx_run_deb on it alone.  flags as for dbx_run_deb (DBX_RUN_FROM_DEBX excepted). */
+/* Folder run (Matlab/multisim.m:17-58): every .xyz in `dir` without its .deb is converted
+ * first (dbx_xyz_to_deb with the default header and table), then every .deb without a .debI,
+ * in name order, runs through one server (dbx_server_open); each file's outputs are those of
+ * dbx_run_deb on it alone.  flags as for dbx_run_deb (DBX_RUN_FROM_DEBX excepted). */
 int dbx_run_folder(const char *dir, int flags);
+
+/* ---- host tools around the driver -------------------------------------------------------
+ * .deb header values of the converter (the defaults of Matlab/xyz2deb.m:13-25: device 0,
+ * block 16, grid 2048, wavelength 1.5418, 2theta 10 to 140 step 0.02, rstep 0.0001, hist 0). */
+typedef struct dbx_deb_header {
+    int device, block, grid;
+    double wavelength, tt_min, tt_max, tt_step, rstep;
+    int hist;
+} dbx_deb_header;
+void dbx_deb_header_defaults(dbx_deb_header *h);
+/* xyz -> deb (Matlab/xyz2deb.m:169-266): lines "element x y z [occupancy [Beq]]" (the first line
+ * fixes the column count; occupancy 1 and Beq 0 where absent); subgroups are the runs of equal
+ * (element, occupancy, Beq) after a stable sort by element name, occupancy and Beq; coordinates
+ * are centred on the middle of each axis' range; Rmax = 2 ceil(1.025 max r); scattering-factor
+ * coefficients from table_path (rows "name a1..a5 c b1..b5" after '#' comments and one column
+ * header; NULL: data/atomic_scattering_factors.txt beside libdebye.so, the Waasmaier-Kirfel
+ * table the reference ships).  deb_path NULL: xyz_path with its last three characters replaced
+ * by "deb".  hdr NULL: dbx_deb_header_defaults.  DBX_ERR_PARSE for an element not in the
+ * table (the reference writes a .deb without its coefficients). */
+int dbx_xyz_to_deb(const char *xyz_path, const char *deb_path, const char *table_path, const dbx_deb_header *hdr);
+/* Average of .debI patterns (Matlab/plotdebi.m:13-63): the common 2theta range [max of the
+ * files' first values (at least 0), min of their last values (at most 180)] at `step`
+ * (0.02 in the reference); each raw pattern through its not-a-knot cubic spline, summed, then
+ * divided by the file count (DBX_AVG_NONE), by its trapezoidal integral (DBX_AVG_INTEGRAL)
+ * or by its maximum (DBX_AVG_MAX).  n_out receives the point count; tt_out / I_out NULL: size
+ * query only, else capacity must hold n_out points. */
+enum { DBX_AVG_NONE = 0, DBX_AVG_INTEGRAL = 1, DBX_AVG_MAX = 2 };
+int dbx_average_debi(const char *const *paths, int n_files, int norm, double step, double *tt_out, double *I_out,
+                     int capacity, int *n_out);
 
 /* ---- diagnostics --------------------------------------------------------------------
  * Self tests of the device arithmetic behind the pass-1 bin rule (no reference

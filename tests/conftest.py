@@ -34,6 +34,8 @@ def golden_dir(tmp_path_factory):
     _gunzip_to(os.path.join(GOLDEN, "gamma", "GammaAl2O3_10nm.debI.gz"), str(d / "GammaAl2O3_10nm.debI.ref"))
     _gunzip_to(os.path.join(GOLDEN, "uh3", "UH3_7nm.deb.gz"), str(d / "UH3_7nm.deb"))
     _gunzip_to(os.path.join(GOLDEN, "uh3", "UH3_7nm.debI.gz"), str(d / "UH3_7nm.debI.ref"))
+    _gunzip_to(os.path.join(GOLDEN, "uh3", "UH3_7nm.xyz.gz"), str(d / "UH3_7nm.xyz"))
+    _gunzip_to(os.path.join(GOLDEN, "uh3", "UH3_7nm_reduced.xyz.gz"), str(d / "UH3_7nm_reduced.xyz"))
     return d
 
 

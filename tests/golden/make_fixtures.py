@@ -9,6 +9,9 @@ runs, Data/ of Martin-Rudolph/cuDebye v1.5):
   gamma/GammaAl2O3_10nm.debI.gz  the reference intensity file (gzip of the CRLF text)
   gamma/log_values.json          numbers printed in Data/cmd-logfile.txt for this run
   uh3/UH3_7nm.deb.gz, uh3/UH3_7nm.debI.gz   the second example (36,903 atoms, 2 subgroups)
+  uh3/UH3_7nm.xyz.gz, uh3/UH3_7nm_reduced.xyz.gz   the converter's inputs for that .deb (six
+                                 and four columns; Data/UH3_7nm_reduced.deb is identical to
+                                 Data/UH3_7nm.deb)
 
 Run in the build container only (it reads /root/reference/Data):
     python tests/golden/make_fixtures.py
@@ -38,6 +41,10 @@ def main():
     gz_copy(f"{SRC}/GammaAl2O3_10nm.debI", f"{HERE}/gamma/GammaAl2O3_10nm.debI.gz")
     gz_copy(f"{SRC}/UH3_7nm.deb", f"{HERE}/uh3/UH3_7nm.deb.gz")
     gz_copy(f"{SRC}/UH3_7nm.debI", f"{HERE}/uh3/UH3_7nm.debI.gz")
+    gz_copy(f"{SRC}/UH3_7nm.xyz", f"{HERE}/uh3/UH3_7nm.xyz.gz")
+    gz_copy(f"{SRC}/UH3_7nm_reduced.xyz", f"{HERE}/uh3/UH3_7nm_reduced.xyz.gz")
+    with open(f"{SRC}/UH3_7nm_reduced.deb", "rb") as a, open(f"{SRC}/UH3_7nm.deb", "rb") as b:
+        assert a.read() == b.read()
     rstep = 1e-4
     arrays, shas = {}, {}
     for r in range(3):

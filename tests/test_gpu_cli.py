@@ -102,9 +102,10 @@ def _write_deb(path, m, hist=2, block=16):
 
 
 def test_folder_run_equals_each_file_alone(tmp_path):
-    """--folder (Matlab/multisim.m): every .deb without a .debI, through one server; each file's
-    .debx equal byte for byte and its .debI within one unit of the printed sixth digit of a run
-    of that file alone (the FP64 sums of a batch split their bins differently)."""
+    """--folder (Matlab/multisim.m): every .xyz without a .deb converted, then every .deb without a
+    .debI, through one server; each file's .debx equal byte for byte and its .debI within one
+    unit of the printed sixth digit of a run of that file alone (the FP64 sums of a batch split
+    their bins differently)."""
     models = [synth.small_random(n, S, 900 + i, spread=sp) for i, (n, S, sp) in
               enumerate([(400, 1, 6.0), (1500, 3, 12.0), (900, 2, 9.0), (2500, 4, 14.0)])]
     solo, batch = tmp_path / "solo", tmp_path / "batch"
@@ -114,8 +115,16 @@ def test_folder_run_equals_each_file_alone(tmp_path):
         for d in (solo, batch):
             _write_deb(d / f"m{i}.deb", m)
     (batch / "m3.debI").write_text("already calculated\n")  # skipped, as multisim.m does
+    # an .xyz without its .deb: converted first with the default header, then calculated
+    rng = np.random.default_rng(77)
+    xyz = "".join(f"{e}  {x:.6f}  {y:.6f}  {z:.6f}\n" for e, (x, y, z) in
+                  zip(rng.choice(["O", "Al"], 300), rng.uniform(-8.0, 8.0, (300, 3))))
+    for d in (solo, batch):
+        (d / "x5.xyz").write_text(xyz)
+    cd.xyz_to_deb(solo / "x5.xyz")
     for i in range(len(models)):
         assert cd.run_deb(solo / f"m{i}.deb") == 0
+    assert cd.run_deb(solo / "x5.deb") == 0
     r = subprocess.run([cd.CLI_PATH, "--folder", str(batch)], capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr
     assert (batch / "m3.debI").read_text() == "already calculated\n"
@@ -129,3 +138,8 @@ def test_folder_run_equals_each_file_alone(tmp_path):
         ref = np.array([[float(v) for v in l.split("\t")] for l in read_debi(str(solo / f"m{i}.debI"))])
         assert got.shape == ref.shape and np.array_equal(got[:, 0], ref[:, 0])
         assert np.max(np.abs(got[:, 1] - ref[:, 1]) / np.abs(ref[:, 1])) <= 1.0001e-5
+    assert (batch / "x5.deb").read_bytes() == (solo / "x5.deb").read_bytes()
+    got = np.array([[float(v) for v in l.split("\t")] for l in read_debi(str(batch / "x5.debI"))])
+    ref = np.array([[float(v) for v in l.split("\t")] for l in read_debi(str(solo / "x5.debI"))])
+    assert got.shape == ref.shape == (6501, 2)
+    assert np.max(np.abs(got[:, 1] - ref[:, 1]) / np.abs(ref[:, 1])) <= 1.0001e-5
