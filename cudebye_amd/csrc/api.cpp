@@ -52,6 +52,20 @@ void *pinned(dbx_ctx *ctx, int slot, size_t bytes)
     return ctx->pin_ptr[slot];
 }
 
+const double *device_q(dbx_ctx *ctx, const double *q, int nq)
+{
+    const size_t b = sizeof(double) * (size_t)nq;
+    double *d = (double *)scratch(ctx, kSlotQin, b);
+    if (d == ctx->q_dev && ctx->q_host.size() == (size_t)nq && std::memcmp(ctx->q_host.data(), q, b) == 0)
+        return d;
+    double *h = (double *)pinned(ctx, kPinQ, b);
+    std::memcpy(h, q, b);
+    DBX_HIP(hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, ctx->stream));
+    ctx->q_host.assign(q, q + nq);
+    ctx->q_dev = d;
+    return d;
+}
+
 template <class F>
 static int guarded(dbx_ctx *ctx, F &&f)
 {
@@ -421,10 +435,10 @@ int dbx_close(dbx_ctx *ctx)
     if (!ctx) return DBX_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (int s = 0; s < 12; s++)
-        if (ctx->slot_ptr[s]) (void)hipFree(ctx->slot_ptr[s]);
-    for (int s = 0; s < 8; s++)
-        if (ctx->pin_ptr[s]) (void)hipHostFree(ctx->pin_ptr[s]);
+    for (void *p : ctx->slot_ptr)
+        if (p) (void)hipFree(p);
+    for (void *p : ctx->pin_ptr)
+        if (p) (void)hipHostFree(p);
     if (ctx->counters) (void)hipFree(ctx->counters);
     for (int i = 0; i < 2; i++)
         if (ctx->ev_mf[i]) (void)hipEventDestroy(ctx->ev_mf[i]);
@@ -588,9 +602,9 @@ int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int
             dh = t;
         }
         const size_t nqd = (size_t)nq;
-        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 2)));
-        double *dq = d, *dff = d + nqd, *dS = dff + P * nqd, *dI = dS + P * nqd;
-        DBX_HIP(hipMemcpyAsync(dq, q, sizeof(double) * nqd, hipMemcpyHostToDevice, ctx->stream));
+        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 1)));
+        double *dff = d, *dS = dff + P * nqd, *dI = dS + P * nqd;
+        const double *dq = device_q(ctx, q, nq);
         DBX_HIP(hipMemcpyAsync(dff, ffoobb, sizeof(double) * P * nqd, hipMemcpyHostToDevice,
                                ctx->stream));
         debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS);
@@ -630,10 +644,10 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
         uint64_t *dh = (uint64_t *)scratch(ctx, kSlotBatchHist, sizeof(uint64_t) * (size_t)total);
         uint64_t *door = (uint64_t *)scratch(ctx, kSlotOor, sizeof(uint64_t) * (size_t)n_items);
         const size_t nqd = (size_t)nq;
-        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * nqd * (1 + 2 * (size_t)n_rows + n_items));
-        double *dq = d, *dff = d + nqd, *dS = dff + n_rows * nqd, *dI = dS + n_rows * nqd;
+        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * nqd * (2 * (size_t)n_rows + n_items));
+        double *dff = d, *dS = dff + n_rows * nqd, *dI = dS + n_rows * nqd;
         DBX_HIP(hipMemsetAsync(door, 0, sizeof(uint64_t) * (size_t)n_items, ctx->stream));
-        DBX_HIP(hipMemcpyAsync(dq, q, sizeof(double) * nqd, hipMemcpyHostToDevice, ctx->stream));
+        const double *dq = device_q(ctx, q, nq);
         std::vector<P1Item> p1;
         std::vector<RowDesc> rows;
         std::vector<GroupPair> pairs;
@@ -697,18 +711,20 @@ int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params, do
         uint64_t *dh = (uint64_t *)scratch(ctx, kSlotHist, hb);
         P1Stats st = pass1_histograms(ctx, m->d, rstep, n_bins, dh, false);
         const size_t nqd = (size_t)nq;
-        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 2)));
-        double *dq = d, *dff = d + nqd, *dS = dff + P * nqd, *dI = dS + P * nqd;
-        DBX_HIP(hipMemcpyAsync(dq, q, sizeof(double) * nqd, hipMemcpyHostToDevice, ctx->stream));
+        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 1)));
+        double *dff = d, *dS = dff + P * nqd, *dI = dS + P * nqd;
+        const double *dq = device_q(ctx, q, nq);
         std::vector<GroupPair> pairs;
         for (int a = 0; a < S; a++)
             for (int b = a; b < S; b++) pairs.push_back(make_group_pair(params[a], params[b]));
         prefactor_rows(ctx, pairs, dq, nq, dff);
         debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS);
         combine_intensity(ctx, dff, dS, P, nq, dI);
-        DBX_HIP(hipMemcpyAsync(I_out, dI, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
+        double *hI = (double *)pinned(ctx, kPinI, sizeof(double) * nqd);
+        DBX_HIP(hipMemcpyAsync(hI, dI, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
         if (hist_out) DBX_HIP(hipMemcpyAsync(hist_out, dh, hb, hipMemcpyDeviceToHost, ctx->stream));
         DBX_HIP(hipStreamSynchronize(ctx->stream));
+        std::memcpy(I_out, hI, sizeof(double) * nqd);
         pass1_read_stats(ctx, st);
         if (st.out_of_range) {
             rc = DBX_ERR_RANGE;

@@ -153,10 +153,13 @@ struct dbx_ctx {
     std::string err;
     std::recursive_mutex mu;
     // grow-only device scratch, one buffer per slot (dbx::kSlot*)
-    void *slot_ptr[12] = {};
-    size_t slot_bytes[12] = {};
-    void *pin_ptr[8] = {};
-    size_t pin_bytes[8] = {};
+    void *slot_ptr[13] = {};
+    size_t slot_bytes[13] = {};
+    void *pin_ptr[10] = {};
+    size_t pin_bytes[10] = {};
+    // the q grid last uploaded to slot kSlotQin (host copy and device address)
+    std::vector<double> q_host;
+    const void *q_dev = nullptr;
     // [0] out-of-range pairs, [1] pairs outside the LDS window (checked kernel),
     // [2] fast jobs queued for the checked kernel; then the retry list (int32)
     uint64_t *counters = nullptr;
@@ -184,14 +187,17 @@ namespace dbx {
 
 enum { kSlotPass1 = 0, kSlotPass2 = 1, kSlotPass2b = 2, kSlotHist = 3, kSlotQ = 4, kSlotDigits = 5,
        kSlotRtab = 6, kSlotBatch = 7, kSlotRows = 8, kSlotPairs = 9, kSlotBatchHist = 10,
-       kSlotOor = 11 };
+       kSlotOor = 11, kSlotQin = 12 };
 // device buffer of at least `bytes` for `slot`; contents are not preserved when it grows
 void *scratch(dbx_ctx *ctx, int slot, size_t bytes);
 // pinned host buffer of at least `bytes` for `slot` (staging of per-call uploads; every call
 // synchronises its stream before returning, so a slot is free again at the next call)
 enum { kPinRefs = 0, kPinList = 1, kPinRows = 2, kPinPairs = 3, kPinMisc = 4, kPinModels = 5, kPinOor = 6,
-       kPinStepI = 7 };
+       kPinStepI = 7, kPinQ = 8, kPinI = 9 };
 void *pinned(dbx_ctx *ctx, int slot, size_t bytes);
+// the q grid on the device (slot kSlotQin), uploaded through pinned memory unless the slot
+// already holds the same values
+const double *device_q(dbx_ctx *ctx, const double *q, int nq);
 
 // pass 1 (pass1.hip): histograms of all subgroup pairs into d_hist[P][n_bins]
 // (zeroed here).  Returns the number of pairs with bin >= n_bins.

@@ -357,33 +357,53 @@ constexpr int kMfFold = 256;   // steps between folds of the 32-bit classes
 
 // balanced count digits as bytes: cdig[i][row][kk] = digit i of 2 N_row,kz[kk] (0 past Kc, past
 // the row's bins or past n_rows), and rsum[row] += the row's sum of 2 N over the compacted bins.
-// Kc_pad is a multiple of 64, so the 64 entries of a wave share one row.
-__global__ void p2_count_digits_kernel(const unsigned long long *__restrict__ hist,
-                                       const RowDesc *__restrict__ rows, int n_rows, const int *__restrict__ kz,
-                                       int Kc, int Kc_pad, int rows_pad, int I, signed char *__restrict__ cdig,
-                                       unsigned long long *__restrict__ rsum)
+// Grid (Kc_pad / 1024 rounded up, rows_pad), 256 threads, four consecutive bins per thread
+// (Kc_pad is a multiple of 64, so a thread's four bins are all inside or all outside it); one
+// atomic per block.
+constexpr int kCdThreads = 256;
+__global__ __launch_bounds__(kCdThreads) void p2_count_digits_kernel(
+    const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows, int n_rows,
+    const int *__restrict__ kz, int Kc, int Kc_pad, int rows_pad, int I, signed char *__restrict__ cdig,
+    unsigned long long *__restrict__ rsum)
 {
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)rows_pad * Kc_pad) return;  // whole waves: the total is a multiple of 64
-    const int row = (int)(t / Kc_pad);
-    const int kk = (int)(t - (int64_t)row * Kc_pad);
-    long long v = 0;
-    if (row < n_rows && kk < Kc) {
+    __shared__ unsigned long long wsum[kCdThreads / 64];
+    const int row = blockIdx.y;
+    const int kk0 = (blockIdx.x * kCdThreads + threadIdx.x) * 4;
+    long long v[4] = {0, 0, 0, 0};
+    if (row < n_rows && kk0 < Kc) {
         const RowDesc rd = rows[row];
-        const int64_t k = kz[kk];
-        if (k < rd.n_bins) v = (long long)(2ull * hist[rd.h_off + k]);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int kk = kk0 + u;
+            if (kk < Kc) {
+                const int64_t k = kz[kk];
+                if (k < rd.n_bins) v[u] = (long long)(2ull * hist[rd.h_off + k]);
+            }
+        }
     }
-    unsigned long long sum = (unsigned long long)v;
+    unsigned long long sum = (unsigned long long)(v[0] + v[1] + v[2] + v[3]);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-    if ((threadIdx.x & 63) == 0 && row < n_rows && sum != 0ull) atomicAdd(&rsum[row], sum);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0 && row < n_rows) {
+        unsigned long long t = 0;
+        for (int w = 0; w < kCdThreads / 64; w++) t += wsum[w];
+        if (t != 0ull) atomicAdd(&rsum[row], t);
+    }
+    if (kk0 >= Kc_pad) return;
     for (int i = 0; i < I; i++) {
-        long long e = v;
-        if (i + 1 < I) {
-            e = ((v + 128) & 255) - 128;
-            v = (v - e) >> 8;
+        uint32_t packed = 0;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            long long e = v[u];
+            if (i + 1 < I) {
+                e = ((v[u] + 128) & 255) - 128;
+                v[u] = (v[u] - e) >> 8;
+            }
+            packed |= (uint32_t)(uint8_t)(signed char)e << (8 * u);
         }
-        cdig[((int64_t)i * rows_pad + row) * Kc_pad + kk] = (signed char)e;
+        *reinterpret_cast<uint32_t *>(cdig + ((int64_t)i * rows_pad + row) * Kc_pad + kk0) = packed;
     }
 }
 
@@ -839,11 +859,11 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
         nsplit = (Kc_pad + kper - 1) / kper;
         signed char *cdig = (signed char *)scratch(ctx, kSlotDigits, (size_t)I * rows_pad * Kc_pad);
         double *Spart = (double *)scratch(ctx, kSlotPass2b, sizeof(double) * (size_t)nsplit * n_rows * nq);
-        const int64_t nd = (int64_t)rows_pad * Kc_pad;
         hipLaunchKernelGGL(p2_zero_kernel, dim3((unsigned)((2 * n_rows + 255) / 256)), dim3(256), 0, ctx->stream,
                            (unsigned int *)rsum, (int64_t)2 * n_rows);
-        hipLaunchKernelGGL(p2_count_digits_kernel, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, ctx->stream,
-                           h, d_rows, n_rows, kz, Kc, Kc_pad, rows_pad, I, cdig, rsum);
+        hipLaunchKernelGGL(p2_count_digits_kernel, dim3((unsigned)((Kc_pad + 4 * kCdThreads - 1) / (4 * kCdThreads)),
+                                                        (unsigned)rows_pad),
+                           dim3(kCdThreads), 0, ctx->stream, h, d_rows, n_rows, kz, Kc, Kc_pad, rows_pad, I, cdig, rsum);
         const int nblocks = rows_pad / 16;
         hipLaunchKernelGGL(p2_kext_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, kz, Kc, d_rows,
                            n_rows, 16, nblocks, kext);
