@@ -39,6 +39,7 @@ sys.path.insert(0, ROOT)
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak
 I8_PEAK_TOPS = 5000.0      # MI355X dense int8 matrix peak
 FLOPS_PER_PAIR = 9         # 3 sub, 3 mul, 2 add, 1 scale-mul
+VALU_ISSUE_PEAK = 256 * 4 * 16 * 2.4e9  # unpacked VALU lane operations/s: 256 CU x 4 SIMD x 16 lanes x 2.4 GHz
 SERVE_LOAD = 0.8           # Poisson arrival rate of the mixed stream, as a fraction of the
                            # server's measured capacity on the same requests (all at once)
 SERVE_N = 600              # requests in the mixed stream
@@ -122,6 +123,7 @@ def primary(args, cd, synth, np):
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
             "traffic": _profile_traffic("p1_histogram_kernel"),
+            "valu_issue": _valu_issue(pairs, k_ms),
             "unit_figure": f"{FLOPS_PER_PAIR} FP32 flops per pair x {pairs} pairs per launch",
         },
         "pass2_roofline": {
@@ -365,6 +367,18 @@ def main():
     if not args.no_cpu:
         cpu_leg(out, m, I, H, pairs, np)
     print(json.dumps(out))
+
+
+def _valu_issue(pairs, k_ms):
+    """What bounds the pass-1 kernel: VALU lane operations issued per second (lane operations per
+    pair from the committed SQ_INSTS_VALU count, profiles/traffic.json, times this run's pair
+    rate) against the unpacked issue peak."""
+    per_pair = _profile_traffic("p1_histogram_kernel_valu_lane_ops_per_pair")
+    if not per_pair:
+        return None
+    rate = per_pair * pairs / (k_ms * 1e-3)
+    return {"lane_ops_per_pair": per_pair, "achieved": round(rate / 1e12, 2), "peak": round(VALU_ISSUE_PEAK / 1e12, 2),
+            "unit": "T lane-ops/s", "frac": round(rate / VALU_ISSUE_PEAK, 4)}
 
 
 def _profile_traffic(kernel):

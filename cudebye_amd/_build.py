@@ -19,6 +19,9 @@ SOURCES = ["pass1.hip", "pass2.hip", "api.cpp", "server.cpp", "deb_run.cpp", "xy
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         # matrix instructions read and write ordinary VGPRs: no per-step copies of the
+         # accumulators into AGPRs in the pass-2 contraction
+         "-mllvm", "-amdgpu-mfma-vgpr-form=1",
          "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
 
 
