@@ -55,9 +55,11 @@ struct Tile {          // 32 bytes
 
 // One pass-1 work item: the pairs of a list of A tiles (subgroup row; 64-, 32- or 16-atom
 // tiles) with the B atoms (subgroup col, 64-atom tiles [b0, b1)) whose distance from the
-// A tile's centre lies in shell `shell` of pair `pair`.  Every pair lands in one LDS window of kP1Bins bins starting
-// at bin k_lo (pairs outside it, which only occur when the window is narrower than the
-// geometry needs, go straight to the global histogram).
+// A tile's centre lies in shell `shell` of pair `pair`, counted in one LDS window of kP1Bins
+// bins starting at bin k_lo.  A shell wider than one window is covered by several jobs with
+// consecutive windows (`win`): each counts only the pairs inside its own window, so every pair
+// is counted by exactly one of them.  Degraded jobs (window narrower than the geometry, no
+// sibling windows) send out-of-window pairs straight to the global histogram.
 struct P1Job {
     int32_t pair;
     int32_t a_off, a_cnt;  // A tiles: alist[a_off .. a_off + a_cnt)
@@ -68,8 +70,18 @@ struct P1Job {
     int32_t e2_off; // offset of the pair's shell-edge table in the edge array
     int32_t degraded;  // 1: the window cannot hold the job's geometry (checked kernel)
     int32_t R;         // B atoms per 64-lane step: 1, 2, 4 for A tiles of 64, 32, 16 atoms
-    int32_t pad;
+    int32_t win;       // kP1WinFirst: no window below this one; kP1WinLast: none above
 };
+constexpr int kP1WinFirst = 1, kP1WinLast = 2, kP1WinOnly = 3;
+// planner: a 64-atom A tile whose shells would be thinner than kP1SingleShell windows in one
+// window takes several windows per shell (instead of smaller tiles and thinner shells), with
+// its level-kP1MultiLevel tiles (0: 64 atoms, 1: 32, 2: 16) and shells kP1MultiShell windows
+// wide.  Measured on the 20 nm rocksalt sphere at 0.0005 A and the spinel sphere at 1e-4 A:
+// level 0 with 0.25 / 0.75 is best (level 1: +14-20%, level 2: +35-60%).
+constexpr bool kP1MultiWindow = true;
+constexpr int kP1MultiLevel = 0;
+constexpr double kP1MultiShell = 0.75;
+constexpr double kP1SingleShell = 0.25;
 
 constexpr int kP1Threads = 1024;   // 16 waves of 64
 constexpr int kP1Bins = 35328;     // LDS window: 138 KiB of uint32 counters (+ guard and sentinel)

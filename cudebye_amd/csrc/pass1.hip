@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -34,30 +35,59 @@
 namespace dbx {
 
 // The LDS window: h[0] guard (bins below the window), h[1 + i] bin k_lo + i for
-// 0 <= i < kP1Bins, h[kP1Bins + 1] sentinel (bins above it).
-struct Window {
-    int k_lo;
+// 0 <= i < kP1Bins, h[kP1Bins + 1] sentinel (bins above it).  v = the pair's distance in bin
+// units (pair_scaled).
+//
+// CountFast (the window holds every pair of the job): v is clamped to the guard and sentinel
+// bins before the round-half-up conversion, so a bin outside the window lands in h[0] or
+// h[kP1Bins + 1] (three instructions: med3, convert, address).
+struct CountFast {
+    uint32_t *h;
     int off;     // 1 - k_lo: h[k + off] holds bin k
-    float vlo;   // k_lo - 1 and k_lo + kP1Bins, exact in FP32 (the planner keeps fast jobs
+    float vlo;   // k_lo - 1 and k_lo + kP1Bins, exact in FP32 (the planner keeps these jobs
     float vhi;   // below 2^24 bins)
+    __device__ __forceinline__ void operator()(float v)
+    {
+        const int k = round_rpi(__builtin_amdgcn_fmed3f(v, vlo, vhi));
+        atomicAdd(&h[k + off], 1u);
+    }
 };
 
-// v = the pair's distance in bin units (pair_scaled).  Fast variant: v clamped to the guard
-// and sentinel bins before the round-half-up conversion, so a bin outside the window lands in
-// h[0] or h[kP1Bins + 1] (three instructions: med3, convert, address).
-template <bool CHECKED>
-__device__ __forceinline__ void count_pair(uint32_t *h, float v, const Window &w, int64_t n_bins,
-                                           unsigned long long *hrow, uint32_t &miss, uint32_t &oor)
-{
-    if (!CHECKED) {
-        const int k = round_rpi(__builtin_amdgcn_fmed3f(v, w.vlo, w.vhi));
-        atomicAdd(&h[k + w.off], 1u);
-    } else {
+// CountWindow (one of several consecutive windows of a job): a pair outside the window belongs
+// to a sibling job and is skipped (its lane is masked off the atomic, so no counter collects
+// the skipped pairs); kmin / kmax record the bins seen, to check that none falls below the
+// first window or above the last.
+struct CountWindow {
+    uint32_t *h;
+    int k_lo;
+    int kmin, kmax;
+    __device__ __forceinline__ void operator()(float v)
+    {
         const int k = round_rpi(v);
-        const unsigned kk = (unsigned)(k - w.k_lo);
+        kmin = min(kmin, k);
+        kmax = max(kmax, k);
+        const unsigned kk = (unsigned)(k - k_lo);
+        if (kk < (unsigned)kP1Bins) atomicAdd(&h[kk + 1], 1u);
+    }
+};
+
+// CountChecked (degraded jobs and re-runs): out-of-window pairs go to the uint64 histogram in
+// HBM, unless a sibling window holds them (below a window that is not the first, above one
+// that is not the last).
+struct CountChecked {
+    uint32_t *h;
+    int k_lo;
+    int win;
+    int64_t n_bins;
+    unsigned long long *hrow;
+    uint32_t miss, oor;
+    __device__ __forceinline__ void operator()(float v)
+    {
+        const int k = round_rpi(v);
+        const unsigned kk = (unsigned)(k - k_lo);
         if (kk < (unsigned)kP1Bins) {
             atomicAdd(&h[kk + 1], 1u);
-        } else {
+        } else if (k < k_lo ? (win & kP1WinFirst) : (win & kP1WinLast)) {
             ++miss;
             if (k >= 0 && (int64_t)k < n_bins)
                 atomicAdd(&hrow[k], 1ull);
@@ -65,7 +95,7 @@ __device__ __forceinline__ void count_pair(uint32_t *h, float v, const Window &w
                 ++oor;
         }
     }
-}
+};
 
 // One workgroup runs one job: it zeroes its LDS window, culls the job's (A tile, B tile)
 // pairs into an LDS candidate list (B tiles with an atom possibly in the shell), then its
@@ -86,7 +116,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     __shared__ uint32_t h[kP1Bins + 2];
     __shared__ uint32_t cand[kP1Cand];
     __shared__ float4 stage[kP1Threads / 64][64];
-    __shared__ int s_ncand, s_next;
+    __shared__ int s_ncand, s_next, s_kmin, s_kmax;
 
     // list entry = (model, job, piece, pieces); the checked launch runs the planned checked
     // jobs, then the fast jobs the fast launch queued for a re-run
@@ -129,11 +159,13 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const float r_lo = e_lo > 0.f ? sqrtf(e_lo) : 0.f;
     const float r_hi = sqrtf(e_hi);  // +inf for the last shell
     const int k_lo = job.k_lo;
-    const Window win{k_lo, 1 - k_lo, (float)(k_lo - 1), (float)(k_lo + kP1Bins)};
     unsigned long long *hrow = M.hist + (int64_t)job.pair * n_bins;
     const int nblk = (job.b1 - job.b0 + 63) >> 6;
     const int units = job.a_cnt * nblk;
-    uint32_t miss = 0, oor = 0;
+    if (threadIdx.x == 0) {
+        s_kmin = INT_MAX;
+        s_kmax = INT_MIN;
+    }
 
     int cur_ia = -1;
     Tile ta = {};
@@ -141,135 +173,137 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     int idx_a = 0;
     float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
 
-    for (int u0 = 0; u0 < units; u0 += kP1Cand / 64) {
-        if (threadIdx.x == 0) {
-            s_ncand = 0;
-            s_next = 0;
-        }
-        __syncthreads();
-        // cull: unit = (A tile, block of 64 B tiles); candidates appended to the LDS list
-        const int u1 = min(units, u0 + kP1Cand / 64);
-        for (int u = u0 + wave; u < u1; u += kP1Threads / 64) {
-            const int ial = u / nblk;
-            const Tile A = tiles[alist[job.a_off + ial]];
-            const int j = job.b0 + (u - ial * nblk) * 64 + lane;
-            bool c = false;
-            if (j < job.b1 && (!job.same || j >= A.parent)) {
-                const Tile tb = tiles[j];
-                const float dx = tb.cx - A.cx, dy = tb.cy - A.cy, dz = tb.cz - A.cz;
-                const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
-                const float mg = 1e-3f + 1e-5f * (d + tb.rad);
-                c = (d + tb.rad + mg >= r_lo) && (d - tb.rad - mg < r_hi);
+    // the job's pairs, each handed to the counter `cnt`
+    auto run = [&](auto &cnt) {
+        for (int u0 = 0; u0 < units; u0 += kP1Cand / 64) {
+            if (threadIdx.x == 0) {
+                s_ncand = 0;
+                s_next = 0;
             }
-            const uint64_t cm = __ballot(c);
-            if (cm) {
-                int base = 0;
-                if (lane == 0) base = atomicAdd(&s_ncand, __popcll(cm));
-                base = __builtin_amdgcn_readfirstlane(base);
-                const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(cm >> 32),
-                                                             __builtin_amdgcn_mbcnt_lo((uint32_t)cm, 0));
-                if (c) cand[base + below] = ((uint32_t)ial << 24) | (uint32_t)(j - job.b0);
-            }
-        }
-        __syncthreads();
-        const int nc = s_ncand;
-        for (;;) {
-            int ci = 0;
-            if (lane == 0) ci = atomicAdd(&s_next, 1);
-            ci = __builtin_amdgcn_readfirstlane(ci);
-            if (ci >= nc) break;
-            const uint32_t e = cand[ci];
-            const int ia = alist[job.a_off + (int)(e >> 24)];
-            const int j2 = job.b0 + (int)(e & 0xFFFFFFu);
-            if (ia != cur_ia) {
-                cur_ia = ia;
-                ta = tiles[ia];
-                a_ok = ai < ta.count;
-                idx_a = ta.start + ai;
-                pa = pos[a_ok ? idx_a : ta.start];
-            }
-            const Tile tb = tiles[j2];
-            const bool b_ok = lane < tb.count;
-            const float4 pb = pos[tb.start + (b_ok ? lane : 0)];
-            const float ex = pb.x - ta.cx, ey = pb.y - ta.cy, ez = pb.z - ta.cz;
-            const float d2c = (ex * ex + ey * ey) + ez * ez;
-            const bool in_shell = b_ok && d2c >= e_lo && d2c < e_hi;
-            uint64_t bm = __ballot(in_shell);
-            const bool diag = job.same && (j2 == ta.parent);  // B tile holds the A tile's atoms
-            if (bm) {
-                // stage the shell's B atoms (with their atom index) in the wave's LDS area; the
-                // lanes then read them back as broadcasts, R per step
-                const int nb = __popcll(bm);
-                if (in_shell) {
-                    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0));
-                    stage[wave][below] = make_float4(pb.x, pb.y, pb.z, __int_as_float(tb.start + lane));
+            __syncthreads();
+            // cull: unit = (A tile, block of 64 B tiles); candidates appended to the LDS list
+            const int u1 = min(units, u0 + kP1Cand / 64);
+            for (int u = u0 + wave; u < u1; u += kP1Threads / 64) {
+                const int ial = u / nblk;
+                const Tile A = tiles[alist[job.a_off + ial]];
+                const int j = job.b0 + (u - ial * nblk) * 64 + lane;
+                bool c = false;
+                if (j < job.b1 && (!job.same || j >= A.parent)) {
+                    const Tile tb = tiles[j];
+                    const float dx = tb.cx - A.cx, dy = tb.cy - A.cy, dz = tb.cz - A.cz;
+                    const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
+                    const float mg = 1e-3f + 1e-5f * (d + tb.rad);
+                    c = (d + tb.rad + mg >= r_lo) && (d - tb.rad - mg < r_hi);
                 }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (R == 1 && !diag) {
-                    if (a_ok) {
-                        // two B atoms per iteration: two independent chains in flight
-                        int t = 0;
-                        for (; t + 1 < nb; t += 2) {
-                            const float4 b0 = stage[wave][t];
-                            const float4 b1 = stage[wave][t + 1];
-                            const float v0 = pair_scaled(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
-                            const float v1 = pair_scaled(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
-                            count_pair<CHECKED>(h, v0, win, n_bins, hrow, miss, oor);
-                            count_pair<CHECKED>(h, v1, win, n_bins, hrow, miss, oor);
-                        }
-                        if (t < nb) {
-                            const float4 b = stage[wave][t];
-                            count_pair<CHECKED>(h, pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), win, n_bins,
-                                                hrow, miss, oor);
-                        }
-                    }
-                } else if (!diag) {
-                    // R B atoms per step (lane slot takes atom t + slot): whole steps need no
-                    // per-lane test, two steps per iteration; then at most one partial step
-                    const int nfull = nb & ~(R - 1);
-                    if (a_ok) {
-                        int t = 0;
-                        for (; t + R < nfull; t += 2 * R) {
-                            const float4 b0 = stage[wave][t + slot];
-                            const float4 b1 = stage[wave][t + R + slot];
-                            const float v0 = pair_scaled(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
-                            const float v1 = pair_scaled(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
-                            count_pair<CHECKED>(h, v0, win, n_bins, hrow, miss, oor);
-                            count_pair<CHECKED>(h, v1, win, n_bins, hrow, miss, oor);
-                        }
-                        if (t < nfull) {
-                            const float4 b = stage[wave][t + slot];
-                            count_pair<CHECKED>(h, pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), win, n_bins,
-                                                hrow, miss, oor);
-                        }
-                        if (nfull + slot < nb) {
-                            const float4 b = stage[wave][nfull + slot];
-                            count_pair<CHECKED>(h, pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), win, n_bins,
-                                                hrow, miss, oor);
-                        }
-                    }
-                } else {
-                    // the B tile holds the A tile's own atoms: pairs with index b >= a only
-                    for (int t = 0; t < nb; t += R) {
-                        const int src = t + slot;
-                        const float4 b = stage[wave][min(src, nb - 1)];
-                        if (a_ok && src < nb && idx_a <= __float_as_int(b.w))
-                            count_pair<CHECKED>(h, pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp), win, n_bins,
-                                                hrow, miss, oor);
-                    }
+                const uint64_t cm = __ballot(c);
+                if (cm) {
+                    int base = 0;
+                    if (lane == 0) base = atomicAdd(&s_ncand, __popcll(cm));
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(cm >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)cm, 0));
+                    if (c) cand[base + below] = ((uint32_t)ial << 24) | (uint32_t)(j - job.b0);
                 }
-                // the stage is rewritten by the next candidate only after every lane's reads
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
             }
+            __syncthreads();
+            const int nc = s_ncand;
+            for (;;) {
+                int ci = 0;
+                if (lane == 0) ci = atomicAdd(&s_next, 1);
+                ci = __builtin_amdgcn_readfirstlane(ci);
+                if (ci >= nc) break;
+                const uint32_t e = cand[ci];
+                const int ia = alist[job.a_off + (int)(e >> 24)];
+                const int j2 = job.b0 + (int)(e & 0xFFFFFFu);
+                if (ia != cur_ia) {
+                    cur_ia = ia;
+                    ta = tiles[ia];
+                    a_ok = ai < ta.count;
+                    idx_a = ta.start + ai;
+                    pa = pos[a_ok ? idx_a : ta.start];
+                }
+                const Tile tb = tiles[j2];
+                const bool b_ok = lane < tb.count;
+                const float4 pb = pos[tb.start + (b_ok ? lane : 0)];
+                const float ex = pb.x - ta.cx, ey = pb.y - ta.cy, ez = pb.z - ta.cz;
+                const float d2c = (ex * ex + ey * ey) + ez * ez;
+                const bool in_shell = b_ok && d2c >= e_lo && d2c < e_hi;
+                uint64_t bm = __ballot(in_shell);
+                const bool diag = job.same && (j2 == ta.parent);  // B tile holds the A tile's atoms
+                if (bm) {
+                    // stage the shell's B atoms (with their atom index) in the wave's LDS area; the
+                    // lanes then read them back as broadcasts, R per step
+                    const int nb = __popcll(bm);
+                    if (in_shell) {
+                        const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0));
+                        stage[wave][below] = make_float4(pb.x, pb.y, pb.z, __int_as_float(tb.start + lane));
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                    if (R == 1 && !diag) {
+                        if (a_ok) {
+                            // two B atoms per iteration: two independent chains in flight
+                            int t = 0;
+                            for (; t + 1 < nb; t += 2) {
+                                const float4 b0 = stage[wave][t];
+                                const float4 b1 = stage[wave][t + 1];
+                                const float v0 = pair_scaled(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
+                                const float v1 = pair_scaled(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
+                                cnt(v0);
+                                cnt(v1);
+                            }
+                            if (t < nb) {
+                                const float4 b = stage[wave][t];
+                                cnt(pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp));
+                            }
+                        }
+                    } else if (!diag) {
+                        // R B atoms per step (lane slot takes atom t + slot): whole steps need no
+                        // per-lane test, two steps per iteration; then at most one partial step
+                        const int nfull = nb & ~(R - 1);
+                        if (a_ok) {
+                            int t = 0;
+                            for (; t + R < nfull; t += 2 * R) {
+                                const float4 b0 = stage[wave][t + slot];
+                                const float4 b1 = stage[wave][t + R + slot];
+                                const float v0 = pair_scaled(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
+                                const float v1 = pair_scaled(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
+                                cnt(v0);
+                                cnt(v1);
+                            }
+                            if (t < nfull) {
+                                const float4 b = stage[wave][t + slot];
+                                cnt(pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp));
+                            }
+                            if (nfull + slot < nb) {
+                                const float4 b = stage[wave][nfull + slot];
+                                cnt(pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp));
+                            }
+                        }
+                    } else {
+                        // the B tile holds the A tile's own atoms: pairs with index b >= a only
+                        for (int t = 0; t < nb; t += R) {
+                            const int src = t + slot;
+                            const float4 b = stage[wave][min(src, nb - 1)];
+                            if (a_ok && src < nb && idx_a <= __float_as_int(b.w))
+                                cnt(pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp));
+                        }
+                    }
+                    // the stage is rewritten by the next candidate only after every lane's reads
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                    __builtin_amdgcn_wave_barrier();
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
-    }
-    __syncthreads();  // zeroing visible to all threads even when the job had no candidates
+    };
     if (CHECKED) {
+        CountChecked cnt{h, k_lo, job.win, n_bins, hrow, 0u, 0u};
+        run(cnt);
+        __syncthreads();  // zeroing visible to all threads even when the job had no candidates
+        uint32_t miss = cnt.miss, oor = cnt.oor;
         for (int off = 32; off > 0; off >>= 1) {
             miss += __shfl_down(miss, off);
             oor += __shfl_down(oor, off);
@@ -278,13 +312,39 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             if (miss) atomicAdd(&counters[1], (unsigned long long)miss);
             if (oor) atomicAdd(M.oor, (unsigned long long)oor);
         }
-    } else if ((h[0] | h[kP1Bins + 1]) != 0u) {
-        // a bin fell outside the window: leave this job to the checked variant
-        if (threadIdx.x == 0) {
-            const unsigned long long r = atomicAdd(&counters[2], 1ull);
-            retry[r] = entry;
+    } else {
+        bool bad;
+        if (job.win == kP1WinOnly) {
+            CountFast cnt{h, 1 - k_lo, (float)(k_lo - 1), (float)(k_lo + kP1Bins)};
+            run(cnt);
+            __syncthreads();
+            // a bin fell outside the window
+            bad = (h[0] | h[kP1Bins + 1]) != 0u;
+        } else {
+            CountWindow cnt{h, k_lo, INT_MAX, INT_MIN};
+            run(cnt);
+            int kmin = cnt.kmin, kmax = cnt.kmax;
+            for (int off = 32; off > 0; off >>= 1) {
+                kmin = min(kmin, __shfl_xor(kmin, off));
+                kmax = max(kmax, __shfl_xor(kmax, off));
+            }
+            if (lane == 0) {
+                atomicMin(&s_kmin, kmin);
+                atomicMax(&s_kmax, kmax);
+            }
+            __syncthreads();
+            // a bin below the first window or above the last one
+            bad = ((job.win & kP1WinFirst) && s_kmin < k_lo) ||
+                  ((job.win & kP1WinLast) && s_kmax >= k_lo + kP1Bins);
         }
-        return;
+        if (bad) {
+            // leave this job to the checked variant
+            if (threadIdx.x == 0) {
+                const unsigned long long r = atomicAdd(&counters[2], 1ull);
+                retry[r] = entry;
+            }
+            return;
+        }
     }
     for (int i = threadIdx.x; i < kP1Bins; i += kP1Threads) {
         const uint32_t v = h[i + 1];
@@ -369,12 +429,32 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
             const double D = std::sqrt(D2) + 1e-3;  // bound on |b - c_A|
             const double mg = 2e-3 + 4e-6 * (D + radA);
             const double h_min = std::max(0.5, 0.02 * usable);
-            const double rad_fit = 0.5 * (usable - h_min - 2 * mg);
-            // A tiles per level
-            std::vector<int32_t> sel[3];
+            // one window per shell for tiles that leave a shell of at least kP1SingleShell
+            // windows' width (h_min without multi-window jobs)
+            const double h_single = kP1MultiWindow ? std::max(h_min, kP1SingleShell * usable) : h_min;
+            const double rad_fit = 0.5 * (usable - h_single - 2 * mg);
+            // A tiles per level (sel[L]: one window per shell; sel[3 + L]: several consecutive
+            // windows per shell).  A 64-atom tile that leaves a shell of useful width in one
+            // window takes level 0; a wider one takes its level-kP1MultiLevel tiles with several
+            // windows when kP1MultiWindow, else its 32-atom halves that fit, else its 16-atom
+            // quarters (checked kernel if even those do not fit)
+            std::vector<int32_t> sel[6];
             for (int t = a0; t < a1; t++) {
                 if (m.tiles_h[t].rad <= rad_fit) {
                     sel[0].push_back(t);
+                    continue;
+                }
+                if (kP1MultiWindow) {
+                    if (kP1MultiLevel == 0) {
+                        sel[3].push_back(t);
+                    } else {
+                        for (int c = m.child1_0[t]; c < m.child1_1[t]; c++) {
+                            if (kP1MultiLevel == 1)
+                                sel[4].push_back(c);
+                            else
+                                for (int q = m.child2_0[c - off1]; q < m.child2_1[c - off1]; q++) sel[5].push_back(q);
+                        }
+                    }
                     continue;
                 }
                 for (int c = m.child1_0[t]; c < m.child1_1[t]; c++) {
@@ -387,14 +467,22 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
             }
             const double nB = (double)m.group_n[gb];
             const int nbt = b1 - b0;
-            for (int L = 0; L < 3; L++) {
-                std::vector<int32_t> &A = sel[L];
+            for (int LL = 0; LL < 6; LL++) {
+                std::vector<int32_t> &A = sel[LL];
                 if (A.empty()) continue;
+                const bool multi = LL >= 3;
+                const int L = LL % 3;
                 std::stable_sort(A.begin(), A.end(), [&](int x, int y) { return m.tiles_h[x].rad < m.tiles_h[y].rad; });
                 const int R = 1 << L;
                 const double ta_atoms = 64.0 / R;
                 const double rad_typ = m.tiles_h[A[A.size() / 2]].rad;
-                const double h_typ = std::max(h_min, usable - 2 * rad_typ - 2 * mg);
+                // windows of a multi-window chunk: the fewest that leave a shell of kP1MultiShell
+                // windows' width, at least h_min
+                auto n_windows = [&](double rad) {
+                    const double hw = std::max(h_min, kP1MultiShell * usable);
+                    return multi ? std::max(1, (int)std::ceil((2 * rad + 2 * mg + hw) / usable)) : 1;
+                };
+                const double h_typ = std::max(h_min, n_windows(rad_typ) * usable - 2 * rad_typ - 2 * mg);
                 const double shell_frac = std::min(1.0, h_typ / std::max(D, h_typ));
                 const double per_tile = ta_atoms * std::max(1.0, nB * shell_frac);
                 int CA = (int)std::ceil(6.0e6 / per_tile);
@@ -413,7 +501,8 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                         rmaxc = std::max(rmaxc, (double)T.rad);
                         na += T.count;
                     }
-                    double h = usable - 2 * rmaxc - 2 * mg;
+                    const int nw = n_windows(rmaxc);
+                    double h = nw * usable - 2 * rmaxc - 2 * mg;
                     const bool degraded = h < h_min;
                     if (degraded) h = h_min;
                     const int ns = (int)(D / h) + 1;
@@ -454,15 +543,20 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                         else
                             klo = std::floor(((sh + 0.5) * h) / rstep) - kP1Bins / 2;
                         if (klo < 0) klo = 0;
-                        J.k_lo = (int32_t)std::min<double>(klo, (double)INT32_MAX - kP1Bins);
-                        // the fast variant's window bounds must be exact in FP32
-                        J.degraded = (degraded || J.k_lo + (double)kP1Bins >= 16777216.0) ? 1 : 0;
-                        jobs.push_back(J);
                         // neighbouring shells share straddling tiles: spread the sampled cost
                         double cst = shell_cost[sh];
                         if (sh > 0) cst += 0.25 * shell_cost[sh - 1];
                         if (sh + 1 < ns) cst += 0.25 * shell_cost[sh + 1];
-                        cost.push_back(na / (c1 - c0) * (cst + 1.0) * (1.0 + 0.15 * (R - 1)));
+                        const double jc = na / (c1 - c0) * (cst + 1.0) * (1.0 + 0.15 * (R - 1));
+                        const int wn = degraded ? 1 : nw;
+                        for (int w = 0; w < wn; w++) {
+                            J.k_lo = (int32_t)std::min<double>(klo + (double)w * kP1Bins, (double)INT32_MAX - kP1Bins);
+                            J.win = (w == 0 ? kP1WinFirst : 0) | (w == wn - 1 ? kP1WinLast : 0);
+                            // the single-window fast variant's bounds must be exact in FP32
+                            J.degraded = (degraded || (wn == 1 && J.k_lo + (double)kP1Bins >= 16777216.0)) ? 1 : 0;
+                            jobs.push_back(J);
+                            cost.push_back(jc);
+                        }
                     }
                 }
             }

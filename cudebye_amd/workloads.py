@@ -4,6 +4,7 @@
 
     c1        single-species FCC sphere, 4 nm (~2k atoms), 4096 Q
     c2        two-species spinel-like sphere, 10 nm (~102k atoms), 4096 Q
+    c2-fine   the c2 sphere at the reference's own bin width, 1e-4 A (1.06M bins), 4096 Q
     c3        three-species rocksalt-like sphere, 20 nm (~453k atoms), 8192 Q (0.3-20 A^-1)
     c4-batch  256 two-species models (1k-60k atoms) in one dbx_batch_intensity call
     c4-plain  the same 256 models, one dbx_intensity call each
@@ -71,7 +72,7 @@ def _c5(reps, mode):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("name", choices=["c1", "c2", "c3", "c4-batch", "c4-plain", "c5-serve"])
+    ap.add_argument("name", choices=["c1", "c2", "c2-fine", "c3", "c4-batch", "c4-plain", "c5-serve"])
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--fp64", action="store_true", help="FP64 pass 2 instead of the matrix cores")
     a = ap.parse_args()
@@ -80,6 +81,8 @@ def main():
         _single(synth.fcc_sphere(), a.reps, mode)
     elif a.name == "c2":
         _single(synth.spinel_sphere(), a.reps, mode)
+    elif a.name == "c2-fine":
+        _single(synth.spinel_sphere(rstep=1e-4), a.reps, mode)
     elif a.name == "c3":
         _single(synth.rocksalt_sphere(), a.reps, mode)
     elif a.name == "c4-batch":

@@ -1,7 +1,9 @@
 """Pass-1 work plan on the host (no device): the planner's jobs cover every pair of every
-subgroup pair exactly once, and every job the fast kernel runs (exact window) holds all of
-its pairs' bins inside its LDS window.  The kernel's shell test and bin rule are emulated in
-NumPy float32 (IEEE, unfused), so these checks see the same per-pair values as the device."""
+subgroup pair exactly once, and every job the fast kernel runs holds its pairs' bins: a
+single-window job all of them inside its LDS window, the consecutive windows of a wide shell
+all of them inside their union (each pair counted by the one window that contains it).  The
+kernel's shell test and bin rule are emulated in NumPy float32 (IEEE, unfused), so these
+checks see the same per-pair values as the device."""
 import ctypes as C
 
 import numpy as np
@@ -55,9 +57,10 @@ def check_plan(m):
     seen = {}
     counted = np.zeros(len(pairs), np.int64)
     for J in jobs:
-        pair, a_off, a_cnt, b0, b1, same, shell, klo, eoff, degraded, R = J[:11]
+        pair, a_off, a_cnt, b0, b1, same, shell, klo, eoff, degraded, R, win = J[:12]
         ga, gb = pairs[pair]
-        assert R in (1, 2, 4)
+        assert R in (1, 2, 4) and win in (0, 1, 2, 3)
+        assert not degraded or win == 3
         lo, hi = e2[eoff + shell], e2[eoff + shell + 1]
         for t in alist[a_off:a_off + a_cnt]:
             assert grp[t] == ga and cnt[t] <= 64 // R
@@ -65,8 +68,8 @@ def check_plan(m):
             ia = np.arange(start[t], start[t] + cnt[t])
             for j in range(max(b0, parent[t]) if same else b0, b1):
                 assert grp[j] == gb and cnt[j] <= 64
-                key = (pair, t, j, int(shell))
-                assert key not in seen, key  # each (A tile, B tile, shell) once
+                key = (pair, t, j, int(shell), int(klo))
+                assert key not in seen, key  # each (A tile, B tile, shell, window) once
                 seen[key] = True
                 B = pos[start[j]:start[j] + cnt[j], :3]
                 e = (B - cen[t]).astype(np.float32)
@@ -80,10 +83,15 @@ def check_plan(m):
                     valid = ia[:, None] <= ib[None, :]
                 else:
                     valid = np.ones(k.shape, bool)
-                counted[pair] += int(valid.sum())
-                if not degraded:
-                    inside = (k >= klo) & (k < klo + W)
-                    assert np.all(inside | ~valid), (pair, t, j, shell)
+                if degraded:
+                    counted[pair] += int(valid.sum())  # the checked kernel counts every pair
+                    continue
+                inside = (k >= klo) & (k < klo + W)
+                counted[pair] += int((valid & inside).sum())
+                if win & 1:
+                    assert np.all((k >= klo) | ~valid), (pair, t, j, shell)
+                if win & 2:
+                    assert np.all((k < klo + W) | ~valid), (pair, t, j, shell)
     n = m.group_n
     want = [n[a] * (n[a] + 1) // 2 if a == b else n[a] * n[b] for a, b in pairs]
     assert list(counted) == want
@@ -104,8 +112,12 @@ def test_plan_covers_every_pair_once_and_windows_hold(maker):
     assert len(jobs) > 0
 
 
-def test_plan_uses_smaller_tiles_where_needed():
-    _, _, jobs, _, _ = host_plan(synth.small_random(1500, 2, 43, spread=45.0))
-    assert set(jobs[:, 10]) >= {2} or set(jobs[:, 10]) >= {4}
+def test_plan_uses_several_windows_where_needed():
+    """Sparse species (wide 64-atom tiles) and fine bins: shells counted in several windows;
+    the bench workload: one window for nearly every job, no checked jobs."""
+    for m in (synth.small_random(1500, 2, 43, spread=45.0),
+              synth.small_random(1200, 1, 44, spread=12.0, rstep=1e-4)):
+        _, _, jobs, _, _ = host_plan(m)
+        assert np.any(jobs[:, 11] != 3) and jobs[:, 9].sum() == 0
     _, _, jobs, _, _ = host_plan(synth.spinel_sphere())
-    assert jobs[:, 9].sum() == 0  # no checked jobs on the bench workload
+    assert jobs[:, 9].sum() == 0 and np.mean(jobs[:, 11] == 3) > 0.97
