@@ -168,6 +168,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     }
 
     int cur_ia = -1;
+    float w_lo2 = -1.f, w_hi2 = INFINITY;  // squared window reach from the A tile's centre
     Tile ta = {};
     bool a_ok = false;
     int idx_a = 0;
@@ -221,13 +222,26 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                     a_ok = ai < ta.count;
                     idx_a = ta.start + ai;
                     pa = pos[a_ok ? idx_a : ta.start];
+                    // a B atom at distance d from the A tile's centre pairs with its atoms at
+                    // distances in [d - rad, d + rad]: one whose interval misses this job's
+                    // window (below it when a window lies below, above it when one lies above)
+                    // has no pair to count here.  Margin: 10 bins plus 1e-5 relative.
+                    if (job.win != kP1WinOnly) {
+                        const float rs = 1.f / rp;
+                        const float lo = (float)(k_lo - 10) * rs - ta.rad;
+                        const float hi = (float)(k_lo + kP1Bins + 10) * rs + ta.rad;
+                        const float lo_m = lo - 1e-5f * fabsf(lo);
+                        const float hi_m = hi + 1e-5f * hi;
+                        w_lo2 = (job.win & kP1WinFirst) || lo_m <= 0.f ? -1.f : lo_m * lo_m;
+                        w_hi2 = (job.win & kP1WinLast) ? INFINITY : hi_m * hi_m;
+                    }
                 }
                 const Tile tb = tiles[j2];
                 const bool b_ok = lane < tb.count;
                 const float4 pb = pos[tb.start + (b_ok ? lane : 0)];
                 const float ex = pb.x - ta.cx, ey = pb.y - ta.cy, ez = pb.z - ta.cz;
                 const float d2c = (ex * ex + ey * ey) + ez * ez;
-                const bool in_shell = b_ok && d2c >= e_lo && d2c < e_hi;
+                const bool in_shell = b_ok && d2c >= e_lo && d2c < e_hi && d2c >= w_lo2 && d2c < w_hi2;
                 uint64_t bm = __ballot(in_shell);
                 const bool diag = job.same && (j2 == ta.parent);  // B tile holds the A tile's atoms
                 if (bm) {
