@@ -70,15 +70,18 @@ struct P1Job {
     int32_t e2_off; // offset of the pair's shell-edge table in the edge array
     int32_t degraded;  // 1: the window cannot hold the job's geometry (checked kernel)
     int32_t R;         // B atoms per 64-lane step: 1, 2, 4 for A tiles of 64, 32, 16 atoms
-    int32_t win;       // kP1WinFirst: no window below this one; kP1WinLast: none above
+    int32_t win;       // kP1WinFirst: no window below this one; kP1WinLast: none above;
+                       // kP1Win16: a window of 2 kP1Bins 16-bit counters
 };
 constexpr int kP1WinFirst = 1, kP1WinLast = 2, kP1WinOnly = 3;
+constexpr int kP1Win16 = 4;  // the window is 2 kP1Bins bins of 16-bit counters
 // planner: a 64-atom A tile whose shells would be thinner than kP1SingleShell windows in one
 // window takes several windows per shell (instead of smaller tiles and thinner shells), with
 // its level-kP1MultiLevel tiles (0: 64 atoms, 1: 32, 2: 16) and shells kP1MultiShell windows
 // wide.  Measured on the 20 nm rocksalt sphere at 0.0005 A and the spinel sphere at 1e-4 A:
 // level 0 with 0.25 / 0.75 is best (level 1: +14-20%, level 2: +35-60%).
 constexpr bool kP1MultiWindow = true;
+constexpr bool kP1Packed16 = true;  // wide-tile chunks count in windows of 16-bit counters
 constexpr int kP1MultiLevel = 0;
 constexpr double kP1MultiShell = 0.75;
 constexpr double kP1SingleShell = 0.25;

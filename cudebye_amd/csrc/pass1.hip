@@ -71,13 +71,79 @@ struct CountWindow {
     }
 };
 
-// CountChecked (degraded jobs and re-runs): out-of-window pairs go to the uint64 histogram in
-// HBM, unless a sibling window holds them (below a window that is not the first, above one
-// that is not the last).
+// Windows of 16-bit counters (jobs with kP1Win16): 2 kP1Bins bins, two per LDS word.  Bin
+// k_lo + i is field (i + 2) & 1 (low, high) of word (i + 2) >> 1, so word 0's high field is the
+// guard and word kP1Bins + 1's low field the sentinel.  A field counts up to 0x7FFF: the lane
+// whose increment sets its top bit (old field 0x7FFF, which exactly one lane sees per 32768
+// counts) takes 0x8000 back off the field and adds 32768 to the bin in HBM, so a field never
+// carries into its neighbour (that would need 32768 more increments of the same bin by the
+// workgroup's other waves before the lane's subtraction lands).  Guard and sentinel overflows
+// are recorded in *err instead.
+__device__ __forceinline__ void add16(uint32_t *h, int i, int k, unsigned long long *hrow, int64_t n_bins,
+                                      unsigned long long *oor, int *err)
+{
+    const int j = i + 2;
+    const int sh = (j & 1) << 4;
+    const uint32_t old = atomicAdd(&h[j >> 1], 1u << sh);
+    if (((old >> sh) & 0xFFFFu) == 0x7FFFu) {
+        atomicSub(&h[j >> 1], 0x8000u << sh);
+        if ((unsigned)i >= (unsigned)(2 * kP1Bins))
+            atomicAdd(err, 1);
+        else if ((int64_t)k < n_bins)
+            atomicAdd(&hrow[k], 32768ull);
+        else
+            atomicAdd(oor, 32768ull);
+    }
+}
+
+// CountFast16: the window holds every pair of the job; out-of-window values are clamped onto
+// the guard or sentinel field
+struct CountFast16 {
+    uint32_t *h;
+    int k_lo;
+    float vlo;   // k_lo - 1 and k_lo + 2 kP1Bins, exact in FP32
+    float vhi;
+    unsigned long long *hrow;
+    int64_t n_bins;
+    unsigned long long *oor;
+    int *err;
+    __device__ __forceinline__ void operator()(float v)
+    {
+        const int k = round_rpi(__builtin_amdgcn_fmed3f(v, vlo, vhi));
+        add16(h, k - k_lo, k, hrow, n_bins, oor, err);
+    }
+};
+
+// CountWindow16: one of several consecutive 16-bit windows of a job, as CountWindow
+struct CountWindow16 {
+    uint32_t *h;
+    int k_lo;
+    int kmin, kmax;
+    unsigned long long *hrow;
+    int64_t n_bins;
+    unsigned long long *oor;
+    int *err;
+    __device__ __forceinline__ void operator()(float v)
+    {
+        const int k = round_rpi(v);
+        kmin = min(kmin, k);
+        kmax = max(kmax, k);
+        const int i = k - k_lo;
+        if ((unsigned)i < (unsigned)(2 * kP1Bins)) add16(h, i, k, hrow, n_bins, oor, err);
+    }
+};
+
+// CountChecked (degraded jobs, and the complements of fast jobs that found pairs outside their
+// window): pairs in the LDS window (kP1Bins bins from k_lo) are counted there, unless
+// `complement`; pairs of the job's own window beyond it (16-bit windows) and pairs outside the
+// job's window that no sibling window holds (below the first window, above the last) go to the
+// uint64 histogram in HBM.  A complement run counts exactly the pairs the fast run could not.
 struct CountChecked {
     uint32_t *h;
     int k_lo;
     int win;
+    int width;   // the job's window: kP1Bins, or 2 kP1Bins for 16-bit windows
+    bool complement;
     int64_t n_bins;
     unsigned long long *hrow;
     uint32_t miss, oor;
@@ -85,9 +151,12 @@ struct CountChecked {
     {
         const int k = round_rpi(v);
         const unsigned kk = (unsigned)(k - k_lo);
-        if (kk < (unsigned)kP1Bins) {
+        const bool mine_hbm = complement ? !(kk < (unsigned)width) && (k < k_lo ? (win & kP1WinFirst) : (win & kP1WinLast))
+                                         : (kk < (unsigned)width ? kk >= (unsigned)kP1Bins
+                                                                 : (k < k_lo ? (win & kP1WinFirst) : (win & kP1WinLast)) != 0);
+        if (!complement && kk < (unsigned)kP1Bins) {
             atomicAdd(&h[kk + 1], 1u);
-        } else if (k < k_lo ? (win & kP1WinFirst) : (win & kP1WinLast)) {
+        } else if (mine_hbm) {
             ++miss;
             if (k >= 0 && (int64_t)k < n_bins)
                 atomicAdd(&hrow[k], 1ull);
@@ -116,7 +185,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     __shared__ uint32_t h[kP1Bins + 2];
     __shared__ uint32_t cand[kP1Cand];
     __shared__ float4 stage[kP1Threads / 64][64];
-    __shared__ int s_ncand, s_next, s_kmin, s_kmax;
+    __shared__ int s_ncand, s_next, s_kmin, s_kmax, s_err;
 
     // list entry = (model, job, piece, pieces); the checked launch runs the planned checked
     // jobs, then the fast jobs the fast launch queued for a re-run
@@ -132,6 +201,9 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             entry = retry[r];
         }
     }
+    // a negative piece count marks the complement of a fast job (see CountChecked)
+    const bool complement = entry.w < 0;
+    if (complement) entry.w = -entry.w;
     const P1Ref M = refs[entry.x];
     P1Job job = M.jobs[entry.y];
     if (entry.w > 1) {
@@ -162,9 +234,12 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     unsigned long long *hrow = M.hist + (int64_t)job.pair * n_bins;
     const int nblk = (job.b1 - job.b0 + 63) >> 6;
     const int units = job.a_cnt * nblk;
+    const bool w16 = (job.win & kP1Win16) != 0;
+    const int width = w16 ? 2 * kP1Bins : kP1Bins;
     if (threadIdx.x == 0) {
         s_kmin = INT_MAX;
         s_kmax = INT_MIN;
+        s_err = 0;
     }
 
     int cur_ia = -1;
@@ -226,10 +301,10 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                     // distances in [d - rad, d + rad]: one whose interval misses this job's
                     // window (below it when a window lies below, above it when one lies above)
                     // has no pair to count here.  Margin: 10 bins plus 1e-5 relative.
-                    if (job.win != kP1WinOnly) {
+                    if ((job.win & kP1WinOnly) != kP1WinOnly) {
                         const float rs = 1.f / rp;
                         const float lo = (float)(k_lo - 10) * rs - ta.rad;
-                        const float hi = (float)(k_lo + kP1Bins + 10) * rs + ta.rad;
+                        const float hi = (float)(k_lo + width + 10) * rs + ta.rad;
                         const float lo_m = lo - 1e-5f * fabsf(lo);
                         const float hi_m = hi + 1e-5f * hi;
                         w_lo2 = (job.win & kP1WinFirst) || lo_m <= 0.f ? -1.f : lo_m * lo_m;
@@ -314,7 +389,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
         }
     };
     if (CHECKED) {
-        CountChecked cnt{h, k_lo, job.win, n_bins, hrow, 0u, 0u};
+        CountChecked cnt{h, k_lo, job.win, width, complement, n_bins, hrow, 0u, 0u};
         run(cnt);
         __syncthreads();  // zeroing visible to all threads even when the job had no candidates
         uint32_t miss = cnt.miss, oor = cnt.oor;
@@ -326,18 +401,12 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             if (miss) atomicAdd(&counters[1], (unsigned long long)miss);
             if (oor) atomicAdd(M.oor, (unsigned long long)oor);
         }
+        if (complement) return;  // nothing counted in the window
     } else {
+        // the window keeps its counts either way; a job with pairs outside it (never, when
+        // the planner's margins hold) queues its complement for the checked variant
         bool bad;
-        if (job.win == kP1WinOnly) {
-            CountFast cnt{h, 1 - k_lo, (float)(k_lo - 1), (float)(k_lo + kP1Bins)};
-            run(cnt);
-            __syncthreads();
-            // a bin fell outside the window
-            bad = (h[0] | h[kP1Bins + 1]) != 0u;
-        } else {
-            CountWindow cnt{h, k_lo, INT_MAX, INT_MIN};
-            run(cnt);
-            int kmin = cnt.kmin, kmax = cnt.kmax;
+        auto kminmax = [&](int kmin, int kmax) {
             for (int off = 32; off > 0; off >>= 1) {
                 kmin = min(kmin, __shfl_xor(kmin, off));
                 kmax = max(kmax, __shfl_xor(kmax, off));
@@ -348,26 +417,56 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             }
             __syncthreads();
             // a bin below the first window or above the last one
-            bad = ((job.win & kP1WinFirst) && s_kmin < k_lo) ||
-                  ((job.win & kP1WinLast) && s_kmax >= k_lo + kP1Bins);
-        }
-        if (bad) {
-            // leave this job to the checked variant
-            if (threadIdx.x == 0) {
-                const unsigned long long r = atomicAdd(&counters[2], 1ull);
-                retry[r] = entry;
+            return ((job.win & kP1WinFirst) && s_kmin < k_lo) || ((job.win & kP1WinLast) && s_kmax >= k_lo + width);
+        };
+        if ((job.win & kP1WinOnly) == kP1WinOnly) {
+            if (!w16) {
+                CountFast cnt{h, 1 - k_lo, (float)(k_lo - 1), (float)(k_lo + kP1Bins)};
+                run(cnt);
+                __syncthreads();
+                bad = (h[0] | h[kP1Bins + 1]) != 0u;
+            } else {
+                CountFast16 cnt{h, k_lo, (float)(k_lo - 1), (float)(k_lo + 2 * kP1Bins), hrow, n_bins, M.oor, &s_err};
+                run(cnt);
+                __syncthreads();
+                bad = (h[0] >> 16) != 0u || (h[kP1Bins + 1] & 0xFFFFu) != 0u || s_err != 0;
             }
-            return;
+        } else if (!w16) {
+            CountWindow cnt{h, k_lo, INT_MAX, INT_MIN};
+            run(cnt);
+            bad = kminmax(cnt.kmin, cnt.kmax);
+        } else {
+            CountWindow16 cnt{h, k_lo, INT_MAX, INT_MIN, hrow, n_bins, M.oor, &s_err};
+            run(cnt);
+            bad = kminmax(cnt.kmin, cnt.kmax) || s_err != 0;
+        }
+        if (bad && threadIdx.x == 0) {
+            const unsigned long long r = atomicAdd(&counters[2], 1ull);
+            retry[r] = make_int4(entry.x, entry.y, entry.z, -entry.w);
         }
     }
-    for (int i = threadIdx.x; i < kP1Bins; i += kP1Threads) {
-        const uint32_t v = h[i + 1];
-        if (v) {
-            const int64_t k = (int64_t)k_lo + i;
-            if (k < n_bins)
-                atomicAdd(&hrow[k], (unsigned long long)v);
-            else
-                atomicAdd(M.oor, (unsigned long long)v);
+    if (!w16) {
+        for (int i = threadIdx.x; i < kP1Bins; i += kP1Threads) {
+            const uint32_t v = h[i + 1];
+            if (v) {
+                const int64_t k = (int64_t)k_lo + i;
+                if (k < n_bins)
+                    atomicAdd(&hrow[k], (unsigned long long)v);
+                else
+                    atomicAdd(M.oor, (unsigned long long)v);
+            }
+        }
+    } else {
+        for (int i = threadIdx.x; i < 2 * kP1Bins; i += kP1Threads) {
+            const int j = i + 2;
+            const uint32_t v = (h[j >> 1] >> ((j & 1) << 4)) & 0xFFFFu;
+            if (v) {
+                const int64_t k = (int64_t)k_lo + i;
+                if (k < n_bins)
+                    atomicAdd(&hrow[k], (unsigned long long)v);
+                else
+                    atomicAdd(M.oor, (unsigned long long)v);
+            }
         }
     }
 }
@@ -418,6 +517,7 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
     DBX_REQUIRE(rstep > 0 && std::isfinite(rstep), DBX_ERR_ARG, "rstep must be positive");
     const int S = m.n_groups;
     const double usable = (kP1Bins - 6) * rstep;
+    const double usable16 = (2 * kP1Bins - 6) * rstep;  // a window of 16-bit counters
 
     std::vector<P1Job> jobs;
     std::vector<double> cost;
@@ -490,13 +590,18 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                 const int R = 1 << L;
                 const double ta_atoms = 64.0 / R;
                 const double rad_typ = m.tiles_h[A[A.size() / 2]].rad;
-                // windows of a multi-window chunk: the fewest that leave a shell of kP1MultiShell
-                // windows' width, at least h_min
+                // windows of a wide-tile chunk (16-bit counters when kP1Packed16: twice the span
+                // per window): one when it leaves a shell of kP1SingleShell windows' width, else
+                // the fewest that leave kP1MultiShell windows' width, at least h_min
+                const bool packed = multi && kP1Packed16;
+                const double uw = packed ? usable16 : usable;
+                const int wbins = packed ? 2 * kP1Bins : kP1Bins;
                 auto n_windows = [&](double rad) {
-                    const double hw = std::max(h_min, kP1MultiShell * usable);
-                    return multi ? std::max(1, (int)std::ceil((2 * rad + 2 * mg + hw) / usable)) : 1;
+                    if (!multi || uw - 2 * rad - 2 * mg >= std::max(h_min, kP1SingleShell * uw)) return 1;
+                    const double hw = std::max(h_min, kP1MultiShell * uw);
+                    return std::max(1, (int)std::ceil((2 * rad + 2 * mg + hw) / uw));
                 };
-                const double h_typ = std::max(h_min, n_windows(rad_typ) * usable - 2 * rad_typ - 2 * mg);
+                const double h_typ = std::max(h_min, n_windows(rad_typ) * uw - 2 * rad_typ - 2 * mg);
                 const double shell_frac = std::min(1.0, h_typ / std::max(D, h_typ));
                 const double per_tile = ta_atoms * std::max(1.0, nB * shell_frac);
                 int CA = (int)std::ceil(6.0e6 / per_tile);
@@ -516,7 +621,7 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                         na += T.count;
                     }
                     const int nw = n_windows(rmaxc);
-                    double h = nw * usable - 2 * rmaxc - 2 * mg;
+                    double h = nw * uw - 2 * rmaxc - 2 * mg;
                     const bool degraded = h < h_min;
                     if (degraded) h = h_min;
                     const int ns = (int)(D / h) + 1;
@@ -564,10 +669,11 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                         const double jc = na / (c1 - c0) * (cst + 1.0) * (1.0 + 0.15 * (R - 1));
                         const int wn = degraded ? 1 : nw;
                         for (int w = 0; w < wn; w++) {
-                            J.k_lo = (int32_t)std::min<double>(klo + (double)w * kP1Bins, (double)INT32_MAX - kP1Bins);
-                            J.win = (w == 0 ? kP1WinFirst : 0) | (w == wn - 1 ? kP1WinLast : 0);
-                            // the single-window fast variant's bounds must be exact in FP32
-                            J.degraded = (degraded || (wn == 1 && J.k_lo + (double)kP1Bins >= 16777216.0)) ? 1 : 0;
+                            J.k_lo = (int32_t)std::min<double>(klo + (double)w * wbins, (double)INT32_MAX - wbins);
+                            J.win = (w == 0 ? kP1WinFirst : 0) | (w == wn - 1 ? kP1WinLast : 0) |
+                                    (packed && !degraded ? kP1Win16 : 0);
+                            // the single-window fast variants' bounds must be exact in FP32
+                            J.degraded = (degraded || (wn == 1 && J.k_lo + (double)wbins >= 16777216.0)) ? 1 : 0;
                             jobs.push_back(J);
                             cost.push_back(jc);
                         }

@@ -59,8 +59,9 @@ def check_plan(m):
     for J in jobs:
         pair, a_off, a_cnt, b0, b1, same, shell, klo, eoff, degraded, R, win = J[:12]
         ga, gb = pairs[pair]
-        assert R in (1, 2, 4) and win in (0, 1, 2, 3)
+        assert R in (1, 2, 4) and (win & ~7) == 0
         assert not degraded or win == 3
+        width = 2 * W if win & 4 else W  # 16-bit counters: twice the bins
         lo, hi = e2[eoff + shell], e2[eoff + shell + 1]
         for t in alist[a_off:a_off + a_cnt]:
             assert grp[t] == ga and cnt[t] <= 64 // R
@@ -86,12 +87,12 @@ def check_plan(m):
                 if degraded:
                     counted[pair] += int(valid.sum())  # the checked kernel counts every pair
                     continue
-                inside = (k >= klo) & (k < klo + W)
+                inside = (k >= klo) & (k < klo + width)
                 counted[pair] += int((valid & inside).sum())
                 if win & 1:
                     assert np.all((k >= klo) | ~valid), (pair, t, j, shell)
                 if win & 2:
-                    assert np.all((k < klo + W) | ~valid), (pair, t, j, shell)
+                    assert np.all((k < klo + width) | ~valid), (pair, t, j, shell)
     n = m.group_n
     want = [n[a] * (n[a] + 1) // 2 if a == b else n[a] * n[b] for a, b in pairs]
     assert list(counted) == want
@@ -118,6 +119,6 @@ def test_plan_uses_several_windows_where_needed():
     for m in (synth.small_random(1500, 2, 43, spread=45.0),
               synth.small_random(1200, 1, 44, spread=12.0, rstep=1e-4)):
         _, _, jobs, _, _ = host_plan(m)
-        assert np.any(jobs[:, 11] != 3) and jobs[:, 9].sum() == 0
+        assert np.any(jobs[:, 11] != 3) and np.any(jobs[:, 11] & 4) and jobs[:, 9].sum() == 0
     _, _, jobs, _, _ = host_plan(synth.spinel_sphere())
     assert jobs[:, 9].sum() == 0 and np.mean(jobs[:, 11] == 3) > 0.97
