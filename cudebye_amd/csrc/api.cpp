@@ -173,10 +173,8 @@ static Tile make_tile(const std::vector<Pt> &pts, int64_t lo, int64_t hi, int g,
 }
 
 // Host part of a model: validation, tiling, tile metadata and the tile-ordered positions.
-// Each subgroup is cut into blocks of 64 atoms (level 0), each block into halves of 32
-// (level 1) and quarters of 16 (level 2), all by median bisection, so every sub-tile is
-// a contiguous, compact run inside its block.  Blocks are stored in ascending size of
-// their bounding box; tiles_h holds all level-0 tiles, then all level-1, then all level-2.
+// Each subgroup is cut into blocks of 64 atoms by median bisection; blocks are stored in
+// ascending size of their bounding box, each a contiguous run of positions.
 void tile_model_host(const float *x, const float *y, const float *z, const int64_t *group_n,
                             int S, ModelDev &m, std::vector<float4> &pos_h)
 {
@@ -194,14 +192,11 @@ void tile_model_host(const float *x, const float *y, const float *z, const int64
     m.n_atoms = N;
     m.group_n.assign(group_n, group_n + S);
     m.bbox.assign(6 * S, 0.f);
+    m.group_tile0.assign(S, 0);
+    m.group_tile1.assign(S, 0);
+    m.tiles_h.clear();
     pos_h.clear();
     pos_h.reserve(N);
-    std::vector<Tile> lv[3];
-    std::vector<int32_t> c1a, c1b, c2a, c2b;  // children ranges (level 1 of each block, level 2 of each half)
-    for (int L = 0; L < 3; L++) {
-        m.lvl_t0[L].assign(S, 0);
-        m.lvl_t1[L].assign(S, 0);
-    }
     int64_t start = 0;
     for (int g = 0; g < S; g++) {
         const int64_t n = group_n[g];
@@ -228,54 +223,15 @@ void tile_model_host(const float *x, const float *y, const float *z, const int64
             key[b] = {(double)t.rad, b};
         }
         std::stable_sort(key.begin(), key.end());
-        for (int L = 0; L < 3; L++) m.lvl_t0[L][g] = (int32_t)lv[L].size();
+        m.group_tile0[g] = (int32_t)m.tiles_h.size();
         for (auto &kb : key) {
             const int64_t lo = blocks[kb.second].first, hi = blocks[kb.second].second;
-            const int64_t shift = (int64_t)pos_h.size() - lo;  // pts index -> position index
-            const int32_t parent = (int32_t)lv[0].size();
-            lv[0].push_back(make_tile(pts, lo, hi, g, lo + shift, parent));
-            std::vector<std::pair<int64_t, int64_t>> halves;
-            kd_split(pts, lo, hi, 32, halves);
-            c1a.push_back((int32_t)lv[1].size());
-            for (auto &hf : halves) {
-                lv[1].push_back(make_tile(pts, hf.first, hf.second, g, hf.first + shift, parent));
-                std::vector<std::pair<int64_t, int64_t>> quarters;
-                kd_split(pts, hf.first, hf.second, 16, quarters);
-                c2a.push_back((int32_t)lv[2].size());
-                for (auto &qt : quarters)
-                    lv[2].push_back(make_tile(pts, qt.first, qt.second, g, qt.first + shift, parent));
-                c2b.push_back((int32_t)lv[2].size());
-            }
-            c1b.push_back((int32_t)lv[1].size());
+            const int32_t self = (int32_t)m.tiles_h.size();
+            m.tiles_h.push_back(make_tile(pts, lo, hi, g, (int64_t)pos_h.size(), self));
             for (int64_t i = lo; i < hi; i++) pos_h.push_back(make_float4(pts[i].x, pts[i].y, pts[i].z, 0.f));
         }
-        for (int L = 0; L < 3; L++) m.lvl_t1[L][g] = (int32_t)lv[L].size();
+        m.group_tile1[g] = (int32_t)m.tiles_h.size();
         start += n;
-    }
-    // one array: level 0, level 1, level 2 (parents are level-0 indices, unchanged)
-    const int32_t off1 = (int32_t)lv[0].size(), off2 = off1 + (int32_t)lv[1].size();
-    m.tiles_h = lv[0];
-    m.tiles_h.insert(m.tiles_h.end(), lv[1].begin(), lv[1].end());
-    m.tiles_h.insert(m.tiles_h.end(), lv[2].begin(), lv[2].end());
-    for (int g = 0; g < S; g++) {
-        m.lvl_t0[1][g] += off1;
-        m.lvl_t1[1][g] += off1;
-        m.lvl_t0[2][g] += off2;
-        m.lvl_t1[2][g] += off2;
-    }
-    m.group_tile0 = m.lvl_t0[0];
-    m.group_tile1 = m.lvl_t1[0];
-    m.child1_0.resize(c1a.size());
-    m.child1_1.resize(c1b.size());
-    for (size_t i = 0; i < c1a.size(); i++) {
-        m.child1_0[i] = c1a[i] + off1;
-        m.child1_1[i] = c1b[i] + off1;
-    }
-    m.child2_0.resize(c2a.size());
-    m.child2_1.resize(c2b.size());
-    for (size_t i = 0; i < c2a.size(); i++) {
-        m.child2_0[i] = c2a[i] + off2;
-        m.child2_1[i] = c2b[i] + off2;
     }
 }
 

@@ -50,16 +50,17 @@ struct Tile {          // 32 bytes
     int32_t start;     // first atom (index into the float4 position array)
     int32_t count;     // atoms in the tile (1..64)
     int32_t group;
-    int32_t parent;    // level-0 tile (64-atom block) containing this tile
+    int32_t parent;    // the tile's own index (pairs within one subgroup: B tiles from here on)
 };
 
-// One pass-1 work item: the pairs of a list of A tiles (subgroup row; 64-, 32- or 16-atom
-// tiles) with the B atoms (subgroup col, 64-atom tiles [b0, b1)) whose distance from the
-// A tile's centre lies in shell `shell` of pair `pair`, counted in one LDS window of kP1Bins
-// bins starting at bin k_lo.  A shell wider than one window is covered by several jobs with
-// consecutive windows (`win`): each counts only the pairs inside its own window, so every pair
-// is counted by exactly one of them.  Degraded jobs (window narrower than the geometry, no
-// sibling windows) send out-of-window pairs straight to the global histogram.
+// One pass-1 work item: the pairs of a list of A tiles (subgroup row, 64-atom tiles) with the
+// B atoms (subgroup col, 64-atom tiles [b0, b1)) whose distance from the A tile's centre lies
+// in shell `shell` of pair `pair`, counted in one LDS window starting at bin k_lo: kP1Bins bins
+// of 32-bit counters, or 2 kP1Bins bins of 16-bit counters (kP1Win16).  A shell wider than one
+// window is covered by several jobs with consecutive windows: each counts only the pairs
+// inside its own window, so every pair is counted by exactly one of them.  Degraded jobs
+// (window narrower than the geometry, no siblings) send out-of-window pairs straight to the
+// global histogram.
 struct P1Job {
     int32_t pair;
     int32_t a_off, a_cnt;  // A tiles: alist[a_off .. a_off + a_cnt)
@@ -69,27 +70,24 @@ struct P1Job {
     int32_t k_lo;   // first bin of the LDS window
     int32_t e2_off; // offset of the pair's shell-edge table in the edge array
     int32_t degraded;  // 1: the window cannot hold the job's geometry (checked kernel)
-    int32_t R;         // B atoms per 64-lane step: 1, 2, 4 for A tiles of 64, 32, 16 atoms
     int32_t win;       // kP1WinFirst: no window below this one; kP1WinLast: none above;
                        // kP1Win16: a window of 2 kP1Bins 16-bit counters
+    int32_t pad;
 };
 constexpr int kP1WinFirst = 1, kP1WinLast = 2, kP1WinOnly = 3;
-constexpr int kP1Win16 = 4;  // the window is 2 kP1Bins bins of 16-bit counters
+constexpr int kP1Win16 = 4;
 // planner: a 64-atom A tile whose shells would be thinner than kP1SingleShell windows in one
-// window takes several windows per shell (instead of smaller tiles and thinner shells), with
-// its level-kP1MultiLevel tiles (0: 64 atoms, 1: 32, 2: 16) and shells kP1MultiShell windows
-// wide.  Measured on the 20 nm rocksalt sphere at 0.0005 A and the spinel sphere at 1e-4 A:
-// level 0 with 0.25 / 0.75 is best (level 1: +14-20%, level 2: +35-60%).
-constexpr bool kP1MultiWindow = true;
-constexpr bool kP1Packed16 = true;  // wide-tile chunks count in windows of 16-bit counters
-constexpr int kP1MultiLevel = 0;
+// 32-bit window counts them in 16-bit windows, one when that leaves a shell of kP1SingleShell
+// windows, else several with shells kP1MultiShell windows wide.  Measured on the 20 nm
+// rocksalt sphere at 0.0005 A and the spinel sphere at 1e-4 A against variants (32- or
+// 16-atom A tiles with thinner shells: +14-60%; other shell fractions: up to +35%).
 constexpr double kP1MultiShell = 0.75;
 constexpr double kP1SingleShell = 0.25;
 
 constexpr int kP1Threads = 1024;   // 16 waves of 64
 constexpr int kP1Bins = 35328;     // LDS window: 138 KiB of uint32 counters (+ guard and sentinel)
 constexpr int kP1Cand = 1024;      // LDS candidate list (A tile, B tile), 4 KiB
-// + a 1 KiB B-atom stage per wave (16 KiB) for the 32- and 16-atom A tiles
+// + a 1 KiB B-atom stage per wave (16 KiB)
 
 // Host-managed suballocator over one device allocation (the serving budget): allocation and
 // release are host bookkeeping only, so releasing memory never synchronises the device.
@@ -115,10 +113,7 @@ struct ModelDev {
     int n_groups = 0;
     int64_t n_atoms = 0;
     std::vector<int64_t> group_n;
-    std::vector<int32_t> group_tile0, group_tile1;  // level-0 tile range per subgroup
-    std::vector<int32_t> lvl_t0[3], lvl_t1[3];      // tile range per level and subgroup
-    std::vector<int32_t> child1_0, child1_1;        // level-1 children of each level-0 tile
-    std::vector<int32_t> child2_0, child2_1;        // level-2 children of each level-1 tile
+    std::vector<int32_t> group_tile0, group_tile1;  // tile range per subgroup
     std::vector<Tile> tiles_h;                      // host copy (job planning)
     std::vector<float> bbox;                        // per subgroup: min xyz, max xyz
     float4 *pos = nullptr;                          // device, spatially ordered

@@ -8,17 +8,15 @@
 // The pair set is the reference's: one subgroup -> pairs col <= row including self pairs
 // (cuDebye-v15.cu:137), two subgroups -> the full n_row x n_col rectangle.
 //
-// MI355X design.  A workgroup (16 waves) owns a window of kP1Bins uint32 counters in LDS.
-// A wave holds the atoms of an A tile in its lanes and streams B atoms past them: every
-// 64-lane step is one pair per lane, ~25 VALU operations and one LDS atomic.  With 64-atom
-// A tiles the B atom is a wave-uniform value (v_readlane); with 32- or 16-atom A tiles the
-// lanes hold 2 or 4 copies of the tile and take 2 or 4 B atoms per step from a small
-// per-wave LDS stage.  To keep all bins of a workgroup inside its LDS window, B atoms are
-// taken shell by shell: a job handles the B atoms whose distance from the A tile's centre
-// lies in [s*h, (s+1)*h); every pair of such an atom with an A atom then lies within
-// [s*h - rad_A, (s+1)*h + rad_A], and the planner picks h (and the A tile size) so that
-// this range fits the window.  At the end of a job the nonzero LDS counters are added to
-// the uint64 histogram in HBM.
+// MI355X design.  A workgroup (16 waves) owns a window of bin counters in LDS (kP1Bins 32-bit
+// counters, or 2 kP1Bins 16-bit ones).  A wave holds the atoms of a 64-atom A tile in its
+// lanes and streams B atoms past them, read as broadcasts from a per-wave LDS stage: every
+// 64-lane step is one pair per lane, ~21 VALU operations and one LDS atomic.  To keep a
+// workgroup's bins inside its window, B atoms are taken shell by shell: a job handles the B
+// atoms whose distance from the A tile's centre lies in [s*h, (s+1)*h); every pair of such an
+// atom with an A atom then lies within [s*h - rad_A, (s+1)*h + rad_A], and the planner picks
+// h so that this range fits the window, or a run of consecutive windows (one job each).  At
+// the end of a job the nonzero LDS counters are added to the uint64 histogram in HBM.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -222,10 +220,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int R = job.R;
-    const int TA = 64 / R;
-    const int slot = lane / TA;
-    const int ai = lane - slot * TA;
+    const int ai = lane;  // the lane's atom of the A tile
     const float e_lo = edge2[job.e2_off + job.shell];
     const float e_hi = edge2[job.e2_off + job.shell + 1];
     const float r_lo = e_lo > 0.f ? sqrtf(e_lo) : 0.f;
@@ -331,7 +326,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                    if (R == 1 && !diag) {
+                    if (!diag) {
                         if (a_ok) {
                             // two B atoms per iteration: two independent chains in flight
                             int t = 0;
@@ -348,35 +343,11 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                                 cnt(pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp));
                             }
                         }
-                    } else if (!diag) {
-                        // R B atoms per step (lane slot takes atom t + slot): whole steps need no
-                        // per-lane test, two steps per iteration; then at most one partial step
-                        const int nfull = nb & ~(R - 1);
-                        if (a_ok) {
-                            int t = 0;
-                            for (; t + R < nfull; t += 2 * R) {
-                                const float4 b0 = stage[wave][t + slot];
-                                const float4 b1 = stage[wave][t + R + slot];
-                                const float v0 = pair_scaled(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
-                                const float v1 = pair_scaled(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
-                                cnt(v0);
-                                cnt(v1);
-                            }
-                            if (t < nfull) {
-                                const float4 b = stage[wave][t + slot];
-                                cnt(pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp));
-                            }
-                            if (nfull + slot < nb) {
-                                const float4 b = stage[wave][nfull + slot];
-                                cnt(pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp));
-                            }
-                        }
                     } else {
-                        // the B tile holds the A tile's own atoms: pairs with index b >= a only
-                        for (int t = 0; t < nb; t += R) {
-                            const int src = t + slot;
-                            const float4 b = stage[wave][min(src, nb - 1)];
-                            if (a_ok && src < nb && idx_a <= __float_as_int(b.w))
+                        // the B tile is the A tile: pairs with index b >= a only
+                        for (int t = 0; t < nb; t++) {
+                            const float4 b = stage[wave][t];
+                            if (a_ok && idx_a <= __float_as_int(b.w))
                                 cnt(pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp));
                         }
                     }
@@ -502,11 +473,12 @@ double point_box_max(const Tile &t, const float *bb)
 }  // namespace
 
 // The plan of one model and bin width.  Per subgroup pair (row ga, col gb):
-//   - A tiles: each 64-atom block of ga whose radius lets a shell of useful width fit the
-//     window, else its 32-atom halves that fit, else its 16-atom quarters (which run in the
-//     checked kernel if even they do not fit: fine bins with wide tiles);
-//   - per level, A tiles sorted by radius and cut into chunks; each chunk gets its own shell
-//     width h = usable - 2 rad_max - margins and one job per shell it can reach;
+//   - A tiles (the 64-atom blocks of ga): narrow ones, whose radius leaves a shell of at least
+//     kP1SingleShell windows in one 32-bit window, and wide ones, counted in 16-bit windows
+//     (one, or several consecutive ones per shell);
+//   - each kind sorted by radius and cut into chunks; each chunk gets its own shell width
+//     h = n_windows usable - 2 rad_max - margins and, per shell it can reach, one job per
+//     window;
 //   - with n_cu > 0, jobs far above a CU's mean share are split along their B-tile range
 //     (n_cu = 0 keeps jobs whole: the launch lists cut them into pieces instead);
 //   - order: exact-window jobs, largest first, then the checked ones.
@@ -523,7 +495,6 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
     std::vector<double> cost;
     edge2.clear();
     alist.clear();
-    const int off1 = m.lvl_t0[1].empty() ? 0 : m.lvl_t0[1][0];
     int p = 0;
     for (int ga = 0; ga < S; ga++) {
         for (int gb = ga; gb < S; gb++, p++) {
@@ -543,67 +514,32 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
             const double D = std::sqrt(D2) + 1e-3;  // bound on |b - c_A|
             const double mg = 2e-3 + 4e-6 * (D + radA);
             const double h_min = std::max(0.5, 0.02 * usable);
-            // one window per shell for tiles that leave a shell of at least kP1SingleShell
-            // windows' width (h_min without multi-window jobs)
-            const double h_single = kP1MultiWindow ? std::max(h_min, kP1SingleShell * usable) : h_min;
+            // A tiles: "narrow" ones leave a shell of at least kP1SingleShell windows in one
+            // 32-bit window; "wide" ones count in 16-bit windows
+            const double h_single = std::max(h_min, kP1SingleShell * usable);
             const double rad_fit = 0.5 * (usable - h_single - 2 * mg);
-            // A tiles per level (sel[L]: one window per shell; sel[3 + L]: several consecutive
-            // windows per shell).  A 64-atom tile that leaves a shell of useful width in one
-            // window takes level 0; a wider one takes its level-kP1MultiLevel tiles with several
-            // windows when kP1MultiWindow, else its 32-atom halves that fit, else its 16-atom
-            // quarters (checked kernel if even those do not fit)
-            std::vector<int32_t> sel[6];
-            for (int t = a0; t < a1; t++) {
-                if (m.tiles_h[t].rad <= rad_fit) {
-                    sel[0].push_back(t);
-                    continue;
-                }
-                if (kP1MultiWindow) {
-                    if (kP1MultiLevel == 0) {
-                        sel[3].push_back(t);
-                    } else {
-                        for (int c = m.child1_0[t]; c < m.child1_1[t]; c++) {
-                            if (kP1MultiLevel == 1)
-                                sel[4].push_back(c);
-                            else
-                                for (int q = m.child2_0[c - off1]; q < m.child2_1[c - off1]; q++) sel[5].push_back(q);
-                        }
-                    }
-                    continue;
-                }
-                for (int c = m.child1_0[t]; c < m.child1_1[t]; c++) {
-                    if (m.tiles_h[c].rad <= rad_fit) {
-                        sel[1].push_back(c);
-                        continue;
-                    }
-                    for (int q = m.child2_0[c - off1]; q < m.child2_1[c - off1]; q++) sel[2].push_back(q);
-                }
-            }
+            std::vector<int32_t> sel[2];
+            for (int t = a0; t < a1; t++) sel[m.tiles_h[t].rad <= rad_fit ? 0 : 1].push_back(t);
             const double nB = (double)m.group_n[gb];
             const int nbt = b1 - b0;
-            for (int LL = 0; LL < 6; LL++) {
-                std::vector<int32_t> &A = sel[LL];
+            for (int wide = 0; wide < 2; wide++) {
+                std::vector<int32_t> &A = sel[wide];
                 if (A.empty()) continue;
-                const bool multi = LL >= 3;
-                const int L = LL % 3;
                 std::stable_sort(A.begin(), A.end(), [&](int x, int y) { return m.tiles_h[x].rad < m.tiles_h[y].rad; });
-                const int R = 1 << L;
-                const double ta_atoms = 64.0 / R;
                 const double rad_typ = m.tiles_h[A[A.size() / 2]].rad;
-                // windows of a wide-tile chunk (16-bit counters when kP1Packed16: twice the span
-                // per window): one when it leaves a shell of kP1SingleShell windows' width, else
-                // the fewest that leave kP1MultiShell windows' width, at least h_min
-                const bool packed = multi && kP1Packed16;
-                const double uw = packed ? usable16 : usable;
-                const int wbins = packed ? 2 * kP1Bins : kP1Bins;
+                // windows of a wide-tile chunk (16-bit counters: twice the span per window): one
+                // when it leaves a shell of kP1SingleShell windows' width, else the fewest that
+                // leave kP1MultiShell windows' width, at least h_min
+                const double uw = wide ? usable16 : usable;
+                const int wbins = wide ? 2 * kP1Bins : kP1Bins;
                 auto n_windows = [&](double rad) {
-                    if (!multi || uw - 2 * rad - 2 * mg >= std::max(h_min, kP1SingleShell * uw)) return 1;
+                    if (!wide || uw - 2 * rad - 2 * mg >= std::max(h_min, kP1SingleShell * uw)) return 1;
                     const double hw = std::max(h_min, kP1MultiShell * uw);
                     return std::max(1, (int)std::ceil((2 * rad + 2 * mg + hw) / uw));
                 };
                 const double h_typ = std::max(h_min, n_windows(rad_typ) * uw - 2 * rad_typ - 2 * mg);
                 const double shell_frac = std::min(1.0, h_typ / std::max(D, h_typ));
-                const double per_tile = ta_atoms * std::max(1.0, nB * shell_frac);
+                const double per_tile = 64.0 * std::max(1.0, nB * shell_frac);
                 int CA = (int)std::ceil(6.0e6 / per_tile);
                 CA = std::max(1, std::min(CA, 64));
                 const int nchunks = ((int)A.size() + CA - 1) / CA;
@@ -655,7 +591,6 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                         J.same = (ga == gb);
                         J.shell = sh;
                         J.e2_off = e2_off;
-                        J.R = R;
                         double klo;
                         if (!degraded)
                             klo = std::floor((sh * h - rmaxc - mg) / rstep) - 2;
@@ -666,12 +601,12 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                         double cst = shell_cost[sh];
                         if (sh > 0) cst += 0.25 * shell_cost[sh - 1];
                         if (sh + 1 < ns) cst += 0.25 * shell_cost[sh + 1];
-                        const double jc = na / (c1 - c0) * (cst + 1.0) * (1.0 + 0.15 * (R - 1));
+                        const double jc = na / (c1 - c0) * (cst + 1.0);
                         const int wn = degraded ? 1 : nw;
                         for (int w = 0; w < wn; w++) {
                             J.k_lo = (int32_t)std::min<double>(klo + (double)w * wbins, (double)INT32_MAX - wbins);
                             J.win = (w == 0 ? kP1WinFirst : 0) | (w == wn - 1 ? kP1WinLast : 0) |
-                                    (packed && !degraded ? kP1Win16 : 0);
+                                    (wide && !degraded ? kP1Win16 : 0);
                             // the single-window fast variants' bounds must be exact in FP32
                             J.degraded = (degraded || (wn == 1 && J.k_lo + (double)wbins >= 16777216.0)) ? 1 : 0;
                             jobs.push_back(J);

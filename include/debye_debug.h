@@ -19,13 +19,14 @@ int dbx_debug_model_tiles(const dbx_model *m, void *out, int cap);
  * copies when cap >= count. */
 int64_t dbx_debug_model_positions(dbx_ctx *ctx, const dbx_model *m, float *out4, int64_t cap);
 /* Pass-1 plan for (m, rstep): work items of 12 int32 each (pair, a_off, a_cnt, b0, b1, same,
- * shell, k_lo, e2_off, degraded, R, pad) in launch order, and the shell-edge table (float, squared A^2).
+ * shell, k_lo, e2_off, degraded, win, pad) in launch order, and the shell-edge table (float,
+ * squared A^2).  win: 1 no window below, 2 none above, 4 a window of 16-bit counters.
  * Returns DBX_OK and the counts; copies when the capacities suffice. */
 int dbx_debug_pass1_plan(const dbx_model *m, double rstep, void *jobs, int cap_jobs, float *edge2,
                          int cap_edge2, int *n_jobs, int *n_edge2);
 
 /* The same tiling and plan computed on the host from arrays (no device call): tiles (all
- * three levels: 64-, 32- and 16-atom tiles, in that order), the tile-ordered positions
+ * 64-atom blocks of every subgroup), the tile-ordered positions
  * (float4), work items, shell edges and the A-tile index list the work items point into,
  * each copied when its capacity suffices; the counts are always returned. */
 int dbx_debug_plan_host(const float *x, const float *y, const float *z, const int64_t *group_n,

@@ -168,10 +168,10 @@ def test_repeatable_and_permutation_invariant(ctx):
 
 
 @pytest.mark.parametrize("maker", ["c3_small", "c4_model5", "c5_req7", "sparse"])
-def test_mixed_tile_sizes_bit_exact(ctx, maker):
-    """Workloads whose sparse subgroups need 32- and 16-atom A tiles (two or four B atoms per
-    step) and fine bins that run checked jobs: a 10 nm sphere of the rocksalt-like three-species
-    lattice at 0.0005 A bins, two request models of the batched/serving streams, a sparse cloud."""
+def test_wide_tiles_and_window_groups_bit_exact(ctx, maker):
+    """Workloads whose sparse subgroups and fine bins need 16-bit windows and several windows per
+    shell: a 10 nm sphere of the rocksalt-like three-species lattice at 0.0005 A bins, two
+    request models of the batched/serving streams, a sparse cloud."""
     m = {"c3_small": lambda: synth.rocksalt_sphere(diameter=100.0, nq=64),
          "c4_model5": lambda: synth.batch_model(5),
          "c5_req7": lambda: synth.serve_request(7),

@@ -57,14 +57,14 @@ def check_plan(m):
     seen = {}
     counted = np.zeros(len(pairs), np.int64)
     for J in jobs:
-        pair, a_off, a_cnt, b0, b1, same, shell, klo, eoff, degraded, R, win = J[:12]
+        pair, a_off, a_cnt, b0, b1, same, shell, klo, eoff, degraded, win = J[:11]
         ga, gb = pairs[pair]
-        assert R in (1, 2, 4) and (win & ~7) == 0
+        assert (win & ~7) == 0
         assert not degraded or win == 3
         width = 2 * W if win & 4 else W  # 16-bit counters: twice the bins
         lo, hi = e2[eoff + shell], e2[eoff + shell + 1]
         for t in alist[a_off:a_off + a_cnt]:
-            assert grp[t] == ga and cnt[t] <= 64 // R
+            assert grp[t] == ga and cnt[t] <= 64
             A = pos[start[t]:start[t] + cnt[t], :3]
             ia = np.arange(start[t], start[t] + cnt[t])
             for j in range(max(b0, parent[t]) if same else b0, b1):
@@ -105,9 +105,9 @@ def test_plan_covers_every_pair_once_and_windows_hold(maker):
          "rand3": lambda: synth.small_random(3000, 3, 41, spread=20.0),
          "batch3": lambda: synth.batch_model(3),
          "batch0": lambda: synth.batch_model(0),
-         # sparse species: a few atoms spread over a large volume (32/16-atom A tiles)
+         # sparse species: a few atoms spread over a large volume (16-bit and multiple windows)
          "sparse": lambda: synth.small_random(1500, 2, 43, spread=45.0),
-         # fine bins: narrow window (16-atom tiles, checked jobs)
+         # fine bins: narrow windows (several per shell)
          "fine": lambda: synth.small_random(1200, 1, 44, spread=12.0, rstep=1e-4)}[maker]()
     jobs = check_plan(m)
     assert len(jobs) > 0
@@ -119,6 +119,6 @@ def test_plan_uses_several_windows_where_needed():
     for m in (synth.small_random(1500, 2, 43, spread=45.0),
               synth.small_random(1200, 1, 44, spread=12.0, rstep=1e-4)):
         _, _, jobs, _, _ = host_plan(m)
-        assert np.any(jobs[:, 11] != 3) and np.any(jobs[:, 11] & 4) and jobs[:, 9].sum() == 0
+        assert np.any(jobs[:, 10] != 3) and np.any(jobs[:, 10] & 4) and jobs[:, 9].sum() == 0
     _, _, jobs, _, _ = host_plan(synth.spinel_sphere())
-    assert jobs[:, 9].sum() == 0 and np.mean(jobs[:, 11] == 3) > 0.97
+    assert jobs[:, 9].sum() == 0 and np.mean(jobs[:, 10] == 3) > 0.97
