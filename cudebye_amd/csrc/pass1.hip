@@ -32,6 +32,14 @@
 
 namespace dbx {
 
+// uint64 add to the histogram in HBM through a global (not flat) atomic: it waits on the
+// vector-memory counter only, so the LDS traffic around it is not serialised behind it
+typedef __attribute__((address_space(1))) unsigned long long gu64;
+__device__ __forceinline__ void hbm_add(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_fetch_add((gu64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // The LDS window: h[0] guard (bins below the window), h[1 + i] bin k_lo + i for
 // 0 <= i < kP1Bins, h[kP1Bins + 1] sentinel (bins above it).  v = the pair's distance in bin
 // units (pair_scaled).
@@ -49,14 +57,16 @@ struct CountFast {
         const int k = round_rpi(__builtin_amdgcn_fmed3f(v, vlo, vhi));
         atomicAdd(&h[k + off], 1u);
     }
+    __device__ __forceinline__ void finish() {}
 };
 
 // CountWindow (one of several consecutive windows of a job): a pair outside the window belongs
-// to a sibling job and is skipped (its lane is masked off the atomic, so no counter collects
-// the skipped pairs); kmin / kmax record the bins seen, to check that none falls below the
-// first window or above the last.
+// to a sibling job; its lane adds to its own slot of `trash` instead (no branch, no shared
+// address); kmin / kmax record the bins seen, to check that none falls below the first window
+// or above the last.
 struct CountWindow {
     uint32_t *h;
+    uint32_t *trash;  // this lane's discard counter
     int k_lo;
     int kmin, kmax;
     __device__ __forceinline__ void operator()(float v)
@@ -65,8 +75,9 @@ struct CountWindow {
         kmin = min(kmin, k);
         kmax = max(kmax, k);
         const unsigned kk = (unsigned)(k - k_lo);
-        if (kk < (unsigned)kP1Bins) atomicAdd(&h[kk + 1], 1u);
+        atomicAdd(kk < (unsigned)kP1Bins ? &h[kk + 1] : trash, 1u);
     }
+    __device__ __forceinline__ void finish() {}
 };
 
 // Windows of 16-bit counters (jobs with kP1Win16): 2 kP1Bins bins, two per LDS word.  Bin
@@ -74,60 +85,73 @@ struct CountWindow {
 // guard and word kP1Bins + 1's low field the sentinel.  A field counts up to 0x7FFF: the lane
 // whose increment sets its top bit (old field 0x7FFF, which exactly one lane sees per 32768
 // counts) takes 0x8000 back off the field and adds 32768 to the bin in HBM, so a field never
-// carries into its neighbour (that would need 32768 more increments of the same bin by the
-// workgroup's other waves before the lane's subtraction lands).  Guard and sentinel overflows
-// are recorded in *err instead.
-__device__ __forceinline__ void add16(uint32_t *h, int i, int k, unsigned long long *hrow, int64_t n_bins,
-                                      unsigned long long *oor, int *err)
-{
-    const int j = i + 2;
-    const int sh = (j & 1) << 4;
-    const uint32_t old = atomicAdd(&h[j >> 1], 1u << sh);
-    if (((old >> sh) & 0xFFFFu) == 0x7FFFu) {
-        atomicSub(&h[j >> 1], 0x8000u << sh);
-        if ((unsigned)i >= (unsigned)(2 * kP1Bins))
-            atomicAdd(err, 1);
-        else if ((int64_t)k < n_bins)
-            atomicAdd(&hrow[k], 32768ull);
-        else
-            atomicAdd(oor, 32768ull);
-    }
-}
-
-// CountFast16: the window holds every pair of the job; out-of-window values are clamped onto
-// the guard or sentinel field
-struct CountFast16 {
+// carries into its neighbour (that would need 32768 more increments of the same bin, by the
+// workgroup's other waves, before the lane's subtraction lands).  The lane looks at the old
+// value one pair later, so that waiting for it does not stall the next pair; finish() looks at
+// the last one.  Guard and sentinel overflows are recorded in *err instead.
+struct Count16 {
     uint32_t *h;
     int k_lo;
-    float vlo;   // k_lo - 1 and k_lo + 2 kP1Bins, exact in FP32
-    float vhi;
     unsigned long long *hrow;
     int64_t n_bins;
     unsigned long long *oor;
     int *err;
+    uint32_t p_old = 0u;  // the previous pair's old word (0: nothing to look at)
+    int p_i = 0, p_k = 0;
+    __device__ __forceinline__ void look()
+    {
+        const int j = p_i + 2;
+        const int sh = (j & 1) << 4;
+        if (((p_old >> sh) & 0xFFFFu) == 0x7FFFu) {
+            atomicSub(&h[j >> 1], 0x8000u << sh);
+            if ((unsigned)p_i >= (unsigned)(2 * kP1Bins))
+                atomicAdd(err, 1);
+            else if ((int64_t)p_k < n_bins)
+                hbm_add(&hrow[p_k], 32768ull);
+            else
+                hbm_add(oor, 32768ull);
+        }
+    }
+    // adds one count to window bin i (bin k), or to `trash` when !in
+    __device__ __forceinline__ void add(int i, int k, bool in, uint32_t *trash)
+    {
+        const int j = i + 2;
+        const uint32_t old = atomicAdd(in ? &h[j >> 1] : trash, 1u << ((j & 1) << 4));
+        look();
+        p_old = in ? old : 0u;
+        p_i = i;
+        p_k = k;
+    }
+    __device__ __forceinline__ void finish()
+    {
+        look();
+        p_old = 0u;
+    }
+};
+
+// CountFast16: the window holds every pair of the job; out-of-window values are clamped onto
+// the guard or sentinel field
+struct CountFast16 : Count16 {
+    float vlo;   // k_lo - 1 and k_lo + 2 kP1Bins, exact in FP32
+    float vhi;
     __device__ __forceinline__ void operator()(float v)
     {
         const int k = round_rpi(__builtin_amdgcn_fmed3f(v, vlo, vhi));
-        add16(h, k - k_lo, k, hrow, n_bins, oor, err);
+        add(k - k_lo, k, true, nullptr);
     }
 };
 
 // CountWindow16: one of several consecutive 16-bit windows of a job, as CountWindow
-struct CountWindow16 {
-    uint32_t *h;
-    int k_lo;
+struct CountWindow16 : Count16 {
+    uint32_t *trash;
     int kmin, kmax;
-    unsigned long long *hrow;
-    int64_t n_bins;
-    unsigned long long *oor;
-    int *err;
     __device__ __forceinline__ void operator()(float v)
     {
         const int k = round_rpi(v);
         kmin = min(kmin, k);
         kmax = max(kmax, k);
         const int i = k - k_lo;
-        if ((unsigned)i < (unsigned)(2 * kP1Bins)) add16(h, i, k, hrow, n_bins, oor, err);
+        add(i, k, (unsigned)i < (unsigned)(2 * kP1Bins), trash);
     }
 };
 
@@ -157,11 +181,12 @@ struct CountChecked {
         } else if (mine_hbm) {
             ++miss;
             if (k >= 0 && (int64_t)k < n_bins)
-                atomicAdd(&hrow[k], 1ull);
+                hbm_add(&hrow[k], 1ull);
             else
                 ++oor;
         }
     }
+    __device__ __forceinline__ void finish() {}
 };
 
 // One workgroup runs one job: it zeroes its LDS window, culls the job's (A tile, B tile)
@@ -184,6 +209,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     __shared__ uint32_t cand[kP1Cand];
     __shared__ float4 stage[kP1Threads / 64][64];
     __shared__ int s_ncand, s_next, s_kmin, s_kmax, s_err;
+    __shared__ uint32_t trash[64];  // per-lane discard counters of multi-window jobs
 
     // list entry = (model, job, piece, pieces); the checked launch runs the planned checked
     // jobs, then the fast jobs the fast launch queued for a re-run
@@ -397,18 +423,24 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                 __syncthreads();
                 bad = (h[0] | h[kP1Bins + 1]) != 0u;
             } else {
-                CountFast16 cnt{h, k_lo, (float)(k_lo - 1), (float)(k_lo + 2 * kP1Bins), hrow, n_bins, M.oor, &s_err};
+                CountFast16 cnt;
+                cnt.h = h, cnt.k_lo = k_lo, cnt.hrow = hrow, cnt.n_bins = n_bins, cnt.oor = M.oor, cnt.err = &s_err;
+                cnt.vlo = (float)(k_lo - 1), cnt.vhi = (float)(k_lo + 2 * kP1Bins);
                 run(cnt);
+                cnt.finish();
                 __syncthreads();
                 bad = (h[0] >> 16) != 0u || (h[kP1Bins + 1] & 0xFFFFu) != 0u || s_err != 0;
             }
         } else if (!w16) {
-            CountWindow cnt{h, k_lo, INT_MAX, INT_MIN};
+            CountWindow cnt{h, &trash[lane], k_lo, INT_MAX, INT_MIN};
             run(cnt);
             bad = kminmax(cnt.kmin, cnt.kmax);
         } else {
-            CountWindow16 cnt{h, k_lo, INT_MAX, INT_MIN, hrow, n_bins, M.oor, &s_err};
+            CountWindow16 cnt;
+            cnt.h = h, cnt.k_lo = k_lo, cnt.hrow = hrow, cnt.n_bins = n_bins, cnt.oor = M.oor, cnt.err = &s_err;
+            cnt.trash = &trash[lane], cnt.kmin = INT_MAX, cnt.kmax = INT_MIN;
             run(cnt);
+            cnt.finish();
             bad = kminmax(cnt.kmin, cnt.kmax) || s_err != 0;
         }
         if (bad && threadIdx.x == 0) {
@@ -422,9 +454,9 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             if (v) {
                 const int64_t k = (int64_t)k_lo + i;
                 if (k < n_bins)
-                    atomicAdd(&hrow[k], (unsigned long long)v);
+                    hbm_add(&hrow[k], (unsigned long long)v);
                 else
-                    atomicAdd(M.oor, (unsigned long long)v);
+                    hbm_add(M.oor, (unsigned long long)v);
             }
         }
     } else {
@@ -434,9 +466,9 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             if (v) {
                 const int64_t k = (int64_t)k_lo + i;
                 if (k < n_bins)
-                    atomicAdd(&hrow[k], (unsigned long long)v);
+                    hbm_add(&hrow[k], (unsigned long long)v);
                 else
-                    atomicAdd(M.oor, (unsigned long long)v);
+                    hbm_add(M.oor, (unsigned long long)v);
             }
         }
     }
