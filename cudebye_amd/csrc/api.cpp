@@ -128,12 +128,20 @@ void kd_split(std::vector<Pt> &pts, int64_t lo, int64_t hi, int64_t unit,
             if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
         const int64_t nt = (n + unit - 1) / unit;
         const int64_t nl = (nt / 2) * unit;
-        auto key = [ax](const Pt &p) { return ax == 0 ? p.x : (ax == 1 ? p.y : p.z); };
-        std::nth_element(pts.begin() + r.lo, pts.begin() + r.lo + nl, pts.begin() + r.hi,
-                         [&](const Pt &a, const Pt &b) {
-                             float ka = key(a), kb = key(b);
-                             return ka < kb || (ka == kb && a.id < b.id);
-                         });
+        // one comparator per axis (no per-comparison axis switch); ties broken by atom index
+        auto split = [&](auto key) {
+            std::nth_element(pts.begin() + r.lo, pts.begin() + r.lo + nl, pts.begin() + r.hi,
+                             [&](const Pt &a, const Pt &b) {
+                                 const float ka = key(a), kb = key(b);
+                                 return ka < kb || (ka == kb && a.id < b.id);
+                             });
+        };
+        if (ax == 0)
+            split([](const Pt &p) { return p.x; });
+        else if (ax == 1)
+            split([](const Pt &p) { return p.y; });
+        else
+            split([](const Pt &p) { return p.z; });
         // depth first, left before right
         stack.push_back({r.lo + nl, r.hi});
         stack.push_back({r.lo, r.lo + nl});

@@ -442,7 +442,11 @@ int dbx_server_open(int device, const dbx_server_config *cfg, dbx_server **out)
     if (cfg) s->cfg = *cfg;
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 256;
     if (!(s->cfg.max_step_pairs > 0)) s->cfg.max_step_pairs = 2e10;
-    if (s->cfg.prep_threads <= 0) s->cfg.prep_threads = 6;
+    if (s->cfg.prep_threads <= 0) {
+        // the host's cores less the two lanes, the caller and one spare, at most 12
+        const int hw = (int)std::thread::hardware_concurrency();
+        s->cfg.prep_threads = hw > 0 ? std::max(2, std::min(12, hw - 4)) : 6;
+    }
     int rc = DBX_OK;
     for (int i = 0; i < 2 && rc == DBX_OK; i++) rc = dbx_open(device, 0, &s->lane[i]);
     if (rc == DBX_OK) rc = dbx_open(device, 0, &s->rw_ctx);

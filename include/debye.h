@@ -176,7 +176,8 @@ typedef struct dbx_server_config {
     size_t hbm_budget;      /* device bytes for resident requests (0: 3/4 of free memory) */
     int max_batch;          /* requests per step (0: 256) */
     double max_step_pairs;  /* pair distances per step (0: 2e10) */
-    int prep_threads;       /* host threads for tiling and planning (0: 6) */
+    int prep_threads;       /* host threads for tiling and planning (0: the host's hardware
+                               threads less 4, between 2 and 12) */
     int pass2_mode;         /* DBX_PASS2_MFMA (default) or DBX_PASS2_FP64 */
 } dbx_server_config;
 typedef struct dbx_request {
