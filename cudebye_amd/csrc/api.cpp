@@ -610,7 +610,6 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
         const size_t nqd = (size_t)nq;
         double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * nqd * (2 * (size_t)n_rows + n_items));
         double *dff = d, *dS = dff + n_rows * nqd, *dI = dS + n_rows * nqd;
-        DBX_HIP(hipMemsetAsync(door, 0, sizeof(uint64_t) * (size_t)n_items, ctx->stream));
         const double *dq = device_q(ctx, q, nq);
         std::vector<P1Item> p1;
         std::vector<RowDesc> rows;

@@ -141,7 +141,7 @@ struct P1Ref {
     unsigned long long *hist;  // this model's [P][n_bins] block
     int64_t n_bins;
     float rp;
-    int32_t pad;
+    int32_t n_rows;            // P, the rows of hist
     unsigned long long *oor;   // this model's count of pairs at or beyond n_bins
 };
 
@@ -225,7 +225,7 @@ struct P1Item {
     ModelDev *m;
     int64_t n_bins;
     uint64_t *d_hist;  // [P][n_bins] of this model (zeroed by the call)
-    uint64_t *d_oor = nullptr;  // its out-of-range counter (caller-zeroed); null: the context's
+    uint64_t *d_oor = nullptr;  // its out-of-range counter (zeroed by pass1_batch); null: the context's
 };
 P1Stats pass1_batch(dbx_ctx *ctx, const std::vector<P1Item> &items, double rstep, bool sync_stats);
 // reads the pass-1 counters of the last launch (after the stream is synchronised)

@@ -200,6 +200,17 @@ struct CountChecked {
 // job whose sentinel is nonzero flushes nothing and is queued for the checked variant.
 // CHECKED = true: out-of-window pairs go straight to the uint64 histogram in HBM (jobs whose
 // window is narrower than their geometry, and the queued ones).
+// zeroes the histogram block and the out-of-range count of every model of a launch:
+// blockIdx.y = model (refs + y0), blocks along x stride over its [P][n_bins] words
+__global__ __launch_bounds__(256) void p1_zero_kernel(const P1Ref *__restrict__ refs, int y0)
+{
+    const P1Ref R = refs[y0 + (int)blockIdx.y];
+    const int64_t n = (int64_t)R.n_rows * R.n_bins;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        R.hist[i] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *R.oor = 0ull;
+}
+
 template <bool CHECKED>
 __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const P1Ref *__restrict__ refs, const int4 *__restrict__ list, int n_fast, int n_checked,
@@ -810,7 +821,7 @@ static P1Ref make_ref(const ModelDev &m, int64_t n_bins, uint64_t *d_hist, doubl
     r.hist = (unsigned long long *)d_hist;
     r.n_bins = n_bins;
     r.rp = (float)(1 / rstep);  // cuDebye-v15.cu:739
-    r.pad = 0;
+    r.n_rows = m.n_groups * (m.n_groups + 1) / 2;
     r.oor = nullptr;
     return r;
 }
@@ -821,14 +832,14 @@ P1Stats pass1_batch(dbx_ctx *ctx, const std::vector<P1Item> &items, double rstep
     P1Stats st;
     std::vector<P1Ref> refs(items.size());
     int n_fast = 0, n_checked = 0;
+    int64_t max_words = 0;
     for (size_t i = 0; i < items.size(); i++) {
         const P1Item &it = items[i];
         DBX_REQUIRE(it.n_bins > 0 && it.n_bins < (int64_t)1 << 40, DBX_ERR_ARG, "bad n_bins");
         ensure_plan(ctx, *it.m, rstep);
         refs[i] = make_ref(*it.m, it.n_bins, it.d_hist, rstep);
         refs[i].oor = it.d_oor ? (unsigned long long *)it.d_oor : (unsigned long long *)ctx->counters;
-        const int P = it.m->n_groups * (it.m->n_groups + 1) / 2;
-        DBX_HIP(hipMemsetAsync(it.d_hist, 0, sizeof(uint64_t) * (size_t)P * it.n_bins, ctx->stream));
+        max_words = std::max<int64_t>(max_words, (int64_t)refs[i].n_rows * it.n_bins);
     }
     // launch list: one model uses its own list; several get one merged list whose pieces are
     // sized for the whole batch, exact-window entries then checked ones, largest first
@@ -864,6 +875,12 @@ P1Stats pass1_batch(dbx_ctx *ctx, const std::vector<P1Item> &items, double rstep
     P1Ref *hr = (P1Ref *)pinned(ctx, kPinRefs, sizeof(P1Ref) * refs.size());
     std::memcpy(hr, refs.data(), sizeof(P1Ref) * refs.size());
     DBX_HIP(hipMemcpyAsync(d_refs, hr, sizeof(P1Ref) * refs.size(), hipMemcpyHostToDevice, ctx->stream));
+    // every histogram and out-of-range count zeroed by one launch (per 65535 models)
+    const unsigned zx = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (max_words + 4095) / 4096));
+    for (int y0 = 0; y0 < (int)refs.size(); y0 += 65535) {
+        const unsigned ny = (unsigned)std::min<int>(65535, (int)refs.size() - y0);
+        hipLaunchKernelGGL(p1_zero_kernel, dim3(zx, ny), dim3(256), 0, ctx->stream, d_refs, y0);
+    }
     DBX_HIP(hipMemsetAsync(ctx->counters, 0, 4 * sizeof(uint64_t), ctx->stream));
     ctx->p1_timed = false;
     if (st.n_jobs > 0) {

@@ -334,7 +334,16 @@ __global__ void p2_reduce_splits_kernel(const double *__restrict__ Spart, int ns
     const int64_t n = (int64_t)nr * nq;
     if (i >= n) return;
     double acc = rsum ? shift * (double)rsum[r0 + i / nq] : 0.0;
-    for (int s = 0; s < nsplit; s++) acc += Spart[(int64_t)s * n + i];
+    // eight loads in flight, the sum kept in split order
+    int s = 0;
+    for (; s + 8 <= nsplit; s += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = Spart[(int64_t)(s + u) * n + i];
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc += v[u];
+    }
+    for (; s < nsplit; s++) acc += Spart[(int64_t)s * n + i];
     S[(int64_t)r0 * nq + i] = acc;
 }
 
@@ -431,8 +440,9 @@ __global__ void p2_rtab_kernel(const int *__restrict__ kz, int Kc, int Kc_pad, d
 //   - Sinc digits: the 16 q x 64 bin values of a step are generated by the WR waves of a q
 //     block together (16 / WR values per thread), through LDS (double-buffered) when WR > 1,
 //     one step ahead of the matrix cores so that both run at once.  The step's r and 1/r come
-//     through the same LDS ring.  With ROTATE, a step whose 64 bins are consecutive takes one
-//     sine-cosine pair per thread and rotates it by q rstep for the following bins.
+//     through the same LDS ring.  A thread whose values lie at consecutive bins takes one
+//     sine-cosine pair and rotates it by q rstep for the following bins.
+//   - The 32-bit sums are folded into FP64 partial sums in Spart every kMfFold steps.
 // The workgroup's bins end at kext of its first row block: rows are ordered by length,
 // longest first.
 template <int RB, int WQ, int WR, int I>
@@ -447,7 +457,6 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     constexpr int C = I + kMfJ - 1;
     constexpr int V = 16 / WR;
     constexpr int NG = V / 4;
-    constexpr bool ROTATE = WR == 1 && RB <= 2;
     __shared__ i32x4 sa[3][NF][64];
     __shared__ double srt[3][2 * kMfK];
     __shared__ i32x4 sd[WR > 1 ? 2 : 1][WR > 1 ? WQ : 1][kMfJ][64];
@@ -464,37 +473,41 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     const double Q = Q0 != 0.0 ? Q0 : 1e-30;
     const double cQ = kMfC / Q;
     const int gk = (gl >> 4) * 16 + gt;
-    // rotation by one bin: (sin, cos) of Q rstep; the wave's smallest Q, for the test that a
-    // rotated run starts at Q r >= 0.5
-    double sdQ = 0.0, cdQ = 1.0, qmin = Q;
-    if constexpr (ROTATE) {
-        sincos_cw(Q * rstep, sdQ, cdQ);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) qmin = fmin(qmin, __shfl_xor(qmin, off));
-    }
+    // rotation by one bin: (sin, cos) of Q rstep
+    double sdQ, cdQ;
+    sincos_cw(Q * rstep, sdQ, cdQ);
     const int kx = (kext[blockIdx.z * (WR * RB)] + kMfK - 1) & ~(kMfK - 1);
     const int kb = blockIdx.y * k_per_split;
     const int ke = min(min(Kc_pad, kx), kb + k_per_split);
     const int nsteps = ke > kb ? (ke - kb) / kMfK : 0;
     i32x4 acc[RB][C];
-    double acc64[RB][4];
 #pragma unroll
-    for (int rb = 0; rb < RB; rb++) {
+    for (int rb = 0; rb < RB; rb++)
 #pragma unroll
         for (int c = 0; c < C; c++) acc[rb][c] = i32x4{0, 0, 0, 0};
-#pragma unroll
-        for (int v = 0; v < 4; v++) acc64[rb][v] = 0.0;
-    }
-    auto fold = [&]() {
+    // The 32-bit sums (digit c weighs 2^(8c)) are folded into the split's FP64 sums, which live
+    // in Spart rather than in registers: each (row, q) has one owner lane, the first fold
+    // stores, later ones add, and the last one scales by 1 / c.
+    const int qq = q0 + col;
+    bool first = true;
+    auto fold = [&](bool last) {
+        constexpr double inv_c = 1.0 / kMfC;
 #pragma unroll
         for (int rb = 0; rb < RB; rb++) {
 #pragma unroll
-            for (int c = 0; c < C; c++) {
+            for (int v = 0; v < 4; v++) {
+                const int row = rowt + (wr * RB + rb) * 16 + kq * 4 + v;
+                const bool own = qq < nq && row < nrows;
+                double *p = Spart + ((int64_t)blockIdx.y * nrows + row) * nq + qq;
+                double a = (own && !first) ? *p : 0.0;
 #pragma unroll
-                for (int v = 0; v < 4; v++) acc64[rb][v] += __builtin_ldexp((double)acc[rb][c][v], 8 * c);
-                acc[rb][c] = i32x4{0, 0, 0, 0};
+                for (int c = 0; c < C; c++) a += __builtin_ldexp((double)acc[rb][c][v], 8 * c);
+                if (own) *p = last ? a * inv_c : a;
             }
+#pragma unroll
+            for (int c = 0; c < C; c++) acc[rb][c] = i32x4{0, 0, 0, 0};
         }
+        first = false;
     };
     // count-digit fragment f of the step at k0: digit f / (WR RB), row block f % (WR RB).  Wave w
     // moves fragments w, w + NW, ... (clamped to the last one: a duplicate writes the same
@@ -523,22 +536,19 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
         srt[slot][lane] = prr;
         srt[slot][kMfK + lane] = pir;
     };
-    // The step in ring slot `slot` may use rotations when its 64 bins are consecutive (the table
-    // is strictly increasing in whole bins, so a span of 63 bins means that) and the wave's
-    // smallest Q r at its first bin is >= 0.5: fifteen rotations add below 4e-15 to sin there,
-    // and so below 8e-15 to sinc, under 1% of the 1/c grid of the sinc digits.  One answer per
-    // wave.
-    auto rotatable = [&](int slot) -> bool {
-        if constexpr (!ROTATE) {
-            return false;
-        } else {
-            const double span = srt[slot][kMfK - 1] - srt[slot][0];
-            const int ok = (span > 62.5 * rstep) && (span < 63.5 * rstep) && (qmin * srt[slot][0] >= 0.5);
-            return __builtin_amdgcn_readfirstlane(ok) != 0;
-        }
-    };
-    // sinc digits of the step whose r table is in srt[slot]: B fragments (WR == 1) or sd[buf]
+    // Sinc digits of the step whose r table is in srt[slot]: B fragments (WR == 1) or sd[buf].
+    // A lane's V values are at bins gk..gk+V-1.  Every lane first takes one sine-cosine pair at
+    // its first bin and rotates it by Q rstep for the next ones.  The result stands where those
+    // bins are consecutive (the table is strictly increasing in whole bins, so a span of V-1
+    // bins means that) and Q r >= 0.5 at the first: at most fifteen rotations add below 4e-15
+    // to sin there, and so below 8e-15 to sinc, under 1% of the 1/c grid of the sinc digits.
+    // Other lanes redo their values with one sine each (fix_gen), a branch that no wave takes
+    // where the histograms are dense.
     i32x4 Bn[kMfJ];
+    auto rot_ok = [&](int slot) -> bool {
+        const double span = srt[slot][gk + V - 1] - srt[slot][gk];
+        return span > (V - 1.5) * rstep && span < (V - 0.5) * rstep && Q * srt[slot][gk] >= 0.5;
+    };
     auto gen = [&](auto rot_tag, int slot, int buf) {
         constexpr bool ROT = decltype(rot_tag)::value;
         double sv = 0.0, cv = 0.0;
@@ -591,11 +601,13 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
         }
     }
     __syncthreads();
+    auto gen_step = [&](int slot, int buf) { gen(std::true_type{}, slot, buf); };
+    auto fix_gen = [&](int slot, int buf) {
+        if (!rot_ok(slot)) gen(std::false_type{}, slot, buf);
+    };
     if (nsteps > 0) {
-        if (rotatable(0))
-            gen(std::true_type{}, 0, 0);
-        else
-            gen(std::false_type{}, 0, 0);
+        gen_step(0, 0);
+        fix_gen(0, 0);
         if (nsteps > 1) load_a(kb + kMfK);
         if (nsteps > 2) load_r(kb + 2 * kMfK);
     }
@@ -603,7 +615,7 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     // one step: the matrix instructions of step s interleaved with the generation of step s+1;
     // branch-free inside (past the last step the generator and the ring take stale or clamped,
     // in-bounds data that no later step reads)
-    auto step = [&](auto rot_tag, int s, int slot, int s1, int s2) {
+    auto step = [&](int s, int slot, int s1, int s2) {
         i32x4 B[kMfJ];
         if constexpr (WR == 1) {
 #pragma unroll
@@ -622,7 +634,7 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
                     acc[rb][i + j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, B[j], acc[rb][i + j], 0, 0, 0);
             }
         }
-        gen(rot_tag, s1, (s + 1) & 1);
+        gen_step(s1, (s + 1) & 1);
         store_a(s1);
         store_r(s2);
         load_a(min(kb + (s + 2) * kMfK, ke - kMfK));
@@ -633,34 +645,22 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
             __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
         }
+        // outside the interleaved block: lanes whose values cannot rotate
+        fix_gen(s1, (s + 1) & 1);
     };
     int slot = 0, nfold = 0;
     for (int s = 0; s < nsteps; s++) {
         const int s1 = slot == 2 ? 0 : slot + 1;
         const int s2 = s1 == 2 ? 0 : s1 + 1;
-        if (rotatable(s1))
-            step(std::true_type{}, s, slot, s1, s2);
-        else
-            step(std::false_type{}, s, slot, s1, s2);
+        step(s, slot, s1, s2);
         if (++nfold == kMfFold) {
-            fold();
+            fold(false);
             nfold = 0;
         }
         __syncthreads();
         slot = s1;
     }
-    fold();
-    const int qq = q0 + col;
-    if (qq < nq) {
-        constexpr double inv_c = 1.0 / kMfC;
-#pragma unroll
-        for (int rb = 0; rb < RB; rb++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int row = rowt + (wr * RB + rb) * 16 + kq * 4 + v;
-                if (row < nrows) Spart[((int64_t)blockIdx.y * nrows + row) * nq + qq] = acc64[rb][v] * inv_c;
-            }
-    }
+    fold(true);
 }
 
 // launch shapes of the matrix-core Debye sum: (RB, WQ, WR) by row count
@@ -671,11 +671,15 @@ struct MfShape {
     int threads() const { return WQ * WR * 64; }
 };
 
+// 32 rows per workgroup (2 row blocks per wave) or 128 (4 row blocks, two waves per q block):
+// the larger tile shares each sinc digit among more rows, at about 6.6 against 7.6 us per row
+// (measured, 4096 q), but pads the rows to a multiple of 128.  Five count digits overfill the
+// larger tile's LDS.
 static MfShape mf_shape(int n_rows, int I)
 {
     if (n_rows <= 16) return {1, 4, 1};
-    if (n_rows <= 32) return {2, 4, 1};
-    if (n_rows <= 64 || I >= 5) return {4, 4, 1};  // (4, 4, 2) with 5 digits overfills LDS
+    const int64_t pad32 = (n_rows + 31) / 32 * 32, pad128 = (n_rows + 127) / 128 * 128;
+    if (I >= 5 || pad32 * 76 <= pad128 * 66) return {2, 4, 1};
     return {4, 4, 2};
 }
 
@@ -711,8 +715,6 @@ static void mf_launch(const MfShape &sh, int I, dim3 grid, hipStream_t st, const
         mf_launch_i<1, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
     else if (sh.RB == 2)
         mf_launch_i<2, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
-    else if (sh.WR == 1)
-        mf_launch_i<4, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
     else
         mf_launch_i<4, 4, 2>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
 }
@@ -853,8 +855,9 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
         const int rows_pad = (n_rows + sh.rows() - 1) / sh.rows() * sh.rows();
         const int qgroups = (nq + sh.qs() - 1) / sh.qs();
         const int rgroups = rows_pad / sh.rows();
-        const int target = 2048 * 4 / (sh.WQ * sh.WR);  // about 8 waves per SIMD over the grid
-        int nsplit = std::max(1, std::min((target + qgroups * rgroups - 1) / (qgroups * rgroups), ksteps));
+        // about 8 waves per SIMD over the grid, at least 8 steps per split
+        const int target = 2048 * 4 / (sh.WQ * sh.WR);
+        int nsplit = std::max(1, std::min((target + qgroups * rgroups - 1) / (qgroups * rgroups), ksteps / 8));
         const int kper = ((ksteps + nsplit - 1) / nsplit) * kMfK;
         nsplit = (Kc_pad + kper - 1) / kper;
         signed char *cdig = (signed char *)scratch(ctx, kSlotDigits, (size_t)I * rows_pad * Kc_pad);

@@ -290,7 +290,6 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
         m.plan_list_fast = ph.list_fast;
         m.plan_cu = ph.cu;
         m.plan_rstep = ph.rstep;
-        DBX_HIP(hipMemsetAsync(r->d_oor, 0, 8, ctx->stream));
     }
     // the images must outlive their copies
     DBX_HIP(hipStreamSynchronize(ctx->stream));

@@ -117,17 +117,21 @@ def test_weight_rule_first_n(mode_ctx):
         assert np.allclose(I, Iref, rtol=1e-13 if mode == cd.DBX_PASS2_FP64 else 1e-9, atol=0)
 
 
-@pytest.mark.parametrize("nq", [16, 100, 1000])
-def test_mfma_error_within_quantisation_bound(ctx, nq):
+@pytest.mark.parametrize("P,cmax,nq", [(5, 2 ** 30, 16), (5, 2 ** 30, 100), (5, 2 ** 30, 1000),
+                                        (24, 2 ** 22, 100), (40, 2 ** 36, 100), (130, 2 ** 30, 100),
+                                        (256, 2 ** 30, 100)])
+def test_mfma_error_within_quantisation_bound(ctx, P, cmax, nq):
     """Exact accumulation: with the sinc table rounded to multiples of 2^-40, the MFMA Debye
     sum can differ from the exact FP64 sum by at most sum_k 2 N_k * 2^-41 (plus FP64 noise).
-    A lossy FP32 accumulation of large counts would exceed that bound by orders of magnitude."""
-    rng = np.random.default_rng(nq)
-    P, nb = 5, 20000
+    A lossy FP32 accumulation of large counts would exceed that bound by orders of magnitude.
+    Rows and count sizes cover every workgroup tile (16, 32 and 128 rows) and 3 to 5 count
+    digits (counts below 2^22, 2^30, 2^36)."""
+    rng = np.random.default_rng(nq + P)
+    nb = 20000
     H = np.zeros((P, nb), np.uint64)
     for p in range(P):
         idx = rng.choice(np.arange(1, nb), 3000, replace=False)
-        H[p, idx] = rng.integers(1, 2 ** 30, 3000).astype(np.uint64)  # 4 count digits
+        H[p, idx] = rng.integers(1, cmax, 3000).astype(np.uint64)
     K = np.linspace(0.3, 20.0, nq)
     ff = np.ones((P, nq))
     ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
