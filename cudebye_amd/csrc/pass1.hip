@@ -40,6 +40,30 @@ __device__ __forceinline__ void hbm_add(unsigned long long *p, unsigned long lon
     __hip_atomic_fetch_add((gu64 *)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// A read-only table entry at a wave-uniform index (a job, a tile, an A-list or shell-edge
+// entry), loaded through the constant address space so that it becomes a scalar load: whole
+// 16-byte words for structs (no constructor takes a constant-space reference), bit-copied.
+typedef int v4i __attribute__((ext_vector_type(4)));
+template <typename T>
+__device__ __forceinline__ T ld_uniform(const T *p)
+{
+    T t;
+    if constexpr (sizeof(T) % 16 == 0) {
+        typedef const __attribute__((address_space(4))) v4i CV4;
+        const CV4 *q = (const CV4 *)p;
+#pragma unroll
+        for (size_t i = 0; i < sizeof(T) / 16; i++) {
+            const v4i w = q[i];
+            __builtin_memcpy(reinterpret_cast<char *>(&t) + 16 * i, &w, 16);
+        }
+    } else {
+        static_assert(sizeof(T) == 4, "a 4-byte scalar or a multiple of 16 bytes");
+        typedef const __attribute__((address_space(4))) T CT;
+        t = *(const CT *)p;
+    }
+    return t;
+}
+
 // The LDS window: h[0] guard (bins below the window), h[1 + i] bin k_lo + i for
 // 0 <= i < kP1Bins, h[kP1Bins + 1] sentinel (bins above it).  v = the pair's distance in bin
 // units (pair_scaled).
@@ -240,7 +264,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const bool complement = entry.w < 0;
     if (complement) entry.w = -entry.w;
     const P1Ref M = refs[entry.x];
-    P1Job job = M.jobs[entry.y];
+    P1Job job = ld_uniform(&M.jobs[entry.y]);
     if (entry.w > 1) {
         // piece z of w of the job's B-tile range
         const int b0 = job.b0, nb = job.b1 - job.b0;
@@ -258,8 +282,8 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int ai = lane;  // the lane's atom of the A tile
-    const float e_lo = edge2[job.e2_off + job.shell];
-    const float e_hi = edge2[job.e2_off + job.shell + 1];
+    const float e_lo = ld_uniform(&edge2[job.e2_off + job.shell]);
+    const float e_hi = ld_uniform(&edge2[job.e2_off + job.shell + 1]);
     const float r_lo = e_lo > 0.f ? sqrtf(e_lo) : 0.f;
     const float r_hi = sqrtf(e_hi);  // +inf for the last shell
     const int k_lo = job.k_lo;
@@ -293,7 +317,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             const int u1 = min(units, u0 + kP1Cand / 64);
             for (int u = u0 + wave; u < u1; u += kP1Threads / 64) {
                 const int ial = u / nblk;
-                const Tile A = tiles[alist[job.a_off + ial]];
+                const Tile A = ld_uniform(&tiles[ld_uniform(&alist[job.a_off + ial])]);
                 const int j = job.b0 + (u - ial * nblk) * 64 + lane;
                 bool c = false;
                 if (j < job.b1 && (!job.same || j >= A.parent)) {
@@ -320,12 +344,13 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                 if (lane == 0) ci = atomicAdd(&s_next, 1);
                 ci = __builtin_amdgcn_readfirstlane(ci);
                 if (ci >= nc) break;
-                const uint32_t e = cand[ci];
-                const int ia = alist[job.a_off + (int)(e >> 24)];
+                // one candidate per wave: its tiles' fields come through scalar loads
+                const uint32_t e = __builtin_amdgcn_readfirstlane(cand[ci]);
+                const int ia = ld_uniform(&alist[job.a_off + (int)(e >> 24)]);
                 const int j2 = job.b0 + (int)(e & 0xFFFFFFu);
                 if (ia != cur_ia) {
                     cur_ia = ia;
-                    ta = tiles[ia];
+                    ta = ld_uniform(&tiles[ia]);
                     a_ok = ai < ta.count;
                     idx_a = ta.start + ai;
                     pa = pos[a_ok ? idx_a : ta.start];
@@ -343,7 +368,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                         w_hi2 = (job.win & kP1WinLast) ? INFINITY : hi_m * hi_m;
                     }
                 }
-                const Tile tb = tiles[j2];
+                const Tile tb = ld_uniform(&tiles[j2]);
                 const bool b_ok = lane < tb.count;
                 const float4 pb = pos[tb.start + (b_ok ? lane : 0)];
                 const float ex = pb.x - ta.cx, ey = pb.y - ta.cy, ez = pb.z - ta.cz;
