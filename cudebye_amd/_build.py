@@ -23,6 +23,9 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=o
          # accumulators into AGPRs in the pass-2 contraction
          "-mllvm", "-amdgpu-mfma-vgpr-form=1",
          "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
+# pass 1 aggregates its uniform-address atomics itself (one lane per wave); the compiler's
+# wave-level atomic rewriting would only add scan code around them, inside the pair loops too
+FILE_FLAGS = {"pass1.hip": ["-mllvm", "-amdgpu-atomic-optimizer-strategy=None"]}
 
 
 def _run(cmd):
@@ -41,7 +44,7 @@ def build(verbose=False):
     for src in SOURCES:
         obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
         lang = ["-x", "hip"] if src.endswith(".hip") else []
-        cmds.append([HIPCC] + FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + FLAGS + FILE_FLAGS.get(src, []) + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     with ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         outs = list(ex.map(_run, cmds))
