@@ -64,6 +64,16 @@ __device__ __forceinline__ T ld_uniform(const T *p)
     return t;
 }
 
+// A position at a per-lane index through the global address space: a global (not flat) load
+// with a 32-bit offset from a scalar base, which waits on the vector-memory counter only
+typedef float v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ld_pos(const float4 *base, int i)
+{
+    typedef const __attribute__((address_space(1))) v4f GV4;
+    const v4f w = ((const GV4 *)base)[i];
+    return make_float4(w.x, w.y, w.z, w.w);
+}
+
 // The LDS window: h[0] guard (bins below the window), h[1 + i] bin k_lo + i for
 // 0 <= i < kP1Bins, h[kP1Bins + 1] sentinel (bins above it).  v = the pair's distance in bin
 // units (pair_scaled).
@@ -106,29 +116,45 @@ struct CountWindow {
 
 // Windows of 16-bit counters (jobs with kP1Win16): 2 kP1Bins bins, two per LDS word.  Bin
 // k_lo + i is field (i + 2) & 1 (low, high) of word (i + 2) >> 1, so word 0's high field is the
-// guard and word kP1Bins + 1's low field the sentinel.  A field counts up to 0x7FFF: the lane
+// guard and word kP1Bins + 1's low field the sentinel.  The planner starts these windows at an
+// even bin, so bin k is field k & 1 of word (k >> 1) + 1 - k_lo / 2, at LDS byte address
+// (2 k + base) & ~3 with base = &h[1 - k_lo / 2] (a multiple of 4): two instructions and no
+// subtraction of k_lo per pair; the field's shift is (k << 4) mod 32, which the shift and
+// bit-field instructions take from k << 4 directly.  A field counts up to 0x7FFF: the lane
 // whose increment sets its top bit (old field 0x7FFF, which exactly one lane sees per 32768
 // counts) takes 0x8000 back off the field and adds 32768 to the bin in HBM, so a field never
 // carries into its neighbour (that would need 32768 more increments of the same bin, by the
 // workgroup's other waves, before the lane's subtraction lands).  The lane looks at the old
 // value one pair later, so that waiting for it does not stall the next pair; finish() looks at
 // the last one.  Guard and sentinel overflows are recorded in *err instead.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+__device__ __forceinline__ uint32_t lds_addr(uint32_t *p) { return (uint32_t)(uintptr_t)(lds_u32 *)p; }
+__device__ __forceinline__ uint32_t lds_add(uint32_t addr, uint32_t v)
+{
+    return __hip_atomic_fetch_add((lds_u32 *)(uintptr_t)addr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 struct Count16 {
-    uint32_t *h;
-    int k_lo;
+    int k_lo;       // even
+    uint32_t base;  // LDS byte address of h[1 - k_lo / 2]
     unsigned long long *hrow;
     int64_t n_bins;
     unsigned long long *oor;
     int *err;
     uint32_t p_old = 0u;  // the previous pair's old word (0: nothing to look at)
-    int p_i = 0, p_k = 0;
+    int p_k = 0;
+    uint32_t p_sh = 0u;   // p_k << 4
+    __device__ __forceinline__ void init(uint32_t *h, int klo)
+    {
+        k_lo = klo;
+        base = lds_addr(h) + 4u * (uint32_t)(1 - (klo >> 1));
+    }
     __device__ __forceinline__ void look()
     {
-        const int j = p_i + 2;
-        const int sh = (j & 1) << 4;
-        if (((p_old >> sh) & 0xFFFFu) == 0x7FFFu) {
-            atomicSub(&h[j >> 1], 0x8000u << sh);
-            if ((unsigned)p_i >= (unsigned)(2 * kP1Bins))
+        if (__builtin_amdgcn_ubfe(p_old, p_sh, 16) == 0x7FFFu) {
+            const uint32_t a = ((uint32_t)(2 * p_k) + base) & ~3u;
+            lds_add(a, 0u - (0x8000u << (p_sh & 16u)));
+            if ((unsigned)(p_k - k_lo) >= (unsigned)(2 * kP1Bins))
                 atomicAdd(err, 1);
             else if ((int64_t)p_k < n_bins)
                 hbm_add(&hrow[p_k], 32768ull);
@@ -136,15 +162,16 @@ struct Count16 {
                 hbm_add(oor, 32768ull);
         }
     }
-    // adds one count to window bin i (bin k), or to `trash` when !in
-    __device__ __forceinline__ void add(int i, int k, bool in, uint32_t *trash)
+    // adds one count to bin k, or to the lane's discard word `trash` (LDS byte address) when !in
+    __device__ __forceinline__ void add(int k, bool in, uint32_t trash)
     {
-        const int j = i + 2;
-        const uint32_t old = atomicAdd(in ? &h[j >> 1] : trash, 1u << ((j & 1) << 4));
+        const uint32_t sh = (uint32_t)k << 4;
+        const uint32_t a = ((uint32_t)(2 * k) + base) & ~3u;
+        const uint32_t old = lds_add(in ? a : trash, 1u << (sh & 31u));
         look();
         p_old = in ? old : 0u;
-        p_i = i;
         p_k = k;
+        p_sh = sh;
     }
     __device__ __forceinline__ void finish()
     {
@@ -161,21 +188,20 @@ struct CountFast16 : Count16 {
     __device__ __forceinline__ void operator()(float v)
     {
         const int k = round_rpi(__builtin_amdgcn_fmed3f(v, vlo, vhi));
-        add(k - k_lo, k, true, nullptr);
+        add(k, true, 0u);
     }
 };
 
 // CountWindow16: one of several consecutive 16-bit windows of a job, as CountWindow
 struct CountWindow16 : Count16 {
-    uint32_t *trash;
+    uint32_t trash;  // LDS byte address of the lane's discard word
     int kmin, kmax;
     __device__ __forceinline__ void operator()(float v)
     {
         const int k = round_rpi(v);
         kmin = min(kmin, k);
         kmax = max(kmax, k);
-        const int i = k - k_lo;
-        add(i, k, (unsigned)i < (unsigned)(2 * kP1Bins), trash);
+        add(k, (unsigned)(k - k_lo) < (unsigned)(2 * kP1Bins), trash);
     }
 };
 
@@ -353,7 +379,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                     ta = ld_uniform(&tiles[ia]);
                     a_ok = ai < ta.count;
                     idx_a = ta.start + ai;
-                    pa = pos[a_ok ? idx_a : ta.start];
+                    pa = ld_pos(pos, a_ok ? idx_a : ta.start);
                     // a B atom at distance d from the A tile's centre pairs with its atoms at
                     // distances in [d - rad, d + rad]: one whose interval misses this job's
                     // window (below it when a window lies below, above it when one lies above)
@@ -370,7 +396,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                 }
                 const Tile tb = ld_uniform(&tiles[j2]);
                 const bool b_ok = lane < tb.count;
-                const float4 pb = pos[tb.start + (b_ok ? lane : 0)];
+                const float4 pb = ld_pos(pos, tb.start + (b_ok ? lane : 0));
                 const float ex = pb.x - ta.cx, ey = pb.y - ta.cy, ez = pb.z - ta.cz;
                 const float d2c = (ex * ex + ey * ey) + ez * ez;
                 const bool in_shell = b_ok && d2c >= e_lo && d2c < e_hi && d2c >= w_lo2 && d2c < w_hi2;
@@ -460,7 +486,8 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                 bad = (h[0] | h[kP1Bins + 1]) != 0u;
             } else {
                 CountFast16 cnt;
-                cnt.h = h, cnt.k_lo = k_lo, cnt.hrow = hrow, cnt.n_bins = n_bins, cnt.oor = M.oor, cnt.err = &s_err;
+                cnt.init(h, k_lo);
+                cnt.hrow = hrow, cnt.n_bins = n_bins, cnt.oor = M.oor, cnt.err = &s_err;
                 cnt.vlo = (float)(k_lo - 1), cnt.vhi = (float)(k_lo + 2 * kP1Bins);
                 run(cnt);
                 cnt.finish();
@@ -473,8 +500,9 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             bad = kminmax(cnt.kmin, cnt.kmax);
         } else {
             CountWindow16 cnt;
-            cnt.h = h, cnt.k_lo = k_lo, cnt.hrow = hrow, cnt.n_bins = n_bins, cnt.oor = M.oor, cnt.err = &s_err;
-            cnt.trash = &trash[lane], cnt.kmin = INT_MAX, cnt.kmax = INT_MIN;
+            cnt.init(h, k_lo);
+            cnt.hrow = hrow, cnt.n_bins = n_bins, cnt.oor = M.oor, cnt.err = &s_err;
+            cnt.trash = lds_addr(&trash[lane]), cnt.kmin = INT_MAX, cnt.kmax = INT_MIN;
             run(cnt);
             cnt.finish();
             bad = kminmax(cnt.kmin, cnt.kmax) || s_err != 0;
@@ -665,6 +693,9 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                         else
                             klo = std::floor(((sh + 0.5) * h) / rstep) - kP1Bins / 2;
                         if (klo < 0) klo = 0;
+                        // 16-bit windows start at an even bin (the counters address bin k's field
+                        // from k alone); the margins above leave room for one bin less
+                        if (wide && !degraded) klo -= std::fmod(klo, 2.0);
                         // neighbouring shells share straddling tiles: spread the sampled cost
                         double cst = shell_cost[sh];
                         if (sh > 0) cst += 0.25 * shell_cost[sh - 1];
@@ -672,7 +703,7 @@ void plan_pass1(const ModelDev &m, double rstep, std::vector<P1Job> &sorted,
                         const double jc = na / (c1 - c0) * (cst + 1.0);
                         const int wn = degraded ? 1 : nw;
                         for (int w = 0; w < wn; w++) {
-                            J.k_lo = (int32_t)std::min<double>(klo + (double)w * wbins, (double)INT32_MAX - wbins);
+                            J.k_lo = (int32_t)std::min<double>(klo + (double)w * wbins, (double)INT32_MAX - 1 - wbins);
                             J.win = (w == 0 ? kP1WinFirst : 0) | (w == wn - 1 ? kP1WinLast : 0) |
                                     (wide && !degraded ? kP1Win16 : 0);
                             // the single-window fast variants' bounds must be exact in FP32
@@ -768,6 +799,9 @@ PlanHost make_plan_host(const ModelDev &m, double rstep, int n_cu)
     ph.rstep = rstep;
     ph.cu = n_cu;
     plan_pass1(m, rstep, ph.jobs, ph.edge2, ph.alist, 0, &ph.cost);
+    for (const P1Job &J : ph.jobs)
+        DBX_REQUIRE(!(J.win & kP1Win16) || (J.k_lo & 1) == 0, DBX_ERR_STATE,
+                    "a 16-bit window must start at an even bin");
     ph.n_fast = 0;
     while (ph.n_fast < (int)ph.jobs.size() && !ph.jobs[ph.n_fast].degraded) ph.n_fast++;
     // the launch list of the model alone

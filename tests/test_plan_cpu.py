@@ -62,6 +62,7 @@ def check_plan(m):
         assert (win & ~7) == 0
         assert not degraded or win == 3
         width = 2 * W if win & 4 else W  # 16-bit counters: twice the bins
+        assert not (win & 4) or klo % 2 == 0  # 16-bit windows address bin k's field from k alone
         lo, hi = e2[eoff + shell], e2[eoff + shell + 1]
         for t in alist[a_off:a_off + a_cnt]:
             assert grp[t] == ga and cnt[t] <= 64
