@@ -64,10 +64,11 @@ __device__ __forceinline__ T ld_uniform(const T *p)
     return t;
 }
 
-// A position at a per-lane index through the global address space: a global (not flat) load
-// with a 32-bit offset from a scalar base, which waits on the vector-memory counter only
+// Position i (per lane) of a run starting at a wave-uniform base, through the global address
+// space: a global (not flat) load with a 32-bit offset from a scalar base, which waits on the
+// vector-memory counter only
 typedef float v4f __attribute__((ext_vector_type(4)));
-__device__ __forceinline__ float4 ld_pos(const float4 *base, int i)
+__device__ __forceinline__ float4 ld_pos(const float4 *base, uint32_t i)
 {
     typedef const __attribute__((address_space(1))) v4f GV4;
     const v4f w = ((const GV4 *)base)[i];
@@ -325,7 +326,9 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     }
 
     int cur_ia = -1;
-    float w_lo2 = -1.f, w_hi2 = INFINITY;  // squared window reach from the A tile's centre
+    // squared distances from the A tile's centre of the B atoms that can pair in this job: the
+    // shell, narrowed to the window's reach in multi-window jobs
+    float s_lo2 = e_lo, s_hi2 = e_hi;
     Tile ta = {};
     bool a_ok = false;
     int idx_a = 0;
@@ -379,7 +382,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                     ta = ld_uniform(&tiles[ia]);
                     a_ok = ai < ta.count;
                     idx_a = ta.start + ai;
-                    pa = ld_pos(pos, a_ok ? idx_a : ta.start);
+                    pa = ld_pos(pos + ta.start, (uint32_t)min(ai, ta.count - 1));
                     // a B atom at distance d from the A tile's centre pairs with its atoms at
                     // distances in [d - rad, d + rad]: one whose interval misses this job's
                     // window (below it when a window lies below, above it when one lies above)
@@ -390,16 +393,18 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                         const float hi = (float)(k_lo + width + 10) * rs + ta.rad;
                         const float lo_m = lo - 1e-5f * fabsf(lo);
                         const float hi_m = hi + 1e-5f * hi;
-                        w_lo2 = (job.win & kP1WinFirst) || lo_m <= 0.f ? -1.f : lo_m * lo_m;
-                        w_hi2 = (job.win & kP1WinLast) ? INFINITY : hi_m * hi_m;
+                        const float w_lo2 = (job.win & kP1WinFirst) || lo_m <= 0.f ? -1.f : lo_m * lo_m;
+                        const float w_hi2 = (job.win & kP1WinLast) ? INFINITY : hi_m * hi_m;
+                        s_lo2 = fmaxf(e_lo, w_lo2);
+                        s_hi2 = fminf(e_hi, w_hi2);
                     }
                 }
                 const Tile tb = ld_uniform(&tiles[j2]);
                 const bool b_ok = lane < tb.count;
-                const float4 pb = ld_pos(pos, tb.start + (b_ok ? lane : 0));
+                const float4 pb = ld_pos(pos + tb.start, (uint32_t)min(lane, tb.count - 1));
                 const float ex = pb.x - ta.cx, ey = pb.y - ta.cy, ez = pb.z - ta.cz;
                 const float d2c = (ex * ex + ey * ey) + ez * ez;
-                const bool in_shell = b_ok && d2c >= e_lo && d2c < e_hi && d2c >= w_lo2 && d2c < w_hi2;
+                const bool in_shell = b_ok && d2c >= s_lo2 && d2c < s_hi2;
                 uint64_t bm = __ballot(in_shell);
                 const bool diag = job.same && (j2 == ta.parent);  // B tile holds the A tile's atoms
                 if (bm) {
