@@ -377,8 +377,14 @@ def _valu_issue(pairs, k_ms):
     if not per_pair:
         return None
     rate = per_pair * pairs / (k_ms * 1e-3)
-    return {"lane_ops_per_pair": per_pair, "achieved": round(rate / 1e12, 2), "peak": round(VALU_ISSUE_PEAK / 1e12, 2),
-            "unit": "T lane-ops/s", "frac": round(rate / VALU_ISSUE_PEAK, 4)}
+    out = {"lane_ops_per_pair": per_pair, "achieved": round(rate / 1e12, 2), "peak": round(VALU_ISSUE_PEAK / 1e12, 2),
+           "unit": "T lane-ops/s", "frac": round(rate / VALU_ISSUE_PEAK, 4)}
+    clock = _profile_traffic("p1_histogram_kernel_clock_ghz")
+    if clock:
+        # the engine clock the profiled run held under this kernel (below the 2.4 GHz peak clock)
+        out["profiled_clock_ghz"] = clock
+        out["frac_at_profiled_clock"] = round(rate / (VALU_ISSUE_PEAK * clock / 2.4), 4)
+    return out
 
 
 def _profile_traffic(kernel):
