@@ -167,6 +167,20 @@ def test_repeatable_and_permutation_invariant(ctx):
     assert np.array_equal(H1, H2) and np.array_equal(H1, H3)
 
 
+def test_bins_beyond_2_24_bit_exact(ctx):
+    """Fine bins over a wide cloud: bin indices above 2^24 (1e-5 A bins, distances to ~200 A),
+    where a single window's clamp bounds are no longer exact in FP32 and the planner sends those
+    jobs to the checked variant."""
+    m = synth.small_random(300, 2, 61, spread=60.0, rstep=1e-5)
+    assert m.n_bins > 2 ** 24
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    H = ctx.histogram(mod, m.rstep, m.n_bins)
+    ref, ov = _oracle_hist(m)
+    assert ov == 0
+    assert np.array_equal(H, ref)
+    assert H[:, 2 ** 24:].sum() > 0
+
+
 @pytest.mark.parametrize("maker", ["c3_small", "c4_model5", "c5_req7", "sparse"])
 def test_wide_tiles_and_window_groups_bit_exact(ctx, maker):
     """Workloads whose sparse subgroups and fine bins need 16-bit windows and several windows per

@@ -100,7 +100,7 @@ def check_plan(m):
     return jobs
 
 
-@pytest.mark.parametrize("maker", ["c1", "rand3", "batch3", "batch0", "sparse", "fine"])
+@pytest.mark.parametrize("maker", ["c1", "rand3", "batch3", "batch0", "sparse", "fine", "huge"])
 def test_plan_covers_every_pair_once_and_windows_hold(maker):
     m = {"c1": synth.fcc_sphere,
          "rand3": lambda: synth.small_random(3000, 3, 41, spread=20.0),
@@ -109,9 +109,14 @@ def test_plan_covers_every_pair_once_and_windows_hold(maker):
          # sparse species: a few atoms spread over a large volume (16-bit and multiple windows)
          "sparse": lambda: synth.small_random(1500, 2, 43, spread=45.0),
          # fine bins: narrow windows (several per shell)
-         "fine": lambda: synth.small_random(1200, 1, 44, spread=12.0, rstep=1e-4)}[maker]()
+         "fine": lambda: synth.small_random(1200, 1, 44, spread=12.0, rstep=1e-4),
+         # bins above 2^24: windows there, any single one degraded (its FP32 clamp is inexact)
+         "huge": lambda: synth.small_random(300, 2, 61, spread=60.0, rstep=1e-5)}[maker]()
     jobs = check_plan(m)
     assert len(jobs) > 0
+    if maker == "huge":
+        single_high = (jobs[:, 10] & 3 == 3) & (jobs[:, 7].astype(np.int64) + np.where(jobs[:, 10] & 4, 2 * W, W) >= 2 ** 24)
+        assert (jobs[:, 7] >= 2 ** 24).any() and np.all(jobs[single_high, 9] == 1)
 
 
 def test_plan_uses_several_windows_where_needed():
