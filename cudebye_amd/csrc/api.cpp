@@ -414,7 +414,15 @@ int dbx_close(dbx_ctx *ctx)
     return DBX_OK;
 }
 
-const char *dbx_last_error(const dbx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *dbx_last_error(const dbx_ctx *ctx)
+{
+    if (!ctx) return "null context";
+    // a copy per calling thread: another thread's call may replace the context's message
+    thread_local std::string msg;
+    std::lock_guard<std::recursive_mutex> lk(const_cast<dbx_ctx *>(ctx)->mu);
+    msg = ctx->err;
+    return msg.c_str();
+}
 
 void *dbx_stream(dbx_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 

@@ -499,7 +499,15 @@ int dbx_server_close(dbx_server *s)
     return DBX_OK;
 }
 
-const char *dbx_server_last_error(dbx_server *s) { return s ? s->err.c_str() : "null server"; }
+const char *dbx_server_last_error(dbx_server *s)
+{
+    if (!s) return "null server";
+    // a copy per calling thread: the lanes and other callers may replace the message
+    thread_local std::string msg;
+    std::lock_guard<std::mutex> lk(s->mu);
+    msg = s->err;
+    return msg.c_str();
+}
 
 int dbx_submit(dbx_server *s, const dbx_request *q, uint64_t *ticket)
 {

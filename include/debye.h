@@ -8,8 +8,8 @@
  *
  * Conventions
  *   - Every function returns an int status: DBX_OK (0) or a negative DBX_ERR_* code;
- *     dbx_last_error() gives the message of the last failure on a context.  Nothing here
- *     calls exit() (the reference's gpuErrchk does, cuDebye-v15.cu:58-66).
+ *     dbx_last_error() gives the message of the context's last call (empty after a success).
+ *     Nothing here calls exit() (the reference's gpuErrchk does, cuDebye-v15.cu:58-66).
  *   - Host buffers belong to the caller.  Device memory belongs to the context and is
  *     released by dbx_model_destroy / dbx_close.
  *   - A context serialises its synchronous calls; distinct contexts are independent.
@@ -58,6 +58,8 @@ typedef struct {
  * hbm_budget: bytes of device memory the context may hold (0 = 90% of free memory). */
 int dbx_open(int device, size_t hbm_budget, dbx_ctx **out);
 int dbx_close(dbx_ctx *ctx);
+/* The message of the context's last call, empty after a success.  The string is the calling
+ * thread's copy, valid until that thread's next dbx_last_error call. */
 const char *dbx_last_error(const dbx_ctx *ctx);
 /* HIP stream the context launches on (hipStream_t as void*). */
 void *dbx_stream(dbx_ctx *ctx);
@@ -202,6 +204,8 @@ typedef struct dbx_server_stats {
 } dbx_server_stats;
 int dbx_server_open(int device, const dbx_server_config *cfg, dbx_server **out);
 int dbx_server_close(dbx_server *s);
+/* The message of the server's last failed call or request (the calling thread's copy, valid
+ * until that thread's next dbx_server_last_error call). */
 const char *dbx_server_last_error(dbx_server *s);
 int dbx_submit(dbx_server *s, const dbx_request *req, uint64_t *ticket);
 /* state: DBX_REQ_QUEUED (waiting for preparation or memory), _RUNNING, _DONE, _FAILED */
