@@ -104,11 +104,12 @@ constexpr int kFlagRows = 32;  // rows per thread of the nonzero-flag pass
 
 // flags[k] = 1 where some row holds a nonzero count at bin k > k_self_max; grid.y over chunks
 // of kFlagRows rows (a bin written by several chunks gets the same value from each).  Also the
-// largest count of any row (atomicMax).
-__global__ void p2_flags_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
-                                int n_rows, int64_t nb_max, int k_self_max, unsigned char *__restrict__ flags,
-                                unsigned long long *__restrict__ max_count)
+// largest count of any row: one atomicMax per block.
+__global__ __launch_bounds__(kScanBlock) void p2_flags_kernel(
+    const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows, int n_rows, int64_t nb_max,
+    int k_self_max, unsigned char *__restrict__ flags, unsigned long long *__restrict__ max_count)
 {
+    __shared__ unsigned long long smx[kScanBlock / 64];
     const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
     const int r0 = blockIdx.y * kFlagRows;
     const int r1 = min(n_rows, r0 + kFlagRows);
@@ -120,12 +121,17 @@ __global__ void p2_flags_kernel(const unsigned long long *__restrict__ hist, con
             nz |= v != 0ull;
             mx = v > mx ? v : mx;
         }
+    if (nz) flags[k] = 1;
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long o = __shfl_down(mx, off);
         mx = o > mx ? o : mx;
     }
-    if ((threadIdx.x & 63) == 0 && mx) atomicMax(max_count, mx);
-    if (nz) flags[k] = 1;
+    if ((threadIdx.x & 63) == 0) smx[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kScanBlock / 64; w++) mx = smx[w] > mx ? smx[w] : mx;
+        if (mx) atomicMax(max_count, mx);
+    }
 }
 
 // profiling: total nonzero bins over all rows (the bins the reference's per-pair sum contracts)
@@ -324,16 +330,23 @@ __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
     }
 }
 
-// S[r0 + i] = sum over splits of Spart[s][i], i over nr rows x nq, fixed order; plus
-// shift * rsum[row] when rsum is given (the q-independent part of the matrix-core sum)
-__global__ void p2_reduce_splits_kernel(const double *__restrict__ Spart, int nsplit, int r0, int nr, int nq,
-                                        const unsigned long long *__restrict__ rsum, double shift,
-                                        double *__restrict__ S)
+// Row r = r0 + i / nq, q index iq = i % nq (i over nr rows x nq):
+//   S[r][iq] = (shift rsum[r] + sum over splits of Spart[s][i], fixed order)
+//            + (self[r] + sum_k low[r][k] sinc(q r_k))
+// rsum (the q-independent part of the matrix-core sum) may be null; nsplit may be 0.  The
+// second term is the reference's weight rule for the entries at r <= 0.0005 among the first
+// first_n (weight 1, no sinc) and their neighbours below first_n (weight 2 with sinc).
+__global__ void p2_finish_rows_kernel(const double *__restrict__ Spart, int nsplit, int r0, int nr, int nq,
+                                      const unsigned long long *__restrict__ rsum, double shift,
+                                      const double *__restrict__ self, const double *__restrict__ low, int nlow,
+                                      double rstep, const double *__restrict__ q, double *__restrict__ S)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)nr * nq;
     if (i >= n) return;
-    double acc = rsum ? shift * (double)rsum[r0 + i / nq] : 0.0;
+    const int r = r0 + (int)(i / nq);
+    const int iq = (int)(i - (int64_t)(r - r0) * nq);
+    double acc = rsum ? shift * (double)rsum[r] : 0.0;
     // eight loads in flight, the sum kept in split order
     int s = 0;
     for (; s + 8 <= nsplit; s += 8) {
@@ -344,7 +357,16 @@ __global__ void p2_reduce_splits_kernel(const double *__restrict__ Spart, int ns
         for (int u = 0; u < 8; u++) acc += v[u];
     }
     for (; s < nsplit; s++) acc += Spart[(int64_t)s * n + i];
-    S[(int64_t)r0 * nq + i] = acc;
+    double lo = self[r];
+    for (int k = 0; k < nlow; k++) {
+        const double w = low[(int64_t)r * nlow + k];
+        if (w != 0.0) {
+            const double rk = (double)k * rstep;
+            const double x = q[iq] * rk;
+            lo += rk == 0.0 ? w : w * sin(x) / x;
+        }
+    }
+    S[(int64_t)r0 * nq + i] = acc + lo;
 }
 
 // ---------------------------------------------------------------------------------
@@ -719,25 +741,6 @@ static void mf_launch(const MfShape &sh, int I, dim3 grid, hipStream_t st, const
         mf_launch_i<4, 4, 2>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
 }
 
-// S[r][q] += self[r] + sum_k low[r][k] sinc(q r_k); grid.y over rows
-__global__ void p2_add_low_kernel(const double *__restrict__ self, const double *__restrict__ low, int nlow,
-                                  double rstep, const double *__restrict__ q, int nq, double *__restrict__ S)
-{
-    const int iq = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = blockIdx.y;
-    if (iq >= nq) return;
-    double s = self[r];
-    for (int k = 0; k < nlow; k++) {
-        const double w = low[(int64_t)r * nlow + k];
-        if (w != 0.0) {
-            const double rk = (double)k * rstep;
-            const double x = q[iq] * rk;
-            s += rk == 0.0 ? w : w * sin(x) / x;
-        }
-    }
-    S[(int64_t)r * nq + iq] += s;
-}
-
 // I[dest][q] = sum over the model's rows of ff S, FP64, rows in order;
 // mrows[m] = (first row, row count, dest, -)
 __global__ void p2_combine_kernel(const double *__restrict__ ff, const double *__restrict__ S,
@@ -776,17 +779,18 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     const int ksm = self_bin_max(rstep);
     const int nlow = ksm + 1;
     const int64_t nblk = (nb_max + kScanBlock - 1) / kScanBlock;
-    // scratch: rows, block counts, total + max count, kz, self, low, kext
+    // scratch: rows, block counts, kz, self, low, kext, then the parts that start at zero
+    // (total + max count, row sums, flags), cleared by one launch
     const int n_rb16 = (n_rows + 15) / 16;
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off = (off + b + 255) & ~(size_t)255; return o; };
     const size_t o_rows = take(sizeof(RowDesc) * n_rows);
     const size_t o_bc = take(sizeof(int) * (nblk + 1));
-    const size_t o_tot = take(sizeof(unsigned long long) * 2);
     const size_t o_kz = take(sizeof(int) * nb_max);
     const size_t o_self = take(sizeof(double) * n_rows);
     const size_t o_low = take(sizeof(double) * (size_t)n_rows * nlow);
     const size_t o_kext = take(sizeof(int) * (n_rb16 + 8));
+    const size_t o_tot = take(sizeof(unsigned long long) * 2);
     const size_t o_rsum = take(sizeof(unsigned long long) * n_rows);
     const size_t o_flags = take((size_t)(nblk * kScanBlock));
     char *base = (char *)scratch(ctx, kSlotPass2, off);
@@ -805,10 +809,9 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     std::copy(rows.begin(), rows.end(), hrows);
     DBX_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     DBX_HIP(hipMemcpyAsync(d_rows, hrows, sizeof(RowDesc) * n_rows, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(p2_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, (unsigned int *)tot, (int64_t)4);
-    const int64_t fwords = nblk * kScanBlock / 4;
-    hipLaunchKernelGGL(p2_zero_kernel, dim3((unsigned)((fwords + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (unsigned int *)flags, fwords);
+    const int64_t zwords = (int64_t)(off - o_tot) / 4;
+    hipLaunchKernelGGL(p2_zero_kernel, dim3((unsigned)((zwords + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (unsigned int *)tot, zwords);
     hipLaunchKernelGGL(p2_flags_kernel, dim3((unsigned)nblk, (unsigned)((n_rows + kFlagRows - 1) / kFlagRows)),
                        dim3(kScanBlock), 0, ctx->stream, h, d_rows, n_rows, nb_max, ksm, flags, tot + 1);
     hipLaunchKernelGGL(p2_count_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, flags, nb_max, bc);
@@ -837,8 +840,9 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
                                r0, np, kz, Kc, A);
             hipLaunchKernelGGL(p2_debye_fp64_kernel, dim3(qblocks, nsplit), dim3(kDsThreads), 0, ctx->stream,
                                kz, Kc, kper, A, np, rstep, d_q, nq, Spart);
-            hipLaunchKernelGGL(p2_reduce_splits_kernel, dim3((unsigned)(((int64_t)np * nq + 255) / 256)),
-                               dim3(256), 0, ctx->stream, Spart, nsplit, r0, np, nq, nullptr, 0.0, d_S);
+            hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)np * nq + 255) / 256)),
+                               dim3(256), 0, ctx->stream, Spart, nsplit, r0, np, nq, nullptr, 0.0, self, low,
+                               nlow, rstep, d_q, d_S);
         }
     } else if (Kc > 0) {
         // balanced count digits: 2 N < 127 256^(I - 1)
@@ -862,8 +866,6 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
         nsplit = (Kc_pad + kper - 1) / kper;
         signed char *cdig = (signed char *)scratch(ctx, kSlotDigits, (size_t)I * rows_pad * Kc_pad);
         double *Spart = (double *)scratch(ctx, kSlotPass2b, sizeof(double) * (size_t)nsplit * n_rows * nq);
-        hipLaunchKernelGGL(p2_zero_kernel, dim3((unsigned)((2 * n_rows + 255) / 256)), dim3(256), 0, ctx->stream,
-                           (unsigned int *)rsum, (int64_t)2 * n_rows);
         hipLaunchKernelGGL(p2_count_digits_kernel, dim3((unsigned)((Kc_pad + 4 * kCdThreads - 1) / (4 * kCdThreads)),
                                                         (unsigned)rows_pad),
                            dim3(kCdThreads), 0, ctx->stream, h, d_rows, n_rows, kz, Kc, Kc_pad, rows_pad, I, cdig, rsum);
@@ -898,13 +900,15 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
             ctx->mf_digits = I;
             ctx->mf_timed = true;
         }
-        hipLaunchKernelGGL(p2_reduce_splits_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
-                           dim3(256), 0, ctx->stream, Spart, nsplit, 0, n_rows, nq, rsum, kMfShift, d_S);
+        hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
+                           dim3(256), 0, ctx->stream, Spart, nsplit, 0, n_rows, nq, rsum, kMfShift, self, low,
+                           nlow, rstep, d_q, d_S);
     } else {
-        DBX_HIP(hipMemsetAsync(d_S, 0, sizeof(double) * (size_t)n_rows * nq, ctx->stream));
+        // no bin beyond the self-pair entries: S is their part alone
+        hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
+                           dim3(256), 0, ctx->stream, nullptr, 0, 0, n_rows, nq, nullptr, 0.0, self, low, nlow,
+                           rstep, d_q, d_S);
     }
-    hipLaunchKernelGGL(p2_add_low_kernel, dim3((nq + 255) / 256, n_rows), dim3(256), 0, ctx->stream, self, low,
-                       nlow, rstep, d_q, nq, d_S);
     DBX_HIP(hipGetLastError());
     DBX_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     ctx->p2_timed = true;
