@@ -252,10 +252,13 @@ struct CountChecked {
 // CHECKED = true: out-of-window pairs go straight to the uint64 histogram in HBM (jobs whose
 // window is narrower than their geometry, and the queued ones).
 // zeroes the histogram block and the out-of-range count of every model of a launch:
-// blockIdx.y = model (refs + y0), blocks along x stride over its [P][n_bins] words
-__global__ __launch_bounds__(256) void p1_zero_kernel(const P1Ref *__restrict__ refs, int y0)
+// blockIdx.y = model (refs + y0, or ref0 alone when refs is null), blocks along x stride over
+// its [P][n_bins] words; the first launch also clears the four launch counters
+__global__ __launch_bounds__(256) void p1_zero_kernel(const P1Ref *__restrict__ refs, P1Ref ref0, int y0,
+                                                      unsigned long long *__restrict__ counters)
 {
-    const P1Ref R = refs[y0 + (int)blockIdx.y];
+    const P1Ref R = refs ? refs[y0 + (int)blockIdx.y] : ref0;
+    if (y0 == 0 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 4) counters[threadIdx.x] = 0ull;
     const int64_t n = (int64_t)R.n_rows * R.n_bins;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
         R.hist[i] = 0ull;
@@ -264,7 +267,7 @@ __global__ __launch_bounds__(256) void p1_zero_kernel(const P1Ref *__restrict__ 
 
 template <bool CHECKED>
 __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
-    const P1Ref *__restrict__ refs, const int4 *__restrict__ list, int n_fast, int n_checked,
+    const P1Ref *__restrict__ refs, P1Ref ref0, const int4 *__restrict__ list, int n_fast, int n_checked,
     unsigned long long *__restrict__ counters, int4 *__restrict__ retry)
 {
     __shared__ uint32_t h[kP1Bins + 2];
@@ -290,7 +293,8 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     // a negative piece count marks the complement of a fast job (see CountChecked)
     const bool complement = entry.w < 0;
     if (complement) entry.w = -entry.w;
-    const P1Ref M = refs[entry.x];
+    // the launch's models (refs), or its one model as an argument (refs null)
+    const P1Ref M = refs ? refs[entry.x] : ref0;
     P1Job job = ld_uniform(&M.jobs[entry.y]);
     if (entry.w > 1) {
         // piece z of w of the job's B-tile range
@@ -893,6 +897,7 @@ static P1Ref make_ref(const ModelDev &m, int64_t n_bins, uint64_t *d_hist, doubl
 P1Stats pass1_batch(dbx_ctx *ctx, const std::vector<P1Item> &items, double rstep, bool sync_stats)
 {
     DBX_REQUIRE(rstep > 0 && std::isfinite(rstep), DBX_ERR_ARG, "rstep must be positive");
+    DBX_REQUIRE(!items.empty(), DBX_ERR_ARG, "no model for pass 1");
     P1Stats st;
     std::vector<P1Ref> refs(items.size());
     int n_fast = 0, n_checked = 0;
@@ -936,26 +941,32 @@ P1Stats pass1_batch(dbx_ctx *ctx, const std::vector<P1Item> &items, double rstep
         DBX_HIP(hipMemcpyAsync((void *)d_list, hl, sizeof(int4) * st.n_jobs, hipMemcpyHostToDevice,
                                ctx->stream));
     }
-    P1Ref *hr = (P1Ref *)pinned(ctx, kPinRefs, sizeof(P1Ref) * refs.size());
-    std::memcpy(hr, refs.data(), sizeof(P1Ref) * refs.size());
-    DBX_HIP(hipMemcpyAsync(d_refs, hr, sizeof(P1Ref) * refs.size(), hipMemcpyHostToDevice, ctx->stream));
-    // every histogram and out-of-range count zeroed by one launch (per 65535 models)
+    // one model travels as a kernel argument, several as a table in device memory
+    const P1Ref ref0 = refs[0];
+    if (items.size() == 1) {
+        d_refs = nullptr;
+    } else {
+        P1Ref *hr = (P1Ref *)pinned(ctx, kPinRefs, sizeof(P1Ref) * refs.size());
+        std::memcpy(hr, refs.data(), sizeof(P1Ref) * refs.size());
+        DBX_HIP(hipMemcpyAsync(d_refs, hr, sizeof(P1Ref) * refs.size(), hipMemcpyHostToDevice, ctx->stream));
+    }
+    // every histogram and out-of-range count, and the launch counters, zeroed by one launch
+    // (per 65535 models)
+    unsigned long long *cnt = (unsigned long long *)ctx->counters;
     const unsigned zx = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (max_words + 4095) / 4096));
     for (int y0 = 0; y0 < (int)refs.size(); y0 += 65535) {
         const unsigned ny = (unsigned)std::min<int>(65535, (int)refs.size() - y0);
-        hipLaunchKernelGGL(p1_zero_kernel, dim3(zx, ny), dim3(256), 0, ctx->stream, d_refs, y0);
+        hipLaunchKernelGGL(p1_zero_kernel, dim3(zx, ny), dim3(256), 0, ctx->stream, d_refs, ref0, y0, cnt);
     }
-    DBX_HIP(hipMemsetAsync(ctx->counters, 0, 4 * sizeof(uint64_t), ctx->stream));
     ctx->p1_timed = false;
     if (st.n_jobs > 0) {
-        unsigned long long *cnt = (unsigned long long *)ctx->counters;
         DBX_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
         if (n_fast > 0)
             hipLaunchKernelGGL(p1_histogram_kernel<false>, dim3((unsigned)n_fast), dim3(kP1Threads), 0,
-                               ctx->stream, d_refs, d_list, n_fast, n_checked, cnt, d_retry);
+                               ctx->stream, d_refs, ref0, d_list, n_fast, n_checked, cnt, d_retry);
         // checked variant: the degraded jobs plus whatever the fast launch queued
         hipLaunchKernelGGL(p1_histogram_kernel<true>, dim3((unsigned)(n_checked + n_fast)),
-                           dim3(kP1Threads), 0, ctx->stream, d_refs, d_list, n_fast, n_checked, cnt, d_retry);
+                           dim3(kP1Threads), 0, ctx->stream, d_refs, ref0, d_list, n_fast, n_checked, cnt, d_retry);
         DBX_HIP(hipGetLastError());
         DBX_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
         ctx->p1_timed = true;

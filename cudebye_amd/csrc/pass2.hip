@@ -223,11 +223,10 @@ __global__ void p2_scatter_kernel(const unsigned char *__restrict__ flags, int64
     }
 }
 
-// kext[blk] = number of compacted bins below the n_bins of the block's first row
-__global__ void p2_kext_kernel(const int *__restrict__ kz, int Kc, const RowDesc *__restrict__ rows, int n_rows,
-                               int rows_per_block, int n_blocks, int *__restrict__ kext)
+// kext[b] = number of compacted bins below the n_bins of row block b's first row
+__device__ __forceinline__ void kext_entry(int b, const int *__restrict__ kz, int Kc, const RowDesc *__restrict__ rows,
+                                           int n_rows, int rows_per_block, int n_blocks, int *__restrict__ kext)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= n_blocks) return;
     const int r = b * rows_per_block;
     if (r >= n_rows) {
@@ -388,18 +387,16 @@ constexpr int kMfFold = 256;   // steps between folds of the 32-bit classes
 
 // balanced count digits as bytes: cdig[i][row][kk] = digit i of 2 N_row,kz[kk] (0 past Kc, past
 // the row's bins or past n_rows), and rsum[row] += the row's sum of 2 N over the compacted bins.
-// Grid (Kc_pad / 1024 rounded up, rows_pad), 256 threads, four consecutive bins per thread
-// (Kc_pad is a multiple of 64, so a thread's four bins are all inside or all outside it); one
-// atomic per block.
+// Block (bx, row) of (Kc_pad / 1024 rounded up) x rows_pad, 256 threads, four consecutive bins
+// per thread (Kc_pad is a multiple of 64, so a thread's four bins are all inside or all outside
+// it); one atomic per block.
 constexpr int kCdThreads = 256;
-__global__ __launch_bounds__(kCdThreads) void p2_count_digits_kernel(
-    const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows, int n_rows,
-    const int *__restrict__ kz, int Kc, int Kc_pad, int rows_pad, int I, signed char *__restrict__ cdig,
-    unsigned long long *__restrict__ rsum)
+__device__ __forceinline__ void count_digits_block(
+    int bx, int row, unsigned long long *wsum, const unsigned long long *__restrict__ hist,
+    const RowDesc *__restrict__ rows, int n_rows, const int *__restrict__ kz, int Kc, int Kc_pad, int rows_pad,
+    int I, signed char *__restrict__ cdig, unsigned long long *__restrict__ rsum)
 {
-    __shared__ unsigned long long wsum[kCdThreads / 64];
-    const int row = blockIdx.y;
-    const int kk0 = (blockIdx.x * kCdThreads + threadIdx.x) * 4;
+    const int kk0 = (bx * kCdThreads + threadIdx.x) * 4;
     long long v[4] = {0, 0, 0, 0};
     if (row < n_rows && kk0 < Kc) {
         const RowDesc rd = rows[row];
@@ -441,15 +438,36 @@ __global__ __launch_bounds__(kCdThreads) void p2_count_digits_kernel(
 // bin distances and their reciprocals: rr[kk] = kz[kk] * rstep, ir[kk] = 1 / rr[kk].  Entries
 // past Kc (zero counts) continue the bins after the last one, so that the table stays strictly
 // increasing in whole bins and 64 entries spanning 63 bins are always 64 consecutive bins.
-__global__ void p2_rtab_kernel(const int *__restrict__ kz, int Kc, int Kc_pad, double rstep,
-                               double *__restrict__ rr, double *__restrict__ ir)
+__device__ __forceinline__ void rtab_entry(int i, const int *__restrict__ kz, int Kc, int Kc_pad, double rstep,
+                                           double *__restrict__ rr, double *__restrict__ ir)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= Kc_pad) return;
     const int k = i < Kc ? kz[i] : (Kc > 0 ? kz[Kc - 1] : 0) + (i - Kc + 1);
     const double r = (double)k * rstep;
     rr[i] = r;
     ir[i] = 1.0 / r;
+}
+
+// The inputs of the contraction in one launch: blocks [0, cd_x rows_pad) write the count digits
+// (block b: bins of chunk b % cd_x of row b / cd_x), the next (Kc_pad + 255) / 256 the r table,
+// the rest the row-block bin extents.
+__global__ __launch_bounds__(kCdThreads) void p2_mf_inputs_kernel(
+    const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows, int n_rows,
+    const int *__restrict__ kz, int Kc, int Kc_pad, int rows_pad, int I, int cd_x, signed char *__restrict__ cdig,
+    unsigned long long *__restrict__ rsum, double rstep, double *__restrict__ rr, double *__restrict__ ir,
+    int n_kblocks, int *__restrict__ kext)
+{
+    __shared__ unsigned long long wsum[kCdThreads / 64];
+    const int n_cd = cd_x * rows_pad;
+    const int n_rt = (Kc_pad + kCdThreads - 1) / kCdThreads;
+    const int b = blockIdx.x;
+    if (b < n_cd) {
+        count_digits_block(b % cd_x, b / cd_x, wsum, hist, rows, n_rows, kz, Kc, Kc_pad, rows_pad, I, cdig, rsum);
+    } else if (b < n_cd + n_rt) {
+        rtab_entry((b - n_cd) * kCdThreads + threadIdx.x, kz, Kc, Kc_pad, rstep, rr, ir);
+    } else {
+        kext_entry((b - n_cd - n_rt) * kCdThreads + threadIdx.x, kz, Kc, rows, n_rows, 16, n_kblocks, kext);
+    }
 }
 
 // One workgroup: WQ x WR waves.  Wave (wq, wr) owns 16 q values (block wq of the workgroup's
@@ -742,14 +760,14 @@ static void mf_launch(const MfShape &sh, int I, dim3 grid, hipStream_t st, const
 }
 
 // I[dest][q] = sum over the model's rows of ff S, FP64, rows in order;
-// mrows[m] = (first row, row count, dest, -)
+// mrows[m] = (first row, row count, dest, -), or m0 for a single model (mrows null)
 __global__ void p2_combine_kernel(const double *__restrict__ ff, const double *__restrict__ S,
-                                  const int4 *__restrict__ mrows, int nq, double *__restrict__ I)
+                                  const int4 *__restrict__ mrows, int4 m0, int nq, double *__restrict__ I)
 {
     const int iq = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = blockIdx.y;
     if (iq >= nq) return;
-    const int4 rr = mrows[m];
+    const int4 rr = mrows ? mrows[m] : m0;
     double acc = 0.0;
     for (int r = rr.x; r < rr.x + rr.y; r++) acc += S[(int64_t)r * nq + iq] * ff[(int64_t)r * nq + iq];
     I[(int64_t)rr.z * nq + iq] = acc;
@@ -853,8 +871,6 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
         const int Kc_pad = (Kc + kMfK - 1) & ~(kMfK - 1);
         const int ksteps = Kc_pad / kMfK;
         double *rtab = (double *)scratch(ctx, kSlotRtab, sizeof(double) * 2 * (size_t)Kc_pad);
-        hipLaunchKernelGGL(p2_rtab_kernel, dim3((Kc_pad + 255) / 256), dim3(256), 0, ctx->stream, kz, Kc,
-                           Kc_pad, rstep, rtab, rtab + Kc_pad);
         const MfShape sh = mf_shape(n_rows, I);
         const int rows_pad = (n_rows + sh.rows() - 1) / sh.rows() * sh.rows();
         const int qgroups = (nq + sh.qs() - 1) / sh.qs();
@@ -866,12 +882,14 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
         nsplit = (Kc_pad + kper - 1) / kper;
         signed char *cdig = (signed char *)scratch(ctx, kSlotDigits, (size_t)I * rows_pad * Kc_pad);
         double *Spart = (double *)scratch(ctx, kSlotPass2b, sizeof(double) * (size_t)nsplit * n_rows * nq);
-        hipLaunchKernelGGL(p2_count_digits_kernel, dim3((unsigned)((Kc_pad + 4 * kCdThreads - 1) / (4 * kCdThreads)),
-                                                        (unsigned)rows_pad),
-                           dim3(kCdThreads), 0, ctx->stream, h, d_rows, n_rows, kz, Kc, Kc_pad, rows_pad, I, cdig, rsum);
         const int nblocks = rows_pad / 16;
-        hipLaunchKernelGGL(p2_kext_kernel, dim3((nblocks + 255) / 256), dim3(256), 0, ctx->stream, kz, Kc, d_rows,
-                           n_rows, 16, nblocks, kext);
+        const int cd_x = (Kc_pad + 4 * kCdThreads - 1) / (4 * kCdThreads);
+        const int64_t n_in = (int64_t)cd_x * rows_pad + (Kc_pad + kCdThreads - 1) / kCdThreads +
+                             (nblocks + kCdThreads - 1) / kCdThreads;
+        DBX_REQUIRE(n_in < INT32_MAX, DBX_ERR_ARG, "pass-2 problem too large");
+        hipLaunchKernelGGL(p2_mf_inputs_kernel, dim3((unsigned)n_in), dim3(kCdThreads), 0, ctx->stream, h, d_rows,
+                           n_rows, kz, Kc, Kc_pad, rows_pad, I, cd_x, cdig, rsum, rstep, rtab, rtab + Kc_pad,
+                           nblocks, kext);
         if (ctx->profiling) DBX_HIP(hipEventRecord(ctx->ev_mf[0], ctx->stream));
         mf_launch(sh, I, dim3((unsigned)qgroups, (unsigned)nsplit, (unsigned)rgroups), ctx->stream, rtab,
                   rtab + Kc_pad, Kc_pad, kper, kext, cdig, rows_pad, d_q, nq, n_rows, rstep, Spart);
@@ -926,12 +944,19 @@ void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const s
                     int nq, double *d_I)
 {
     const int nm = (int)mrows.size();
+    if (nm == 1) {
+        // one model: its row range as an argument, no copy
+        hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, 1), dim3(256), 0, ctx->stream, d_ff, d_S,
+                           nullptr, mrows[0], nq, d_I);
+        DBX_HIP(hipGetLastError());
+        return;
+    }
     int4 *hm = (int4 *)pinned(ctx, kPinModels, sizeof(int4) * nm);
     std::copy(mrows.begin(), mrows.end(), hm);
     int4 *dm = (int4 *)scratch(ctx, kSlotRows, sizeof(int4) * nm);
     DBX_HIP(hipMemcpyAsync(dm, hm, sizeof(int4) * nm, hipMemcpyHostToDevice, ctx->stream));
     hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, ctx->stream, d_ff, d_S, dm,
-                       nq, d_I);
+                       make_int4(0, 0, 0, 0), nq, d_I);
     DBX_HIP(hipGetLastError());
 }
 
@@ -950,7 +975,8 @@ void launch_prefactor(hipStream_t st, const GroupPair *d_pairs, int n, const dou
 void launch_combine(hipStream_t st, const double *d_ff, const double *d_S, const int4 *d_mrows, int nm, int nq,
                     double *d_I)
 {
-    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, st, d_ff, d_S, d_mrows, nq, d_I);
+    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, st, d_ff, d_S, d_mrows,
+                       make_int4(0, 0, 0, 0), nq, d_I);
     DBX_HIP(hipGetLastError());
 }
 
