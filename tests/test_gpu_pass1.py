@@ -108,6 +108,26 @@ def test_empty_subgroup_and_duplicates(ctx):
     assert H[5, 0] == 3 and H[5].sum() == 3
 
 
+def test_single_atom_and_far_from_origin(ctx):
+    """One atom (its self pair only); a cluster about 7000 A from the origin, where FP32
+    coordinate differences carry ~5e-4 A of rounding and the bin of each pair depends on the
+    exact FP32 operation order."""
+    one = np.array([3.25], np.float32)
+    mod = ctx.model(one, one, one, np.array([1], np.int64))
+    H = ctx.histogram(mod, 1e-3, 10)
+    assert H.shape == (1, 10) and H[0, 0] == 1 and H.sum() == 1
+    m = synth.small_random(2000, 2, 77, spread=15.0)
+    x = (m.x.astype(np.float64) + 5000.0).astype(np.float32)
+    y = (m.y.astype(np.float64) - 3000.0).astype(np.float32)
+    z = (m.z.astype(np.float64) + 4000.0).astype(np.float32)
+    m.x, m.y, m.z = x, y, z
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    H = ctx.histogram(mod, m.rstep, m.n_bins)
+    ref, ov = _oracle_hist(m)
+    assert ov == 0
+    assert np.array_equal(H, ref)
+
+
 def test_rmax_too_small_reports_range_error(ctx):
     m = synth.small_random(500, 2, 9, spread=20.0)
     mod = ctx.model(m.x, m.y, m.z, m.group_n)
