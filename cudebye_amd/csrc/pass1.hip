@@ -65,8 +65,7 @@ __device__ __forceinline__ T ld_uniform(const T *p)
 }
 
 // Position i (per lane) of a run starting at a wave-uniform base, through the global address
-// space: a global (not flat) load with a 32-bit offset from a scalar base, which waits on the
-// vector-memory counter only
+// space: a global (not flat) load, which waits on the vector-memory counter only
 typedef float v4f __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ float4 ld_pos(const float4 *base, uint32_t i)
 {
