@@ -424,8 +424,19 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     if (!diag) {
                         if (a_ok) {
-                            // two B atoms per iteration: two independent chains in flight
+                            // four, then two B atoms per iteration: independent chains in
+                            // flight, the stage address advanced once per four pairs
                             int t = 0;
+                            for (; t + 3 < nb; t += 4) {
+                                float v[4];
+#pragma unroll
+                                for (int u = 0; u < 4; u++) {
+                                    const float4 b = stage[wave][t + u];
+                                    v[u] = pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp);
+                                }
+#pragma unroll
+                                for (int u = 0; u < 4; u++) cnt(v[u]);
+                            }
                             for (; t + 1 < nb; t += 2) {
                                 const float4 b0 = stage[wave][t];
                                 const float4 b1 = stage[wave][t + 1];
