@@ -341,6 +341,45 @@ def cpu_leg(out, m, I, H, pairs, np):
     }
 
 
+def cpu_batched(out, synth, np):
+    """The oracle on the batched configuration: every 16th of the 256 models, each end to end
+    (all partial histograms, then I at all 4096 Q), for models/s beside the GPU batch."""
+    from oracle import oracle as O
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    models = synth.batch(256)[::16]
+    t0 = time.perf_counter()
+    pairs = 0
+    for m in models:
+        st = np.concatenate([[0], np.cumsum(m.group_n)[:-1]])
+        rp = O.rp_of(m.rstep)
+        S = len(m.group_n)
+        H = np.zeros((S * (S + 1) // 2, m.n_bins), np.uint64)
+        p = 0
+        for r in range(S):
+            for c in range(r, S):
+                O.histogram(m.x, m.y, m.z, st[r], m.group_n[r], st[c], m.group_n[c], r == c, rp, m.n_bins,
+                            nthreads=nthreads, out=H[p])
+                p += 1
+
+        class Pm:
+            group_n = m.group_n
+            occ = np.array([q["occ"] for q in m.params])
+            beq = np.array([q["beq"] for q in m.params])
+            a = np.array([q["a"] for q in m.params])
+            b = np.array([q["b"] for q in m.params])
+            c = np.array([q["c"] for q in m.params])
+
+        O.intensity_from_histograms(H, Pm, m.q, m.rstep, first_n=m.first_n, nthreads=nthreads)
+        pairs += m.pair_count
+    dt = time.perf_counter() - t0
+    out["cpu_baseline"]["batched"] = {
+        "value": round(len(models) / dt, 2), "unit": "models/s", "cores": nthreads, "kind": "port",
+        "pair_distances_per_s": round(pairs / dt / 1e9, 3),
+        "sample": f"every 16th of the 256 batched models ({len(models)} models, {pairs:.3g} pair "
+                  f"distances), each end to end: histograms then I at 4096 Q ({dt:.1f} s)",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -365,6 +404,8 @@ def main():
         out["serving"] = stream_c5(cd, synth, np)
     if not args.no_cpu:
         cpu_leg(out, m, I, H, pairs, np)
+        if not args.no_streams:
+            cpu_batched(out, synth, np)
     print(json.dumps(out))
 
 
