@@ -5,7 +5,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "fastmath.h"
@@ -98,55 +100,91 @@ struct Pt {
     int64_t id;
 };
 
-// Recursive median bisection of pts[lo, hi) along the longest bounding-box axis into
-// leaves of at most `unit` atoms; the left part always receives a whole number of full
-// leaves, so only the last leaf of a range can be short.
+// One median split of pts[lo, hi) (more than `unit` atoms) along its longest bounding-box
+// axis, ties broken by atom index; returns the split point.  The left part receives a whole
+// number of full leaves, so only the last leaf of a range can be short.
+int64_t split_range(std::vector<Pt> &pts, int64_t lo, int64_t hi, int64_t unit)
+{
+    const int64_t n = hi - lo;
+    float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int64_t i = lo; i < hi; i++) {
+        const float v[3] = {pts[i].x, pts[i].y, pts[i].z};
+        for (int a = 0; a < 3; a++) {
+            mn[a] = std::min(mn[a], v[a]);
+            mx[a] = std::max(mx[a], v[a]);
+        }
+    }
+    int ax = 0;
+    for (int a = 1; a < 3; a++)
+        if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
+    const int64_t nt = (n + unit - 1) / unit;
+    const int64_t nl = (nt / 2) * unit;
+    // one comparator per axis (no per-comparison axis switch); ties broken by atom index
+    auto split = [&](auto key) {
+        std::nth_element(pts.begin() + lo, pts.begin() + lo + nl, pts.begin() + hi,
+                         [&](const Pt &a, const Pt &b) {
+                             const float ka = key(a), kb = key(b);
+                             return ka < kb || (ka == kb && a.id < b.id);
+                         });
+    };
+    if (ax == 0)
+        split([](const Pt &p) { return p.x; });
+    else if (ax == 1)
+        split([](const Pt &p) { return p.y; });
+    else
+        split([](const Pt &p) { return p.z; });
+    return lo + nl;
+}
+
+// Recursive median bisection of pts[lo, hi) into leaves of at most `unit` atoms, appended
+// to `out` depth first, left before right.
 void kd_split(std::vector<Pt> &pts, int64_t lo, int64_t hi, int64_t unit,
               std::vector<std::pair<int64_t, int64_t>> &out)
 {
-    struct Range { int64_t lo, hi; };
-    std::vector<Range> stack{{lo, hi}};
-    std::vector<Range> leaves;
+    std::vector<std::pair<int64_t, int64_t>> stack{{lo, hi}};
     while (!stack.empty()) {
-        Range r = stack.back();
+        const auto r = stack.back();
         stack.pop_back();
-        const int64_t n = r.hi - r.lo;
-        if (n <= unit) {
-            leaves.push_back(r);
+        if (r.second - r.first <= unit) {
+            out.push_back(r);
             continue;
         }
-        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
-        for (int64_t i = r.lo; i < r.hi; i++) {
-            const float v[3] = {pts[i].x, pts[i].y, pts[i].z};
-            for (int a = 0; a < 3; a++) {
-                mn[a] = std::min(mn[a], v[a]);
-                mx[a] = std::max(mx[a], v[a]);
-            }
-        }
-        int ax = 0;
-        for (int a = 1; a < 3; a++)
-            if (mx[a] - mn[a] > mx[ax] - mn[ax]) ax = a;
-        const int64_t nt = (n + unit - 1) / unit;
-        const int64_t nl = (nt / 2) * unit;
-        // one comparator per axis (no per-comparison axis switch); ties broken by atom index
-        auto split = [&](auto key) {
-            std::nth_element(pts.begin() + r.lo, pts.begin() + r.lo + nl, pts.begin() + r.hi,
-                             [&](const Pt &a, const Pt &b) {
-                                 const float ka = key(a), kb = key(b);
-                                 return ka < kb || (ka == kb && a.id < b.id);
-                             });
-        };
-        if (ax == 0)
-            split([](const Pt &p) { return p.x; });
-        else if (ax == 1)
-            split([](const Pt &p) { return p.y; });
-        else
-            split([](const Pt &p) { return p.z; });
-        // depth first, left before right
-        stack.push_back({r.lo + nl, r.hi});
-        stack.push_back({r.lo, r.lo + nl});
+        const int64_t m = split_range(pts, r.first, r.second, unit);
+        stack.push_back({m, r.second});
+        stack.push_back({r.first, m});
     }
-    for (auto &l : leaves) out.push_back({l.lo, l.hi});
+}
+
+// The same leaves for a large range with the work on several threads: the range is split
+// here, its right half goes to a new thread while this one takes the left, `depth` levels
+// deep (2^depth threads at most); the leaves are joined in the same depth-first order.
+void kd_split_parallel(std::vector<Pt> &pts, int64_t lo, int64_t hi, int64_t unit, int depth,
+                       std::vector<std::pair<int64_t, int64_t>> &out)
+{
+    constexpr int64_t kParallelAtoms = 32768;
+    if (depth == 0 || hi - lo < kParallelAtoms) {
+        kd_split(pts, lo, hi, unit, out);
+        return;
+    }
+    const int64_t m = split_range(pts, lo, hi, unit);
+    std::vector<std::pair<int64_t, int64_t>> right;
+    std::exception_ptr err;
+    std::thread t([&] {
+        try {
+            kd_split_parallel(pts, m, hi, unit, depth - 1, right);
+        } catch (...) {
+            err = std::current_exception();
+        }
+    });
+    try {
+        kd_split_parallel(pts, lo, m, unit, depth - 1, out);
+    } catch (...) {
+        t.join();
+        throw;
+    }
+    t.join();
+    if (err) std::rethrow_exception(err);
+    out.insert(out.end(), right.begin(), right.end());
 }
 
 }  // namespace
@@ -223,7 +261,7 @@ void tile_model_host(const float *x, const float *y, const float *z, const int64
             m.bbox[6 * g + 3 + a] = n ? mx[a] : 0.f;
         }
         std::vector<std::pair<int64_t, int64_t>> blocks;
-        if (n) kd_split(pts, 0, n, kTileAtoms, blocks);
+        if (n) kd_split_parallel(pts, 0, n, kTileAtoms, 3, blocks);
         // blocks in ascending bounding-box diagonal
         std::vector<std::pair<double, size_t>> key(blocks.size());
         for (size_t b = 0; b < blocks.size(); b++) {
