@@ -481,7 +481,7 @@ __global__ __launch_bounds__(kCdThreads) void p2_mf_inputs_kernel(
 //     block together (16 / WR values per thread), through LDS (double-buffered) when WR > 1,
 //     one step ahead of the matrix cores so that both run at once.  The step's r and 1/r come
 //     through the same LDS ring.  A thread whose values lie at consecutive bins takes one
-//     sine-cosine pair and rotates it by q rstep for the following bins.
+//     sine-cosine pair and steps along the following bins with a second-order recurrence.
 //   - The 32-bit sums are folded into FP64 partial sums in Spart every kMfFold steps.
 // The workgroup's bins end at kext of its first row block: rows are ordered by length,
 // longest first.
@@ -513,9 +513,11 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     const double Q = Q0 != 0.0 ? Q0 : 1e-30;
     const double cQ = kMfC / Q;
     const int gk = (gl >> 4) * 16 + gt;
-    // rotation by one bin: (sin, cos) of Q rstep
-    double sdQ, cdQ;
-    sincos_cw(Q * rstep, sdQ, cdQ);
+    // one bin further: sin(Q rstep), and lam = 2 - 2 cos(Q rstep) = 4 sin^2(Q rstep / 2), formed
+    // from the half-angle sine so that it keeps full relative precision for small Q rstep
+    const double sdQ = sin_cw(Q * rstep);
+    const double shQ = sin_cw(0.5 * (Q * rstep));
+    const double lamQ = 4.0 * (shQ * shQ);
     const int kx = (kext[blockIdx.z * (WR * RB)] + kMfK - 1) & ~(kMfK - 1);
     const int kb = blockIdx.y * k_per_split;
     const int ke = min(min(Kc_pad, kx), kb + k_per_split);
@@ -577,13 +579,17 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
         srt[slot][kMfK + lane] = pir;
     };
     // Sinc digits of the step whose r table is in srt[slot]: B fragments (WR == 1) or sd[buf].
-    // A lane's V values are at bins gk..gk+V-1.  Every lane first takes one sine-cosine pair at
-    // its first bin and rotates it by Q rstep for the next ones.  The result stands where those
-    // bins are consecutive (the table is strictly increasing in whole bins, so a span of V-1
-    // bins means that) and Q r >= 0.5 at the first: at most fifteen rotations add below 4e-15
-    // to sin there, and so below 8e-15 to sinc, under 1% of the 1/c grid of the sinc digits.
-    // Other lanes redo their values with one sine each (fix_gen), a branch that no wave takes
-    // where the histograms are dense.
+    // A lane's V values are at bins gk..gk+V-1.  Every lane first takes one sine-cosine pair
+    // (s0, c0) at its first bin and steps along the next ones with the second-order recurrence
+    // in Reinsch's form: d1 = c0 sin(h) - s0 lam / 2, s1 = s0 + d1, then d(k+1) = d(k) -
+    // lam s(k), s(k+1) = s(k) + d(k+1), with h = Q rstep and lam = 4 sin^2(h / 2) (two operations
+    // per bin).  Each step rounds s once (<= 1.1e-16) and the difference d is small (|d| <= h),
+    // so fifteen steps add below 2e-15 to the sine of the sine-cosine pair, and so below 4e-15
+    // to sinc where Q r >= 0.5, under 1% of the 1/c grid of the sinc digits.  That holds where
+    // the lane's bins are consecutive (the table is strictly increasing in whole bins, so a
+    // span of V-1 bins means that) and Q r >= 0.5 at the first; other lanes redo their values
+    // with one sine each (fix_gen), a branch that no wave takes where the histograms are
+    // dense.
     i32x4 Bn[kMfJ];
     auto rot_ok = [&](int slot) -> bool {
         const double span = srt[slot][gk + V - 1] - srt[slot][gk];
@@ -591,7 +597,7 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     };
     auto gen = [&](auto rot_tag, int slot, int buf) {
         constexpr bool ROT = decltype(rot_tag)::value;
-        double sv = 0.0, cv = 0.0;
+        double sv = 0.0, cv = 0.0, dv = 0.0;
         if constexpr (ROT) sincos_cw(Q * srt[slot][gk], sv, cv);
         int dw[NG][kMfJ];
 #pragma unroll
@@ -602,10 +608,12 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
                 const int t = g4 * 4 + u;
                 double s;
                 if constexpr (ROT) {
-                    if (t > 0) {
-                        const double sn = fma(sv, cdQ, cv * sdQ);
-                        cv = fma(cv, cdQ, -(sv * sdQ));
-                        sv = sn;
+                    if (t == 1) {
+                        dv = fma(cv, sdQ, -(sv * (0.5 * lamQ)));
+                        sv = sv + dv;
+                    } else if (t > 1) {
+                        dv = fma(-lamQ, sv, dv);
+                        sv = sv + dv;
                     }
                     s = sv;
                 } else {
@@ -685,7 +693,7 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
             __builtin_amdgcn_sched_group_barrier(0x002, 3, 0);
         }
-        // outside the interleaved block: lanes whose values cannot rotate
+        // outside the interleaved block: lanes whose values cannot use the recurrence
         fix_gen(s1, (s + 1) & 1);
     };
     int slot = 0, nfold = 0;
