@@ -128,3 +128,60 @@ def test_plan_uses_several_windows_where_needed():
         assert np.any(jobs[:, 10] != 3) and np.any(jobs[:, 10] & 4) and jobs[:, 9].sum() == 0
     _, _, jobs, _, _ = host_plan(synth.spinel_sphere())
     assert jobs[:, 9].sum() == 0 and np.mean(jobs[:, 10] == 3) > 0.97
+
+
+def _leaf_sets(x, y, z, ids, unit=64):
+    """The tiling rule restated: median bisection along the longest float32 bounding-box axis
+    (the first of equal extents), the left part the nt // 2 * unit atoms first in (coordinate,
+    atom index) order, depth first, left before right."""
+    xyz = np.stack([x, y, z], 1).astype(np.float32)
+    out = []
+    stack = [ids]
+    while stack:
+        r = stack.pop()
+        if len(r) <= unit:
+            out.append(r)
+            continue
+        ext = xyz[r].max(0) - xyz[r].min(0)
+        ax = 0
+        for a in (1, 2):
+            if ext[a] > ext[ax]:
+                ax = a
+        nl = (-(-len(r) // unit) // 2) * unit
+        o = r[np.lexsort((r, xyz[r, ax]))]
+        stack.append(o[nl:])
+        stack.append(o[:nl])
+    return out
+
+
+def test_tiles_are_the_median_bisection_leaves():
+    """Every 64-atom tile holds exactly the atoms of one leaf of the median bisection, for a
+    subgroup large enough to be split on several threads (a 46k-atom lattice with exact
+    coordinate ties) and a small one."""
+    rng = np.random.default_rng(12)
+    a = 4.08
+    g = np.arange(-14, 14)
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    cells = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    p = ((cells[:, None, :] + base[None]) * a).reshape(-1, 3)
+    p = p[(p ** 2).sum(1) <= (14 * a) ** 2]
+    small = rng.uniform(-10, 10, (700, 3))
+    xyz = np.concatenate([p, small]).astype(np.float32)
+    gn = np.array([len(p), len(small)], np.int64)
+
+    class M:
+        pass
+    m = M()
+    m.x, m.y, m.z = xyz[:, 0].copy(), xyz[:, 1].copy(), xyz[:, 2].copy()
+    m.group_n, m.rstep, m.n_atoms = gn, 1e-3, len(xyz)
+    assert gn[0] > 32768
+    tiles, pos, _, _, _ = host_plan(m)
+    got = sorted(tuple(sorted(map(tuple, pos[t[4]:t[4] + t[5], :3].tolist()))) for t in tiles)
+    want = []
+    start = 0
+    for n in gn:
+        ids = np.arange(start, start + n)
+        for leaf in _leaf_sets(m.x, m.y, m.z, ids):
+            want.append(tuple(sorted(map(tuple, xyz[leaf].tolist()))))
+        start += n
+    assert got == sorted(want)
