@@ -28,10 +28,12 @@ def _params(m):
     return [cd.GroupParams.make(p["occ"], p["beq"], p["a"], p["b"], p["c"]) for p in m.params]
 
 
-def test_rocksalt_sphere_20nm_full_size(ctx):
-    """452,719 atoms, 1.02e11 pair distances, 6 partials at 0.0005 A, 8192 Q."""
-    m = synth.rocksalt_sphere()
-    print(f"rocksalt sphere: {m.n_atoms} atoms, {m.n_bins} bins, {len(m.q)} Q", flush=True)
+@pytest.mark.parametrize("diameter", [200.0, 256.0])
+def test_rocksalt_sphere_full_size(ctx, diameter):
+    """The three-species rocksalt-like sphere, 6 partials at 0.0005 A, 8192 Q: 20 nm
+    (452,719 atoms, 1.02e11 pair distances) and 25.6 nm (948,557 atoms, 4.5e11)."""
+    m = synth.rocksalt_sphere(diameter=diameter)
+    print(f"rocksalt sphere D={diameter}: {m.n_atoms} atoms, {m.n_bins} bins, {len(m.q)} Q", flush=True)
     mod = ctx.model(m.x, m.y, m.z, m.group_n)
     H = ctx.histogram(mod, m.rstep, m.n_bins)
     assert [int(h.sum()) for h in H] == _totals(m.group_n)
