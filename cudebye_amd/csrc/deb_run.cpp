@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -291,7 +292,7 @@ extern "C" int dbx_read_debx(const char *path, double rstep, int64_t n_bins, uin
     return rc;
 }
 
-extern "C" int dbx_run_deb(const char *path, int flags)
+static int run_deb_body(const char *path, int flags)
 {
     if (!path) return DBX_ERR_ARG;
     const bool quiet = flags & DBX_RUN_QUIET;
@@ -313,8 +314,18 @@ extern "C" int dbx_run_deb(const char *path, int flags)
                 "hist=%d (%.3f s)\n",
          path, (long long)d.x.size(), d.S, nTT, K[0], K[nTT - 1], (long long)channels, d.hist, now_s() - t0);
 
-    dbx_ctx *ctx = nullptr;
-    rc = dbx_open(d.device, 0, &ctx);
+    // the context and the model are released on every way out, exceptions included
+    struct Owned {
+        dbx_ctx *ctx = nullptr;
+        dbx_model *m = nullptr;
+        ~Owned()
+        {
+            if (m) dbx_model_destroy(ctx, m);
+            if (ctx) dbx_close(ctx);
+        }
+    } own;
+    rc = dbx_open(d.device, 0, &own.ctx);
+    dbx_ctx *const ctx = own.ctx;
     if (rc != DBX_OK) {
         std::fprintf(stderr, "ERROR: cannot open device %d\n", d.device);
         return rc;
@@ -327,7 +338,6 @@ extern "C" int dbx_run_deb(const char *path, int flags)
     std::vector<double> I(nTT, 0.0);
     const int first_n = d.block > 0 ? d.block : 1;
     double t1 = now_s();
-    dbx_model *m = nullptr;
     if (from_debx) {
         // pass 2 only, from the histograms a previous run wrote (<path>x<r>x<c>)
         int p = 0;
@@ -348,36 +358,27 @@ extern "C" int dbx_run_deb(const char *path, int flags)
                                nullptr);
             if (rc != DBX_OK) std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
         }
-        if (rc != DBX_OK) {
-            dbx_close(ctx);
-            return rc;
-        }
+        if (rc != DBX_OK) return rc;
     } else {
-        rc = dbx_model_create(ctx, d.x.data(), d.y.data(), d.z.data(), d.n.data(), d.S, &m);
+        rc = dbx_model_create(ctx, d.x.data(), d.y.data(), d.z.data(), d.n.data(), d.S, &own.m);
         if (rc != DBX_OK) {
             std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
-            dbx_close(ctx);
             return rc;
         }
+        dbx_model *const m = own.m;
         if (d.hist != 1)
             rc = dbx_intensity(ctx, m, d.gp.data(), d.rstep, channels, first_n, K.data(), nTT, I.data(), H.data());
         else
             rc = dbx_histogram(ctx, m, d.rstep, channels, H.data(), 0, nullptr);
         if (rc != DBX_OK) {
             std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
-            if (rc != DBX_ERR_RANGE) {
-                dbx_model_destroy(ctx, m);
-                dbx_close(ctx);
-                return rc;
-            }
+            if (rc != DBX_ERR_RANGE) return rc;
         }
     }
     const double t_calc = now_s() - t1;
     const int wrc = write_outputs(path, d, tt, H.data(), channels, I, !from_debx, eol, quiet);
     if (rc == DBX_OK) rc = wrc;
     logf(quiet, "Time for Calculation of Distances and Evaluation of the Debye Double Sum: %.3f s\n", t_calc);
-    if (m) dbx_model_destroy(ctx, m);
-    dbx_close(ctx);
     return rc;
 }
 
@@ -386,7 +387,7 @@ extern "C" int dbx_run_deb(const char *path, int flags)
 // once, so small models share steps and large ones overlap.  Each file's outputs are those of
 // dbx_run_deb on it alone.  Returns DBX_OK, or the first failing file's status (the other
 // files still run).
-extern "C" int dbx_run_folder(const char *dir, int flags)
+static int run_folder_body(const char *dir, int flags)
 {
     if (!dir) return DBX_ERR_ARG;
     const bool quiet = flags & DBX_RUN_QUIET;
@@ -453,8 +454,16 @@ extern "C" int dbx_run_folder(const char *dir, int flags)
     }
     dbx_server_config cfg{};
     cfg.pass2_mode = (flags & DBX_RUN_MFMA) ? DBX_PASS2_MFMA : DBX_PASS2_FP64;
-    dbx_server *srv = nullptr;
-    int rc = dbx_server_open(device, &cfg, &srv);
+    // the server (and its threads) is closed on every way out, exceptions included
+    struct Owned {
+        dbx_server *srv = nullptr;
+        ~Owned()
+        {
+            if (srv) dbx_server_close(srv);
+        }
+    } own;
+    int rc = dbx_server_open(device, &cfg, &own.srv);
+    dbx_server *const srv = own.srv;
     if (rc != DBX_OK) {
         std::fprintf(stderr, "ERROR: cannot open device %d\n", device);
         return rc;
@@ -498,6 +507,34 @@ extern "C" int dbx_run_folder(const char *dir, int flags)
         }
         if (j.rc != DBX_OK && first_bad == DBX_OK) first_bad = j.rc;
     }
-    dbx_server_close(srv);
     return first_bad;
+}
+
+// The C boundary: no exception leaves it (a failed host allocation, say for the histograms of
+// a header whose rmax is huge, returns DBX_ERR_NOMEM instead of ending the caller's process)
+template <class F>
+static int no_throw(F &&f)
+{
+    try {
+        return f();
+    } catch (const dbx::Error &e) {
+        std::fprintf(stderr, "ERROR: %s\n", e.what());
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        std::fprintf(stderr, "ERROR: host allocation failed\n");
+        return DBX_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "ERROR: %s\n", e.what());
+        return DBX_ERR_ARG;
+    }
+}
+
+extern "C" int dbx_run_deb(const char *path, int flags)
+{
+    return no_throw([&] { return run_deb_body(path, flags); });
+}
+
+extern "C" int dbx_run_folder(const char *dir, int flags)
+{
+    return no_throw([&] { return run_folder_body(dir, flags); });
 }

@@ -10,7 +10,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <new>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -237,6 +239,22 @@ bool read_debi(const char *path, std::vector<double> &tt, std::vector<double> &I
     return !tt.empty();
 }
 
+// The C boundary: no exception leaves it (a failed host allocation returns DBX_ERR_NOMEM
+// instead of ending the caller's process)
+template <class F>
+int no_throw(F &&f)
+{
+    try {
+        return f();
+    } catch (const std::bad_alloc &) {
+        std::fprintf(stderr, "ERROR: host allocation failed\n");
+        return DBX_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        std::fprintf(stderr, "ERROR: %s\n", e.what());
+        return DBX_ERR_ARG;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -255,7 +273,8 @@ void dbx_deb_header_defaults(dbx_deb_header *h)
     h->hist = 0;
 }
 
-int dbx_xyz_to_deb(const char *xyz_path, const char *deb_path, const char *table_path, const dbx_deb_header *hdr)
+static int xyz_to_deb_body(const char *xyz_path, const char *deb_path, const char *table_path,
+                           const dbx_deb_header *hdr)
 {
     if (!xyz_path) return DBX_ERR_ARG;
     dbx_deb_header H;
@@ -406,8 +425,8 @@ int dbx_xyz_to_deb(const char *xyz_path, const char *deb_path, const char *table
     return ok ? DBX_OK : DBX_ERR_IO;
 }
 
-int dbx_average_debi(const char *const *paths, int n_files, int norm, double step, double *tt_out, double *I_out,
-                     int capacity, int *n_out)
+static int average_debi_body(const char *const *paths, int n_files, int norm, double step, double *tt_out,
+                             double *I_out, int capacity, int *n_out)
 {
     if (!paths || n_files < 1 || !n_out || !(step > 0) || norm < DBX_AVG_NONE || norm > DBX_AVG_MAX)
         return DBX_ERR_ARG;
@@ -450,6 +469,17 @@ int dbx_average_debi(const char *const *paths, int n_files, int norm, double ste
         I_out[k] = acc[k] / div;
     }
     return DBX_OK;
+}
+
+int dbx_xyz_to_deb(const char *xyz_path, const char *deb_path, const char *table_path, const dbx_deb_header *hdr)
+{
+    return no_throw([&] { return xyz_to_deb_body(xyz_path, deb_path, table_path, hdr); });
+}
+
+int dbx_average_debi(const char *const *paths, int n_files, int norm, double step, double *tt_out, double *I_out,
+                     int capacity, int *n_out)
+{
+    return no_throw([&] { return average_debi_body(paths, n_files, norm, step, tt_out, I_out, capacity, n_out); });
 }
 
 }  // extern "C"

@@ -34,7 +34,7 @@ enum {
     DBX_ERR_IO = -3,       /* file missing / unwritable */
     DBX_ERR_PARSE = -4,    /* malformed .deb, or atom count mismatch (cuDebye-v15.cu:830-833) */
     DBX_ERR_RANGE = -5,    /* a pair distance fell at or beyond the last bin (rmax too small) */
-    DBX_ERR_NOMEM = -6,    /* device allocation failed */
+    DBX_ERR_NOMEM = -6,    /* device or host allocation failed */
     DBX_ERR_BUDGET = -7,   /* request larger than the context's HBM budget */
     DBX_ERR_STATE = -8     /* unknown ticket / wrong state */
 };
