@@ -23,6 +23,7 @@
 #include <set>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -436,7 +437,8 @@ int dbx_server_open(int device, const dbx_server_config *cfg, dbx_server **out)
 {
     if (!out) return DBX_ERR_ARG;
     *out = nullptr;
-    dbx_server *s = new dbx_server;
+    dbx_server *s = new (std::nothrow) dbx_server;
+    if (!s) return DBX_ERR_NOMEM;
     s->device = device;
     if (cfg) s->cfg = *cfg;
     if (s->cfg.max_batch <= 0) s->cfg.max_batch = 256;
@@ -468,8 +470,14 @@ int dbx_server_open(int device, const dbx_server_config *cfg, dbx_server **out)
         delete s;
         return rc;
     }
-    for (int i = 0; i < s->cfg.prep_threads; i++) s->threads.emplace_back(prep_loop, s);
-    for (int i = 0; i < 2; i++) s->threads.emplace_back(lane_loop, s, i);
+    try {
+        for (int i = 0; i < s->cfg.prep_threads; i++) s->threads.emplace_back(prep_loop, s);
+        for (int i = 0; i < 2; i++) s->threads.emplace_back(lane_loop, s, i);
+    } catch (const std::exception &) {
+        // a thread that could not start: stop and join those that did, release everything
+        dbx_server_close(s);
+        return DBX_ERR_NOMEM;
+    }
     *out = s;
     return DBX_OK;
 }
@@ -523,13 +531,20 @@ int dbx_submit(dbx_server *s, const dbx_request *q, uint64_t *ticket)
     if (n > 0 && (!q->x || !q->y || !q->z)) return DBX_ERR_ARG;
     for (int64_t i = 0; i < n; i++)
         if (!std::isfinite(q->x[i]) || !std::isfinite(q->y[i]) || !std::isfinite(q->z[i])) return DBX_ERR_ARG;
-    std::unique_ptr<Request> r(new Request);
-    r->x.assign(q->x, q->x + n);
-    r->y.assign(q->y, q->y + n);
-    r->z.assign(q->z, q->z + n);
-    r->group_n.assign(q->group_n, q->group_n + q->n_groups);
-    r->params.assign(q->params, q->params + q->n_groups);
-    r->q.assign(q->q, q->q + q->nq);
+    std::unique_ptr<Request> r;
+    try {
+        r.reset(new Request);
+        r->x.assign(q->x, q->x + n);
+        r->y.assign(q->y, q->y + n);
+        r->z.assign(q->z, q->z + n);
+        r->group_n.assign(q->group_n, q->group_n + q->n_groups);
+        r->params.assign(q->params, q->params + q->n_groups);
+        r->q.assign(q->q, q->q + q->nq);
+    } catch (const std::bad_alloc &) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = "host allocation failed";
+        return DBX_ERR_NOMEM;
+    }
     r->rstep = q->rstep;
     r->n_bins = q->n_bins;
     r->first_n = q->first_n;
@@ -543,12 +558,23 @@ int dbx_submit(dbx_server *s, const dbx_request *q, uint64_t *ticket)
         return DBX_ERR_BUDGET;
     }
     std::lock_guard<std::mutex> lk(s->mu);
-    r->id = s->next_id++;
-    r->t_submit = now_ms();
-    *ticket = r->id;
-    s->preparing.insert(r->id);
-    s->prep_q.push_back(r.get());
-    s->reqs[r->id] = std::move(r);
+    const uint64_t id = s->next_id;
+    Request *rp = r.get();
+    rp->id = id;
+    rp->t_submit = now_ms();
+    // registered in all three places or in none
+    try {
+        s->reqs[id] = std::move(r);
+        s->preparing.insert(id);
+        s->prep_q.push_back(rp);
+    } catch (const std::bad_alloc &) {
+        s->preparing.erase(id);
+        s->reqs.erase(id);
+        s->err = "host allocation failed";
+        return DBX_ERR_NOMEM;
+    }
+    s->next_id++;
+    *ticket = id;
     s->st.submitted++;
     s->cv_prep.notify_one();
     return DBX_OK;
@@ -570,10 +596,15 @@ int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms)
 {
     if (!s) return DBX_ERR_ARG;
     std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->reqs.find(ticket);
-    if (it == s->reqs.end()) return DBX_ERR_ARG;
-    Request *r = it->second.get();
-    s->cv_done.wait(lk, [&] { return r->state == kDone || r->state == kFailed || s->stop; });
+    if (s->reqs.find(ticket) == s->reqs.end()) return DBX_ERR_ARG;
+    // looked up again at every wake-up: another thread may release the request meanwhile
+    Request *r = nullptr;
+    s->cv_done.wait(lk, [&] {
+        auto it = s->reqs.find(ticket);
+        r = it == s->reqs.end() ? nullptr : it->second.get();
+        return !r || r->state == kDone || r->state == kFailed || s->stop;
+    });
+    if (!r) return DBX_ERR_ARG;
     if (latency_ms) *latency_ms = r->t_done - r->t_submit;
     if (r->state != kDone && r->state != kFailed) return DBX_ERR_STATE;
     if (I_out && !r->I.empty()) std::memcpy(I_out, r->I.data(), sizeof(double) * r->I.size());
@@ -587,6 +618,9 @@ int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms)
 int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params, double *I_out)
 {
     if (!s || !params || !I_out) return DBX_ERR_ARG;
+    // the decode lane is held from the lookup to the last use of the request: dbx_release
+    // takes it too before it frees one
+    std::lock_guard<std::mutex> rl(s->rw_mu);
     Request *r = nullptr;
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -598,7 +632,6 @@ int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params,
             return DBX_ERR_STATE;
         }
     }
-    std::lock_guard<std::mutex> rl(s->rw_mu);
     try {
         (void)hipSetDevice(s->device);
         const int nq = (int)r->q.size();
@@ -632,6 +665,10 @@ int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params,
         std::lock_guard<std::mutex> lk(s->mu);
         s->err = e.what();
         return e.code;
+    } catch (const std::bad_alloc &) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = "host allocation failed";
+        return DBX_ERR_NOMEM;
     }
     return DBX_OK;
 }
@@ -639,6 +676,7 @@ int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params,
 int dbx_request_histogram(dbx_server *s, uint64_t ticket, uint64_t *hist_out)
 {
     if (!s || !hist_out) return DBX_ERR_ARG;
+    std::lock_guard<std::mutex> rl(s->rw_mu);  // as in dbx_reweight
     Request *r = nullptr;
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -650,7 +688,6 @@ int dbx_request_histogram(dbx_server *s, uint64_t ticket, uint64_t *hist_out)
             return DBX_ERR_STATE;
         }
     }
-    std::lock_guard<std::mutex> rl(s->rw_mu);
     (void)hipSetDevice(s->device);
     if (hipMemcpyAsync(hist_out, r->d_hist, sizeof(uint64_t) * (size_t)r->P * r->n_bins, hipMemcpyDeviceToHost,
                        s->rw_ctx->stream) != hipSuccess ||

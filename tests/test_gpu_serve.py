@@ -141,6 +141,38 @@ def test_serve_reweight_replays_match_direct(ctx):
             srv.reweight(t, [_gp(p) for p in m.params])
 
 
+
+def test_serve_release_while_another_thread_reweights(ctx):
+    """dbx_release on one thread while another reweights the same request: each reweight
+    either returns the request's pattern or, once the request is gone, fails with an argument
+    or state error (never reads freed memory)."""
+    m = _models(1, 9, nqs=(128,))[0]
+    params = [_gp(p) for p in m.params]
+    with cd.Server(0) as srv:
+        t = _submit(srv, m)
+        I0, _, _ = _finish(srv, t)
+        ok, errs = [], []
+        go = threading.Event()
+
+        def loop():
+            go.wait()
+            for _ in range(300):
+                try:
+                    ok.append(_rel(srv.reweight(t, params), I0))
+                except cd.DebyeError as e:
+                    errs.append(e.code)
+                    break
+
+        th = threading.Thread(target=loop)
+        th.start()
+        go.set()
+        time.sleep(0.003)
+        srv.release(t)
+        th.join()
+        assert all(r < 1e-12 for r in ok)
+        assert all(c in (cd.DBX_ERR_ARG, cd.DBX_ERR_STATE) for c in errs)
+        assert srv.stats()["resident_bytes"] == 0
+
 def test_serve_admission_waits_for_budget():
     # five requests of nearly the same size; a budget that holds two of them resident
     models = [synth.small_random(1500, 2, 500 + i, spread=10.0, nq=64) for i in range(5)]
