@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <new>
 #include <numeric>
 #include <thread>
 #include <vector>
@@ -407,11 +408,13 @@ int dbx_open(int device, size_t hbm_budget, dbx_ctx **out)
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return DBX_ERR_HIP;
     if (hipSetDevice(device) != hipSuccess) return DBX_ERR_HIP;
-    dbx_ctx *c = new dbx_ctx;
+    dbx_ctx *c = new (std::nothrow) dbx_ctx;
+    if (!c) return DBX_ERR_NOMEM;
     c->device = device;
+    // on failure dbx_close releases whatever was created (members start null)
     size_t fr = 0, tot = 0;
     if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
-        delete c;
+        dbx_close(c);
         return DBX_ERR_HIP;
     }
     c->budget = hbm_budget ? hbm_budget : (size_t)(0.9 * (double)fr);
@@ -420,12 +423,12 @@ int dbx_open(int device, size_t hbm_budget, dbx_ctx **out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->counters, 64) != hipSuccess ||
         hipHostMalloc((void **)&c->counters_h, 64, hipHostMallocDefault) != hipSuccess) {
-        delete c;
+        dbx_close(c);
         return DBX_ERR_HIP;
     }
     for (int i = 0; i < 4; i++)
         if (hipEventCreate(&c->ev[i]) != hipSuccess) {
-            delete c;
+            dbx_close(c);
             return DBX_ERR_HIP;
         }
     *out = c;
@@ -436,7 +439,7 @@ int dbx_close(dbx_ctx *ctx)
 {
     if (!ctx) return DBX_ERR_ARG;
     (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (void *p : ctx->slot_ptr)
         if (p) (void)hipFree(p);
     for (void *p : ctx->pin_ptr)
@@ -447,7 +450,7 @@ int dbx_close(dbx_ctx *ctx)
     if (ctx->counters_h) (void)hipHostFree(ctx->counters_h);
     for (int i = 0; i < 4; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
-    (void)hipStreamDestroy(ctx->stream);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return DBX_OK;
 }
