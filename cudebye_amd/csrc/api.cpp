@@ -440,6 +440,11 @@ int dbx_close(dbx_ctx *ctx)
     if (!ctx) return DBX_ERR_ARG;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (dbx_model *m : ctx->models) {
+        free_model(m->d);
+        delete m;
+    }
+    ctx->models.clear();
     for (void *p : ctx->slot_ptr)
         if (p) (void)hipFree(p);
     for (void *p : ctx->pin_ptr)
@@ -502,6 +507,7 @@ int dbx_model_create(dbx_ctx *ctx, const float *x, const float *y, const float *
         dbx_model *m = new dbx_model;
         try {
             build_model(ctx, x, y, z, group_n, n_groups, m->d);
+            ctx->models.insert(m);
         } catch (...) {
             free_model(m->d);
             delete m;
@@ -515,6 +521,8 @@ int dbx_model_destroy(dbx_ctx *ctx, dbx_model *m)
 {
     if (!m) return DBX_ERR_ARG;
     return guarded(ctx, [&] {
+        // only a live model of this context (a second destroy is an argument error)
+        DBX_REQUIRE(ctx->models.erase(m) == 1, DBX_ERR_ARG, "not a live model of this context");
         DBX_HIP(hipStreamSynchronize(ctx->stream));
         free_model(m->d);
         delete m;

@@ -6,6 +6,7 @@
 
 #include <map>
 #include <mutex>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -162,6 +163,7 @@ struct dbx_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     std::recursive_mutex mu;
+    std::set<dbx_model *> models;  // live models, released by dbx_close if still there
     // grow-only device scratch, one buffer per slot (dbx::kSlot*)
     void *slot_ptr[13] = {};
     size_t slot_bytes[13] = {};

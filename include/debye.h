@@ -11,7 +11,9 @@
  *     dbx_last_error() gives the message of the context's last call (empty after a success).
  *     Nothing here calls exit() (the reference's gpuErrchk does, cuDebye-v15.cu:58-66).
  *   - Host buffers belong to the caller.  Device memory belongs to the context and is
- *     released by dbx_model_destroy / dbx_close.
+ *     released by dbx_model_destroy / dbx_close (which also releases the context's models
+ *     still alive: their handles are invalid afterwards).  dbx_model_destroy of a handle
+ *     that is not a live model of the context returns DBX_ERR_ARG.
  *   - A context serialises its synchronous calls; distinct contexts are independent.
  *   - Subgroup pairs are numbered in the reference's order (cuDebye-v15.cu:941-947):
  *     (0,0),(0,1),...,(0,S-1),(1,1),...,(S-1,S-1); P = S(S+1)/2.

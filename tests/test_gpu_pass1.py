@@ -219,3 +219,22 @@ def test_wide_tiles_and_window_groups_bit_exact(ctx, maker):
     ref, ov = _oracle_hist(m)
     assert ov == 0
     assert np.array_equal(H, ref)
+
+
+def test_model_lifetime_destroy_twice_and_close_with_live_models():
+    """A second dbx_model_destroy of a handle is an argument error (no double free), and
+    dbx_close releases the models still alive on the context."""
+    m = synth.small_random(500, 2, 5)
+    with cd.Context(0) as c:
+        mod = c.model(m.x, m.y, m.z, m.group_n)
+        h = mod.handle
+        assert cd.lib().dbx_model_destroy(c.handle, h) == cd.DBX_OK
+        mod.handle = None
+        assert cd.lib().dbx_model_destroy(c.handle, h) == cd.DBX_ERR_ARG
+        keep = [c.model(m.x, m.y, m.z, m.group_n) for _ in range(3)]
+        H = c.histogram(keep[0], m.rstep, m.n_bins)
+        a, b = (int(v) for v in m.group_n)
+        assert [int(r.sum()) for r in H] == [a * (a + 1) // 2, a * b, b * (b + 1) // 2]
+    # closed with three live models; a later close() of them is a no-op
+    for k in keep:
+        k.close()
