@@ -424,10 +424,20 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                     if (!diag) {
                         if (a_ok) {
-                            // four, then two B atoms per iteration: independent chains in
-                            // flight, the stage address advanced once per four pairs
+                            // eight, four, then two B atoms per iteration: independent chains
+                            // in flight, the stage address advanced once per eight pairs
                             int t = 0;
-                            for (; t + 3 < nb; t += 4) {
+                            for (; t + 7 < nb; t += 8) {
+                                float v[8];
+#pragma unroll
+                                for (int u = 0; u < 8; u++) {
+                                    const float4 b = stage[wave][t + u];
+                                    v[u] = pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp);
+                                }
+#pragma unroll
+                                for (int u = 0; u < 8; u++) cnt(v[u]);
+                            }
+                            if (t + 3 < nb) {
                                 float v[4];
 #pragma unroll
                                 for (int u = 0; u < 4; u++) {
@@ -436,14 +446,16 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                                 }
 #pragma unroll
                                 for (int u = 0; u < 4; u++) cnt(v[u]);
+                                t += 4;
                             }
-                            for (; t + 1 < nb; t += 2) {
+                            if (t + 1 < nb) {
                                 const float4 b0 = stage[wave][t];
                                 const float4 b1 = stage[wave][t + 1];
                                 const float v0 = pair_scaled(pa.x, pa.y, pa.z, b0.x, b0.y, b0.z, rp);
                                 const float v1 = pair_scaled(pa.x, pa.y, pa.z, b1.x, b1.y, b1.z, rp);
                                 cnt(v0);
                                 cnt(v1);
+                                t += 2;
                             }
                             if (t < nb) {
                                 const float4 b = stage[wave][t];
