@@ -172,6 +172,13 @@ def test_serve_release_while_another_thread_reweights(ctx):
         assert all(r < 1e-12 for r in ok)
         assert all(c in (cd.DBX_ERR_ARG, cd.DBX_ERR_STATE) for c in errs)
         assert srv.stats()["resident_bytes"] == 0
+        # a released ticket is unknown to every call that takes one
+        _, _, rc = srv.wait(t, allow=(cd.DBX_ERR_ARG,))
+        assert rc == cd.DBX_ERR_ARG
+        with pytest.raises(cd.DebyeError):
+            srv.histogram(t)
+        with pytest.raises(cd.DebyeError):
+            srv.release(t)
 
 def test_serve_admission_waits_for_budget():
     # five requests of nearly the same size; a budget that holds two of them resident
