@@ -122,11 +122,13 @@ struct CountWindow {
 // subtraction of k_lo per pair; the field's shift is (k << 4) mod 32, which the shift and
 // bit-field instructions take from k << 4 directly.  A field counts up to 0x7FFF: the lane
 // whose increment sets its top bit (old field 0x7FFF, which exactly one lane sees per 32768
-// counts) takes 0x8000 back off the field and adds 32768 to the bin in HBM, so a field never
-// carries into its neighbour (that would need 32768 more increments of the same bin, by the
-// workgroup's other waves, before the lane's subtraction lands).  The lane looks at the old
-// value one pair later, so that waiting for it does not stall the next pair; finish() looks at
-// the last one.  Guard and sentinel overflows are recorded in *err instead.
+// counts) takes 0x8000 back off the field and adds 32768 to the bin in HBM.  The lane looks at
+// the old value at its next pair, so that waiting for it does not stall that pair, or at
+// finish(), which the pair loops call where a lane skips a pair and at the end of every
+// candidate: a field that reached 0x8000 is drained within one iteration of its wave's pair
+// loop, with no barrier and no wait on another wave in between.  A carry into the neighbouring
+// field would need 32768 further increments of that bin by the other waves inside that one
+// iteration.  Guard and sentinel overflows are recorded in *err instead.
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 __device__ __forceinline__ uint32_t lds_addr(uint32_t *p) { return (uint32_t)(uintptr_t)(lds_u32 *)p; }
 __device__ __forceinline__ uint32_t lds_add(uint32_t addr, uint32_t v)
@@ -463,13 +465,19 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                             }
                         }
                     } else {
-                        // the B tile is the A tile: pairs with index b >= a only
+                        // the B tile is the A tile: pairs with index b >= a only; a lane that
+                        // skips a pair settles its previous one, as an add would
                         for (int t = 0; t < nb; t++) {
                             const float4 b = stage[wave][t];
                             if (a_ok && idx_a <= __float_as_int(b.w))
                                 cnt(pair_scaled(pa.x, pa.y, pa.z, b.x, b.y, b.z, rp));
+                            else
+                                cnt.finish();
                         }
                     }
+                    // the candidate's last pair is settled here, before the wave can wait on
+                    // other waves (next candidate, chunk barrier)
+                    cnt.finish();
                     // the stage is rewritten by the next candidate only after every lane's reads
                     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                     __builtin_amdgcn_wave_barrier();
@@ -521,7 +529,6 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                 cnt.hrow = hrow, cnt.n_bins = n_bins, cnt.oor = M.oor, cnt.err = &s_err;
                 cnt.vlo = (float)(k_lo - 1), cnt.vhi = (float)(k_lo + 2 * kP1Bins);
                 run(cnt);
-                cnt.finish();
                 __syncthreads();
                 bad = (h[0] >> 16) != 0u || (h[kP1Bins + 1] & 0xFFFFu) != 0u || s_err != 0;
             }
@@ -535,7 +542,6 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             cnt.hrow = hrow, cnt.n_bins = n_bins, cnt.oor = M.oor, cnt.err = &s_err;
             cnt.trash = lds_addr(&trash[lane]), cnt.kmin = INT_MAX, cnt.kmax = INT_MIN;
             run(cnt);
-            cnt.finish();
             bad = kminmax(cnt.kmin, cnt.kmax) || s_err != 0;
         }
         if (bad && threadIdx.x == 0) {

@@ -229,6 +229,32 @@ def serve_request(index, seed_base=5000, n_lo=500, n_hi=200000, rstep=1e-3):
                    q_grid_2theta(nq), dict(config=5, index=index, nq=nq, seed=seed_base + index))
 
 
+def stacked_lattice(spacing=9.0, radius=100.0, stack=8, rstep=1e-3, seed=6001, nq=64):
+    """Perfect simple cubic lattice (sigma = 0) cut to a sphere, `stack` coincident atoms on
+    every site, one species: many pairs per bin (shell multiplicity x stack^2 per site pair),
+    and 64-atom tiles wide enough for 16-bit pass-1 windows."""
+    rng = np.random.default_rng(seed)
+    n = int(math.ceil(radius / spacing)) + 1
+    g = np.arange(-n, n + 1, dtype=np.float64) * spacing
+    p = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    p = np.repeat(p[(p ** 2).sum(1) <= radius * radius], stack, axis=0)
+    return _finish(f"stack-{spacing:g}-{radius:g}-{stack}", [p], [cromer_mann(rng)], rstep, q_grid_2theta(nq),
+                   dict(spacing=spacing, radius=radius, stack=stack, seed=seed))
+
+
+def shell_around_stack(n_shell=2000, n_stack=60000, rho=20.00003, rstep=1e-4, seed=6002, nq=64):
+    """Subgroup 0: n_shell atoms at distance rho (FP64, then rounded to FP32) from the origin in
+    seeded random directions; subgroup 1: n_stack atoms all at the origin.  Every cross pair
+    falls in the one bin of rho, so the (0, 1) partial puts all its counts into a single bin,
+    and its A tiles (patches of the sphere) are wide enough for 16-bit windows."""
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n_shell, 3))
+    d *= rho / np.sqrt((d ** 2).sum(1))[:, None]
+    return _finish(f"shell-{n_shell}-stack-{n_stack}", [d, np.zeros((n_stack, 3))],
+                   [cromer_mann(rng), cromer_mann(rng)], rstep, q_grid_2theta(nq),
+                   dict(n_shell=n_shell, n_stack=n_stack, rho=rho, seed=seed))
+
+
 def small_random(n, n_groups=2, seed=0, spread=8.0, rstep=1e-3, nq=64):
     """Small random cluster for tests."""
     rng = np.random.default_rng(seed)

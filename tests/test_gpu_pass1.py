@@ -238,3 +238,51 @@ def test_model_lifetime_destroy_twice_and_close_with_live_models():
     # closed with three live models; a later close() of them is a no-op
     for k in keep:
         k.close()
+
+
+def _device_plan_windows(m, mod):
+    """win field of every pass-1 job the model's plan runs (dbx_debug_pass1_plan)."""
+    import ctypes as C
+    fn = cd.lib().dbx_debug_pass1_plan
+    fn.restype = C.c_int
+    nj, ne = C.c_int(), C.c_int()
+    assert fn(mod.handle, C.c_double(m.rstep), None, 0, None, 0, C.byref(nj), C.byref(ne)) == 0
+    jobs = np.zeros((nj.value, 12), np.int32)
+    e2 = np.zeros(ne.value, np.float32)
+    assert fn(mod.handle, C.c_double(m.rstep), jobs.ctypes.data_as(C.c_void_p), nj.value,
+              e2.ctypes.data_as(C.c_void_p), ne.value, C.byref(nj), C.byref(ne)) == 0
+    return jobs
+
+
+@pytest.mark.parametrize("maker", ["stacked_lattice", "shell_around_stack"])
+def test_16bit_windows_under_heavy_counts_bit_exact(ctx, maker):
+    """16-bit LDS counters whose fields overflow again and again within one workgroup: a perfect
+    lattice with eight coincident atoms per site (single 16-bit windows, over 65536 counts in a
+    bin per job), and atoms on a sphere around 60,000 coincident atoms (multi-window 16-bit
+    jobs; every cross pair of a workgroup lands in the one field of bin rho, over 10^6 per job,
+    incremented by all 1024 threads at once).  Bit-exact against the oracle, closed forms."""
+    m = getattr(synth, maker)()
+    print(f"{m.name}: {m.n_atoms} atoms, {m.pair_count:.3g} pairs, {m.n_bins} bins", flush=True)
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    jobs = _device_plan_windows(m, mod)
+    w = jobs[:, 10]
+    if maker == "stacked_lattice":
+        assert np.sum((w & 4 != 0) & (w & 3 == 3)) > 100
+    else:
+        cross = w[jobs[:, 0] == 1]
+        assert np.all(cross & 4) and np.all(cross & 3 != 3)
+    H = ctx.histogram(mod, m.rstep, m.n_bins)
+    n = [int(v) for v in m.group_n]
+    S = len(n)
+    assert [int(h.sum()) for h in H] == [n[a] * (n[a] + 1) // 2 if a == b else n[a] * n[b]
+                                         for a in range(S) for b in range(a, S)]
+    if maker == "shell_around_stack":
+        k_rho = int(np.float32(m.meta["rho"]) * np.float32(1.0 / m.rstep) + np.float32(0.5))
+        assert int(H[1, k_rho]) == n[0] * n[1]           # all cross pairs in one bin
+        assert int(H[2, 0]) == n[1] * (n[1] + 1) // 2    # the stack: every pair at distance 0
+    else:
+        assert int(H[0, 0]) == n[0] * (8 + 1) // 2       # each site: its 8 atoms' 36 pairs
+    ref, ov = _oracle_hist(m)
+    assert ov == 0
+    assert np.array_equal(H, ref)
+    mod.close()

@@ -185,3 +185,64 @@ def test_tiles_are_the_median_bisection_leaves():
             want.append(tuple(sorted(map(tuple, xyz[leaf].tolist()))))
         start += n
     assert got == sorted(want)
+
+
+def _job_bin_counts(m, tiles, pos, J, e2, alist):
+    """Counts per window bin that one job adds (its shell's pairs inside its window), emulated
+    like check_plan: the kernel's shell test and bin rule in float32."""
+    tf = tiles.view(np.float32)
+    cen, start, cnt, parent = tf[:, :3], tiles[:, 4], tiles[:, 5], tiles[:, 7]
+    rp = np.float32(1.0 / m.rstep)
+    pair, a_off, a_cnt, b0, b1, same, shell, klo, eoff, degraded, win = J[:11]
+    lo, hi = e2[eoff + shell], e2[eoff + shell + 1]
+    width = 2 * W if win & 4 else W
+    acc = np.zeros(width, np.int64)
+    for t in alist[a_off:a_off + a_cnt]:
+        j0 = max(b0, parent[t]) if same else b0
+        if j0 >= b1:
+            continue
+        A = pos[start[t]:start[t] + cnt[t], :3]
+        ia = np.arange(start[t], start[t] + cnt[t])
+        ib = np.arange(start[j0], start[b1 - 1] + cnt[b1 - 1])  # tiles j0..b1-1 are contiguous
+        B = pos[ib, :3]
+        e = (B - cen[t]).astype(np.float32)
+        d2c = (e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]) + e[:, 2] * e[:, 2]
+        sel = (d2c >= lo) & (d2c < hi)
+        if not sel.any():
+            continue
+        k = _bins(A, B[sel], rp)
+        valid = ia[:, None] <= ib[sel][None, :] if same else np.ones(k.shape, bool)
+        kk = k[valid] - klo
+        kk = kk[(kk >= 0) & (kk < width)]
+        acc += np.bincount(kk, minlength=width)[:width]
+    return acc
+
+
+def test_plan_16bit_windows_take_heavy_counts():
+    """The workloads of the 16-bit counter stress tests on the GPU: a perfect lattice with eight
+    coincident atoms per site runs single 16-bit windows in which one job adds more than 65536
+    counts to a bin (every such field drains at least twice), and atoms on a sphere around a
+    stack of coincident atoms run multi-window 16-bit jobs whose every cross pair falls into
+    the same bin, more than 10^6 of them per job."""
+    m = synth.stacked_lattice()
+    tiles, pos, jobs, e2, al = host_plan(m)
+    w = jobs[:, 10]
+    single16 = np.nonzero((w & 4 != 0) & (w & 3 == 3))[0]
+    assert len(single16) > 100 and jobs[:, 9].sum() == 0
+    heavy = max(int(_job_bin_counts(m, tiles, pos, jobs[j], e2, al).max()) for j in single16[:48])
+    assert heavy > 65536, heavy
+
+    m = synth.shell_around_stack()
+    tiles, pos, jobs, e2, al = host_plan(m)
+    cross = jobs[jobs[:, 0] == 1]
+    assert np.all(cross[:, 10] & 4) and np.all(cross[:, 10] & 3 != 3) and cross[:, 9].sum() == 0
+    cnt = tiles[:, 5]
+    # every cross pair is at distance rho: the window of a shell that holds rho's bin counts
+    # all the job's cross pairs in one field, A atoms x B atoms of its range
+    k_rho = int(np.float32(m.meta["rho"]) * np.float32(1.0 / m.rstep) + np.float32(0.5))
+    mine = cross[(cross[:, 7] <= k_rho) & (k_rho < cross[:, 7] + 2 * W)]
+    per_job = [int(cnt[al[J[1]:J[1] + J[2]]].sum()) * int(cnt[J[3]:J[4]].sum()) for J in mine]
+    assert len(mine) > 0 and max(per_job) > 10 ** 6, max(per_job)
+    for J, n in list(zip(mine, per_job))[:6]:
+        h = _job_bin_counts(m, tiles, pos, J, e2, al)
+        assert h.max() == n == h.sum() and h[k_rho - J[7]] == n
