@@ -52,7 +52,7 @@ class BatchItem(C.Structure):
 class ServerConfig(C.Structure):
     """dbx_server_config."""
     _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", C.c_int), ("max_step_pairs", C.c_double),
-                ("prep_threads", C.c_int), ("pass2_mode", C.c_int)]
+                ("prep_threads", C.c_int), ("pass2_mode", C.c_int), ("batch_window_ms", C.c_double)]
 
 
 class RequestDesc(C.Structure):
@@ -67,7 +67,8 @@ class ServerStats(C.Structure):
     """dbx_server_stats."""
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("steps", C.c_uint64), ("pairs", C.c_double), ("resident_bytes", C.c_uint64),
-                ("budget_bytes", C.c_uint64), ("max_step_requests", C.c_double)]
+                ("budget_bytes", C.c_uint64), ("max_step_requests", C.c_double),
+                ("max_step_qgrids", C.c_double)]
 
 
 DBX_REQ_KEEP = 1
@@ -461,8 +462,8 @@ class Server:
     their intensities, re-weight resident ones with new form factors."""
 
     def __init__(self, device=0, hbm_budget=0, max_batch=0, max_step_pairs=0.0, prep_threads=0,
-                 pass2_mode=DBX_PASS2_MFMA):
-        cfg = ServerConfig(hbm_budget, max_batch, max_step_pairs, prep_threads, pass2_mode)
+                 pass2_mode=DBX_PASS2_MFMA, batch_window_ms=0.0):
+        cfg = ServerConfig(hbm_budget, max_batch, max_step_pairs, prep_threads, pass2_mode, batch_window_ms)
         h = C.c_void_p()
         rc = lib().dbx_server_open(device, C.byref(cfg), C.byref(h))
         if rc != DBX_OK:
@@ -508,17 +509,24 @@ class Server:
             self._err(rc, "poll")
         return st.value
 
+    def _info(self, ticket):
+        """(nq, n_groups, n_bins) of a ticket submitted here and not yet released"""
+        if ticket not in self._nq:
+            raise DebyeError(DBX_ERR_ARG, f"unknown ticket {ticket}")
+        return self._nq[ticket]
+
     def wait(self, ticket, allow=()):
-        """(I, latency_ms, status)"""
-        I = np.zeros(self._nq[ticket][0])
+        """(I, latency_ms, status).  DBX_ERR_BUDGET: the request cannot be admitted until
+        finished DBX_REQ_KEEP requests are released (it stays queued)."""
+        I = np.zeros(self._nq[ticket][0]) if ticket in self._nq else None
         lat = C.c_double()
-        rc = lib().dbx_wait(self.handle, ticket, _p(I), C.byref(lat))
+        rc = lib().dbx_wait(self.handle, ticket, None if I is None else _p(I), C.byref(lat))
         if rc != DBX_OK and rc not in allow:
             self._err(rc, f"request {ticket}")
         return I, lat.value, rc
 
     def reweight(self, ticket, params):
-        nq, S, _ = self._nq[ticket]
+        nq, S, _ = self._info(ticket)
         pa = (GroupParams * S)(*params)
         I = np.zeros(nq)
         rc = lib().dbx_reweight(self.handle, ticket, pa, _p(I))
@@ -527,7 +535,7 @@ class Server:
         return I
 
     def histogram(self, ticket):
-        nq, S, nb = self._nq[ticket]
+        nq, S, nb = self._info(ticket)
         H = np.zeros((S * (S + 1) // 2, nb), np.uint64)
         rc = lib().dbx_request_histogram(self.handle, ticket, _p(H))
         if rc != DBX_OK:
@@ -538,6 +546,7 @@ class Server:
         rc = lib().dbx_release(self.handle, ticket)
         if rc != DBX_OK:
             self._err(rc, "release")
+        self._nq.pop(ticket, None)
 
     def stats(self):
         st = ServerStats()

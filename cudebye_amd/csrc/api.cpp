@@ -522,10 +522,15 @@ int dbx_model_destroy(dbx_ctx *ctx, dbx_model *m)
     if (!m) return DBX_ERR_ARG;
     return guarded(ctx, [&] {
         // only a live model of this context (a second destroy is an argument error)
-        DBX_REQUIRE(ctx->models.erase(m) == 1, DBX_ERR_ARG, "not a live model of this context");
-        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        DBX_REQUIRE(ctx->models.count(m) == 1, DBX_ERR_ARG, "not a live model of this context");
+        // the model is released whatever the synchronisation returns (a failed one has ended
+        // the stream's work); its error is reported after
+        const hipError_t e = hipStreamSynchronize(ctx->stream);
+        ctx->models.erase(m);
         free_model(m->d);
         delete m;
+        if (e != hipSuccess)
+            throw Error(DBX_ERR_HIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
     });
 }
 

@@ -3,14 +3,18 @@
 // No reference counterpart in-process: the reference runs one process per model
 // (Matlab/multisim.m:42-58 launching cuDebye.exe, cuDebye-v15.cu:599-1003).
 //
-//   submit -> [prep threads: tiling + pass-1 plan on the host] -> ready queue (FIFO)
+//   submit -> [prep threads: tiling + pass-1 plan on the host] -> ready queue
 //          -> admission: every device buffer of the request from the budget arena
 //          -> [lane threads, one stream each: one step = the admitted requests that fit the
 //              step caps; pass 1 over all of them in one launch pair, then per q grid one
 //              prefactor + compaction + Debye contraction + combine] -> done
 // A request's device memory comes from one arena sized by the HBM budget; a request waits
-// in the ready queue until its buffers fit.  Memory is returned at completion (or at
-// dbx_release for DBX_REQ_KEEP requests, whose histograms and S_p(q) stay resident).
+// in the ready queue until its buffers fit.  Prepared requests are admitted in submission
+// order, except that one may pass earlier requests still being prepared or waiting for
+// memory while the oldest waiting request is younger than kAdmitHoldMs; past that, nothing
+// passes it, so the memory that finishing steps return goes to it.  Memory is returned at
+// completion (or at dbx_release for DBX_REQ_KEEP requests, whose histograms and S_p(q) stay
+// resident).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +37,9 @@
 namespace dbx {
 
 enum { kQueued = 0, kReady = 1, kAdmitted = 2, kRunning = 3, kDone = 4, kFailed = 5 };
+
+// how long the oldest waiting request may be passed by later ones at admission
+constexpr double kAdmitHoldMs = 50.0;
 
 static double now_ms()
 {
@@ -76,7 +83,7 @@ struct Request {
     int state = kQueued;
     int err = DBX_OK;
     std::string msg;
-    double t_submit = 0, t_done = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
     // reweight: captured graph, pinned staging
     hipGraph_t rw_graph = nullptr;
     hipGraphExec_t rw_exec = nullptr;
@@ -105,6 +112,9 @@ struct dbx_server {
     std::mutex rw_mu;
     std::string err;
     dbx_server_stats st{};
+    int running_steps = 0;  // lanes inside a step
+    int in_api = 0;         // callers inside an entry point (dbx_server_close waits for none)
+    std::condition_variable cv_api;
 };
 
 namespace dbx {
@@ -255,26 +265,63 @@ static void prep_loop(dbx_server *s)
         r->state = kReady;
         s->ready[r->id] = r;
         s->cv_sched.notify_all();
+        s->cv_done.notify_all();  // a waiter looks again whether its request can still be admitted
     }
 }
 
 // admission (caller holds s->mu): prepared requests in submission order while their buffers
-// fit; a request also waits while an earlier one is still being prepared
-static void admit(dbx_server *s)
+// fit.  While the oldest waiting request (being prepared, or prepared and waiting for memory)
+// is younger than kAdmitHoldMs, later prepared requests that fit may pass it; after that,
+// none does.  Returns the number admitted.
+static int admit(dbx_server *s)
 {
-    while (!s->ready.empty()) {
-        auto it = s->ready.begin();
-        if (!s->preparing.empty() && *s->preparing.begin() < it->first) break;
-        Request *r = it->second;
-        if (!alloc_request(*s->arena, *r)) break;
-        s->ready.erase(it);
-        r->state = kAdmitted;
-        s->admitted_q.push_back(r);
+    const double now = now_ms();
+    int n = 0;
+    for (;;) {
+        uint64_t oldest = UINT64_MAX;
+        if (!s->preparing.empty()) oldest = *s->preparing.begin();
+        if (!s->ready.empty()) oldest = std::min(oldest, s->ready.begin()->first);
+        bool hold = false;
+        if (oldest != UINT64_MAX) {
+            auto it = s->reqs.find(oldest);
+            hold = it != s->reqs.end() && now - it->second->t_submit > kAdmitHoldMs;
+        }
+        bool held_in = false;
+        for (auto it = s->ready.begin(); it != s->ready.end();) {
+            // a request held in line: nothing after it is admitted before it
+            if (hold && it->first != oldest) break;
+            Request *r = it->second;
+            if (!alloc_request(*s->arena, *r)) {
+                if (hold) break;
+                ++it;
+                continue;
+            }
+            it = s->ready.erase(it);
+            r->state = kAdmitted;
+            r->t_admit = now;
+            s->admitted_q.push_back(r);
+            n++;
+            if (hold) {
+                held_in = true;  // the line moves on: look at the next oldest
+                break;
+            }
+        }
+        if (!held_in) break;
     }
     s->st.resident_bytes = s->arena->used();
+    return n;
 }
 
-// one step on a lane: uploads, pass 1 over all requests, pass 2 per q grid, results
+// true when request r waits for memory that nothing but a dbx_release can free: it is
+// prepared but not admitted, and no step runs, none is admitted and none is being prepared
+static bool admission_stalled(dbx_server *s, const Request *r)
+{
+    return r->state == kReady && s->running_steps == 0 && s->admitted_q.empty() && s->preparing.empty();
+}
+
+// one step on a lane: uploads, pass 1 over all requests, pass 2 per q grid, results.  The
+// host waits for the device once, at the end (pass 2 sizes its contraction from values it
+// reads back, see debye_structure_rows).
 static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
 {
     const double rstep = batch[0]->rstep;
@@ -283,18 +330,15 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
         ModelDev &m = r->m;
         PlanHost &ph = r->plan;
         DBX_HIP(hipMemcpyAsync(m.pos, r->image.data(), r->image.size(), hipMemcpyHostToDevice, ctx->stream));
-        m.plan_h = ph.jobs;
-        m.plan_cost = ph.cost;
-        m.plan_jobs = (int)ph.jobs.size();
+        m.plan_h = std::move(ph.jobs);
+        m.plan_cost = std::move(ph.cost);
+        m.plan_jobs = (int)m.plan_h.size();
         m.plan_fast = ph.n_fast;
         m.plan_list_n = (int)ph.list.size();
         m.plan_list_fast = ph.list_fast;
         m.plan_cu = ph.cu;
         m.plan_rstep = ph.rstep;
     }
-    // the images must outlive their copies
-    DBX_HIP(hipStreamSynchronize(ctx->stream));
-    for (Request *r : batch) std::vector<char>().swap(r->image);
     std::vector<P1Item> items;
     for (Request *r : batch) {
         P1Item it{&r->m, r->n_bins, r->d_hist};
@@ -302,29 +346,56 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
         items.push_back(it);
     }
     pass1_batch(ctx, items, rstep, false);
+    // the launch list is on the device: the host copies of the plan and tiles are not needed
+    // any more
+    for (Request *r : batch) {
+        r->plan = PlanHost();
+        std::vector<P1Job>().swap(r->m.plan_h);
+        std::vector<double>().swap(r->m.plan_cost);
+        std::vector<Tile>().swap(r->m.tiles_h);
+    }
     // pass 2: one contraction per q grid, rows longest first
     std::vector<Request *> order(batch);
     std::stable_sort(order.begin(), order.end(), [](Request *a, Request *b) {
         if (a->q.size() != b->q.size()) return a->q.size() < b->q.size();
         return a->n_bins > b->n_bins;
     });
-    size_t g0 = 0;
-    while (g0 < order.size()) {
+    std::vector<std::pair<size_t, size_t>> groups;
+    for (size_t g0 = 0; g0 < order.size();) {
         size_t g1 = g0 + 1;
         while (g1 < order.size() && order[g1]->q == order[g0]->q) g1++;
+        groups.push_back({g0, g1});
+        g0 = g1;
+    }
+    // every group's intensities and out-of-range counts land in one pinned block each, read
+    // after the one synchronisation of the step
+    size_t nI = 0;
+    for (Request *r : batch) nI += r->q.size();
+    double *hI = (double *)pinned(ctx, kPinStepI, sizeof(double) * nI);
+    uint64_t *hoor = (uint64_t *)pinned(ctx, kPinOor, sizeof(uint64_t) * batch.size());
+    std::vector<size_t> I_off(order.size());
+    size_t io = 0;
+    for (size_t i = 0; i < order.size(); i++) {
+        I_off[i] = io;
+        io += order[i]->q.size();
+        DBX_HIP(hipMemcpyAsync(hoor + i, order[i]->d_oor, 8, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    for (auto &g : groups) {
         // rows longest first within the group
-        std::vector<Request *> grp(order.begin() + g0, order.begin() + g1);
-        std::stable_sort(grp.begin(), grp.end(), [](Request *a, Request *b) { return a->n_bins > b->n_bins; });
-        const int nq = (int)grp[0]->q.size();
-        uint64_t *base = grp[0]->d_hist;
-        for (Request *r : grp) base = std::min(base, r->d_hist);
+        std::vector<size_t> grp;
+        for (size_t i = g.first; i < g.second; i++) grp.push_back(i);
+        std::stable_sort(grp.begin(), grp.end(),
+                         [&](size_t a, size_t b) { return order[a]->n_bins > order[b]->n_bins; });
+        const int nq = (int)order[grp[0]]->q.size();
+        uint64_t *base = order[grp[0]]->d_hist;
+        for (size_t i : grp) base = std::min(base, order[i]->d_hist);
         std::vector<RowDesc> rows;
         std::vector<GroupPair> pairs;
         std::vector<int4> mrows;
-        for (size_t i = 0; i < grp.size(); i++) {
-            Request *r = grp[i];
+        for (size_t gi = 0; gi < grp.size(); gi++) {
+            Request *r = order[grp[gi]];
             const int S = (int)r->group_n.size();
-            mrows.push_back(make_int4((int)rows.size(), r->P, (int)i, 0));
+            mrows.push_back(make_int4((int)rows.size(), r->P, (int)gi, 0));
             int p = 0;
             for (int a = 0; a < S; a++)
                 for (int b = a; b < S; b++, p++) {
@@ -336,27 +407,32 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
         const size_t R = rows.size();
         double *buf = (double *)scratch(ctx, kSlotQ, sizeof(double) * (size_t)nq * (2 * R + grp.size()));
         double *dff = buf, *dS = buf + R * nq, *dI = dS + R * nq;
-        prefactor_rows(ctx, pairs, grp[0]->d_q, nq, dff);
-        debye_structure_rows(ctx, base, rows, rstep, grp[0]->d_q, nq, dS);
+        const double *dq = order[grp[0]]->d_q;
+        prefactor_rows(ctx, pairs, dq, nq, dff);
+        debye_structure_rows(ctx, base, rows, rstep, dq, nq, dS);
         combine_models(ctx, dff, dS, mrows, nq, dI);
-        double *hI = (double *)pinned(ctx, kPinStepI, sizeof(double) * (size_t)nq * grp.size());
-        DBX_HIP(hipMemcpyAsync(hI, dI, sizeof(double) * (size_t)nq * grp.size(), hipMemcpyDeviceToHost, ctx->stream));
-        for (size_t i = 0; i < grp.size(); i++) {
-            Request *r = grp[i];
-            DBX_HIP(hipMemcpyAsync(r->d_S, dS + (size_t)mrows[i].x * nq, sizeof(double) * (size_t)r->P * nq,
+        for (size_t gi = 0; gi < grp.size(); gi++) {
+            Request *r = order[grp[gi]];
+            DBX_HIP(hipMemcpyAsync(hI + I_off[grp[gi]], dI + gi * (size_t)nq, sizeof(double) * (size_t)nq,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            DBX_HIP(hipMemcpyAsync(r->d_S, dS + (size_t)mrows[gi].x * nq, sizeof(double) * (size_t)r->P * nq,
                                    hipMemcpyDeviceToDevice, ctx->stream));
         }
-        uint64_t *hoor = (uint64_t *)pinned(ctx, kPinOor, sizeof(uint64_t) * grp.size());
-        for (size_t i = 0; i < grp.size(); i++)
-            DBX_HIP(hipMemcpyAsync(hoor + i, grp[i]->d_oor, 8, hipMemcpyDeviceToHost, ctx->stream));
-        DBX_HIP(hipStreamSynchronize(ctx->stream));
-        for (size_t i = 0; i < grp.size(); i++) {
-            Request *r = grp[i];
-            r->I.assign(hI + i * (size_t)nq, hI + (i + 1) * (size_t)nq);
-            r->oor = hoor[i];
-        }
-        g0 = g1;
     }
+    DBX_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < order.size(); i++) {
+        Request *r = order[i];
+        r->I.assign(hI + I_off[i], hI + I_off[i] + r->q.size());
+        r->oor = hoor[i];
+        std::vector<char>().swap(r->image);  // its upload is complete
+    }
+}
+
+// pair distances of a request: N(N+1)/2
+static double request_pairs(const Request *r)
+{
+    const double n = (double)r->n_atoms;
+    return 0.5 * n * (n + 1);
 }
 
 static void lane_loop(dbx_server *s, int li)
@@ -370,7 +446,19 @@ static void lane_loop(dbx_server *s, int li)
             for (;;) {
                 if (s->stop) return;
                 admit(s);
-                if (!s->admitted_q.empty()) break;
+                if (!s->admitted_q.empty()) {
+                    // batching window: while requests are still being prepared and the step is
+                    // not full, wait for them up to batch_window_ms after the first admission
+                    const double left = s->admitted_q.front()->t_admit + s->cfg.batch_window_ms - now_ms();
+                    double pairs = 0;
+                    for (Request *r : s->admitted_q) pairs += request_pairs(r);
+                    const bool full = (int)s->admitted_q.size() >= s->cfg.max_batch || pairs >= s->cfg.max_step_pairs;
+                    if (left > 0 && !full && !s->preparing.empty()) {
+                        s->cv_sched.wait_for(lk, std::chrono::duration<double, std::milli>(left));
+                        continue;
+                    }
+                    break;
+                }
                 s->cv_sched.wait(lk);
             }
             // the step: admitted requests with the head's bin width, within the caps
@@ -378,8 +466,7 @@ static void lane_loop(dbx_server *s, int li)
             double pairs = 0;
             for (auto it = s->admitted_q.begin(); it != s->admitted_q.end();) {
                 Request *r = *it;
-                const double n = (double)r->n_atoms;
-                const double pr = 0.5 * n * (n + 1);
+                const double pr = request_pairs(r);
                 if (r->rstep != rstep || (!batch.empty() && (pairs + pr > s->cfg.max_step_pairs ||
                                                              (int)batch.size() >= s->cfg.max_batch))) {
                     ++it;
@@ -390,9 +477,18 @@ static void lane_loop(dbx_server *s, int li)
                 r->state = kRunning;
                 it = s->admitted_q.erase(it);
             }
+            // distinct q grids of the step (one contraction each)
+            std::vector<const std::vector<double> *> grids;
+            for (Request *r : batch) {
+                bool seen = false;
+                for (auto *g : grids) seen = seen || *g == r->q;
+                if (!seen) grids.push_back(&r->q);
+            }
             s->st.steps++;
             s->st.max_step_requests = std::max(s->st.max_step_requests, (double)batch.size());
+            s->st.max_step_qgrids = std::max(s->st.max_step_qgrids, (double)grids.size());
             s->st.pairs += pairs;
+            s->running_steps++;
         }
         int code = DBX_OK;
         std::string msg;
@@ -406,6 +502,7 @@ static void lane_loop(dbx_server *s, int li)
             msg = e.what();
         }
         std::lock_guard<std::mutex> lk(s->mu);
+        s->running_steps--;
         const double t = now_ms();
         for (Request *r : batch) {
             r->t_done = t;
@@ -426,6 +523,27 @@ static void lane_loop(dbx_server *s, int li)
         s->cv_sched.notify_all();
     }
 }
+
+// A caller inside an entry point of a server: refused once dbx_server_close has begun;
+// dbx_server_close waits until every admitted caller has left before it frees the server.
+struct ApiCall {
+    dbx_server *s;
+    bool ok;
+    explicit ApiCall(dbx_server *sv) : s(sv)
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        ok = !s->stop;
+        if (ok) s->in_api++;
+    }
+    ~ApiCall()
+    {
+        if (!ok) return;
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (--s->in_api == 0) s->cv_api.notify_all();
+    }
+    ApiCall(const ApiCall &) = delete;
+    ApiCall &operator=(const ApiCall &) = delete;
+};
 
 }  // namespace dbx
 
@@ -492,6 +610,12 @@ int dbx_server_close(dbx_server *s)
     s->cv_prep.notify_all();
     s->cv_sched.notify_all();
     s->cv_done.notify_all();
+    {
+        // callers inside an entry point leave (waiters return DBX_ERR_STATE) before anything
+        // they may touch is freed
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_api.wait(lk, [&] { return s->in_api == 0; });
+    }
     for (auto &t : s->threads) t.join();
     (void)hipSetDevice(s->device);
     for (auto &kv : s->reqs) {
@@ -510,6 +634,8 @@ int dbx_server_close(dbx_server *s)
 const char *dbx_server_last_error(dbx_server *s)
 {
     if (!s) return "null server";
+    ApiCall call(s);
+    if (!call.ok) return "server closing";
     // a copy per calling thread: the lanes and other callers may replace the message
     thread_local std::string msg;
     std::lock_guard<std::mutex> lk(s->mu);
@@ -520,6 +646,8 @@ const char *dbx_server_last_error(dbx_server *s)
 int dbx_submit(dbx_server *s, const dbx_request *q, uint64_t *ticket)
 {
     if (!s || !q || !ticket) return DBX_ERR_ARG;
+    ApiCall call(s);
+    if (!call.ok) return DBX_ERR_STATE;
     if (q->n_groups <= 0 || !q->group_n || !q->params || !q->q || q->nq <= 0 || q->n_bins <= 0 ||
         !(q->rstep > 0) || q->first_n < 0)
         return DBX_ERR_ARG;
@@ -583,6 +711,8 @@ int dbx_submit(dbx_server *s, const dbx_request *q, uint64_t *ticket)
 int dbx_poll(dbx_server *s, uint64_t ticket, int *state)
 {
     if (!s || !state) return DBX_ERR_ARG;
+    ApiCall call(s);
+    if (!call.ok) return DBX_ERR_STATE;
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->reqs.find(ticket);
     if (it == s->reqs.end()) return DBX_ERR_ARG;
@@ -595,16 +725,31 @@ int dbx_poll(dbx_server *s, uint64_t ticket, int *state)
 int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms)
 {
     if (!s) return DBX_ERR_ARG;
+    ApiCall call(s);
+    if (!call.ok) return DBX_ERR_STATE;
     std::unique_lock<std::mutex> lk(s->mu);
     if (s->reqs.find(ticket) == s->reqs.end()) return DBX_ERR_ARG;
     // looked up again at every wake-up: another thread may release the request meanwhile
     Request *r = nullptr;
+    bool stalled = false;
     s->cv_done.wait(lk, [&] {
         auto it = s->reqs.find(ticket);
         r = it == s->reqs.end() ? nullptr : it->second.get();
-        return !r || r->state == kDone || r->state == kFailed || s->stop;
+        if (!r || r->state == kDone || r->state == kFailed || s->stop) return true;
+        if (r->state == kReady) {
+            if (admit(s) > 0) s->cv_sched.notify_all();
+            stalled = admission_stalled(s, r);
+        }
+        return stalled;
     });
     if (!r) return DBX_ERR_ARG;
+    if (stalled) {
+        // nothing runs or waits that could free memory: only a dbx_release can; the request
+        // stays queued and a later wait returns its result
+        s->err = "request " + std::to_string(ticket) +
+                 " waits for device memory held by finished DBX_REQ_KEEP requests: release some first";
+        return DBX_ERR_BUDGET;
+    }
     if (latency_ms) *latency_ms = r->t_done - r->t_submit;
     if (r->state != kDone && r->state != kFailed) return DBX_ERR_STATE;
     if (I_out && !r->I.empty()) std::memcpy(I_out, r->I.data(), sizeof(double) * r->I.size());
@@ -618,6 +763,8 @@ int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms)
 int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params, double *I_out)
 {
     if (!s || !params || !I_out) return DBX_ERR_ARG;
+    ApiCall call(s);
+    if (!call.ok) return DBX_ERR_STATE;
     // the decode lane is held from the lookup to the last use of the request: dbx_release
     // takes it too before it frees one
     std::lock_guard<std::mutex> rl(s->rw_mu);
@@ -676,6 +823,8 @@ int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params,
 int dbx_request_histogram(dbx_server *s, uint64_t ticket, uint64_t *hist_out)
 {
     if (!s || !hist_out) return DBX_ERR_ARG;
+    ApiCall call(s);
+    if (!call.ok) return DBX_ERR_STATE;
     std::lock_guard<std::mutex> rl(s->rw_mu);  // as in dbx_reweight
     Request *r = nullptr;
     {
@@ -699,6 +848,8 @@ int dbx_request_histogram(dbx_server *s, uint64_t ticket, uint64_t *hist_out)
 int dbx_release(dbx_server *s, uint64_t ticket)
 {
     if (!s) return DBX_ERR_ARG;
+    ApiCall call(s);
+    if (!call.ok) return DBX_ERR_STATE;
     std::unique_ptr<Request> r;
     {
         std::lock_guard<std::mutex> lk(s->mu);
@@ -727,6 +878,8 @@ int dbx_release(dbx_server *s, uint64_t ticket)
 int dbx_server_stats_get(dbx_server *s, dbx_server_stats *out)
 {
     if (!s || !out) return DBX_ERR_ARG;
+    ApiCall call(s);
+    if (!call.ok) return DBX_ERR_STATE;
     std::lock_guard<std::mutex> lk(s->mu);
     *out = s->st;
     return DBX_OK;

@@ -183,6 +183,9 @@ typedef struct dbx_server_config {
     int prep_threads;       /* host threads for tiling and planning (0: the host's hardware
                                threads less 4, between 2 and 12) */
     int pass2_mode;         /* DBX_PASS2_MFMA (default) or DBX_PASS2_FP64 */
+    double batch_window_ms; /* a lane that finds admitted requests waits up to this long for
+                               requests still being prepared or admitted, so that a burst of
+                               submissions forms one step (0: start at once) */
 } dbx_server_config;
 typedef struct dbx_request {
     const float *x, *y, *z;           /* positions in subgroup order (copied) */
@@ -203,6 +206,7 @@ typedef struct dbx_server_stats {
     double pairs;                 /* pair distances histogrammed */
     uint64_t resident_bytes, budget_bytes;
     double max_step_requests;     /* most requests in one step */
+    double max_step_qgrids;       /* most distinct q grids in one step */
 } dbx_server_stats;
 int dbx_server_open(int device, const dbx_server_config *cfg, dbx_server **out);
 int dbx_server_close(dbx_server *s);

@@ -252,3 +252,125 @@ def test_serve_poisson_stream_small(ctx):
         st = srv.stats()
         print(f"p50 {np.percentile(lats, 50):.2f} ms p99 {np.percentile(lats, 99):.2f} ms steps {st['steps']}")
         assert st["completed"] == len(models) and st["resident_bytes"] == 0
+
+
+# Requests of the mixed serving stream (synth.serve_request): every q grid (2048, 4096, 8192
+# points), 1-4 species, one of 121k atoms and two below 1k.
+C5_REQUESTS = (7, 6, 13, 1, 2, 3, 15, 18, 0, 57)
+
+
+@pytest.fixture(scope="module")
+def c5_reference(ctx):
+    """The C5 requests, each run alone through dbx_intensity, and the oracle's histograms and
+    FP64 intensities of every one."""
+    models = [synth.serve_request(i) for i in C5_REQUESTS]
+    assert {len(m.q) for m in models} == {2048, 4096, 8192}
+    assert {len(m.group_n) for m in models} == {1, 2, 3, 4}
+    assert max(m.n_atoms for m in models) >= 100000 and min(m.n_atoms for m in models) <= 1000
+    alone = [_alone(ctx, m) for m in models]
+    oracle = []
+    for m, (_, H) in zip(models, alone):
+        print(f"C5 request {m.name}: {m.n_atoms} atoms, {len(m.group_n)} species, {len(m.q)} Q", flush=True)
+        ref, ov = _oracle_hist(m)
+        assert ov == 0
+        oracle.append((ref, _oracle_I(m, ref)))
+    return models, alone, oracle
+
+
+def _check_c5(srv, models, tickets, alone, oracle):
+    worst = 0.0
+    for m, t, (Ia, Ha), (Ho, Io) in zip(models, tickets, alone, oracle):
+        I, lat, rc = _finish(srv, t, timeout=120.0)
+        assert rc == cd.DBX_OK and lat > 0
+        H = srv.histogram(t)
+        assert np.array_equal(H, Ha), m.name
+        assert np.array_equal(H, Ho), m.name
+        assert _rel(I, Ia) < MIX_TOL, m.name
+        worst = max(worst, _rel(I, Io))
+        srv.release(t)
+    assert worst < ORACLE_TOL, worst
+    return worst
+
+
+def test_serve_c5_requests_in_one_step(c5_reference):
+    """The C5 requests submitted at once form one step (a batching window covers their host
+    preparation): one pass-1 launch over all of them and one contraction per q grid, three grids
+    in the step.  Each result: histograms bit-exact against the request alone and the oracle, I
+    within 1e-9 of the request alone and 1e-6 of the oracle's FP64 sums."""
+    models, alone, oracle = c5_reference
+    with cd.Server(0, batch_window_ms=5000.0) as srv:
+        ts = [_submit(srv, m) for m in models]
+        worst = _check_c5(srv, models, ts, alone, oracle)
+        st = srv.stats()
+        print(f"steps {st['steps']} requests/step {st['max_step_requests']} q grids/step "
+              f"{st['max_step_qgrids']} worst vs oracle {worst:.2e}")
+        assert st["steps"] == 1 and st["max_step_requests"] == len(models) and st["max_step_qgrids"] == 3
+        assert st["resident_bytes"] == 0
+
+
+def test_serve_c5_requests_arriving_over_time(c5_reference):
+    """The same requests arriving one by one from another thread, largest first, into a server
+    that starts steps at once: whatever steps they fall into, each result is the same as alone."""
+    models, alone, oracle = c5_reference
+    order = sorted(range(len(models)), key=lambda i: -models[i].n_atoms)
+    with cd.Server(0) as srv:
+        ts = [None] * len(models)
+
+        def feed():
+            for i in order:
+                ts[i] = _submit(srv, models[i])
+                time.sleep(0.003)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        _check_c5(srv, models, ts, alone, oracle)
+        print(f"steps {srv.stats()['steps']}")
+
+
+def test_serve_kept_requests_over_budget_do_not_block_wait():
+    """More DBX_REQ_KEEP requests than the budget holds, waited on in order without releasing:
+    once nothing runs that could free memory, dbx_wait on a request still waiting for memory
+    returns DBX_ERR_BUDGET instead of blocking; releasing a kept request admits it and a later
+    wait returns its result."""
+    models = [synth.small_random(1500, 2, 600 + i, spread=10.0, nq=64) for i in range(4)]
+    with cd.Server(0) as srv:
+        t = _submit(srv, models[0])
+        _finish(srv, t)
+        size = srv.stats()["resident_bytes"]
+        srv.release(t)
+    with cd.Server(0, hbm_budget=int(2.5 * size)) as srv:
+        ts = [_submit(srv, m) for m in models]
+        rcs = []
+        for t in ts:
+            _, _, rc = srv.wait(t, allow=(cd.DBX_ERR_BUDGET,))
+            rcs.append(rc)
+        assert rcs[:2] == [cd.DBX_OK] * 2 and rcs[2:] == [cd.DBX_ERR_BUDGET] * 2, rcs
+        assert [srv.poll(t) for t in ts[2:]] == [cd.DBX_REQ_QUEUED] * 2
+        srv.release(ts[0])
+        srv.release(ts[1])
+        for t in ts[2:]:
+            _, _, rc = _finish(srv, t)
+            assert rc == cd.DBX_OK
+            srv.release(t)
+        assert srv.stats()["resident_bytes"] == 0
+
+
+def test_serve_close_while_another_thread_waits():
+    """dbx_server_close while another thread is blocked in dbx_wait: the waiter returns (the
+    result, or DBX_ERR_STATE once the server stops) and the close completes."""
+    m = synth.serve_request(6)
+    srv = cd.Server(0)
+    t = _submit(srv, m)
+    out = []
+
+    def waiter():
+        _, _, rc = srv.wait(t, allow=(cd.DBX_ERR_STATE,))
+        out.append(rc)
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    time.sleep(0.001)
+    srv.close()
+    th.join(timeout=60)
+    assert not th.is_alive() and out and out[0] in (cd.DBX_OK, cd.DBX_ERR_STATE)
