@@ -116,6 +116,7 @@ def lib():
             "dbx_histogram": (i32, [vp, vp, f64, i64, vp, i32, P(u64)]),
             "dbx_prefactor": (i32, [vp, vp, vp, vp, i32, vp]),
             "dbx_debye": (i32, [vp, vp, i32, i32, i64, f64, i32, vp, i32, vp, vp, vp]),
+            "dbx_debye_noalloc": (i32, [vp, vp, i32, i64, f64, i32, vp, i32, vp, vp, vp]),
             "dbx_intensity": (i32, [vp, vp, vp, f64, i64, i32, vp, i32, vp, vp]),
             "dbx_run_deb": (i32, [C.c_char_p, i32]),
             "dbx_run_folder": (i32, [C.c_char_p, i32]),
@@ -158,7 +159,7 @@ EXPORTS = ("dbx_open", "dbx_close", "dbx_last_error", "dbx_stream", "dbx_q_from_
            "dbx_batch_intensity", "dbx_server_open", "dbx_server_close", "dbx_server_last_error",
            "dbx_submit", "dbx_poll", "dbx_wait", "dbx_reweight", "dbx_request_histogram",
            "dbx_release", "dbx_server_stats_get", "dbx_read_debx", "dbx_run_folder",
-           "dbx_deb_header_defaults", "dbx_xyz_to_deb", "dbx_average_debi")
+           "dbx_deb_header_defaults", "dbx_xyz_to_deb", "dbx_average_debi", "dbx_debye_noalloc")
 
 
 def _p(a):

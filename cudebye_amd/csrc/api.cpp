@@ -69,28 +69,6 @@ const double *device_q(dbx_ctx *ctx, const double *q, int nq)
     return d;
 }
 
-template <class F>
-static int guarded(dbx_ctx *ctx, F &&f)
-{
-    if (!ctx) return DBX_ERR_ARG;
-    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
-    try {
-        DBX_HIP(hipSetDevice(ctx->device));
-        f();
-        ctx->err.clear();
-        return DBX_OK;
-    } catch (const Error &e) {
-        ctx->err = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        ctx->err = "host allocation failed";
-        return DBX_ERR_NOMEM;
-    } catch (const std::exception &e) {
-        ctx->err = e.what();
-        return DBX_ERR_ARG;
-    }
-}
-
 // ---------------------------------------------------------------------------------
 // spatial tiling of a model (host)
 // ---------------------------------------------------------------------------------
@@ -422,7 +400,15 @@ int dbx_open(int device, size_t hbm_budget, dbx_ctx **out)
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->counters, 64) != hipSuccess ||
-        hipHostMalloc((void **)&c->counters_h, 64, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&c->counters_h, 64, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->p2_readback_h, 64, hipHostMallocDefault) != hipSuccess) {
+        dbx_close(c);
+        return DBX_ERR_HIP;
+    }
+    // the allocation-free pass 2's scratch region on this device (device current)
+    try {
+        noalloc_init(device);
+    } catch (const Error &) {
         dbx_close(c);
         return DBX_ERR_HIP;
     }
@@ -453,6 +439,7 @@ int dbx_close(dbx_ctx *ctx)
     for (int i = 0; i < 2; i++)
         if (ctx->ev_mf[i]) (void)hipEventDestroy(ctx->ev_mf[i]);
     if (ctx->counters_h) (void)hipHostFree(ctx->counters_h);
+    if (ctx->p2_readback_h) (void)hipHostFree(ctx->p2_readback_h);
     for (int i = 0; i < 4; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <map>
+#include <new>
 #include <mutex>
 #include <set>
 #include <stdexcept>
@@ -187,6 +188,7 @@ struct dbx_ctx {
     bool p2_timed = false;
     int pass2_mode = DBX_PASS2_MFMA;  // DBX_PASS2_MFMA or DBX_PASS2_FP64
     int last_p2_bins = 0;             // compacted bins of the last pass 2
+    uint64_t *p2_readback_h = nullptr;  // pinned host words for pass 2's read-backs, from dbx_open
     // profiling of the matrix-core contraction (dbx_debug_set_profiling)
     bool profiling = false;
     hipEvent_t ev_mf[2] = {};
@@ -277,9 +279,22 @@ void prefactor_dev(dbx_ctx *ctx, const dbx_group_params &ga, const dbx_group_par
 // ff[r][q] for every row r
 void prefactor_rows(dbx_ctx *ctx, const std::vector<GroupPair> &rows, const double *d_q, int nq,
                     double *d_ff);
+// Where pass 2's device buffers and its read-back words come from: the context's grow-only
+// slots (ctx_pass2_mem), or a fixed device region (the allocation-free entry point).  dev()
+// returns a buffer of at least `bytes` that no other dev() call of the same pass 2 overlaps.
+struct P2Mem {
+    virtual ~P2Mem() = default;
+    virtual void *dev(int slot, size_t bytes) = 0;
+    virtual unsigned long long *readback() = 0;  // two words of pinned host memory
+};
+P2Mem &ctx_pass2_mem(dbx_ctx *ctx);
 // S[r][q] for every row r (rows of any models sharing rstep, first_n and the q grid)
 void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::vector<RowDesc> &rows,
                           double rstep, const double *d_q, int nq, double *d_S);
+// S[p][q] for the P partials of one model, d_hist[P][n_bins], buffers from mem (the row
+// descriptors are written on the device: no host staging)
+void debye_structure_mem(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
+                         int first_n, const double *d_q, int nq, double *d_S);
 // I[mrows[m].z][q] = sum over rows [mrows[m].x, mrows[m].x + mrows[m].y) of ff S
 void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const std::vector<int4> &mrows,
                     int nq, double *d_I);
@@ -288,9 +303,39 @@ void debye_structure(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins
                      int first_n, const double *d_q, int nq, double *d_S);
 // I[q] = sum_p ff[p][q] S[p][q]
 void launch_prefactor(hipStream_t st, const GroupPair *d_pairs, int n, const double *d_q, int nq, double *d_ff);
+// d_mrows null: one model, rows m0.x .. m0.x + m0.y - 1 into I[m0.z]
 void launch_combine(hipStream_t st, const double *d_ff, const double *d_S, const int4 *d_mrows, int nm, int nq,
-                    double *d_I);
+                    double *d_I, int4 m0 = make_int4(0, 0, 0, 0));
 void combine_intensity(dbx_ctx *ctx, const double *d_ff, const double *d_S, int P, int nq,
                        double *d_I);
+
+
+// An entry point's body under the context's lock, with its device current: C++ errors become
+// the returned status and the context's message (no exception crosses the C ABI).
+template <class F>
+int guarded(dbx_ctx *ctx, F &&f)
+{
+    if (!ctx) return DBX_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    try {
+        DBX_HIP(hipSetDevice(ctx->device));
+        f();
+        ctx->err.clear();
+        return DBX_OK;
+    } catch (const Error &e) {
+        ctx->err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        ctx->err = "host allocation failed";
+        return DBX_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        ctx->err = e.what();
+        return DBX_ERR_ARG;
+    }
+}
+
+// the allocation-free pass 2 (noalloc.hip): the device address of this device's fixed scratch
+// region, looked up once per device (called by dbx_open with the device current)
+void noalloc_init(int device);
 
 }  // namespace dbx

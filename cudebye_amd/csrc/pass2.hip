@@ -790,18 +790,21 @@ static int self_bin_max(double rstep)
     return k;
 }
 
-void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::vector<RowDesc> &rows,
-                          double rstep, const double *d_q, int nq, double *d_S)
+// row r of a single model's P partials: its histogram at r * n_bins, all of equal length
+__global__ void p2_rows_fill_kernel(RowDesc *__restrict__ rows, int n_rows, int64_t n_bins, int first_n)
 {
-    const int n_rows = (int)rows.size();
-    DBX_REQUIRE(n_rows > 0 && nq > 0, DBX_ERR_ARG, "empty pass-2 problem");
-    for (int r = 0; r < n_rows; r++) {
-        DBX_REQUIRE(rows[r].n_bins > 0, DBX_ERR_ARG, "pass-2 row without bins");
-        DBX_REQUIRE(r == 0 || rows[r].n_bins <= rows[r - 1].n_bins, DBX_ERR_ARG,
-                    "pass-2 rows must be ordered by length, longest first");
-    }
-    DBX_REQUIRE(rows[0].n_bins < (int64_t)INT32_MAX, DBX_ERR_ARG, "too many bins");
-    const int64_t nb_max = rows[0].n_bins;
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n_rows) rows[r] = RowDesc{(int64_t)r * n_bins, n_bins, first_n, 0};
+}
+
+// The pass-2 structure sums S[r][q] of n_rows rows (longest first; nb_max = the first row's
+// length), buffers from mem.  h_rows: the row descriptors on the host (uploaded through pinned
+// memory), or null for the P equal rows of one model (nb_max bins each, uni_first_n), written
+// on the device.
+static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base, const RowDesc *h_rows,
+                           int n_rows, int64_t nb_max, int uni_first_n, double rstep, const double *d_q, int nq,
+                           double *d_S)
+{
     const int ksm = self_bin_max(rstep);
     const int nlow = ksm + 1;
     const int64_t nblk = (nb_max + kScanBlock - 1) / kScanBlock;
@@ -819,7 +822,7 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     const size_t o_tot = take(sizeof(unsigned long long) * 2);
     const size_t o_rsum = take(sizeof(unsigned long long) * n_rows);
     const size_t o_flags = take((size_t)(nblk * kScanBlock));
-    char *base = (char *)scratch(ctx, kSlotPass2, off);
+    char *base = (char *)mem.dev(kSlotPass2, off);
     RowDesc *d_rows = (RowDesc *)(base + o_rows);
     int *bc = (int *)(base + o_bc);
     unsigned long long *tot = (unsigned long long *)(base + o_tot);  // [0] Kc, [1] max count
@@ -831,10 +834,15 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     unsigned char *flags = (unsigned char *)(base + o_flags);
     const unsigned long long *h = (const unsigned long long *)d_hist_base;
 
-    RowDesc *hrows = (RowDesc *)pinned(ctx, kPinRows, sizeof(RowDesc) * n_rows);
-    std::copy(rows.begin(), rows.end(), hrows);
     DBX_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
-    DBX_HIP(hipMemcpyAsync(d_rows, hrows, sizeof(RowDesc) * n_rows, hipMemcpyHostToDevice, ctx->stream));
+    if (h_rows) {
+        RowDesc *hrows = (RowDesc *)pinned(ctx, kPinRows, sizeof(RowDesc) * n_rows);
+        std::copy(h_rows, h_rows + n_rows, hrows);
+        DBX_HIP(hipMemcpyAsync(d_rows, hrows, sizeof(RowDesc) * n_rows, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        hipLaunchKernelGGL(p2_rows_fill_kernel, dim3((n_rows + 63) / 64), dim3(64), 0, ctx->stream, d_rows, n_rows,
+                           nb_max, uni_first_n);
+    }
     const int64_t zwords = (int64_t)(off - o_tot) / 4;
     hipLaunchKernelGGL(p2_zero_kernel, dim3((unsigned)((zwords + 255) / 256)), dim3(256), 0, ctx->stream,
                        (unsigned int *)tot, zwords);
@@ -846,7 +854,7 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
                        bc, kz);
     hipLaunchKernelGGL(p2_self_kernel, dim3((n_rows + 63) / 64), dim3(64), 0, ctx->stream, h, d_rows, n_rows,
                        ksm, self, low);
-    unsigned long long *hv = (unsigned long long *)pinned(ctx, kPinMisc, sizeof(unsigned long long) * 2);
+    unsigned long long *hv = mem.readback();
     DBX_HIP(hipMemcpyAsync(hv, tot, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, ctx->stream));
     DBX_HIP(hipStreamSynchronize(ctx->stream));
     const int Kc = (int)(hv[0] & 0xFFFFFFFFull);
@@ -858,8 +866,8 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
         const int kper = (Kc + nsplit - 1) / nsplit;
         nsplit = (Kc + kper - 1) / kper;
         const int npass = std::min(n_rows, kDsPmax);
-        double *A = (double *)scratch(ctx, kSlotDigits, sizeof(double) * (size_t)npass * Kc);
-        double *Spart = (double *)scratch(ctx, kSlotPass2b, sizeof(double) * (size_t)nsplit * npass * nq);
+        double *A = (double *)mem.dev(kSlotDigits, sizeof(double) * (size_t)npass * Kc);
+        double *Spart = (double *)mem.dev(kSlotPass2b, sizeof(double) * (size_t)nsplit * npass * nq);
         for (int r0 = 0; r0 < n_rows; r0 += kDsPmax) {
             const int np = std::min(kDsPmax, n_rows - r0);
             hipLaunchKernelGGL(p2_weights_kernel, dim3((Kc + 255) / 256), dim3(256), 0, ctx->stream, h, d_rows,
@@ -878,7 +886,7 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
                     "bin count too large for the matrix-core Debye sum");
         const int Kc_pad = (Kc + kMfK - 1) & ~(kMfK - 1);
         const int ksteps = Kc_pad / kMfK;
-        double *rtab = (double *)scratch(ctx, kSlotRtab, sizeof(double) * 2 * (size_t)Kc_pad);
+        double *rtab = (double *)mem.dev(kSlotRtab, sizeof(double) * 2 * (size_t)Kc_pad);
         const MfShape sh = mf_shape(n_rows, I);
         const int rows_pad = (n_rows + sh.rows() - 1) / sh.rows() * sh.rows();
         const int qgroups = (nq + sh.qs() - 1) / sh.qs();
@@ -888,8 +896,8 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
         int nsplit = std::max(1, std::min((target + qgroups * rgroups - 1) / (qgroups * rgroups), ksteps / 8));
         const int kper = ((ksteps + nsplit - 1) / nsplit) * kMfK;
         nsplit = (Kc_pad + kper - 1) / kper;
-        signed char *cdig = (signed char *)scratch(ctx, kSlotDigits, (size_t)I * rows_pad * Kc_pad);
-        double *Spart = (double *)scratch(ctx, kSlotPass2b, sizeof(double) * (size_t)nsplit * n_rows * nq);
+        signed char *cdig = (signed char *)mem.dev(kSlotDigits, (size_t)I * rows_pad * Kc_pad);
+        double *Spart = (double *)mem.dev(kSlotPass2b, sizeof(double) * (size_t)nsplit * n_rows * nq);
         const int nblocks = rows_pad / 16;
         const int cd_x = (Kc_pad + 4 * kCdThreads - 1) / (4 * kCdThreads);
         const int64_t n_in = (int64_t)cd_x * rows_pad + (Kc_pad + kCdThreads - 1) / kCdThreads +
@@ -940,12 +948,52 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     ctx->p2_timed = true;
 }
 
+// the context's grow-only slots and pinned words
+namespace {
+struct CtxMem : P2Mem {
+    dbx_ctx *ctx = nullptr;
+    void *dev(int slot, size_t bytes) override { return scratch(ctx, slot, bytes); }
+    unsigned long long *readback() override
+    {
+        return (unsigned long long *)pinned(ctx, kPinMisc, sizeof(unsigned long long) * 2);
+    }
+};
+}  // namespace
+
+P2Mem &ctx_pass2_mem(dbx_ctx *ctx)
+{
+    thread_local CtxMem m;
+    m.ctx = ctx;
+    return m;
+}
+
+void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::vector<RowDesc> &rows,
+                          double rstep, const double *d_q, int nq, double *d_S)
+{
+    const int n_rows = (int)rows.size();
+    DBX_REQUIRE(n_rows > 0 && nq > 0, DBX_ERR_ARG, "empty pass-2 problem");
+    for (int r = 0; r < n_rows; r++) {
+        DBX_REQUIRE(rows[r].n_bins > 0, DBX_ERR_ARG, "pass-2 row without bins");
+        DBX_REQUIRE(r == 0 || rows[r].n_bins <= rows[r - 1].n_bins, DBX_ERR_ARG,
+                    "pass-2 rows must be ordered by length, longest first");
+    }
+    DBX_REQUIRE(rows[0].n_bins < (int64_t)INT32_MAX, DBX_ERR_ARG, "too many bins");
+    structure_core(ctx, ctx_pass2_mem(ctx), d_hist_base, rows.data(), n_rows, rows[0].n_bins, 0, rstep, d_q, nq,
+                   d_S);
+}
+
+void debye_structure_mem(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
+                         int first_n, const double *d_q, int nq, double *d_S)
+{
+    DBX_REQUIRE(P > 0 && nq > 0 && n_bins > 0, DBX_ERR_ARG, "empty pass-2 problem");
+    DBX_REQUIRE(n_bins < (int64_t)INT32_MAX, DBX_ERR_ARG, "too many bins");
+    structure_core(ctx, mem, d_hist, nullptr, P, n_bins, first_n, rstep, d_q, nq, d_S);
+}
+
 void debye_structure(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
                      int first_n, const double *d_q, int nq, double *d_S)
 {
-    std::vector<RowDesc> rows(P);
-    for (int p = 0; p < P; p++) rows[p] = RowDesc{(int64_t)p * n_bins, n_bins, first_n, 0};
-    debye_structure_rows(ctx, d_hist, rows, rstep, d_q, nq, d_S);
+    debye_structure_mem(ctx, ctx_pass2_mem(ctx), d_hist, P, n_bins, rstep, first_n, d_q, nq, d_S);
 }
 
 void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const std::vector<int4> &mrows,
@@ -981,10 +1029,10 @@ void launch_prefactor(hipStream_t st, const GroupPair *d_pairs, int n, const dou
 }
 
 void launch_combine(hipStream_t st, const double *d_ff, const double *d_S, const int4 *d_mrows, int nm, int nq,
-                    double *d_I)
+                    double *d_I, int4 m0)
 {
-    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, st, d_ff, d_S, d_mrows,
-                       make_int4(0, 0, 0, 0), nq, d_I);
+    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, st, d_ff, d_S, d_mrows, m0,
+                       nq, d_I);
     DBX_HIP(hipGetLastError());
 }
 

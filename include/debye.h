@@ -121,6 +121,18 @@ int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int
               double rstep, int first_n, const double *q, int nq, const double *ffoobb,
               double *I_out, double *S_out);
 
+/* dbx_debye_noalloc: dbx_debye on device memory that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device or pinned memory during the call
+ * (tight serving loops, harnesses that forbid hipMalloc).  d_hist [P][n_bins], d_q [nq],
+ * d_ffoobb [P][nq] and d_I [nq] (and d_S [P][nq], optional) are device pointers; the result
+ * equals dbx_debye's on the same inputs.  Every buffer pass 2 uses comes from one fixed 64 MiB
+ * device region per device, set up by dbx_open; a problem whose buffers exceed it returns
+ * DBX_ERR_BUDGET (dbx_debye has no such limit).  The region is shared by every call on a
+ * device and keeps the last call's contents: calls that overlap on one device (two contexts
+ * or threads at once) race.  The call waits for its own work before returning. */
+int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep, int first_n,
+                      const double *d_q, int nq, const double *d_ffoobb, double *d_I, double *d_S);
+
 /* Pass-2 arithmetic of a context (default DBX_PASS2_MFMA):
  *   DBX_PASS2_MFMA: matrix-core contraction on signed 8-bit digits with exact 32-bit integer
  *     accumulation; sinc is evaluated in FP64, shifted by 3/8 and rounded to a multiple of
