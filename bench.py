@@ -1,6 +1,6 @@
 """Benchmark of the Debye hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-streams]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-streams] [--no-workloads]
 
 Primary line (`value`): one step = one structure model through the whole hot path on the
 device: pass 1 (all subgroup-pair distance histograms, uint64), compaction, prefactors and
@@ -25,6 +25,11 @@ against the plain path (one model per call: model creation + dbx_intensity, sequ
 arrivals at a fixed rate (80% of the server's capacity measured on the same requests), server
 against the plain path fed the same arrival times: models/s and p50/p99 latency (submission to
 result).
+`workloads`: the other configurations, each timed on its own: the full rocksalt-like sphere
+(6 partials, 0.0005 A bins, 8192 Q) with its matrix-core I(Q) against FP64 sums, the primary
+sphere at the reference's 1e-4 A bins, the primary model cold (tiling, upload and planning
+inside the timing), and the reference's own Gamma example through the `debye` program against
+the reference's logged times.
 """
 import argparse
 import json
@@ -65,7 +70,9 @@ def primary(args, cd, synth, np):
             p1_ms.append(a)
             p2_ms.append(b)
         elapsed = time.perf_counter() - t0  # intensity() synchronises the stream on return
-        I, H = ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n, with_hist=True)
+        I, H, bound = ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n, with_hist=True,
+                                    with_err=True)
+        refined = ctx.pass2_refined()
         n_jobs, misses = ctx.pass1_stats()
         ctx.set_profiling(True)
         ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n)
@@ -142,7 +149,9 @@ def primary(args, cd, synth, np):
                     "(FP64 VALU), not by the matrix cores; see the batched stream for many rows",
         },
     }
-    return out, m, I, H, pairs
+    out["error_bound"] = {"max_rel_err_bound_I": float(bound.max()), "pass2_refined_models": refined,
+                          "target": 1e-6}
+    return out, m, (I, bound), H, pairs
 
 
 def stream_c4(cd, synth, np):
@@ -152,16 +161,30 @@ def stream_c4(cd, synth, np):
     with cd.Context(0) as ctx:
         items = [(ctx.model(m.x, m.y, m.z, m.group_n), pa, m.n_bins, m.first_n) for m, pa in zip(models, params)]
         ctx.batch_intensity(items, models[0].rstep, models[0].q)
-        ctx.set_profiling(True)
         t0 = time.perf_counter()
         ctx.batch_intensity(items, models[0].rstep, models[0].q)
         t_batch = time.perf_counter() - t0
         p1_b, p2_b = ctx.kernel_times()
+        ctx.set_profiling(True)
+        _, batch_bound = ctx.batch_intensity(items, models[0].rstep, models[0].q, with_err=True)
+        batch_refined = ctx.pass2_refined()
         mf_ms, mf_flops, mf_useful, kc, digits = ctx.pass2_profile()
+        ctx.set_profiling(False)
+        # the same resident models, one dbx_intensity call each (plans already built): the
+        # batching gain without host tiling on either side
+        for m, it in zip(models, items):
+            ctx.intensity(it[0], it[1], m.rstep, it[2], m.q, first_n=it[3])
+        t0 = time.perf_counter()
+        for m, it in zip(models, items):
+            ctx.intensity(it[0], it[1], m.rstep, it[2], m.q, first_n=it[3])
+        t_seq = time.perf_counter() - t0
         for it in items:
             it[0].close()
     batch_call = {
         "models_per_s": round(len(models) / t_batch, 1),
+        "resident_one_call_per_model_models_per_s": round(len(models) / t_seq, 1),
+        "gpu_only_ratio": round(t_seq / t_batch, 2),
+        "max_rel_err_bound_I": float(batch_bound.max()), "pass2_refined_models": batch_refined,
         "pass1_kernel_ms": round(p1_b, 3),
         "pass2_ms": round(p2_b, 3),
         "pass2_roofline": {"bound": "mfma", "kernel": "p2_debye_mfma_kernel", "kernel_ms": round(mf_ms, 3),
@@ -184,24 +207,32 @@ def stream_c4(cd, synth, np):
                 I_plain.append(ctx.intensity(mod, pa, m.rstep, m.n_bins, m.q, first_n=m.first_n))
                 mod.close()
             t_plain = time.perf_counter() - t0
+    # server: the median of three timed passes (its steps depend on when the host threads finish
+    # preparing each request, which varies from pass to pass)
+    passes = []
     with cd.Server(0) as srv:
-        for rep in range(2):
+        for rep in range(4):
             steps0 = srv.stats()["steps"]
             t0 = time.perf_counter()
             ts = [srv.submit(m.x, m.y, m.z, m.group_n, pa, m.rstep, m.n_bins, m.q, first_n=m.first_n)
                   for m, pa in zip(models, params)]
             res = [srv.wait(t) for t in ts]
-            t_srv = time.perf_counter() - t0
+            t_pass = time.perf_counter() - t0
             steps = srv.stats()["steps"] - steps0
             for t in ts:
                 srv.release(t)
+            if rep:
+                passes.append((t_pass, res, steps))
+    passes.sort(key=lambda p: p[0])
+    t_srv, res, steps = passes[len(passes) // 2]
     lats = np.array([r[1] for r in res])
     rel = max(float(np.max(np.abs(r[0] - ip) / np.abs(ip))) for r, ip in zip(res, I_plain))
     n = len(models)
     return {
         "workload": "256 two-species models (seeded FCC/HCP spheres and ellipsoids, stacking faults p=0.1, "
                     "sigma 0.01-0.08 A, 1k-60k atoms log-uniform), bin 0.001 A, 4096 Q; all submitted at once; "
-                    "each path timed on its second pass over the stream (steady state)",
+                    "after one untimed pass of each path (steady state): plain timed once, server the median "
+                    "of three passes",
         "pairs": float(sum(m.pair_count for m in models)),
         "plain_models_per_s": round(n / t_plain, 1),
         "server_models_per_s": round(n / t_srv, 1),
@@ -244,13 +275,13 @@ def stream_c5(cd, synth, np):
         ctx.intensity(mod, params[0], m0.rstep, m0.n_bins, m0.q, first_n=m0.first_n)
         mod.close()
         t0 = time.perf_counter()
-        lat_plain = []
+        lat_plain, I_plain = [], []
         for m, pa, a in zip(models, params, arrivals):
             wait = t0 + a - time.perf_counter()
             if wait > 0:
                 time.sleep(wait)
             mod = ctx.model(m.x, m.y, m.z, m.group_n)
-            ctx.intensity(mod, pa, m.rstep, m.n_bins, m.q, first_n=m.first_n)
+            I_plain.append(ctx.intensity(mod, pa, m.rstep, m.n_bins, m.q, first_n=m.first_n))
             mod.close()
             lat_plain.append((time.perf_counter() - t0 - a) * 1e3)
         t_plain = time.perf_counter() - t0
@@ -271,8 +302,9 @@ def stream_c5(cd, synth, np):
         th = threading.Thread(target=feed)
         th.start()
         th.join()
-        lat_srv = [srv.wait(t)[1] for t in ts]
+        res = [srv.wait(t, with_err=True) for t in ts]
         t_srv = time.perf_counter() - t0
+        lat_srv = [r[1] for r in res]
         st = srv.stats()
         for t in ts:
             srv.release(t)
@@ -291,10 +323,140 @@ def stream_c5(cd, synth, np):
                   "p50_ms": round(float(np.percentile(lat_plain, 50)), 2),
                   "p99_ms": round(float(np.percentile(lat_plain, 99)), 2)},
         "ratio": round(t_plain / t_srv, 2),
+        "max_rel_I_server_vs_plain": max(float(np.max(np.abs(r[0] - ip) / np.abs(ip)))
+                                         for r, ip in zip(res, I_plain)),
+        "max_rel_err_bound_I": max(float(r[3].max()) for r in res),
+        "max_step_qgrids": int(st["max_step_qgrids"]),
     }
 
 
-def cpu_leg(out, m, I, H, pairs, np):
+def _resident_steps(cd, ctx, m, params, reps=3):
+    """(median wall ms, median pass-1 kernel ms, median pass-2 ms, last (I, bound)) of `reps` steps of a
+    resident model after one warm-up step (which builds its pass-1 plan)."""
+    import numpy as np
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n)
+    wall, p1, p2 = [], [], []
+    for _ in range(reps):
+        t = time.perf_counter()
+        I = ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n, with_err=True)
+        wall.append((time.perf_counter() - t) * 1e3)
+        a, b = ctx.kernel_times()
+        p1.append(a)
+        p2.append(b)
+    mod.close()
+    return float(np.median(wall)), float(np.median(p1)), float(np.median(p2)), I
+
+
+def workload_legs(cd, synth, np):
+    """The other configurations, each timed on its own (not part of `value`)."""
+    out = {}
+    # the three-species rocksalt-like sphere at full size, and its I(Q) on the matrix cores
+    # against FP64 arithmetic on the same histograms
+    m = synth.rocksalt_sphere()
+    params = [_gp(cd, p) for p in m.params]
+    with cd.Context(0) as ctx:
+        wall, p1, p2, (I, bound) = _resident_steps(cd, ctx, m, params)
+        refined = ctx.pass2_refined()
+        mod = ctx.model(m.x, m.y, m.z, m.group_n)
+        # the matrix-core sums alone (no FP64 recomputation) and FP64 sums, each with its bound
+        ctx.set_rel_err_target(0)
+        Imf, bmf = ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n, with_err=True)
+        ctx.set_pass2_mode(cd.DBX_PASS2_FP64)
+        I64, b64 = ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n, with_err=True)
+        mod.close()
+    out["rocksalt_sphere_20nm"] = {
+        "workload": "rocksalt-like sphere D=20 nm (a=4.2 A, sigma 0), species 0.5/0.3/0.2 -> 6 partials, "
+                    "bin 0.0005 A, 8192 Q uniform in 0.3-20 A^-1",
+        "atoms": m.n_atoms, "pairs_per_step": m.pair_count, "bins": m.n_bins,
+        "ms_per_step": round(wall, 2), "gpairs_per_s": round(m.pair_count / wall / 1e6, 1),
+        "pass1_kernel_ms": round(p1, 2), "pass1_gpairs_per_s": round(m.pair_count / p1 / 1e6, 1),
+        "pass2_ms": round(p2, 3),
+        "max_rel_err_bound_I": float(bound.max()), "pass2_refined_models": refined,
+        "matrix_cores_alone": {"max_rel_I_vs_fp64": float(np.max(np.abs(Imf - I64) / np.abs(I64))),
+                               "max_rel_err_bound_I": float(bmf.max())},
+        "fp64_max_rel_err_bound_I": float(b64.max()),
+    }
+    # the spinel-like sphere at the reference's default bin width (1e-4 A: 1.06M bins)
+    m = synth.spinel_sphere(rstep=1e-4)
+    params = [_gp(cd, p) for p in m.params]
+    with cd.Context(0) as ctx:
+        wall, p1, p2, _ = _resident_steps(cd, ctx, m, params)
+    out["spinel_sphere_1e-4"] = {
+        "workload": "the primary spinel-like sphere at bin 1e-4 A (the reference's default), 4096 Q",
+        "bins": m.n_bins, "ms_per_step": round(wall, 2), "gpairs_per_s": round(m.pair_count / wall / 1e6, 1),
+        "pass1_kernel_ms": round(p1, 2), "pass1_gpairs_per_s": round(m.pair_count / p1 / 1e6, 1),
+        "pass2_ms": round(p2, 3),
+    }
+    # the primary model cold: host tiling, upload, pass-1 planning and the step, per model
+    m = synth.spinel_sphere()
+    params = [_gp(cd, p) for p in m.params]
+    with cd.Context(0) as ctx:
+        walls, creates = [], []
+        for rep in range(4):
+            t = time.perf_counter()
+            mod = ctx.model(m.x, m.y, m.z, m.group_n)
+            t1 = time.perf_counter()
+            ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n)
+            t2 = time.perf_counter()
+            mod.close()
+            if rep:
+                walls.append((t2 - t) * 1e3)
+                creates.append((t1 - t) * 1e3)
+    w = float(np.median(walls))
+    out["spinel_sphere_cold"] = {
+        "workload": "the primary model not resident: dbx_model_create (host tiling + upload) + dbx_intensity "
+                    "(pass-1 plan built on first use) per step",
+        "ms_per_step": round(w, 2), "model_create_ms": round(float(np.median(creates)), 2),
+        "gpairs_per_s": round(m.pair_count / w / 1e6, 1),
+    }
+    out["gamma_example_cli"] = _gamma_cli(cd)
+    return out
+
+
+def _gamma_cli(cd):
+    """The reference's own example (Data/GammaAl2O3_10nm.deb: 117,936 atoms, 3 subgroups,
+    1e-4 A bins, 6501 2-theta points, hist=2) through the `debye` program, FP64 pass 2 as the
+    reference, CRLF outputs; the program's own timers against the reference's log."""
+    import gzip
+    import re
+    import subprocess
+    import tempfile
+    gold = os.path.join(ROOT, "tests", "golden", "gamma")
+    with tempfile.TemporaryDirectory() as d:
+        deb = os.path.join(d, "GammaAl2O3_10nm.deb")
+        with gzip.open(os.path.join(gold, "GammaAl2O3_10nm.deb.gz"), "rb") as f, open(deb, "wb") as g:
+            g.write(f.read())
+        runs = []
+        for _ in range(2):
+            t = time.perf_counter()
+            r = subprocess.run([cd.CLI_PATH, deb, "--crlf"], capture_output=True, text=True, check=True)
+            wall = time.perf_counter() - t
+            v = {k: float(x) for k, x in re.findall(r"Time for ([^:]+): ([0-9.]+) s", r.stdout)}
+            tot = re.search(r"Total time: ([0-9.]+) s", r.stdout)
+            runs.append((wall, v, float(tot.group(1)) if tot else None))
+        with gzip.open(os.path.join(gold, "GammaAl2O3_10nm.debI.gz"), "rb") as f:
+            ref = f.read().decode().replace("\r", "").splitlines()
+        with open(deb + "I", "rb") as f:
+            got = f.read().decode().replace("\r", "").splitlines()
+    wall, v, tot = runs[-1]
+    calc = v.get("Calculation of Distances and Evaluation of the Debye Double Sum")
+    region = v.get("Distances, Debye Double Sum and .debx files")
+    return {
+        "workload": "Data/GammaAl2O3_10nm.deb (117,936 atoms, 6 partials, 3.56M bins of 1e-4 A, 6501 2-theta), "
+                    "hist=2, `debye --crlf` (FP64 pass 2); second of two runs",
+        "pairs": 6954509016,
+        "calc_s": calc, "debx_write_s": v.get("writing the .debx files"), "timed_region_s": region,
+        "program_s": tot, "process_wall_s": round(wall, 3),
+        "reference_calc_excl_debx_s": 6.832, "reference_timed_region_s": 11.563,
+        "speedup_timed_region": round(11.563 / region, 1) if region else None,
+        "speedup_calc": round(6.832 / calc, 1) if calc else None,
+        "debI_identical_to_reference": got == ref,
+    }
+
+
+def cpu_leg(out, m, I_bound, H, pairs, np):
+    """The oracle on the primary model; I_bound = (I, its error bound) from the device."""
     from oracle import oracle as O
     nthreads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
     st = np.concatenate([[0], np.cumsum(m.group_n)[:-1]])
@@ -335,9 +497,13 @@ def cpu_leg(out, m, I, H, pairs, np):
                   f"timed on every 8th of the 4096 Q points and scaled x8 ({t2:.2f} s est.)",
         "cpu": _cpu_model(),
     }
+    I, bound = I_bound
+    err = np.abs(I[qs] - Iref) / np.abs(Iref)
     out["parity"] = {
         "histograms_bit_exact": bool(np.array_equal(H, Href)),
-        "max_rel_err_I_vs_fp64_oracle": float(np.max(np.abs(I[qs] - Iref) / np.abs(Iref))),
+        "max_rel_err_I_vs_fp64_oracle": float(err.max()),
+        "max_rel_err_bound_I": float(bound.max()),
+        "error_within_bound": bool(np.all(err <= bound[qs] + 1e-14)),
     }
 
 
@@ -387,6 +553,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-streams", action="store_true", help="skip the batched and serving streams")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the other configurations (full rocksalt sphere, fine bins, cold model, Gamma example)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -398,12 +566,14 @@ def main():
     import cudebye_amd as cd
     from cudebye_amd import synth
 
-    out, m, I, H, pairs = primary(args, cd, synth, np)
+    out, m, I_bound, H, pairs = primary(args, cd, synth, np)
+    if not args.no_workloads:
+        out["workloads"] = workload_legs(cd, synth, np)
     if not args.no_streams:
         out["batched"] = stream_c4(cd, synth, np)
         out["serving"] = stream_c5(cd, synth, np)
     if not args.no_cpu:
-        cpu_leg(out, m, I, H, pairs, np)
+        cpu_leg(out, m, I_bound, H, pairs, np)
         if not args.no_streams:
             cpu_batched(out, synth, np)
     print(json.dumps(out))

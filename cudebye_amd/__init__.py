@@ -52,7 +52,8 @@ class BatchItem(C.Structure):
 class ServerConfig(C.Structure):
     """dbx_server_config."""
     _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", C.c_int), ("max_step_pairs", C.c_double),
-                ("prep_threads", C.c_int), ("pass2_mode", C.c_int), ("batch_window_ms", C.c_double)]
+                ("prep_threads", C.c_int), ("pass2_mode", C.c_int), ("batch_window_ms", C.c_double),
+                ("rel_err_target", C.c_double)]
 
 
 class RequestDesc(C.Structure):
@@ -115,9 +116,11 @@ def lib():
             "dbx_model_info": (i32, [vp, P(i64), P(i32), P(i32)]),
             "dbx_histogram": (i32, [vp, vp, f64, i64, vp, i32, P(u64)]),
             "dbx_prefactor": (i32, [vp, vp, vp, vp, i32, vp]),
-            "dbx_debye": (i32, [vp, vp, i32, i32, i64, f64, i32, vp, i32, vp, vp, vp]),
-            "dbx_debye_noalloc": (i32, [vp, vp, i32, i64, f64, i32, vp, i32, vp, vp, vp]),
-            "dbx_intensity": (i32, [vp, vp, vp, f64, i64, i32, vp, i32, vp, vp]),
+            "dbx_debye": (i32, [vp, vp, i32, i32, i64, f64, i32, vp, i32, vp, vp, vp, vp]),
+            "dbx_debye_noalloc": (i32, [vp, vp, i32, i64, f64, i32, vp, i32, vp, vp, vp, vp]),
+            "dbx_intensity": (i32, [vp, vp, vp, f64, i64, i32, vp, i32, vp, vp, vp]),
+            "dbx_set_rel_err_target": (i32, [vp, f64]),
+            "dbx_debug_pass2_refined": (i32, [vp, P(i32)]),
             "dbx_run_deb": (i32, [C.c_char_p, i32]),
             "dbx_run_folder": (i32, [C.c_char_p, i32]),
             "dbx_read_debx": (i32, [C.c_char_p, f64, i64, vp, P(i64)]),
@@ -127,14 +130,14 @@ def lib():
             "dbx_last_kernel_times": (i32, [vp, P(f64), P(f64)]),
             "dbx_set_pass2_mode": (i32, [vp, i32]),
             "dbx_selftest_sin": (i32, [vp, i32, vp]),
-            "dbx_batch_intensity": (i32, [vp, vp, i32, f64, vp, i32, vp, vp, vp]),
+            "dbx_batch_intensity": (i32, [vp, vp, i32, f64, vp, i32, vp, vp, vp, vp]),
             "dbx_server_open": (i32, [i32, vp, P(vp)]),
             "dbx_server_close": (i32, [vp]),
             "dbx_server_last_error": (C.c_char_p, [vp]),
             "dbx_submit": (i32, [vp, vp, P(u64)]),
             "dbx_poll": (i32, [vp, u64, P(i32)]),
-            "dbx_wait": (i32, [vp, u64, vp, P(f64)]),
-            "dbx_reweight": (i32, [vp, u64, vp, vp]),
+            "dbx_wait": (i32, [vp, u64, vp, P(f64), vp]),
+            "dbx_reweight": (i32, [vp, u64, vp, vp, vp]),
             "dbx_request_histogram": (i32, [vp, u64, vp]),
             "dbx_release": (i32, [vp, u64]),
             "dbx_server_stats_get": (i32, [vp, vp]),
@@ -159,11 +162,18 @@ EXPORTS = ("dbx_open", "dbx_close", "dbx_last_error", "dbx_stream", "dbx_q_from_
            "dbx_batch_intensity", "dbx_server_open", "dbx_server_close", "dbx_server_last_error",
            "dbx_submit", "dbx_poll", "dbx_wait", "dbx_reweight", "dbx_request_histogram",
            "dbx_release", "dbx_server_stats_get", "dbx_read_debx", "dbx_run_folder",
-           "dbx_deb_header_defaults", "dbx_xyz_to_deb", "dbx_average_debi", "dbx_debye_noalloc")
+           "dbx_deb_header_defaults", "dbx_xyz_to_deb", "dbx_average_debi", "dbx_debye_noalloc",
+           "dbx_set_rel_err_target")
 
 
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
+
+
+def _results(first, *optional):
+    """first alone, or a tuple of first and the (value, wanted) items that are wanted"""
+    out = (first,) + tuple(v for v, wanted in optional if wanted)
+    return out[0] if len(out) == 1 else out
 
 
 def q_from_2theta(lam, tt_min, tt_max, tt_step):
@@ -380,6 +390,17 @@ class Context:
         """DBX_PASS2_MFMA (8-bit integer digits on the matrix cores, default) or DBX_PASS2_FP64."""
         self._check(lib().dbx_set_pass2_mode(self.handle, mode), "set_pass2_mode")
 
+    def set_rel_err_target(self, target):
+        """Models whose I(q) error bound exceeds `target` (default 1e-6) are recomputed with FP64
+        arithmetic; target <= 0: never."""
+        self._check(lib().dbx_set_rel_err_target(self.handle, float(target)), "set_rel_err_target")
+
+    def pass2_refined(self):
+        """Models the last pass 2 recomputed with FP64 arithmetic to meet the error target."""
+        n = C.c_int(0)
+        self._check(lib().dbx_debug_pass2_refined(self.handle, C.byref(n)), "pass2_refined")
+        return n.value
+
     def kernel_times(self):
         """(pass-1 histogram kernel ms, pass-2 structure ms) of the last calls (HIP events)."""
         a, b = C.c_double(0), C.c_double(0)
@@ -406,27 +427,33 @@ class Context:
                                         _p(ff)), "prefactor")
         return ff
 
-    def debye(self, hist, rstep, q, ffoobb, first_n=8, with_S=False):
+    def debye(self, hist, rstep, q, ffoobb, first_n=8, with_S=False, with_err=False):
+        """I(q) [, S_p(q)] [, relative error bound of each I(q)]."""
         hist = np.ascontiguousarray(hist, np.uint64)
         P, nb = hist.shape
         q = np.ascontiguousarray(q, np.float64)
         ff = np.ascontiguousarray(ffoobb, np.float64).reshape(P, len(q))
         I = np.zeros(len(q))
         S = np.zeros((P, len(q))) if with_S else None
+        rel = np.zeros(len(q)) if with_err else None
         self._check(lib().dbx_debye(self.handle, _p(hist), 0, P, nb, rstep, first_n, _p(q), len(q),
-                                    _p(ff), _p(I), None if S is None else _p(S)), "debye")
-        return (I, S) if with_S else I
+                                    _p(ff), _p(I), None if S is None else _p(S),
+                                    None if rel is None else _p(rel)), "debye")
+        return _results(I, (S, with_S), (rel, with_err))
 
-    def intensity(self, m: Model, params, rstep, n_bins, q, first_n=8, with_hist=False):
+    def intensity(self, m: Model, params, rstep, n_bins, q, first_n=8, with_hist=False, with_err=False):
+        """I(q) [, histograms] [, relative error bound of each I(q)]."""
         arr = (GroupParams * m.n_groups)(*params)
         q = np.ascontiguousarray(q, np.float64)
         I = np.zeros(len(q))
         H = np.zeros((m.n_pairs, n_bins), np.uint64) if with_hist else None
+        rel = np.zeros(len(q)) if with_err else None
         self._check(lib().dbx_intensity(self.handle, m.handle, arr, rstep, n_bins, first_n, _p(q),
-                                        len(q), _p(I), None if H is None else _p(H)), "intensity")
-        return (I, H) if with_hist else I
+                                        len(q), _p(I), None if H is None else _p(H),
+                                        None if rel is None else _p(rel)), "intensity")
+        return _results(I, (H, with_hist), (rel, with_err))
 
-    def batch_intensity(self, items, rstep, q, with_hist=False, return_oor=False):
+    def batch_intensity(self, items, rstep, q, with_hist=False, return_oor=False, with_err=False):
         """Many models in one pass-1 launch and one pass-2 contraction (dbx_batch_intensity).
         items: sequence of (Model, params, n_bins, first_n).  Returns I[n_items, nq]
         (+ list of histograms, + out-of-range counts per item)."""
@@ -447,15 +474,11 @@ class Context:
         I = np.zeros((n, len(q)))
         oor = np.zeros(n, np.uint64)
         hp = (C.c_void_p * n)(*[h.ctypes.data for h in hists]) if with_hist else None
+        rel = np.zeros((n, len(q))) if with_err else None
         self._check(lib().dbx_batch_intensity(self.handle, arr, n, rstep, _p(q), len(q), _p(I), _p(oor),
-                                              hp), "batch_intensity",
+                                              hp, None if rel is None else _p(rel)), "batch_intensity",
                     allow=(DBX_ERR_RANGE,) if return_oor else ())
-        out = (I,)
-        if with_hist:
-            out += (hists,)
-        if return_oor:
-            out += ([int(v) for v in oor],)
-        return out[0] if len(out) == 1 else out
+        return _results(I, (hists, with_hist), ([int(v) for v in oor], return_oor), (rel, with_err))
 
 
 class Server:
@@ -463,8 +486,9 @@ class Server:
     their intensities, re-weight resident ones with new form factors."""
 
     def __init__(self, device=0, hbm_budget=0, max_batch=0, max_step_pairs=0.0, prep_threads=0,
-                 pass2_mode=DBX_PASS2_MFMA, batch_window_ms=0.0):
-        cfg = ServerConfig(hbm_budget, max_batch, max_step_pairs, prep_threads, pass2_mode, batch_window_ms)
+                 pass2_mode=DBX_PASS2_MFMA, batch_window_ms=0.0, rel_err_target=0.0):
+        cfg = ServerConfig(hbm_budget, max_batch, max_step_pairs, prep_threads, pass2_mode, batch_window_ms,
+                           rel_err_target)
         h = C.c_void_p()
         rc = lib().dbx_server_open(device, C.byref(cfg), C.byref(h))
         if rc != DBX_OK:
@@ -516,24 +540,30 @@ class Server:
             raise DebyeError(DBX_ERR_ARG, f"unknown ticket {ticket}")
         return self._nq[ticket]
 
-    def wait(self, ticket, allow=()):
-        """(I, latency_ms, status).  DBX_ERR_BUDGET: the request cannot be admitted until
-        finished DBX_REQ_KEEP requests are released (it stays queued)."""
-        I = np.zeros(self._nq[ticket][0]) if ticket in self._nq else None
+    def wait(self, ticket, allow=(), with_err=False):
+        """(I, latency_ms, status[, relative error bound of each I(q)]).  DBX_ERR_BUDGET: the
+        request cannot be admitted until finished DBX_REQ_KEEP requests are released (it stays
+        queued)."""
+        known = ticket in self._nq
+        I = np.zeros(self._nq[ticket][0]) if known else None
+        rel = np.zeros(self._nq[ticket][0]) if known and with_err else None
         lat = C.c_double()
-        rc = lib().dbx_wait(self.handle, ticket, None if I is None else _p(I), C.byref(lat))
+        rc = lib().dbx_wait(self.handle, ticket, None if I is None else _p(I), C.byref(lat),
+                            None if rel is None else _p(rel))
         if rc != DBX_OK and rc not in allow:
             self._err(rc, f"request {ticket}")
-        return I, lat.value, rc
+        return (I, lat.value, rc, rel) if with_err else (I, lat.value, rc)
 
-    def reweight(self, ticket, params):
+    def reweight(self, ticket, params, with_err=False):
+        """I(q) for new form factors [, its relative error bound]."""
         nq, S, _ = self._info(ticket)
         pa = (GroupParams * S)(*params)
         I = np.zeros(nq)
-        rc = lib().dbx_reweight(self.handle, ticket, pa, _p(I))
+        rel = np.zeros(nq) if with_err else None
+        rc = lib().dbx_reweight(self.handle, ticket, pa, _p(I), None if rel is None else _p(rel))
         if rc != DBX_OK:
             self._err(rc, "reweight")
-        return I
+        return (I, rel) if with_err else I
 
     def histogram(self, ticket):
         nq, S, nb = self._info(ticket)

@@ -373,6 +373,32 @@ void free_model(ModelDev &m)
     m.d_alist = nullptr;
 }
 
+// The I(q) of one model (rows [row0, row0 + P) of d_S, d_B; destination dest) and its bound to
+// the host (hI, hrel: nq each).  When the bound exceeds the context's target at some q and the
+// rows were summed on the matrix cores, they are recomputed with FP64 arithmetic from the
+// model's histograms d_hist[P][n_bins] and combined again.  Synchronises the stream.
+static void finish_model(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep, int first_n,
+                         const double *dq, int nq, const double *dff, double *dS, double *dB, int row0, int dest,
+                         double *dI, double *drel, double *hI, double *hrel)
+{
+    const size_t nqd = (size_t)nq;
+    auto fetch = [&] {
+        DBX_HIP(hipMemcpyAsync(hI, dI + dest * nqd, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
+        DBX_HIP(hipMemcpyAsync(hrel, drel + dest * nqd, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
+        DBX_HIP(hipStreamSynchronize(ctx->stream));
+    };
+    fetch();
+    if (ctx->pass2_mode != DBX_PASS2_MFMA) return;
+    bool bad = false;
+    for (int i = 0; i < nq && !bad; i++) bad = exceeds_target(hrel[i], ctx->rel_err_target);
+    if (!bad) return;
+    debye_structure(ctx, d_hist, P, n_bins, rstep, first_n, dq, nq, dS + (size_t)row0 * nqd, dB + row0,
+                    DBX_PASS2_FP64);
+    launch_combine(ctx->stream, dff, dS, dB, nullptr, 1, nq, dI, drel, make_int4(row0, P, dest, 0));
+    ctx->last_p2_refined++;
+    fetch();
+}
+
 }  // namespace dbx
 
 using namespace dbx;
@@ -529,6 +555,14 @@ int dbx_set_pass2_mode(dbx_ctx *ctx, int mode)
     return DBX_OK;
 }
 
+int dbx_set_rel_err_target(dbx_ctx *ctx, double target)
+{
+    if (!ctx || std::isnan(target)) return DBX_ERR_ARG;
+    std::lock_guard<std::recursive_mutex> lk(ctx->mu);
+    ctx->rel_err_target = target > 0 ? target : 0.0;
+    return DBX_OK;
+}
+
 int dbx_last_pass1_stats(const dbx_ctx *ctx, int64_t *n_jobs, uint64_t *window_misses, uint64_t *retried)
 {
     if (!ctx) return DBX_ERR_ARG;
@@ -603,7 +637,7 @@ int dbx_prefactor(dbx_ctx *ctx, const dbx_group_params *ga, const dbx_group_para
 
 int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int64_t n_bins,
               double rstep, int first_n, const double *q, int nq, const double *ffoobb,
-              double *I_out, double *S_out)
+              double *I_out, double *S_out, double *rel_err_out)
 {
     if (!hist || !q || !ffoobb || !I_out || P <= 0 || nq <= 0 || n_bins <= 0) return DBX_ERR_ARG;
     return guarded(ctx, [&] {
@@ -615,24 +649,30 @@ int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int
             dh = t;
         }
         const size_t nqd = (size_t)nq;
-        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 1)));
-        double *dff = d, *dS = dff + P * nqd, *dI = dS + P * nqd;
+        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 2) + P));
+        double *dff = d, *dS = dff + P * nqd, *dI = dS + P * nqd, *drel = dI + nqd, *dB = drel + nqd;
         const double *dq = device_q(ctx, q, nq);
         DBX_HIP(hipMemcpyAsync(dff, ffoobb, sizeof(double) * P * nqd, hipMemcpyHostToDevice,
                                ctx->stream));
-        debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS);
-        combine_intensity(ctx, dff, dS, P, nq, dI);
-        DBX_HIP(hipMemcpyAsync(I_out, dI, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
-        if (S_out)
+        ctx->last_p2_refined = 0;
+        debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS, dB);
+        combine_intensity(ctx, dff, dS, dB, P, nq, dI, drel);
+        double *hI = (double *)pinned(ctx, kPinI, sizeof(double) * nqd);
+        double *hrel = (double *)pinned(ctx, kPinRel, sizeof(double) * nqd);
+        finish_model(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dff, dS, dB, 0, 0, dI, drel, hI, hrel);
+        std::memcpy(I_out, hI, sizeof(double) * nqd);
+        if (rel_err_out) std::memcpy(rel_err_out, hrel, sizeof(double) * nqd);
+        if (S_out) {
             DBX_HIP(hipMemcpyAsync(S_out, dS, sizeof(double) * P * nqd, hipMemcpyDeviceToHost,
                                    ctx->stream));
-        DBX_HIP(hipStreamSynchronize(ctx->stream));
+            DBX_HIP(hipStreamSynchronize(ctx->stream));
+        }
     });
 }
 
 int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, double rstep,
                         const double *q, int nq, double *I_out, uint64_t *n_out_of_range,
-                        uint64_t *const *hist_out)
+                        uint64_t *const *hist_out, double *rel_err_out)
 {
     if (!items || n_items <= 0 || !q || nq <= 0 || !I_out) return DBX_ERR_ARG;
     for (int i = 0; i < n_items; i++)
@@ -657,8 +697,10 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
         uint64_t *dh = (uint64_t *)scratch(ctx, kSlotBatchHist, sizeof(uint64_t) * (size_t)total);
         uint64_t *door = (uint64_t *)scratch(ctx, kSlotOor, sizeof(uint64_t) * (size_t)n_items);
         const size_t nqd = (size_t)nq;
-        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * nqd * (2 * (size_t)n_rows + n_items));
-        double *dff = d, *dS = dff + n_rows * nqd, *dI = dS + n_rows * nqd;
+        double *d = (double *)scratch(ctx, kSlotQ,
+                                      sizeof(double) * (nqd * (2 * (size_t)n_rows + 2 * n_items) + n_rows));
+        double *dff = d, *dS = dff + n_rows * nqd, *dI = dS + n_rows * nqd, *drel = dI + n_items * nqd;
+        double *dB = drel + n_items * nqd;
         const double *dq = device_q(ctx, q, nq);
         std::vector<P1Item> p1;
         std::vector<RowDesc> rows;
@@ -678,11 +720,18 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
                     pairs.push_back(make_group_pair(it.params[a], it.params[b]));
                 }
         }
+        ctx->last_p2_refined = 0;
         pass1_batch(ctx, p1, rstep, false);
         prefactor_rows(ctx, pairs, dq, nq, dff);
-        debye_structure_rows(ctx, dh, rows, rstep, dq, nq, dS);
-        combine_models(ctx, dff, dS, mrows, nq, dI);
-        DBX_HIP(hipMemcpyAsync(I_out, dI, sizeof(double) * nqd * n_items, hipMemcpyDeviceToHost, ctx->stream));
+        debye_structure_rows(ctx, dh, rows, rstep, dq, nq, dS, dB);
+        combine_models(ctx, dff, dS, dB, mrows, nq, dI, drel);
+        double *hI = (double *)pinned(ctx, kPinStepI, sizeof(double) * nqd * n_items);
+        double *hrel = (double *)pinned(ctx, kPinStepRel, sizeof(double) * nqd * n_items);
+        auto fetch = [&] {
+            DBX_HIP(hipMemcpyAsync(hI, dI, sizeof(double) * nqd * n_items, hipMemcpyDeviceToHost, ctx->stream));
+            DBX_HIP(hipMemcpyAsync(hrel, drel, sizeof(double) * nqd * n_items, hipMemcpyDeviceToHost, ctx->stream));
+        };
+        fetch();
         if (hist_out)
             for (int i = 0; i < n_items; i++)
                 if (hist_out[i]) {
@@ -694,6 +743,25 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
         uint64_t *hoor = (uint64_t *)pinned(ctx, kPinOor, sizeof(uint64_t) * (size_t)n_items);
         DBX_HIP(hipMemcpyAsync(hoor, door, sizeof(uint64_t) * (size_t)n_items, hipMemcpyDeviceToHost, ctx->stream));
         DBX_HIP(hipStreamSynchronize(ctx->stream));
+        // models whose bound misses the target: their partials again with FP64 arithmetic
+        if (ctx->pass2_mode == DBX_PASS2_MFMA) {
+            for (const int4 &mr : mrows) {
+                const int i = mr.z;
+                bool bad = false;
+                for (int k = 0; k < nq && !bad; k++) bad = exceeds_target(hrel[i * nqd + k], ctx->rel_err_target);
+                if (!bad) continue;
+                debye_structure(ctx, dh + hoff[i], mr.y, items[i].n_bins, rstep, items[i].first_n, dq, nq,
+                                dS + (size_t)mr.x * nqd, dB + mr.x, DBX_PASS2_FP64);
+                launch_combine(ctx->stream, dff, dS, dB, nullptr, 1, nq, dI, drel, mr);
+                ctx->last_p2_refined++;
+            }
+            if (ctx->last_p2_refined) {
+                fetch();
+                DBX_HIP(hipStreamSynchronize(ctx->stream));
+            }
+        }
+        std::memcpy(I_out, hI, sizeof(double) * nqd * n_items);
+        if (rel_err_out) std::memcpy(rel_err_out, hrel, sizeof(double) * nqd * n_items);
         P1Stats st;
         pass1_read_stats(ctx, st);
         int bad = 0;
@@ -712,7 +780,7 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
 
 int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params, double rstep,
                   int64_t n_bins, int first_n, const double *q, int nq, double *I_out,
-                  uint64_t *hist_out)
+                  uint64_t *hist_out, double *rel_err_out)
 {
     if (!m || !params || !q || !I_out || nq <= 0) return DBX_ERR_ARG;
     int rc = DBX_OK;
@@ -723,20 +791,22 @@ int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params, do
         uint64_t *dh = (uint64_t *)scratch(ctx, kSlotHist, hb);
         P1Stats st = pass1_histograms(ctx, m->d, rstep, n_bins, dh, false);
         const size_t nqd = (size_t)nq;
-        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 1)));
-        double *dff = d, *dS = dff + P * nqd, *dI = dS + P * nqd;
+        double *d = (double *)scratch(ctx, kSlotQ, sizeof(double) * (nqd * (2 * P + 2) + P));
+        double *dff = d, *dS = dff + P * nqd, *dI = dS + P * nqd, *drel = dI + nqd, *dB = drel + nqd;
         const double *dq = device_q(ctx, q, nq);
         std::vector<GroupPair> pairs;
         for (int a = 0; a < S; a++)
             for (int b = a; b < S; b++) pairs.push_back(make_group_pair(params[a], params[b]));
+        ctx->last_p2_refined = 0;
         prefactor_rows(ctx, pairs, dq, nq, dff);
-        debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS);
-        combine_intensity(ctx, dff, dS, P, nq, dI);
-        double *hI = (double *)pinned(ctx, kPinI, sizeof(double) * nqd);
-        DBX_HIP(hipMemcpyAsync(hI, dI, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
+        debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS, dB);
+        combine_intensity(ctx, dff, dS, dB, P, nq, dI, drel);
         if (hist_out) DBX_HIP(hipMemcpyAsync(hist_out, dh, hb, hipMemcpyDeviceToHost, ctx->stream));
-        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        double *hI = (double *)pinned(ctx, kPinI, sizeof(double) * nqd);
+        double *hrel = (double *)pinned(ctx, kPinRel, sizeof(double) * nqd);
+        finish_model(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dff, dS, dB, 0, 0, dI, drel, hI, hrel);
         std::memcpy(I_out, hI, sizeof(double) * nqd);
+        if (rel_err_out) std::memcpy(rel_err_out, hrel, sizeof(double) * nqd);
         pass1_read_stats(ctx, st);
         if (st.out_of_range) {
             rc = DBX_ERR_RANGE;
@@ -858,6 +928,13 @@ int dbx_debug_pass2_profile(dbx_ctx *ctx, double *kernel_ms, double *mfma_flops,
         if (n_bins_compacted) *n_bins_compacted = ctx->last_p2_bins;
         if (count_digits) *count_digits = ctx->mf_digits;
     });
+}
+
+int dbx_debug_pass2_refined(const dbx_ctx *ctx, int *n_models)
+{
+    if (!ctx || !n_models) return DBX_ERR_ARG;
+    *n_models = ctx->last_p2_refined;
+    return DBX_OK;
 }
 
 int dbx_debug_sin_host(const double *x, int n, double *out)

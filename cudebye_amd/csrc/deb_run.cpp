@@ -162,9 +162,11 @@ double now_s()
 // per-pair checks (cuDebye-v15.cu:360-410), .debx files (cuDebye-v15.cu:413-430, unless the
 // histograms were read from them) and the .debI file (cuDebye-v15.cu:973-998)
 int write_outputs(const char *path, const DebFile &d, const std::vector<double> &tt, const uint64_t *H,
-                  int64_t channels, const std::vector<double> &I, bool write_debx, const char *eol, bool quiet)
+                  int64_t channels, const std::vector<double> &I, bool write_debx, const char *eol, bool quiet,
+                  double *t_debx)
 {
     int rc = DBX_OK;
+    double t_w = 0;
     int p = 0;
     for (int row = 0; row < d.S; row++) {
         for (int col = row; col < d.S; col++, p++) {
@@ -204,6 +206,7 @@ int write_outputs(const char *path, const DebFile &d, const std::vector<double> 
                      (unsigned long long)near);
             }
             if (d.hist > 0 && write_debx) {
+                const double tw0 = now_s();
                 const std::string fn = std::string(path) + "x" + std::to_string(row) + "x" + std::to_string(col);
                 FILE *f = std::fopen(fn.c_str(), "wb");
                 if (!f) {
@@ -214,6 +217,7 @@ int write_outputs(const char *path, const DebFile &d, const std::vector<double> 
                 for (int64_t k = 0; k < channels; k++)
                     if (h[k]) std::fprintf(f, "%.7g\t%llu%s", (double)k * d.rstep, (unsigned long long)h[k], eol);
                 std::fclose(f);
+                t_w += now_s() - tw0;
             }
         }
     }
@@ -228,6 +232,7 @@ int write_outputs(const char *path, const DebFile &d, const std::vector<double> 
             std::fclose(f);
         }
     }
+    if (t_debx) *t_debx = t_w;
     return rc;
 }
 
@@ -335,7 +340,7 @@ static int run_deb_body(const char *path, int flags)
     dbx_set_pass2_mode(ctx, (flags & DBX_RUN_MFMA) ? DBX_PASS2_MFMA : DBX_PASS2_FP64);
     const int P = d.S * (d.S + 1) / 2;
     std::vector<uint64_t> H((size_t)P * channels);
-    std::vector<double> I(nTT, 0.0);
+    std::vector<double> I(nTT, 0.0), rel(nTT, 0.0);
     const int first_n = d.block > 0 ? d.block : 1;
     double t1 = now_s();
     if (from_debx) {
@@ -355,7 +360,7 @@ static int run_deb_body(const char *path, int flags)
                     rc = dbx_prefactor(ctx, &d.gp[a], &d.gp[b], K.data(), nTT, &ff[(size_t)p * nTT]);
             if (rc == DBX_OK)
                 rc = dbx_debye(ctx, H.data(), 0, P, channels, d.rstep, first_n, K.data(), nTT, ff.data(), I.data(),
-                               nullptr);
+                               nullptr, rel.data());
             if (rc != DBX_OK) std::fprintf(stderr, "ERROR: %s\n", dbx_last_error(ctx));
         }
         if (rc != DBX_OK) return rc;
@@ -367,7 +372,8 @@ static int run_deb_body(const char *path, int flags)
         }
         dbx_model *const m = own.m;
         if (d.hist != 1)
-            rc = dbx_intensity(ctx, m, d.gp.data(), d.rstep, channels, first_n, K.data(), nTT, I.data(), H.data());
+            rc = dbx_intensity(ctx, m, d.gp.data(), d.rstep, channels, first_n, K.data(), nTT, I.data(), H.data(),
+                               rel.data());
         else
             rc = dbx_histogram(ctx, m, d.rstep, channels, H.data(), 0, nullptr);
         if (rc != DBX_OK) {
@@ -376,9 +382,20 @@ static int run_deb_body(const char *path, int flags)
         }
     }
     const double t_calc = now_s() - t1;
-    const int wrc = write_outputs(path, d, tt, H.data(), channels, I, !from_debx, eol, quiet);
+    double t_debx = 0;
+    const int wrc = write_outputs(path, d, tt, H.data(), channels, I, !from_debx, eol, quiet, &t_debx);
     if (rc == DBX_OK) rc = wrc;
+    if (d.hist != 1) {
+        double worst = 0;
+        for (double v : rel) worst = std::max(worst, v);
+        logf(quiet, "Largest relative error bound of I(2theta): %.3g\n", worst);
+    }
     logf(quiet, "Time for Calculation of Distances and Evaluation of the Debye Double Sum: %.3f s\n", t_calc);
+    // the reference times its subgroup-pair loop, which writes the .debx files inside it
+    // (cuDebye-v15.cu:928-971)
+    logf(quiet, "Time for writing the .debx files: %.3f s\n", t_debx);
+    logf(quiet, "Time for Distances, Debye Double Sum and .debx files: %.3f s\n", t_calc + t_debx);
+    logf(quiet, "Total time: %.3f s\n", now_s() - t0);
     return rc;
 }
 
@@ -491,14 +508,15 @@ static int run_folder_body(const char *dir, int flags)
         if (j.rc == DBX_OK) {
             std::vector<double> I(j.K.size());
             double ms = 0;
-            j.rc = dbx_wait(srv, j.ticket, I.data(), &ms);
+            j.rc = dbx_wait(srv, j.ticket, I.data(), &ms, nullptr);
             if (j.rc == DBX_OK) {
                 const int P = j.d.S * (j.d.S + 1) / 2;
                 std::vector<uint64_t> H((size_t)P * j.channels);
                 j.rc = dbx_request_histogram(srv, j.ticket, H.data());
                 if (j.rc == DBX_OK) {
                     logf(quiet, "Calculated %s (%.3f s)\n", j.path.c_str(), ms * 1e-3);
-                    j.rc = write_outputs(j.path.c_str(), j.d, j.tt, H.data(), j.channels, I, true, eol, quiet);
+                    j.rc = write_outputs(j.path.c_str(), j.d, j.tt, H.data(), j.channels, I, true, eol, quiet,
+                                         nullptr);
                 }
             } else {
                 std::fprintf(stderr, "ERROR: %s: %s\n", j.path.c_str(), dbx_server_last_error(srv));

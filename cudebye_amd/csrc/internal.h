@@ -168,8 +168,8 @@ struct dbx_ctx {
     // grow-only device scratch, one buffer per slot (dbx::kSlot*)
     void *slot_ptr[13] = {};
     size_t slot_bytes[13] = {};
-    void *pin_ptr[10] = {};
-    size_t pin_bytes[10] = {};
+    void *pin_ptr[12] = {};
+    size_t pin_bytes[12] = {};
     // the q grid last uploaded to slot kSlotQin (host copy and device address)
     std::vector<double> q_host;
     const void *q_dev = nullptr;
@@ -187,6 +187,8 @@ struct dbx_ctx {
     hipEvent_t ev[4] = {};
     bool p2_timed = false;
     int pass2_mode = DBX_PASS2_MFMA;  // DBX_PASS2_MFMA or DBX_PASS2_FP64
+    double rel_err_target = 1e-6;     // models whose I bound exceeds it are redone in FP64 (0: never)
+    int last_p2_refined = 0;          // models redone in FP64 by the last call
     int last_p2_bins = 0;             // compacted bins of the last pass 2
     uint64_t *p2_readback_h = nullptr;  // pinned host words for pass 2's read-backs, from dbx_open
     // profiling of the matrix-core contraction (dbx_debug_set_profiling)
@@ -207,7 +209,7 @@ void *scratch(dbx_ctx *ctx, int slot, size_t bytes);
 // pinned host buffer of at least `bytes` for `slot` (staging of per-call uploads; every call
 // synchronises its stream before returning, so a slot is free again at the next call)
 enum { kPinRefs = 0, kPinList = 1, kPinRows = 2, kPinPairs = 3, kPinMisc = 4, kPinModels = 5, kPinOor = 6,
-       kPinStepI = 7, kPinQ = 8, kPinI = 9 };
+       kPinStepI = 7, kPinQ = 8, kPinI = 9, kPinRel = 10, kPinStepRel = 11 };
 void *pinned(dbx_ctx *ctx, int slot, size_t bytes);
 // the q grid on the device (slot kSlotQin), uploaded through pinned memory unless the slot
 // already holds the same values
@@ -289,25 +291,31 @@ struct P2Mem {
 };
 P2Mem &ctx_pass2_mem(dbx_ctx *ctx);
 // S[r][q] for every row r (rows of any models sharing rstep, first_n and the q grid)
+// d_B (optional) receives each row's absolute error bound of S (the same for every q); mode:
+// DBX_PASS2_MFMA or DBX_PASS2_FP64, -1 for the context's
 void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::vector<RowDesc> &rows,
-                          double rstep, const double *d_q, int nq, double *d_S);
+                          double rstep, const double *d_q, int nq, double *d_S, double *d_B = nullptr,
+                          int mode = -1);
 // S[p][q] for the P partials of one model, d_hist[P][n_bins], buffers from mem (the row
 // descriptors are written on the device: no host staging)
 void debye_structure_mem(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
-                         int first_n, const double *d_q, int nq, double *d_S);
-// I[mrows[m].z][q] = sum over rows [mrows[m].x, mrows[m].x + mrows[m].y) of ff S
-void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const std::vector<int4> &mrows,
-                    int nq, double *d_I);
+                         int first_n, const double *d_q, int nq, double *d_S, double *d_B = nullptr, int mode = -1);
+// I[mrows[m].z][q] = sum over rows [mrows[m].x, mrows[m].x + mrows[m].y) of ff S; with the row
+// bounds d_B, d_rel[mrows[m].z][q] (optional) receives the relative error bound of each I
+void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const double *d_B,
+                    const std::vector<int4> &mrows, int nq, double *d_I, double *d_rel);
 // S_p(q) for all p into d_S[P][nq] from device histograms d_hist[P][n_bins].
 void debye_structure(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
-                     int first_n, const double *d_q, int nq, double *d_S);
+                     int first_n, const double *d_q, int nq, double *d_S, double *d_B = nullptr, int mode = -1);
 // I[q] = sum_p ff[p][q] S[p][q]
 void launch_prefactor(hipStream_t st, const GroupPair *d_pairs, int n, const double *d_q, int nq, double *d_ff);
 // d_mrows null: one model, rows m0.x .. m0.x + m0.y - 1 into I[m0.z]
-void launch_combine(hipStream_t st, const double *d_ff, const double *d_S, const int4 *d_mrows, int nm, int nq,
-                    double *d_I, int4 m0 = make_int4(0, 0, 0, 0));
-void combine_intensity(dbx_ctx *ctx, const double *d_ff, const double *d_S, int P, int nq,
-                       double *d_I);
+void launch_combine(hipStream_t st, const double *d_ff, const double *d_S, const double *d_B, const int4 *d_mrows,
+                    int nm, int nq, double *d_I, double *d_rel, int4 m0 = make_int4(0, 0, 0, 0));
+void combine_intensity(dbx_ctx *ctx, const double *d_ff, const double *d_S, const double *d_B, int P, int nq,
+                       double *d_I, double *d_rel);
+// *d_out = the bit pattern of the largest of the non-negative doubles d_v[0..n)
+void launch_max_bits(hipStream_t st, const double *d_v, int n, unsigned long long *d_out);
 
 
 // An entry point's body under the context's lock, with its device current: C++ errors become
@@ -333,6 +341,9 @@ int guarded(dbx_ctx *ctx, F &&f)
         return DBX_ERR_ARG;
     }
 }
+
+// true when an intensity bound rel (relative) does not meet target (> 0); NaN does not
+inline bool exceeds_target(double rel, double target) { return target > 0 && !(rel <= target); }
 
 // the allocation-free pass 2 (noalloc.hip): the device address of this device's fixed scratch
 // region, looked up once per device (called by dbx_open with the device current)

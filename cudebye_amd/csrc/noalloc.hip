@@ -9,6 +9,7 @@
 // device (two contexts, two threads) race; one context serialises its own calls.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <mutex>
 #include <string>
 
@@ -73,7 +74,8 @@ using namespace dbx;
 extern "C" {
 
 int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep, int first_n,
-                      const double *d_q, int nq, const double *d_ffoobb, double *d_I, double *d_S)
+                      const double *d_q, int nq, const double *d_ffoobb, double *d_I, double *d_S,
+                      double *d_rel_err)
 {
     if (!d_hist || !d_q || !d_ffoobb || !d_I || P <= 0 || nq <= 0 || n_bins <= 0 || first_n < 0 ||
         !(rstep > 0))
@@ -82,10 +84,31 @@ int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bin
         char *base = g_scratch[ctx->device];
         DBX_REQUIRE(base != nullptr, DBX_ERR_STATE, "fixed scratch region not set up on this device");
         FixedMem mem(base, (unsigned long long *)ctx->p2_readback_h);
-        // the structure sums S[p][q]: the caller's buffer, or the region's
+        // the structure sums S[p][q], their bounds and the bound of I: the caller's buffers, or
+        // the region's
         double *S = d_S ? d_S : (double *)mem.dev(kSlotQ, sizeof(double) * (size_t)P * (size_t)nq);
-        debye_structure_mem(ctx, mem, d_hist, P, n_bins, rstep, first_n, d_q, nq, S);
-        launch_combine(ctx->stream, d_ffoobb, S, nullptr, 1, nq, d_I, make_int4(0, P, 0, 0));
+        double *rel = d_rel_err ? d_rel_err : (double *)mem.dev(kSlotQ, sizeof(double) * (size_t)nq);
+        double *B = (double *)mem.dev(kSlotQ, sizeof(double) * (size_t)P);
+        unsigned long long *worst = (unsigned long long *)mem.dev(kSlotQ, sizeof(unsigned long long));
+        const size_t kept = mem.used;  // the pass-2 buffers above this mark are reused below
+        ctx->last_p2_refined = 0;
+        debye_structure_mem(ctx, mem, d_hist, P, n_bins, rstep, first_n, d_q, nq, S, B);
+        launch_combine(ctx->stream, d_ffoobb, S, B, nullptr, 1, nq, d_I, rel, make_int4(0, P, 0, 0));
+        if (ctx->pass2_mode == DBX_PASS2_MFMA && ctx->rel_err_target > 0) {
+            // the largest bound, read back; past the target the partials are redone in FP64
+            launch_max_bits(ctx->stream, rel, nq, worst);
+            DBX_HIP(hipMemcpyAsync(ctx->p2_readback_h, worst, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            DBX_HIP(hipStreamSynchronize(ctx->stream));
+            double w;
+            std::memcpy(&w, ctx->p2_readback_h, sizeof(w));
+            if (exceeds_target(w, ctx->rel_err_target)) {
+                mem.used = kept;
+                debye_structure_mem(ctx, mem, d_hist, P, n_bins, rstep, first_n, d_q, nq, S, B, DBX_PASS2_FP64);
+                launch_combine(ctx->stream, d_ffoobb, S, B, nullptr, 1, nq, d_I, rel, make_int4(0, P, 0, 0));
+                ctx->last_p2_refined = 1;
+            }
+        }
         DBX_HIP(hipStreamSynchronize(ctx->stream));
     });
 }

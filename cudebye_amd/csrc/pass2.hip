@@ -245,16 +245,21 @@ __device__ __forceinline__ void kext_entry(int b, const int *__restrict__ kz, in
     kext[b] = lo;
 }
 
-// weights of the compacted bins, FP64: A[r][i] = 2 N_r,kz[i]
+// weights of the compacted bins, FP64: A[r][i] = 2 N_r,kz[i]; rsum[r0 + t] += the row's sum of
+// them (one atomic per wave and row)
 __global__ void p2_weights_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
-                                  int r0, int nr, const int *__restrict__ kz, int Kc, double *__restrict__ A)
+                                  int r0, int nr, const int *__restrict__ kz, int Kc, double *__restrict__ A,
+                                  unsigned long long *__restrict__ rsum)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= Kc) return;
-    const int64_t k = kz[i];
+    const int64_t k = i < Kc ? kz[i] : -1;
     for (int t = 0; t < nr; t++) {
         const RowDesc rd = rows[r0 + t];
-        A[(int64_t)t * Kc + i] = k < rd.n_bins ? 2.0 * (double)hist[rd.h_off + k] : 0.0;
+        const unsigned long long n = (k >= 0 && k < rd.n_bins) ? 2ull * hist[rd.h_off + k] : 0ull;
+        if (i < Kc) A[(int64_t)t * Kc + i] = (double)n;
+        unsigned long long w = n;
+        for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
+        if ((threadIdx.x & 63) == 0 && w) atomicAdd(&rsum[r0 + t], w);
     }
 }
 
@@ -366,6 +371,22 @@ __global__ void p2_finish_rows_kernel(const double *__restrict__ Spart, int nspl
         }
     }
     S[(int64_t)r0 * nq + i] = acc + lo;
+}
+
+// Absolute error bound of each row's structure sum S[r][q], the same for every q:
+//   B[r] = e rsum[r] + 5 u (self[r] + sum_k |low[r][k]|)
+// rsum[r] = sum of the weights 2 N over the row's compacted bins (exact), e = the error per unit
+// weight of the mode's sum over them (pass2_unit_error); the weight-1 bins are exact integers and
+// each low bin's FP64 sinc term is within 5 u of its value.  rsum null: no compacted bin.
+__global__ void p2_row_bounds_kernel(const unsigned long long *__restrict__ rsum, const double *__restrict__ self,
+                                     const double *__restrict__ low, int nlow, int n_rows, double e,
+                                     double *__restrict__ B)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    double lo = self[r];
+    for (int k = 0; k < nlow; k++) lo += fabs(low[(int64_t)r * nlow + k]);
+    B[r] = (rsum ? e * (double)rsum[r] : 0.0) + 5.0 * 0x1p-53 * lo;
 }
 
 // ---------------------------------------------------------------------------------
@@ -768,17 +789,30 @@ static void mf_launch(const MfShape &sh, int I, dim3 grid, hipStream_t st, const
 }
 
 // I[dest][q] = sum over the model's rows of ff S, FP64, rows in order;
-// mrows[m] = (first row, row count, dest, -), or m0 for a single model (mrows null)
+// mrows[m] = (first row, row count, dest, -), or m0 for a single model (mrows null).
+// rel (optional) receives the relative error bound of each I[dest][q] against the exact sum of
+// the same histograms and prefactors:
+//   (sum_r |ff_r| B_r + (P + 8) u sum_r |ff_r S_r|) / |I|
+// with B_r the row bounds (p2_row_bounds_kernel); the second term covers the FP64 rounding of
+// this sum, of S_r's last additions and of the prefactors (a few units each).
 __global__ void p2_combine_kernel(const double *__restrict__ ff, const double *__restrict__ S,
-                                  const int4 *__restrict__ mrows, int4 m0, int nq, double *__restrict__ I)
+                                  const double *__restrict__ B, const int4 *__restrict__ mrows, int4 m0, int nq,
+                                  double *__restrict__ I, double *__restrict__ rel)
 {
     const int iq = blockIdx.x * blockDim.x + threadIdx.x;
     const int m = blockIdx.y;
     if (iq >= nq) return;
     const int4 rr = mrows ? mrows[m] : m0;
-    double acc = 0.0;
-    for (int r = rr.x; r < rr.x + rr.y; r++) acc += S[(int64_t)r * nq + iq] * ff[(int64_t)r * nq + iq];
+    double acc = 0.0, mag = 0.0, bnd = 0.0;
+    for (int r = rr.x; r < rr.x + rr.y; r++) {
+        const double f = ff[(int64_t)r * nq + iq];
+        const double t = S[(int64_t)r * nq + iq] * f;
+        acc += t;
+        mag += fabs(t);
+        if (B) bnd += fabs(f) * B[r];
+    }
     I[(int64_t)rr.z * nq + iq] = acc;
+    if (rel) rel[(int64_t)rr.z * nq + iq] = (bnd + (double)(rr.y + 8) * 0x1p-53 * mag) / fabs(acc);
 }
 
 // largest k with double(k)*rstep <= 0.0005 (the reference's test r[k] <= 0.0005)
@@ -801,9 +835,29 @@ __global__ void p2_rows_fill_kernel(RowDesc *__restrict__ rows, int n_rows, int6
 // length), buffers from mem.  h_rows: the row descriptors on the host (uploaded through pinned
 // memory), or null for the P equal rows of one model (nb_max bins each, uni_first_n), written
 // on the device.
+// Error per unit weight of a row's sum over its compacted bins, sum_k A_k T_k with A_k = 2 N_k:
+//   matrix cores: the sinc digits round c (T - 3/8) to an integer, |dT| <= 1 / (2c); the FP64 sine
+//     and its recurrence add at most 4e-15 (sin_cw alone, below 1e-15), the product forming c T
+//     3 u; the FP64 partial sums hold at most 0.625 sum_k A_k (|T - 3/8| <= 0.625) and take
+//     nsplit + classes x folds + 2 rounded additions each, and the 3/8 sum_k A_k term one
+//     conversion;
+//   FP64: a sinc value within 4 u (sine, quotient, product), then kper + nsplit + 2 rounded
+//     additions of partial sums below sum_k A_k (the classical gamma_n bound).
+// u = 2^-53.  Every term is an upper bound, whatever the signs of the errors.
+static double pass2_unit_error(int mode, int kper_bins, int nsplit, int classes)
+{
+    const double u = 0x1p-53;
+    if (mode == DBX_PASS2_FP64) {
+        const double n = (double)kper_bins + nsplit + 2;
+        return 4 * u + n * u / (1.0 - n * u);
+    }
+    const int folds = (kper_bins / kMfK + kMfFold - 1) / kMfFold;
+    return 0.5 / kMfC + 4e-15 + 3 * u + 0.625 * u * (double)(nsplit + classes * folds + 2) + 0.375 * u;
+}
+
 static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base, const RowDesc *h_rows,
                            int n_rows, int64_t nb_max, int uni_first_n, double rstep, const double *d_q, int nq,
-                           double *d_S)
+                           double *d_S, int mode, double *d_B)
 {
     const int ksm = self_bin_max(rstep);
     const int nlow = ksm + 1;
@@ -860,7 +914,8 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
     const int Kc = (int)(hv[0] & 0xFFFFFFFFull);
     const unsigned long long max_count = hv[1];
     ctx->last_p2_bins = Kc;
-    if (Kc > 0 && ctx->pass2_mode == DBX_PASS2_FP64) {
+    double unit_err = 0.0;  // error per unit weight of the compacted bins' sum (pass2_unit_error)
+    if (Kc > 0 && mode == DBX_PASS2_FP64) {
         const int qblocks = (nq + kDsThreads - 1) / kDsThreads;
         int nsplit = std::max(1, std::min((2048 + qblocks - 1) / qblocks, (Kc + 511) / 512));
         const int kper = (Kc + nsplit - 1) / nsplit;
@@ -871,13 +926,14 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
         for (int r0 = 0; r0 < n_rows; r0 += kDsPmax) {
             const int np = std::min(kDsPmax, n_rows - r0);
             hipLaunchKernelGGL(p2_weights_kernel, dim3((Kc + 255) / 256), dim3(256), 0, ctx->stream, h, d_rows,
-                               r0, np, kz, Kc, A);
+                               r0, np, kz, Kc, A, rsum);
             hipLaunchKernelGGL(p2_debye_fp64_kernel, dim3(qblocks, nsplit), dim3(kDsThreads), 0, ctx->stream,
                                kz, Kc, kper, A, np, rstep, d_q, nq, Spart);
             hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)np * nq + 255) / 256)),
                                dim3(256), 0, ctx->stream, Spart, nsplit, r0, np, nq, nullptr, 0.0, self, low,
                                nlow, rstep, d_q, d_S);
         }
+        unit_err = pass2_unit_error(mode, kper, nsplit, 1);
     } else if (Kc > 0) {
         // balanced count digits: 2 N < 127 256^(I - 1)
         int I = 1;
@@ -937,12 +993,16 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
         hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
                            dim3(256), 0, ctx->stream, Spart, nsplit, 0, n_rows, nq, rsum, kMfShift, self, low,
                            nlow, rstep, d_q, d_S);
+        unit_err = pass2_unit_error(mode, kper, nsplit, I + kMfJ - 1);
     } else {
         // no bin beyond the self-pair entries: S is their part alone
         hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
                            dim3(256), 0, ctx->stream, nullptr, 0, 0, n_rows, nq, nullptr, 0.0, self, low, nlow,
                            rstep, d_q, d_S);
     }
+    if (d_B)
+        hipLaunchKernelGGL(p2_row_bounds_kernel, dim3((n_rows + 63) / 64), dim3(64), 0, ctx->stream,
+                           Kc > 0 ? rsum : nullptr, self, low, nlow, n_rows, unit_err, d_B);
     DBX_HIP(hipGetLastError());
     DBX_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     ctx->p2_timed = true;
@@ -968,7 +1028,7 @@ P2Mem &ctx_pass2_mem(dbx_ctx *ctx)
 }
 
 void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::vector<RowDesc> &rows,
-                          double rstep, const double *d_q, int nq, double *d_S)
+                          double rstep, const double *d_q, int nq, double *d_S, double *d_B, int mode)
 {
     const int n_rows = (int)rows.size();
     DBX_REQUIRE(n_rows > 0 && nq > 0, DBX_ERR_ARG, "empty pass-2 problem");
@@ -979,31 +1039,32 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     }
     DBX_REQUIRE(rows[0].n_bins < (int64_t)INT32_MAX, DBX_ERR_ARG, "too many bins");
     structure_core(ctx, ctx_pass2_mem(ctx), d_hist_base, rows.data(), n_rows, rows[0].n_bins, 0, rstep, d_q, nq,
-                   d_S);
+                   d_S, mode < 0 ? ctx->pass2_mode : mode, d_B);
 }
 
 void debye_structure_mem(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
-                         int first_n, const double *d_q, int nq, double *d_S)
+                         int first_n, const double *d_q, int nq, double *d_S, double *d_B, int mode)
 {
     DBX_REQUIRE(P > 0 && nq > 0 && n_bins > 0, DBX_ERR_ARG, "empty pass-2 problem");
     DBX_REQUIRE(n_bins < (int64_t)INT32_MAX, DBX_ERR_ARG, "too many bins");
-    structure_core(ctx, mem, d_hist, nullptr, P, n_bins, first_n, rstep, d_q, nq, d_S);
+    structure_core(ctx, mem, d_hist, nullptr, P, n_bins, first_n, rstep, d_q, nq, d_S,
+                   mode < 0 ? ctx->pass2_mode : mode, d_B);
 }
 
 void debye_structure(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,
-                     int first_n, const double *d_q, int nq, double *d_S)
+                     int first_n, const double *d_q, int nq, double *d_S, double *d_B, int mode)
 {
-    debye_structure_mem(ctx, ctx_pass2_mem(ctx), d_hist, P, n_bins, rstep, first_n, d_q, nq, d_S);
+    debye_structure_mem(ctx, ctx_pass2_mem(ctx), d_hist, P, n_bins, rstep, first_n, d_q, nq, d_S, d_B, mode);
 }
 
-void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const std::vector<int4> &mrows,
-                    int nq, double *d_I)
+void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const double *d_B,
+                    const std::vector<int4> &mrows, int nq, double *d_I, double *d_rel)
 {
     const int nm = (int)mrows.size();
     if (nm == 1) {
         // one model: its row range as an argument, no copy
-        hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, 1), dim3(256), 0, ctx->stream, d_ff, d_S,
-                           nullptr, mrows[0], nq, d_I);
+        hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, 1), dim3(256), 0, ctx->stream, d_ff, d_S, d_B,
+                           nullptr, mrows[0], nq, d_I, d_rel);
         DBX_HIP(hipGetLastError());
         return;
     }
@@ -1011,14 +1072,44 @@ void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const s
     std::copy(mrows.begin(), mrows.end(), hm);
     int4 *dm = (int4 *)scratch(ctx, kSlotRows, sizeof(int4) * nm);
     DBX_HIP(hipMemcpyAsync(dm, hm, sizeof(int4) * nm, hipMemcpyHostToDevice, ctx->stream));
-    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, ctx->stream, d_ff, d_S, dm,
-                       make_int4(0, 0, 0, 0), nq, d_I);
+    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, ctx->stream, d_ff, d_S, d_B, dm,
+                       make_int4(0, 0, 0, 0), nq, d_I, d_rel);
     DBX_HIP(hipGetLastError());
 }
 
-void combine_intensity(dbx_ctx *ctx, const double *d_ff, const double *d_S, int P, int nq, double *d_I)
+void combine_intensity(dbx_ctx *ctx, const double *d_ff, const double *d_S, const double *d_B, int P, int nq,
+                       double *d_I, double *d_rel)
 {
-    combine_models(ctx, d_ff, d_S, std::vector<int4>{make_int4(0, P, 0, 0)}, nq, d_I);
+    combine_models(ctx, d_ff, d_S, d_B, std::vector<int4>{make_int4(0, P, 0, 0)}, nq, d_I, d_rel);
+}
+
+// *out = the largest of v[0..n) as its bit pattern (non-negative doubles order like their bits;
+// a NaN sorts above every number), one workgroup
+__global__ __launch_bounds__(1024) void p2_max_bits_kernel(const double *__restrict__ v, int n,
+                                                            unsigned long long *__restrict__ out)
+{
+    __shared__ unsigned long long sm[1024 / 64];
+    unsigned long long m = 0;
+    for (int i = threadIdx.x; i < n; i += 1024) {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, v[i]);
+        m = b > m ? b : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 1024 / 64; w++) m = sm[w] > m ? sm[w] : m;
+        *out = m;
+    }
+}
+
+void launch_max_bits(hipStream_t st, const double *d_v, int n, unsigned long long *d_out)
+{
+    hipLaunchKernelGGL(p2_max_bits_kernel, dim3(1), dim3(1024), 0, st, d_v, n, d_out);
+    DBX_HIP(hipGetLastError());
 }
 
 // stream-explicit launches for callers that record them into a graph (no allocation, no copy)
@@ -1028,11 +1119,11 @@ void launch_prefactor(hipStream_t st, const GroupPair *d_pairs, int n, const dou
     DBX_HIP(hipGetLastError());
 }
 
-void launch_combine(hipStream_t st, const double *d_ff, const double *d_S, const int4 *d_mrows, int nm, int nq,
-                    double *d_I, int4 m0)
+void launch_combine(hipStream_t st, const double *d_ff, const double *d_S, const double *d_B, const int4 *d_mrows,
+                    int nm, int nq, double *d_I, double *d_rel, int4 m0)
 {
-    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, st, d_ff, d_S, d_mrows, m0,
-                       nq, d_I);
+    hipLaunchKernelGGL(p2_combine_kernel, dim3((nq + 255) / 256, nm), dim3(256), 0, st, d_ff, d_S, d_B, d_mrows,
+                       m0, nq, d_I, d_rel);
     DBX_HIP(hipGetLastError());
 }
 

@@ -72,13 +72,15 @@ struct Request {
     uint64_t *d_oor = nullptr;
     double *d_S = nullptr;  // [P][nq]
     double *d_q = nullptr;
-    double *d_ff = nullptr;  // [P][nq], reweight
-    double *d_I = nullptr;   // [nq], reweight
+    double *d_ff = nullptr;  // [P][nq]: the step's prefactors, then the last reweight's
+    double *d_I = nullptr;   // [nq]
+    double *d_B = nullptr;   // [P] error bound of each S_p
+    double *d_rel = nullptr; // [nq] relative error bound of d_I
     GroupPair *d_pairs = nullptr;
     int4 *d_mrows = nullptr;
     size_t bytes = 0;
     // results
-    std::vector<double> I;
+    std::vector<double> I, rel;
     uint64_t oor = 0;
     int state = kQueued;
     int err = DBX_OK;
@@ -88,7 +90,7 @@ struct Request {
     hipGraph_t rw_graph = nullptr;
     hipGraphExec_t rw_exec = nullptr;
     GroupPair *h_pairs = nullptr;
-    double *h_I = nullptr;
+    double *h_I = nullptr;   // [2 nq]: I, then its relative error bound
 };
 
 }  // namespace dbx
@@ -147,8 +149,8 @@ static size_t request_bytes(const Request &r)
 {
     const size_t nq = r.q.size(), P = (size_t)r.P;
     return al256(r.image.size()) + al256(sizeof(uint64_t) * P * (size_t)r.n_bins) + al256(8) +
-           2 * al256(sizeof(double) * P * nq) + al256(sizeof(double) * nq) + al256(sizeof(GroupPair) * P) +
-           al256(sizeof(int4));
+           2 * al256(sizeof(double) * P * nq) + 2 * al256(sizeof(double) * nq) + al256(sizeof(double) * P) +
+           al256(sizeof(GroupPair) * P) + al256(sizeof(int4));
 }
 
 // every device buffer of the request from the arena; false (nothing kept) when one does not fit
@@ -177,6 +179,8 @@ static bool alloc_request(DevArena &A, Request &r)
     r.d_S = (double *)need(take(sizeof(double) * P * nq));
     r.d_ff = (double *)need(take(sizeof(double) * P * nq));
     r.d_I = (double *)need(take(sizeof(double) * nq));
+    r.d_rel = (double *)need(take(sizeof(double) * nq));
+    r.d_B = (double *)need(take(sizeof(double) * P));
     r.d_pairs = (GroupPair *)need(take(sizeof(GroupPair) * P));
     r.d_mrows = (int4 *)need(take(sizeof(int4)));
     if (!ok) {
@@ -204,6 +208,7 @@ static void free_request_device(DevArena &A, Request &r)
     r.m.d_alist = nullptr;
     r.d_hist = nullptr;
     r.d_S = nullptr;
+    r.d_B = nullptr;
     if (r.rw_exec) (void)hipGraphExecDestroy(r.rw_exec);
     if (r.rw_graph) (void)hipGraphDestroy(r.rw_graph);
     r.rw_exec = nullptr;
@@ -367,11 +372,12 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
         groups.push_back({g0, g1});
         g0 = g1;
     }
-    // every group's intensities and out-of-range counts land in one pinned block each, read
-    // after the one synchronisation of the step
+    // every group's intensities, their bounds and the out-of-range counts land in one pinned
+    // block each, read after the step's synchronisation
     size_t nI = 0;
     for (Request *r : batch) nI += r->q.size();
     double *hI = (double *)pinned(ctx, kPinStepI, sizeof(double) * nI);
+    double *hrel = (double *)pinned(ctx, kPinStepRel, sizeof(double) * nI);
     uint64_t *hoor = (uint64_t *)pinned(ctx, kPinOor, sizeof(uint64_t) * batch.size());
     std::vector<size_t> I_off(order.size());
     size_t io = 0;
@@ -404,25 +410,51 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
                     pairs.push_back(make_group_pair(r->params[a], r->params[b]));
                 }
         }
-        const size_t R = rows.size();
-        double *buf = (double *)scratch(ctx, kSlotQ, sizeof(double) * (size_t)nq * (2 * R + grp.size()));
-        double *dff = buf, *dS = buf + R * nq, *dI = dS + R * nq;
+        const size_t R = rows.size(), G = grp.size();
+        double *buf = (double *)scratch(ctx, kSlotQ, sizeof(double) * ((size_t)nq * (2 * R + 2 * G) + R));
+        double *dff = buf, *dS = buf + R * nq, *dI = dS + R * nq, *drel = dI + G * nq, *dB = drel + G * nq;
         const double *dq = order[grp[0]]->d_q;
         prefactor_rows(ctx, pairs, dq, nq, dff);
-        debye_structure_rows(ctx, base, rows, rstep, dq, nq, dS);
-        combine_models(ctx, dff, dS, mrows, nq, dI);
-        for (size_t gi = 0; gi < grp.size(); gi++) {
+        debye_structure_rows(ctx, base, rows, rstep, dq, nq, dS, dB);
+        combine_models(ctx, dff, dS, dB, mrows, nq, dI, drel);
+        for (size_t gi = 0; gi < G; gi++) {
             Request *r = order[grp[gi]];
+            const size_t rn = (size_t)r->P * nq, r0 = (size_t)mrows[gi].x;
             DBX_HIP(hipMemcpyAsync(hI + I_off[grp[gi]], dI + gi * (size_t)nq, sizeof(double) * (size_t)nq,
                                    hipMemcpyDeviceToHost, ctx->stream));
-            DBX_HIP(hipMemcpyAsync(r->d_S, dS + (size_t)mrows[gi].x * nq, sizeof(double) * (size_t)r->P * nq,
-                                   hipMemcpyDeviceToDevice, ctx->stream));
+            DBX_HIP(hipMemcpyAsync(hrel + I_off[grp[gi]], drel + gi * (size_t)nq, sizeof(double) * (size_t)nq,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            // the request keeps its own S_p, prefactors and bounds (reweighting, FP64 recomputation)
+            DBX_HIP(hipMemcpyAsync(r->d_S, dS + r0 * nq, sizeof(double) * rn, hipMemcpyDeviceToDevice, ctx->stream));
+            DBX_HIP(hipMemcpyAsync(r->d_ff, dff + r0 * nq, sizeof(double) * rn, hipMemcpyDeviceToDevice, ctx->stream));
+            DBX_HIP(hipMemcpyAsync(r->d_B, dB + r0, sizeof(double) * r->P, hipMemcpyDeviceToDevice, ctx->stream));
         }
     }
     DBX_HIP(hipStreamSynchronize(ctx->stream));
+    // requests whose bound misses the target: their partials again with FP64 arithmetic
+    bool refined = false;
+    if (ctx->pass2_mode == DBX_PASS2_MFMA) {
+        for (size_t i = 0; i < order.size(); i++) {
+            Request *r = order[i];
+            const int nq = (int)r->q.size();
+            bool bad = false;
+            for (int k = 0; k < nq && !bad; k++) bad = exceeds_target(hrel[I_off[i] + k], ctx->rel_err_target);
+            if (!bad || hoor[i]) continue;
+            debye_structure(ctx, r->d_hist, r->P, r->n_bins, rstep, r->first_n, r->d_q, nq, r->d_S, r->d_B,
+                            DBX_PASS2_FP64);
+            launch_combine(ctx->stream, r->d_ff, r->d_S, r->d_B, nullptr, 1, nq, r->d_I, r->d_rel,
+                           make_int4(0, r->P, 0, 0));
+            DBX_HIP(hipMemcpyAsync(hI + I_off[i], r->d_I, sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream));
+            DBX_HIP(hipMemcpyAsync(hrel + I_off[i], r->d_rel, sizeof(double) * nq, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            refined = true;
+        }
+        if (refined) DBX_HIP(hipStreamSynchronize(ctx->stream));
+    }
     for (size_t i = 0; i < order.size(); i++) {
         Request *r = order[i];
         r->I.assign(hI + I_off[i], hI + I_off[i] + r->q.size());
+        r->rel.assign(hrel + I_off[i], hrel + I_off[i] + r->q.size());
         r->oor = hoor[i];
         std::vector<char>().swap(r->image);  // its upload is complete
     }
@@ -570,7 +602,12 @@ int dbx_server_open(int device, const dbx_server_config *cfg, dbx_server **out)
     for (int i = 0; i < 2 && rc == DBX_OK; i++) rc = dbx_open(device, 0, &s->lane[i]);
     if (rc == DBX_OK) rc = dbx_open(device, 0, &s->rw_ctx);
     if (rc == DBX_OK) {
-        for (int i = 0; i < 2; i++) s->lane[i]->pass2_mode = s->cfg.pass2_mode;
+        // 0: the default target; negative: never recompute
+        const double target = s->cfg.rel_err_target == 0.0 ? 1e-6 : std::max(0.0, s->cfg.rel_err_target);
+        for (int i = 0; i < 2; i++) {
+            s->lane[i]->pass2_mode = s->cfg.pass2_mode;
+            s->lane[i]->rel_err_target = target;
+        }
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) rc = DBX_ERR_HIP;
         size_t budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : (size_t)(0.75 * (double)fr);
@@ -722,7 +759,7 @@ int dbx_poll(dbx_server *s, uint64_t ticket, int *state)
     return DBX_OK;
 }
 
-int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms)
+int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms, double *rel_err_out)
 {
     if (!s) return DBX_ERR_ARG;
     ApiCall call(s);
@@ -753,6 +790,7 @@ int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms)
     if (latency_ms) *latency_ms = r->t_done - r->t_submit;
     if (r->state != kDone && r->state != kFailed) return DBX_ERR_STATE;
     if (I_out && !r->I.empty()) std::memcpy(I_out, r->I.data(), sizeof(double) * r->I.size());
+    if (rel_err_out && !r->rel.empty()) std::memcpy(rel_err_out, r->rel.data(), sizeof(double) * r->rel.size());
     if (r->state == kFailed) {
         s->err = r->msg;
         return r->err;
@@ -760,7 +798,8 @@ int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms)
     return DBX_OK;
 }
 
-int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params, double *I_out)
+int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params, double *I_out,
+                 double *rel_err_out)
 {
     if (!s || !params || !I_out) return DBX_ERR_ARG;
     ApiCall call(s);
@@ -785,7 +824,7 @@ int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params,
         hipStream_t st = s->rw_ctx->stream;
         if (!r->h_pairs) {
             DBX_HIP(hipHostMalloc((void **)&r->h_pairs, sizeof(GroupPair) * r->P, hipHostMallocDefault));
-            DBX_HIP(hipHostMalloc((void **)&r->h_I, sizeof(double) * nq, hipHostMallocDefault));
+            DBX_HIP(hipHostMalloc((void **)&r->h_I, sizeof(double) * 2 * nq, hipHostMallocDefault));
         }
         const int S = (int)r->group_n.size();
         int p = 0;
@@ -800,14 +839,16 @@ int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params,
             DBX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
             DBX_HIP(hipMemcpyAsync(r->d_pairs, r->h_pairs, sizeof(GroupPair) * r->P, hipMemcpyHostToDevice, st));
             launch_prefactor(st, r->d_pairs, r->P, r->d_q, nq, r->d_ff);
-            launch_combine(st, r->d_ff, r->d_S, r->d_mrows, 1, nq, r->d_I);
+            launch_combine(st, r->d_ff, r->d_S, r->d_B, r->d_mrows, 1, nq, r->d_I, r->d_rel);
             DBX_HIP(hipMemcpyAsync(r->h_I, r->d_I, sizeof(double) * nq, hipMemcpyDeviceToHost, st));
+            DBX_HIP(hipMemcpyAsync(r->h_I + nq, r->d_rel, sizeof(double) * nq, hipMemcpyDeviceToHost, st));
             DBX_HIP(hipStreamEndCapture(st, &r->rw_graph));
             DBX_HIP(hipGraphInstantiate(&r->rw_exec, r->rw_graph, nullptr, nullptr, 0));
         }
         DBX_HIP(hipGraphLaunch(r->rw_exec, st));
         DBX_HIP(hipStreamSynchronize(st));
         std::memcpy(I_out, r->h_I, sizeof(double) * nq);
+        if (rel_err_out) std::memcpy(rel_err_out, r->h_I + nq, sizeof(double) * nq);
     } catch (const Error &e) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->err = e.what();

@@ -116,22 +116,26 @@ int dbx_prefactor(dbx_ctx *ctx, const dbx_group_params *ga, const dbx_group_para
  *   count N_pk once with sinc 1, every other nonzero bin counts 2 N_pk sin(q r_k)/(q r_k);
  *   I(q) = sum_p ffoobb_p(q) S_p(q).
  * hist: uint64[P][n_bins] (host or device memory per hist_is_device).  ffoobb: [P][nq]
- * host FP64.  I_out[nq] host FP64 (overwritten).  S_out ([P][nq], optional) receives S_p. */
+ * host FP64.  I_out[nq] host FP64 (overwritten).  S_out ([P][nq], optional) receives S_p.
+ * rel_err_out ([nq], optional) receives for every I(q) a bound of its relative error against
+ * the exact (FP64-oracle) sum over the same histograms and prefactors, derived from the
+ * arithmetic used (see dbx_set_rel_err_target); every I returned here is within that bound. */
 int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int64_t n_bins,
               double rstep, int first_n, const double *q, int nq, const double *ffoobb,
-              double *I_out, double *S_out);
+              double *I_out, double *S_out, double *rel_err_out);
 
 /* dbx_debye_noalloc: dbx_debye on device memory that allocates nothing.  No counterpart in the
  * reference: it is for callers that cannot allocate device or pinned memory during the call
  * (tight serving loops, harnesses that forbid hipMalloc).  d_hist [P][n_bins], d_q [nq],
- * d_ffoobb [P][nq] and d_I [nq] (and d_S [P][nq], optional) are device pointers; the result
- * equals dbx_debye's on the same inputs.  Every buffer pass 2 uses comes from one fixed 64 MiB
+ * d_ffoobb [P][nq] and d_I [nq] (and d_S [P][nq], d_rel_err [nq], optional) are device
+ * pointers; the results equal dbx_debye's on the same inputs.  Every buffer pass 2 uses comes from one fixed 64 MiB
  * device region per device, set up by dbx_open; a problem whose buffers exceed it returns
  * DBX_ERR_BUDGET (dbx_debye has no such limit).  The region is shared by every call on a
  * device and keeps the last call's contents: calls that overlap on one device (two contexts
  * or threads at once) race.  The call waits for its own work before returning. */
 int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep, int first_n,
-                      const double *d_q, int nq, const double *d_ffoobb, double *d_I, double *d_S);
+                      const double *d_q, int nq, const double *d_ffoobb, double *d_I, double *d_S,
+                      double *d_rel_err);
 
 /* Pass-2 arithmetic of a context (default DBX_PASS2_MFMA):
  *   DBX_PASS2_MFMA: matrix-core contraction on signed 8-bit digits with exact 32-bit integer
@@ -140,6 +144,13 @@ int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bin
  *   DBX_PASS2_FP64: FP64 vector arithmetic throughout (the reference's debyeRowSum). */
 enum { DBX_PASS2_MFMA = 0, DBX_PASS2_FP64 = 1 };
 int dbx_set_pass2_mode(dbx_ctx *ctx, int mode);
+/* Error target of a context's pass 2 (default 1e-6).  Every I(q) comes with a bound of its
+ * relative error: (sum_p |ffoobb_p(q)| B_p + rounding of the final sum) / |I(q)|, where B_p is
+ * the sum of the partial's weights 2 N_pk times the largest error per unit weight of the
+ * arithmetic (matrix cores: the sinc digits' rounding to 1 / (3 2^39) plus the FP64 sine and
+ * sums; FP64: the rounding of the sums).  A model whose bound exceeds the target at any q
+ * has its partials recomputed with FP64 arithmetic (target <= 0: never). */
+int dbx_set_rel_err_target(dbx_ctx *ctx, double target);
 
 /* ---- whole model ----------------------------------------------------------------------
  * One model end to end on the device: pass 1, compaction, prefactors, pass 2.
@@ -147,7 +158,7 @@ int dbx_set_pass2_mode(dbx_ctx *ctx, int mode);
  * histograms.  Equivalent to the reference's subgroup-pair loop (cuDebye-v15.cu:941-975). */
 int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params,
                   double rstep, int64_t n_bins, int first_n, const double *q, int nq,
-                  double *I_out, uint64_t *hist_out);
+                  double *I_out, uint64_t *hist_out, double *rel_err_out);
 
 /* ---- batch of models ------------------------------------------------------------------
  * Many models end to end in one pass-1 launch (a merged work list over all models) and one
@@ -167,7 +178,7 @@ typedef struct dbx_batch_item {
 } dbx_batch_item;
 int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, double rstep,
                         const double *q, int nq, double *I_out, uint64_t *n_out_of_range,
-                        uint64_t *const *hist_out);
+                        uint64_t *const *hist_out, double *rel_err_out);
 
 /* ---- serving ---------------------------------------------------------------------------
  * A stream of structure models of uneven size (refinement / multi-simulation style), served
@@ -198,6 +209,7 @@ typedef struct dbx_server_config {
     double batch_window_ms; /* a lane that finds admitted requests waits up to this long for
                                requests still being prepared or admitted, so that a burst of
                                submissions forms one step (0: start at once) */
+    double rel_err_target;  /* as dbx_set_rel_err_target (0: 1e-6; negative: never recompute) */
 } dbx_server_config;
 typedef struct dbx_request {
     const float *x, *y, *z;           /* positions in subgroup order (copied) */
@@ -229,10 +241,13 @@ int dbx_submit(dbx_server *s, const dbx_request *req, uint64_t *ticket);
 /* state: DBX_REQ_QUEUED (waiting for preparation or memory), _RUNNING, _DONE, _FAILED */
 int dbx_poll(dbx_server *s, uint64_t ticket, int *state);
 /* Blocks until the request ends; I_out[nq] (optional) receives I(q), latency_ms (optional)
- * the time from submission to completion.  Returns the request's status. */
-int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms);
-/* I_out[nq] = intensity of a finished DBX_REQ_KEEP request for new params[n_groups]. */
-int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params, double *I_out);
+ * the time from submission to completion, rel_err_out[nq] (optional) the relative error bound
+ * of each I(q) (as dbx_debye's).  Returns the request's status. */
+int dbx_wait(dbx_server *s, uint64_t ticket, double *I_out, double *latency_ms, double *rel_err_out);
+/* I_out[nq] = intensity of a finished DBX_REQ_KEEP request for new params[n_groups];
+ * rel_err_out[nq] (optional): its relative error bound. */
+int dbx_reweight(dbx_server *s, uint64_t ticket, const dbx_group_params *params, double *I_out,
+                 double *rel_err_out);
 /* hist_out[P][n_bins] of a finished DBX_REQ_KEEP request (host). */
 int dbx_request_histogram(dbx_server *s, uint64_t ticket, uint64_t *hist_out);
 /* Frees a finished request (and its resident memory). */

@@ -65,6 +65,10 @@ int dbx_debug_set_profiling(dbx_ctx *ctx, int on);
 int dbx_debug_pass2_profile(dbx_ctx *ctx, double *kernel_ms, double *mfma_flops, double *useful_flops,
                             int *n_bins_compacted, int *count_digits);
 
+/* Models whose partials the last pass 2 on ctx recomputed with FP64 arithmetic because their
+ * matrix-core error bound exceeded the context's target (dbx_set_rel_err_target). */
+int dbx_debug_pass2_refined(const dbx_ctx *ctx, int *n_models);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,15 +53,25 @@ def test_rocksalt_sphere_full_size(ctx, diameter):
         for b in range(a, len(pr)):
             ff[p] = ctx.prefactor(pr[a], pr[b], m.q)
             p += 1
+    # the matrix-core sums with their error bound (no FP64 recomputation), against FP64 sums
     ctx.set_pass2_mode(cd.DBX_PASS2_FP64)
     try:
-        I64 = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n)
+        I64, b64 = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n, with_err=True)
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+        ctx.set_rel_err_target(0)
+        I, bmf = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n, with_err=True)
     finally:
         ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
-    I = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n)
-    rel = np.max(np.abs(I - I64) / np.abs(I64))
-    print(f"max rel I(Q) matrix cores vs FP64: {rel:.3e}", flush=True)
-    assert rel < 1e-6
+        ctx.set_rel_err_target(1e-6)
+    err = np.abs(I - I64) / np.abs(I64)
+    print(f"max rel I(Q) matrix cores vs FP64: {err.max():.3e}, matrix-core bound {bmf.max():.3e}, "
+          f"FP64 bound {b64.max():.3e}", flush=True)
+    assert err.max() < 1e-6
+    assert np.all(err <= bmf + b64)
+    # the default target: every I within 1e-6 by its bound, recomputed in FP64 where needed
+    Id, bd = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n, with_err=True)
+    assert bd.max() <= 1e-6 and ctx.pass2_refined() == int(bmf.max() > 1e-6)
+    assert np.all(np.abs(Id - I64) <= (bd + b64) * np.abs(I64))
 
 
 def test_batch_of_256_models_full_size(ctx):

@@ -157,3 +157,114 @@ def test_device_sin_equals_host_build():
     fn.restype = C.c_int
     assert fn(x.ctypes.data_as(C.c_void_p), len(x), host.ctypes.data_as(C.c_void_p)) == 0
     assert np.array_equal(dev.view(np.int64), host.view(np.int64))
+
+
+@pytest.mark.parametrize("maker", ["c1", "rand2", "rand5"])
+def test_error_bound_covers_error_against_oracle(ctx, maker):
+    """Every I(q) comes with a relative error bound; the error against the oracle's compensated
+    FP64 sums stays within it, on the matrix cores (no FP64 recomputation) and in FP64."""
+    m = {"c1": lambda: synth.fcc_sphere(nq=1024),
+         "rand2": lambda: synth.small_random(3000, 2, 31, nq=512),
+         "rand5": lambda: synth.small_random(4000, 5, 32, nq=300)}[maker]()
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    params = [_gp(p) for p in m.params]
+    H = ctx.histogram(mod, m.rstep, m.n_bins)
+    Iref, mag = _oracle_I(m, H, m.first_n)
+    try:
+        ctx.set_rel_err_target(0)
+        for mode in MODES:
+            ctx.set_pass2_mode(mode)
+            I, rel = ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n, with_err=True)
+            err = np.abs(I - Iref) / np.abs(Iref)
+            print(f"{maker} mode {mode}: error {err.max():.2e}, bound {rel.max():.2e}, worst ratio "
+                  f"{(err / rel).max():.3f}")
+            # 1e-14: the oracle's prefactors (its own exp) differ from the device's by an ulp or two
+            assert np.all(err <= rel + 1e-14), (mode, np.argmax(err / rel))
+            assert ctx.pass2_refined() == 0
+    finally:
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+        ctx.set_rel_err_target(1e-6)
+    mod.close()
+
+
+@pytest.mark.parametrize("P,cmax,nq", [(5, 2 ** 30, 100), (40, 2 ** 36, 100), (130, 2 ** 30, 100)])
+def test_error_bound_covers_quantisation_error(ctx, P, cmax, nq):
+    """Rows of 3000 bins with counts up to 2^36 (up to 5 count digits, every workgroup tile): the
+    matrix-core sum of each I(q) = sum_p S_p(q) is within its bound of the oracle's FP64 sum."""
+    rng = np.random.default_rng(nq + P)
+    nb = 20000
+    H = np.zeros((P, nb), np.uint64)
+    for p in range(P):
+        idx = rng.choice(np.arange(1, nb), 3000, replace=False)
+        H[p, idx] = rng.integers(1, cmax, 3000).astype(np.uint64)
+    K = np.linspace(0.3, 20.0, nq)
+    ff = np.ones((P, nq))
+    Iref = np.zeros(nq)
+    for p in range(P):
+        nz = np.nonzero(H[p])[0]
+        Iref += O.debye_sum(nz, H[p][nz].astype(float), 1e-3, K, np.ones(nq), first_n=8)
+    ctx.set_rel_err_target(0)
+    try:
+        I, rel = ctx.debye(H, 1e-3, K, ff, first_n=8, with_err=True)
+    finally:
+        ctx.set_rel_err_target(1e-6)
+    err = np.abs(I - Iref) / np.abs(Iref)
+    print(f"P={P}: error {err.max():.2e}, bound {rel.max():.2e}, worst ratio {(err / rel).max():.3f}")
+    assert np.all(err <= rel)
+
+
+def test_error_target_recomputes_in_fp64(ctx):
+    """A target below the matrix-core bound: the model's partials are recomputed with FP64
+    arithmetic, giving the FP64 mode's I bit for bit and its (smaller) bound; a loose target
+    leaves the matrix-core result."""
+    m = synth.small_random(3000, 3, 33, nq=400)
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    params = [_gp(p) for p in m.params]
+    args = (mod, params, m.rstep, m.n_bins, m.q)
+    try:
+        ctx.set_pass2_mode(cd.DBX_PASS2_FP64)
+        I64, rel64 = ctx.intensity(*args, first_n=m.first_n, with_err=True)
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+        ctx.set_rel_err_target(1.0)
+        Imf, relmf = ctx.intensity(*args, first_n=m.first_n, with_err=True)
+        assert ctx.pass2_refined() == 0 and relmf.max() > rel64.max()
+        ctx.set_rel_err_target(relmf.min() / 2)
+        I, rel = ctx.intensity(*args, first_n=m.first_n, with_err=True)
+        assert ctx.pass2_refined() == 1
+        assert np.array_equal(I, I64) and np.array_equal(rel, rel64)
+    finally:
+        ctx.set_rel_err_target(1e-6)
+    mod.close()
+
+
+def test_error_target_in_a_batch_recomputes_only_the_models_that_miss_it(ctx):
+    """In a batch, only the models whose bound misses the target are recomputed in FP64: each
+    result within 1e-12 of the FP64 mode on that model alone, the others equal to the batch
+    without a target."""
+    models = [synth.small_random(n, s, 40 + i, nq=200) for i, (n, s) in enumerate([(800, 2), (3000, 3), (1500, 1),
+                                                                                     (2500, 2)])]
+    items = [(ctx.model(m.x, m.y, m.z, m.group_n), [_gp(p) for p in m.params], m.n_bins, m.first_n)
+             for m in models]
+    q = models[0].q
+    try:
+        ctx.set_rel_err_target(0)
+        I0, rel0 = ctx.batch_intensity(items, 1e-3, q, with_err=True)
+        worst = rel0.max(axis=1)
+        target = float(np.median(worst))
+        ctx.set_rel_err_target(target)
+        I1, rel1 = ctx.batch_intensity(items, 1e-3, q, with_err=True)
+        redo = worst > target
+        assert ctx.pass2_refined() == int(redo.sum()) > 0
+        ctx.set_pass2_mode(cd.DBX_PASS2_FP64)
+        for i, (m, it) in enumerate(zip(models, items)):
+            if redo[i]:
+                I64 = ctx.intensity(it[0], it[1], 1e-3, it[2], q, first_n=it[3])
+                assert np.max(np.abs(I1[i] - I64) / np.abs(I64)) < 1e-12, i
+                assert rel1[i].max() <= target
+            else:
+                assert np.array_equal(I1[i], I0[i]) and np.array_equal(rel1[i], rel0[i]), i
+    finally:
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+        ctx.set_rel_err_target(1e-6)
+        for it in items:
+            it[0].close()

@@ -280,12 +280,15 @@ def c5_reference(ctx):
 def _check_c5(srv, models, tickets, alone, oracle):
     worst = 0.0
     for m, t, (Ia, Ha), (Ho, Io) in zip(models, tickets, alone, oracle):
-        I, lat, rc = _finish(srv, t, timeout=120.0)
+        _finish(srv, t, timeout=120.0)
+        I, lat, rc, bound = srv.wait(t, with_err=True)
         assert rc == cd.DBX_OK and lat > 0
         H = srv.histogram(t)
         assert np.array_equal(H, Ha), m.name
         assert np.array_equal(H, Ho), m.name
         assert _rel(I, Ia) < MIX_TOL, m.name
+        # each I within its reported bound of the oracle (1e-14: the oracle's own prefactors)
+        assert bound.max() <= ORACLE_TOL and np.all(np.abs(I - Io) / np.abs(Io) <= bound + 1e-14), m.name
         worst = max(worst, _rel(I, Io))
         srv.release(t)
     assert worst < ORACLE_TOL, worst
@@ -374,3 +377,31 @@ def test_serve_close_while_another_thread_waits():
     srv.close()
     th.join(timeout=60)
     assert not th.is_alive() and out and out[0] in (cd.DBX_OK, cd.DBX_ERR_STATE)
+
+
+def test_serve_error_target_recomputes_in_fp64(ctx):
+    """A server whose error target lies below the matrix-core bounds: every request's partials
+    are recomputed with FP64 arithmetic, giving the FP64 result of the request alone, with its
+    bound; a reweight of a kept request reports the bound for its new form factors."""
+    models = _models(5, 41, nqs=(96, 160))
+    ctx.set_pass2_mode(cd.DBX_PASS2_FP64)
+    try:
+        ref = []
+        for m in models:
+            mod = ctx.model(m.x, m.y, m.z, m.group_n)
+            ref.append(ctx.intensity(mod, [_gp(p) for p in m.params], m.rstep, m.n_bins, m.q, first_n=m.first_n,
+                                     with_err=True))
+            mod.close()
+    finally:
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+    with cd.Server(0, rel_err_target=1e-15) as srv:
+        ts = [_submit(srv, m) for m in models]
+        for m, t, (I64, b64) in zip(models, ts, ref):
+            _finish(srv, t)
+            I, _, rc, bound = srv.wait(t, with_err=True)
+            assert rc == cd.DBX_OK
+            assert _rel(I, I64) < 1e-12 and np.allclose(bound, b64, rtol=1e-6, atol=0)
+        Ir, br = srv.reweight(ts[0], [_gp(p) for p in models[0].params], with_err=True)
+        assert _rel(Ir, ref[0][0]) < 1e-12 and np.allclose(br, ref[0][1], rtol=1e-6, atol=0)
+        for t in ts:
+            srv.release(t)

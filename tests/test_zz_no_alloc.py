@@ -73,11 +73,12 @@ def _noalloc(ctx, dev, H, m, ff, with_S=True):
     dH, dq, dff = dev.put(H), dev.put(m.q), dev.put(ff)
     dI = dev.alloc(8 * nq)
     dS = dev.alloc(8 * P * nq) if with_S else None
-    rc = cd.lib().dbx_debye_noalloc(ctx.handle, dH, P, nb, m.rstep, m.first_n, dq, nq, dff, dI, dS)
+    drel = dev.alloc(8 * nq)
+    rc = cd.lib().dbx_debye_noalloc(ctx.handle, dH, P, nb, m.rstep, m.first_n, dq, nq, dff, dI, dS, drel)
     assert rc == cd.DBX_OK, cd.lib().dbx_last_error(ctx.handle).decode()
     I = dev.get(dI, nq, np.float64)
     S = dev.get(dS, (P, nq), np.float64) if with_S else None
-    return I, S
+    return I, S, dev.get(drel, nq, np.float64)
 
 
 @pytest.mark.parametrize("mode", [cd.DBX_PASS2_MFMA, cd.DBX_PASS2_FP64], ids=["mfma", "fp64"])
@@ -91,9 +92,10 @@ def test_noalloc_equals_debye_two_calls_second_smaller(mode):
         assert probs[1][1].shape[0] < probs[0][1].shape[0] and probs[1][1].shape[1] < probs[0][1].shape[1]
         try:
             for i, (m, H, ff) in enumerate(probs):
-                Iref, Sref = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n, with_S=True)
-                I, S = _noalloc(ctx, dev, H, m, ff, with_S=(i == 0))
+                Iref, Sref, rref = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n, with_S=True, with_err=True)
+                I, S, rel = _noalloc(ctx, dev, H, m, ff, with_S=(i == 0))
                 assert np.array_equal(I, Iref), (i, np.max(np.abs(I - Iref) / np.abs(Iref)))
+                assert np.array_equal(rel, rref), i
                 if S is not None:
                     assert np.array_equal(S, Sref), i
         finally:
@@ -112,13 +114,13 @@ def test_noalloc_refuses_problems_beyond_its_region():
             q = np.linspace(0.5, 5.0, nq)
             dq, dff = dev.put(q), dev.put(np.ones(nq))
             dI = dev.alloc(8 * nq)
-            rc = cd.lib().dbx_debye_noalloc(ctx.handle, dH, 1, nb, 1e-3, 8, dq, nq, dff, dI, None)
+            rc = cd.lib().dbx_debye_noalloc(ctx.handle, dH, 1, nb, 1e-3, 8, dq, nq, dff, dI, None, None)
             assert rc == cd.DBX_ERR_BUDGET
             assert "fixed scratch region" in cd.lib().dbx_last_error(ctx.handle).decode()
             m = synth.small_random(500, 1, 73, nq=64)
             H, ff = _problem(ctx, m)
             Iref = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n)
-            I, _ = _noalloc(ctx, dev, H, m, ff, with_S=False)
+            I, _, _ = _noalloc(ctx, dev, H, m, ff, with_S=False)
             assert np.array_equal(I, Iref)
         finally:
             dev.free()
