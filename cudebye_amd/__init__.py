@@ -120,6 +120,7 @@ def lib():
             "dbx_debye_noalloc": (i32, [vp, vp, i32, i64, f64, i32, vp, i32, vp, vp, vp, vp]),
             "dbx_intensity": (i32, [vp, vp, vp, f64, i64, i32, vp, i32, vp, vp, vp]),
             "dbx_set_rel_err_target": (i32, [vp, f64]),
+            "dbx_view_deb": (i32, [C.c_char_p, vp, C.c_char_p, C.c_char_p, C.c_char_p, P(i64)]),
             "dbx_debug_pass2_refined": (i32, [vp, P(i32)]),
             "dbx_run_deb": (i32, [C.c_char_p, i32]),
             "dbx_run_folder": (i32, [C.c_char_p, i32]),
@@ -163,7 +164,7 @@ EXPORTS = ("dbx_open", "dbx_close", "dbx_last_error", "dbx_stream", "dbx_q_from_
            "dbx_submit", "dbx_poll", "dbx_wait", "dbx_reweight", "dbx_request_histogram",
            "dbx_release", "dbx_server_stats_get", "dbx_read_debx", "dbx_run_folder",
            "dbx_deb_header_defaults", "dbx_xyz_to_deb", "dbx_average_debi", "dbx_debye_noalloc",
-           "dbx_set_rel_err_target")
+           "dbx_set_rel_err_target", "dbx_view_deb")
 
 
 def _p(a):
@@ -259,6 +260,24 @@ def average_debi(paths, norm="none", step=0.02):
     if rc != DBX_OK:
         raise DebyeError(rc, "average_debi")
     return tt, I
+
+
+def view_deb(deb, limits=None, svg=None, xyz=None, radii=None):
+    """Structure view of a .deb file (Matlab/viewdeb.m): the atoms inside `limits` (lo, hi) for
+    every axis or ((xlo, xhi), (ylo, yhi), (zlo, zhi)); default -10..10) drawn into `svg` and/or
+    listed in `xyz`.  Returns the number of atoms kept."""
+    lim = None
+    if limits is not None:
+        a = np.asarray(limits, np.float64)
+        a = np.tile(a.reshape(1, 2), (3, 1)) if a.size == 2 else a.reshape(3, 2)
+        lim = np.ascontiguousarray(a.ravel())
+    enc = lambda v: None if v is None else os.fspath(v).encode()
+    n = C.c_int64(0)
+    rc = lib().dbx_view_deb(os.fspath(deb).encode(), None if lim is None else _p(lim), enc(svg), enc(xyz),
+                            enc(radii), C.byref(n))
+    if rc != DBX_OK:
+        raise DebyeError(rc, f"cannot view {deb}")
+    return int(n.value)
 
 
 def lds_atomic_rate():

@@ -16,7 +16,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libdebye.so")
 CLI = os.path.join(HERE, "bin", "debye")
 SOURCES = ["pass1.hip", "pass2.hip", "noalloc.hip", "api.cpp", "server.cpp", "deb_run.cpp", "xyz2deb.cpp",
-           "selftest.hip"]
+           "view.cpp", "selftest.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -5,6 +5,9 @@
 // `debye --xyz2deb FILE.xyz [--table PATH] [--rstep R]`: the converter (Matlab/xyz2deb.m).
 // `debye --average [--norm none|int|max] [--step S] -o OUT A.debI B.debI ...`: the averaged
 // pattern of several .debI files (Matlab/plotdebi.m), written as "%.10g\t%.10g" lines.
+// `debye --view FILE.deb [--limits LO HI | --limits XLO XHI YLO YHI ZLO ZHI] [--svg OUT]
+// [--xyz OUT] [--radii PATH]`: the structure view (Matlab/viewdeb.m); without --svg or --xyz it
+// draws FILE.deb.svg.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -19,8 +22,10 @@ static int usage(const char *prog)
                  "usage: %s file.deb [--crlf] [--quiet] [--mfma] [--from-debx]\n"
                  "       %s --folder DIR [--crlf] [--quiet] [--mfma]\n"
                  "       %s --xyz2deb FILE.xyz [--table PATH] [--rstep R]\n"
-                 "       %s --average [--norm none|int|max] [--step S] -o OUT A.debI B.debI ...\n",
-                 prog, prog, prog, prog);
+                 "       %s --average [--norm none|int|max] [--step S] -o OUT A.debI B.debI ...\n"
+                 "       %s --view FILE.deb [--limits LO HI | --limits XLO XHI YLO YHI ZLO ZHI] [--svg OUT] "
+                 "[--xyz OUT] [--radii PATH]\n",
+                 prog, prog, prog, prog, prog);
     return 2;
 }
 
@@ -63,9 +68,49 @@ static int average(int argc, char **argv)
     return std::fclose(f) == 0 ? 0 : 1;
 }
 
+static int view(int argc, char **argv)
+{
+    const char *deb = nullptr, *svg = nullptr, *xyz = nullptr, *radii = nullptr;
+    std::vector<double> lim;
+    for (int i = 2; i < argc; i++) {
+        if (!std::strcmp(argv[i], "--limits")) {
+            // two numbers (every axis) or six (per axis)
+            while (i + 1 < argc && lim.size() < 6) {
+                char *e = nullptr;
+                const double v = std::strtod(argv[i + 1], &e);
+                if (e == argv[i + 1] || *e) break;
+                lim.push_back(v);
+                i++;
+            }
+            if (lim.size() != 2 && lim.size() != 6) return usage(argv[0]);
+        } else if (!std::strcmp(argv[i], "--svg") && i + 1 < argc) {
+            svg = argv[++i];
+        } else if (!std::strcmp(argv[i], "--xyz") && i + 1 < argc) {
+            xyz = argv[++i];
+        } else if (!std::strcmp(argv[i], "--radii") && i + 1 < argc) {
+            radii = argv[++i];
+        } else {
+            deb = argv[i];
+        }
+    }
+    if (!deb) return usage(argv[0]);
+    std::string svg_default;
+    if (!svg && !xyz) {
+        svg_default = std::string(deb) + ".svg";
+        svg = svg_default.c_str();
+    }
+    if (lim.size() == 2) lim = {lim[0], lim[1], lim[0], lim[1], lim[0], lim[1]};
+    int64_t n = 0;
+    const int rc = dbx_view_deb(deb, lim.empty() ? nullptr : lim.data(), svg, xyz, radii, &n);
+    if (rc != DBX_OK) return 1;
+    std::printf("%lld atoms inside the limits\n", (long long)n);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc > 1 && !std::strcmp(argv[1], "--average")) return average(argc, argv);
+    if (argc > 1 && !std::strcmp(argv[1], "--view")) return view(argc, argv);
     const char *path = nullptr;
     const char *table = nullptr;
     double rstep = 0;

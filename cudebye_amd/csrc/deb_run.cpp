@@ -20,18 +20,12 @@
 #include <string>
 #include <vector>
 
+#include "debfile.h"
 #include "internal.h"
 
-namespace {
+namespace dbx {
 
-struct DebFile {
-    int device = 0, block = 0, grid = 0, hist = 0, S = 0;
-    double lambda = 0, tt_min = 0, tt_max = 0, tt_step = 0, rmax = 0, rstep = 0;
-    std::vector<int64_t> n;
-    std::vector<std::string> type;
-    std::vector<dbx_group_params> gp;
-    std::vector<float> x, y, z;
-};
+namespace {
 
 bool blank(const char *s)
 {
@@ -55,6 +49,8 @@ struct Lines {
         return true;
     }
 };
+
+}  // namespace
 
 int parse_deb(const char *path, DebFile &d, std::string &err)
 {
@@ -144,6 +140,8 @@ int parse_deb(const char *path, DebFile &d, std::string &err)
     }
     return DBX_OK;
 }
+
+namespace {
 
 __attribute__((format(printf, 2, 3))) void logf(bool quiet, const char *fmt, ...)
 {
@@ -556,3 +554,5 @@ extern "C" int dbx_run_folder(const char *dir, int flags)
 {
     return no_throw([&] { return run_folder_body(dir, flags); });
 }
+
+}  // namespace dbx

@@ -305,6 +305,17 @@ enum { DBX_AVG_NONE = 0, DBX_AVG_INTEGRAL = 1, DBX_AVG_MAX = 2 };
 int dbx_average_debi(const char *const *paths, int n_files, int norm, double step, double *tt_out, double *I_out,
                      int capacity, int *n_out);
 
+/* Structure view of a .deb file (Matlab/viewdeb.m:1-194): the atoms with lo < coordinate < hi on
+ * every axis (limits[6] = xlo xhi ylo yhi zlo zhi; NULL: -10..10 on every axis, as the
+ * reference) are drawn into svg_path as the reference's 3-D scatter (Matlab's default view,
+ * discs of diameter 2 r 5 points in the element's colour, r and colour from radii_path: rows
+ * "symbol radius red green blue"; NULL: data/atomic_radii.txt beside libdebye.so) and/or
+ * listed in xyz_path ("symbol x y z" lines after a count and a title line).  Elements are the
+ * subgroups' type fields without charge and label suffixes ("Al3+" -> "Al").  n_shown
+ * (optional) receives the number of atoms kept.  Either output path may be NULL, not both. */
+int dbx_view_deb(const char *deb_path, const double *limits, const char *svg_path, const char *xyz_path,
+                 const char *radii_path, int64_t *n_shown);
+
 /* ---- diagnostics --------------------------------------------------------------------
  * Self tests of the device arithmetic behind the pass-1 bin rule (no reference
  * counterpart: the reference relies on CUDA's sqrtf and roundf).

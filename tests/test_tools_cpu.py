@@ -128,3 +128,93 @@ def test_average_cli_and_spline_small_cases(tmp_path):
     subprocess.run([CLI, "--average", "--norm", "max", "-o", str(out), str(tmp_path / "p9.debI")], check=True)
     d = np.loadtxt(out)
     assert abs(d[:, 1].max() - 1.0) < 1e-9
+
+
+# ---- structure view (Matlab/viewdeb.m) ------------------------------------------------------
+def _radii_table():
+    rows = {}
+    path = os.path.join(os.path.dirname(cd.__file__), "data", "atomic_radii.txt")
+    for line in open(path):
+        if line.startswith("#") or not line.strip():
+            continue
+        s, r, *rgb = line.split()
+        rows[s.lower()] = (float(r), tuple(float(v) for v in rgb))
+    return rows
+
+
+def _svg(path):
+    import xml.etree.ElementTree as ET
+    ns = "{http://www.w3.org/2000/svg}"
+    root = ET.parse(path).getroot()
+    circles = [(float(c.get("cx")), float(c.get("cy")), float(c.get("r")), c.get("fill")) for c in root.iter(ns + "circle")]
+    texts = [(t.text, t.get("fill")) for t in root.iter(ns + "text") if t.get("font-size") == "24"]
+    return circles, texts
+
+
+def _hex(rgb):
+    return "#" + "".join(f"{int(round(min(1, max(0, v)) * 255)):02x}" for v in rgb)
+
+
+def test_view_deb_clips_and_draws_the_reference_example(golden_dir, tmp_path):
+    """Data/UH3_7nm.deb with the default limits (-10 < coordinate < 10 on every axis): the kept
+    atoms are exactly those of the file inside the box, listed in the .xyz in file order with
+    their elements; the SVG draws one disc per atom, of radius 5 r (r from the radius table) in
+    the element's colour, and lists H and U in their colours."""
+    deb = golden_dir / "UH3_7nm.deb"
+    head, groups, pos = _deb_groups(deb)
+    n = [int(g[0]) for g in groups]
+    elem = np.repeat([g[1] for g in groups], n)
+    p32 = pos.astype(np.float32)
+    keep = np.all((p32 > -10) & (p32 < 10), axis=1)
+    svg, xyz = tmp_path / "v.svg", tmp_path / "v.xyz"
+    assert cd.view_deb(deb, svg=svg, xyz=xyz) == int(keep.sum()) > 0
+    lines = open(xyz).read().splitlines()
+    assert int(lines[0]) == keep.sum() and lines[1] == "UH3_7nm.deb"
+    got = [l.split() for l in lines[2:]]
+    assert [g[0] for g in got] == list(elem[keep])
+    assert np.allclose(np.array([[float(v) for v in g[1:]] for g in got]), p32[keep], rtol=1e-6, atol=1e-6)
+    table = _radii_table()
+    circles, texts = _svg(svg)
+    assert len(circles) == keep.sum()
+    want = {e: (5 * table[e.lower()][0], _hex(table[e.lower()][1])) for e in ("H", "U")}
+    assert sorted(set((round(r, 3), f) for _, _, r, f in circles)) == sorted(set((round(r, 3), f) for r, f in want.values()))
+    assert texts == [("H", want["H"][1]), ("U", want["U"][1])]
+
+
+def test_view_deb_limits_types_and_projection(tmp_path):
+    """Per-axis limits (open intervals), subgroup types with charge and label suffixes mapped to
+    their elements, an element missing from the table kept in the .xyz but not drawn, and the
+    view's vertical z axis: atoms that differ in z alone are drawn one above the other, higher z
+    higher on the page."""
+    types = ["Al3+", "O2-", "Fe:oct", "Cu1", "Xx"]
+    pts = [[(0, 0, -1), (0, 0, 1), (0, 0, 2.5)], [(1, 2, 3)], [(5, 0, 0), (-4, 0, 0)], [(0, 3.9, 0)], [(1, 1, 1)]]
+    deb = tmp_path / "m.deb"
+    with open(deb, "w") as f:
+        f.write("0 16 2048\n1.5418 10 140 0.02\n20 0.001 0\n%d\n" % len(types))
+        for t, p in zip(types, pts):
+            f.write(f"{len(p)} {t} 1 0 " + " ".join(["1"] * 11) + "\n")
+        for p in pts:
+            for v in p:
+                f.write("%g\t%g\t%g\n" % v)
+    xyz, svg = tmp_path / "m.xyz", tmp_path / "m.svg"
+    # x in (-4.5, 4.5), y in (-10, 3.9) (3.9 itself excluded), z in (-2, 2.5) (2.5 excluded)
+    n = cd.view_deb(deb, limits=((-4.5, 4.5), (-10, 3.9), (-2, 2.5)), svg=svg, xyz=xyz)
+    rows = [l.split() for l in open(xyz).read().splitlines()[2:]]
+    assert n == 4 and [r[0] for r in rows] == ["Al", "Al", "Fe", "Xx"]
+    circles, texts = _svg(svg)
+    assert [t for t, _ in texts] == ["Al", "O", "Fe", "Cu", "Xx"]
+    assert len(circles) == 3  # Xx: no radius, not drawn
+    # the two Al atoms (z = -1 and z = 1) in drawing order, farthest first: from above the xy
+    # plane the z = 1 atom is the nearer one, drawn second, straight above the other (smaller cy)
+    al = [c for c in circles if c[3] == _hex(_radii_table()["al"][1])]
+    assert len(al) == 2 and abs(al[0][0] - al[1][0]) < 1e-6 and al[1][1] < al[0][1]
+    assert [float(r[3]) for r in rows[:2]] == [-1.0, 1.0]  # the .xyz keeps the file's order
+
+
+def test_cli_view(golden_dir, tmp_path):
+    deb = tmp_path / "UH3_7nm.deb"
+    shutil.copy(golden_dir / "UH3_7nm.deb", deb)
+    r = subprocess.run([CLI, "--view", str(deb), "--limits", "-5", "5"], check=True, capture_output=True, text=True)
+    n = int(r.stdout.split()[0])
+    circles, _ = _svg(str(deb) + ".svg")
+    assert n == len(circles) == cd.view_deb(deb, limits=(-5, 5), xyz=tmp_path / "a.xyz")
