@@ -76,7 +76,7 @@ def primary(args, cd, synth, np):
         n_jobs, misses = ctx.pass1_stats()
         ctx.set_profiling(True)
         ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n)
-        mf_ms, mf_flops, mf_useful, kc, digits = ctx.pass2_profile()
+        mf_ms, mf_flops, mf_useful, kc, digits, mean_digits = ctx.pass2_profile()
         ctx.set_profiling(False)
         mod.close()
     lds_peak = cd.lds_atomic_rate()
@@ -168,7 +168,7 @@ def stream_c4(cd, synth, np):
         ctx.set_profiling(True)
         _, batch_bound = ctx.batch_intensity(items, models[0].rstep, models[0].q, with_err=True)
         batch_refined = ctx.pass2_refined()
-        mf_ms, mf_flops, mf_useful, kc, digits = ctx.pass2_profile()
+        mf_ms, mf_flops, mf_useful, kc, digits, mean_digits = ctx.pass2_profile()
         ctx.set_profiling(False)
         # the same resident models, one dbx_intensity call each (plans already built): the
         # batching gain without host tiling on either side

@@ -145,7 +145,7 @@ def lib():
             "dbx_debug_set_profiling": (i32, [vp, i32]),
             "dbx_selftest_round": (i32, [C.c_uint32, C.c_uint32, P(u64)]),
             "dbx_selftest_lds_atomic_rate": (i32, [P(f64)]),
-            "dbx_debug_pass2_profile": (i32, [vp, P(f64), P(f64), P(f64), P(i32), P(i32)]),
+            "dbx_debug_pass2_profile": (i32, [vp, P(f64), P(f64), P(f64), P(i32), P(i32), P(f64)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -431,13 +431,13 @@ class Context:
         self._check(lib().dbx_debug_set_profiling(self.handle, int(on)), "set_profiling")
 
     def pass2_profile(self):
-        """(kernel ms, issued int8 matrix ops, useful flops, compacted bins, count digits) of
-        the last profiled matrix-core pass 2."""
-        ms, fl, us = C.c_double(), C.c_double(), C.c_double()
+        """(kernel ms, issued int8 matrix ops, useful flops, compacted bins, largest count-digit
+        number, mean count digits over the row tiles) of the last profiled matrix-core pass 2."""
+        ms, fl, us, md = C.c_double(), C.c_double(), C.c_double(), C.c_double()
         kc, dg = C.c_int(), C.c_int()
         self._check(lib().dbx_debug_pass2_profile(self.handle, C.byref(ms), C.byref(fl), C.byref(us), C.byref(kc),
-                                                  C.byref(dg)), "pass2_profile")
-        return ms.value, fl.value, us.value, kc.value, dg.value
+                                                  C.byref(dg), C.byref(md)), "pass2_profile")
+        return ms.value, fl.value, us.value, kc.value, dg.value, md.value
 
     def prefactor(self, ga: GroupParams, gb: GroupParams, q):
         q = np.ascontiguousarray(q, np.float64)

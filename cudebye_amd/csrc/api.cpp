@@ -392,6 +392,7 @@ static void finish_model(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_
     bool bad = false;
     for (int i = 0; i < nq && !bad; i++) bad = exceeds_target(hrel[i], ctx->rel_err_target);
     if (!bad) return;
+    P2Refining mark(ctx);
     debye_structure(ctx, d_hist, P, n_bins, rstep, first_n, dq, nq, dS + (size_t)row0 * nqd, dB + row0,
                     DBX_PASS2_FP64);
     launch_combine(ctx->stream, dff, dS, dB, nullptr, 1, nq, dI, drel, make_int4(row0, P, dest, 0));
@@ -427,7 +428,7 @@ int dbx_open(int device, size_t hbm_budget, dbx_ctx **out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->counters, 64) != hipSuccess ||
         hipHostMalloc((void **)&c->counters_h, 64, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->p2_readback_h, 64, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&c->p2_readback_h, kReadbackBytes, hipHostMallocDefault) != hipSuccess) {
         dbx_close(c);
         return DBX_ERR_HIP;
     }
@@ -744,18 +745,25 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
         DBX_HIP(hipMemcpyAsync(hoor, door, sizeof(uint64_t) * (size_t)n_items, hipMemcpyDeviceToHost, ctx->stream));
         DBX_HIP(hipStreamSynchronize(ctx->stream));
         // models whose bound misses the target: their partials again with FP64 arithmetic
+        // (all of them in one FP64 pass 2, written over their rows of dS and dB)
         if (ctx->pass2_mode == DBX_PASS2_MFMA) {
+            std::vector<int4> redo;
+            std::vector<int2> redo_rows;
+            std::vector<double *> Sd, Bd;
             for (const int4 &mr : mrows) {
                 const int i = mr.z;
                 bool bad = false;
                 for (int k = 0; k < nq && !bad; k++) bad = exceeds_target(hrel[i * nqd + k], ctx->rel_err_target);
                 if (!bad) continue;
-                debye_structure(ctx, dh + hoff[i], mr.y, items[i].n_bins, rstep, items[i].first_n, dq, nq,
-                                dS + (size_t)mr.x * nqd, dB + mr.x, DBX_PASS2_FP64);
-                launch_combine(ctx->stream, dff, dS, dB, nullptr, 1, nq, dI, drel, mr);
-                ctx->last_p2_refined++;
+                redo.push_back(mr);
+                redo_rows.push_back(make_int2(mr.x, mr.y));
+                Sd.push_back(dS + (size_t)mr.x * nqd);
+                Bd.push_back(dB + mr.x);
             }
-            if (ctx->last_p2_refined) {
+            if (!redo.empty()) {
+                structure_fp64_models(ctx, dh, rows, redo_rows, rstep, dq, nq, Sd, Bd);
+                combine_models(ctx, dff, dS, dB, redo, nq, dI, drel);
+                ctx->last_p2_refined = (int)redo.size();
                 fetch();
                 DBX_HIP(hipStreamSynchronize(ctx->stream));
             }
@@ -915,7 +923,7 @@ int dbx_debug_set_profiling(dbx_ctx *ctx, int on)
 }
 
 int dbx_debug_pass2_profile(dbx_ctx *ctx, double *kernel_ms, double *mfma_flops, double *useful_flops,
-                            int *n_bins_compacted, int *count_digits)
+                            int *n_bins_compacted, int *count_digits, double *mean_digits)
 {
     if (!ctx) return DBX_ERR_ARG;
     return guarded(ctx, [&] {
@@ -925,8 +933,9 @@ int dbx_debug_pass2_profile(dbx_ctx *ctx, double *kernel_ms, double *mfma_flops,
         if (kernel_ms) *kernel_ms = ms;
         if (mfma_flops) *mfma_flops = ctx->mf_flops;
         if (useful_flops) *useful_flops = ctx->mf_useful;
-        if (n_bins_compacted) *n_bins_compacted = ctx->last_p2_bins;
+        if (n_bins_compacted) *n_bins_compacted = ctx->mf_bins;
         if (count_digits) *count_digits = ctx->mf_digits;
+        if (mean_digits) *mean_digits = ctx->mf_mean_digits;
     });
 }
 

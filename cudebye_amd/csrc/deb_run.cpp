@@ -334,7 +334,7 @@ static int run_deb_body(const char *path, int flags)
         return rc;
     }
     // the reference evaluates the Debye sum in FP64; the driver does too unless asked for the
-    // matrix-core path (which rounds sinc to a multiple of 2^-40 and may move a printed 6th digit)
+    // matrix-core path (integer digits of sinc, as accurate as FP64 but summed in another order)
     dbx_set_pass2_mode(ctx, (flags & DBX_RUN_MFMA) ? DBX_PASS2_MFMA : DBX_PASS2_FP64);
     const int P = d.S * (d.S + 1) / 2;
     std::vector<uint64_t> H((size_t)P * channels);

@@ -166,8 +166,8 @@ struct dbx_ctx {
     std::recursive_mutex mu;
     std::set<dbx_model *> models;  // live models, released by dbx_close if still there
     // grow-only device scratch, one buffer per slot (dbx::kSlot*)
-    void *slot_ptr[13] = {};
-    size_t slot_bytes[13] = {};
+    void *slot_ptr[14] = {};
+    size_t slot_bytes[14] = {};
     void *pin_ptr[12] = {};
     size_t pin_bytes[12] = {};
     // the q grid last uploaded to slot kSlotQin (host copy and device address)
@@ -186,24 +186,26 @@ struct dbx_ctx {
     float last_p2_ms = 0.f;         // hipEvent time of the last pass-2 structure computation
     hipEvent_t ev[4] = {};
     bool p2_timed = false;
+    bool p2_refining = false;  // an FP64 recomputation: the pass-2 time keeps the first pass's start
     int pass2_mode = DBX_PASS2_MFMA;  // DBX_PASS2_MFMA or DBX_PASS2_FP64
     double rel_err_target = 1e-6;     // models whose I bound exceeds it are redone in FP64 (0: never)
     int last_p2_refined = 0;          // models redone in FP64 by the last call
     int last_p2_bins = 0;             // compacted bins of the last pass 2
-    uint64_t *p2_readback_h = nullptr;  // pinned host words for pass 2's read-backs, from dbx_open
+    uint64_t *p2_readback_h = nullptr;  // kReadbackBytes of pinned host memory for pass 2, from dbx_open
     // profiling of the matrix-core contraction (dbx_debug_set_profiling)
     bool profiling = false;
     hipEvent_t ev_mf[2] = {};
     bool mf_timed = false;
     double mf_flops = 0, mf_useful = 0;
-    int mf_digits = 0;
+    int mf_digits = 0, mf_bins = 0;  // the profiled contraction's largest digit count, compacted bins
+    double mf_mean_digits = 0;      // and its digit count averaged over the row tiles
 };
 
 namespace dbx {
 
 enum { kSlotPass1 = 0, kSlotPass2 = 1, kSlotPass2b = 2, kSlotHist = 3, kSlotQ = 4, kSlotDigits = 5,
        kSlotRtab = 6, kSlotBatch = 7, kSlotRows = 8, kSlotPairs = 9, kSlotBatchHist = 10,
-       kSlotOor = 11, kSlotQin = 12 };
+       kSlotOor = 11, kSlotQin = 12, kSlotRefine = 13 };
 // device buffer of at least `bytes` for `slot`; contents are not preserved when it grows
 void *scratch(dbx_ctx *ctx, int slot, size_t bytes);
 // pinned host buffer of at least `bytes` for `slot` (staging of per-call uploads; every call
@@ -284,10 +286,24 @@ void prefactor_rows(dbx_ctx *ctx, const std::vector<GroupPair> &rows, const doub
 // Where pass 2's device buffers and its read-back words come from: the context's grow-only
 // slots (ctx_pass2_mem), or a fixed device region (the allocation-free entry point).  dev()
 // returns a buffer of at least `bytes` that no other dev() call of the same pass 2 overlaps.
+// pinned host memory that dbx_open allocates for the allocation-free pass 2's read-backs and
+// uploads: the row digit classes and the reordered row list of up to 2340 rows
+constexpr size_t kReadbackBytes = 64 << 10;
 struct P2Mem {
     virtual ~P2Mem() = default;
     virtual void *dev(int slot, size_t bytes) = 0;
-    virtual unsigned long long *readback() = 0;  // two words of pinned host memory
+    // `words` words of pinned host memory for read-backs, or null when this provider has fewer
+    // (two always fit)
+    virtual unsigned long long *readback(size_t words) = 0;
+    // pinned host memory for uploads, or null when this provider has less; may be the read-back
+    // memory
+    virtual void *staging(size_t bytes) = 0;
+};
+// marks an FP64 recomputation of pass 2 for its lifetime
+struct P2Refining {
+    dbx_ctx *ctx;
+    explicit P2Refining(dbx_ctx *c) : ctx(c) { ctx->p2_refining = true; }
+    ~P2Refining() { ctx->p2_refining = false; }
 };
 P2Mem &ctx_pass2_mem(dbx_ctx *ctx);
 // S[r][q] for every row r (rows of any models sharing rstep, first_n and the q grid)
@@ -296,6 +312,12 @@ P2Mem &ctx_pass2_mem(dbx_ctx *ctx);
 void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::vector<RowDesc> &rows,
                           double rstep, const double *d_q, int nq, double *d_S, double *d_B = nullptr,
                           int mode = -1);
+// The rows of several models again with FP64 arithmetic, in one pass 2: model m is rows
+// [first, first + count) of rows (longest first) given as models[m] = (first, count); its sums go
+// to S_dst[m] (count x nq) and its bounds to B_dst[m] (count).
+void structure_fp64_models(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::vector<RowDesc> &rows,
+                           const std::vector<int2> &models, double rstep, const double *d_q, int nq,
+                           const std::vector<double *> &S_dst, const std::vector<double *> &B_dst);
 // S[p][q] for the P partials of one model, d_hist[P][n_bins], buffers from mem (the row
 // descriptors are written on the device: no host staging)
 void debye_structure_mem(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,

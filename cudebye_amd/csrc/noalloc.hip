@@ -62,7 +62,11 @@ struct FixedMem : P2Mem {
         used += a;
         return p;
     }
-    unsigned long long *readback() override { return rb; }
+    unsigned long long *readback(size_t words) override
+    {
+        return words * sizeof(unsigned long long) <= kReadbackBytes ? rb : nullptr;
+    }
+    void *staging(size_t bytes) override { return bytes <= kReadbackBytes ? rb : nullptr; }
 };
 
 }  // namespace
@@ -104,6 +108,7 @@ int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bin
             std::memcpy(&w, ctx->p2_readback_h, sizeof(w));
             if (exceeds_target(w, ctx->rel_err_target)) {
                 mem.used = kept;
+                P2Refining mark(ctx);
                 debye_structure_mem(ctx, mem, d_hist, P, n_bins, rstep, first_n, d_q, nq, S, B, DBX_PASS2_FP64);
                 launch_combine(ctx->stream, d_ffoobb, S, B, nullptr, 1, nq, d_I, rel, make_int4(0, P, 0, 0));
                 ctx->last_p2_refined = 1;

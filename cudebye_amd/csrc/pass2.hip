@@ -103,35 +103,32 @@ __device__ __forceinline__ bool row_holds(const RowDesc *rows, int r, int64_t k)
 constexpr int kFlagRows = 32;  // rows per thread of the nonzero-flag pass
 
 // flags[k] = 1 where some row holds a nonzero count at bin k > k_self_max; grid.y over chunks
-// of kFlagRows rows (a bin written by several chunks gets the same value from each).  Also the
-// largest count of any row: one atomicMax per block.
+// of kFlagRows rows (a bin written by several chunks gets the same value from each).  Also each
+// row's digit class: the count digits the matrix-core sum needs for its largest count beyond the
+// self bins, the smallest I with 2 N < 127 256^(I - 1) (6: more than five), from four ballots per
+// wave and row and one atomic per wave and row that needs more than one digit.
 __global__ __launch_bounds__(kScanBlock) void p2_flags_kernel(
     const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows, int n_rows, int64_t nb_max,
-    int k_self_max, unsigned char *__restrict__ flags, unsigned long long *__restrict__ max_count)
+    int k_self_max, unsigned char *__restrict__ flags, int *__restrict__ rowcls)
 {
-    __shared__ unsigned long long smx[kScanBlock / 64];
     const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
     const int r0 = blockIdx.y * kFlagRows;
     const int r1 = min(n_rows, r0 + kFlagRows);
+    const bool kin = k < nb_max && k > k_self_max;
     bool nz = false;
-    unsigned long long mx = 0;
-    if (k < nb_max && k > k_self_max)
-        for (int r = r0; r < r1 && row_holds(rows, r, k); r++) {
-            const unsigned long long v = hist[rows[r].h_off + k];
-            nz |= v != 0ull;
-            mx = v > mx ? v : mx;
-        }
+    for (int r = r0; r < r1; r++) {
+        const RowDesc rd = rows[r];
+        const unsigned long long v2 = (kin && k < rd.n_bins) ? 2ull * hist[rd.h_off + k] : 0ull;
+        nz |= v2 != 0ull;
+        const uint64_t c2 = __ballot(v2 >= (unsigned long long)mf_count_limit(1));
+        if (!c2) continue;
+        const int c = 2 + (__ballot(v2 >= (unsigned long long)mf_count_limit(2)) != 0) +
+                      (__ballot(v2 >= (unsigned long long)mf_count_limit(3)) != 0) +
+                      (__ballot(v2 >= (unsigned long long)mf_count_limit(4)) != 0) +
+                      (__ballot(v2 >= (unsigned long long)mf_count_limit(5)) != 0);
+        if ((threadIdx.x & 63) == 0) atomicMax(&rowcls[r], c);
+    }
     if (nz) flags[k] = 1;
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_down(mx, off);
-        mx = o > mx ? o : mx;
-    }
-    if ((threadIdx.x & 63) == 0) smx[threadIdx.x >> 6] = mx;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < kScanBlock / 64; w++) mx = smx[w] > mx ? smx[w] : mx;
-        if (mx) atomicMax(max_count, mx);
-    }
 }
 
 // profiling: total nonzero bins over all rows (the bins the reference's per-pair sum contracts)
@@ -169,8 +166,10 @@ __global__ void p2_count_kernel(const unsigned char *__restrict__ flags, int64_t
     }
 }
 
-// exclusive scan of the block counts, one workgroup
-__global__ void p2_scan_kernel(int *__restrict__ block_counts, int n_blocks, int *__restrict__ total)
+// exclusive scan of the block counts, one workgroup; total[0] = the sum (compacted bins),
+// total[2] = the largest row digit class (at least 1)
+__global__ void p2_scan_kernel(int *__restrict__ block_counts, int n_blocks, int *__restrict__ total,
+                               const int *__restrict__ rowcls, int n_rows)
 {
     __shared__ int part[1024];
     const int t = threadIdx.x;
@@ -179,6 +178,8 @@ __global__ void p2_scan_kernel(int *__restrict__ block_counts, int n_blocks, int
     int s = 0;
     for (int i = b0; i < b1; i++) s += block_counts[i];
     part[t] = s;
+    int cls = 1;
+    for (int r = t; r < n_rows; r += 1024) cls = max(cls, rowcls[r]);
     __syncthreads();
     if (t == 0) {
         int acc = 0;
@@ -195,6 +196,13 @@ __global__ void p2_scan_kernel(int *__restrict__ block_counts, int n_blocks, int
         const int c = block_counts[i];
         block_counts[i] = acc;
         acc += c;
+    }
+    __syncthreads();
+    part[t] = cls;
+    __syncthreads();
+    if (t == 0) {
+        for (int i = 1; i < 1024; i++) cls = max(cls, part[i]);
+        total[2] = cls;
     }
 }
 
@@ -223,17 +231,18 @@ __global__ void p2_scatter_kernel(const unsigned char *__restrict__ flags, int64
     }
 }
 
-// kext[b] = number of compacted bins below the n_bins of row block b's first row
+// kext[b] = number of compacted bins below the n_bins of row block b's longest row
 __device__ __forceinline__ void kext_entry(int b, const int *__restrict__ kz, int Kc, const RowDesc *__restrict__ rows,
                                            int n_rows, int rows_per_block, int n_blocks, int *__restrict__ kext)
 {
     if (b >= n_blocks) return;
-    const int r = b * rows_per_block;
-    if (r >= n_rows) {
+    const int r0 = b * rows_per_block, r1 = min(n_rows, r0 + rows_per_block);
+    if (r0 >= n_rows) {
         kext[b] = 0;
         return;
     }
-    const int64_t nb = rows[r].n_bins;
+    int64_t nb = 0;  // the block's longest row
+    for (int r = r0; r < r1; r++) nb = max(nb, rows[r].n_bins);
     int lo = 0, hi = Kc;  // first index with kz >= nb
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
@@ -334,22 +343,25 @@ __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
     }
 }
 
-// Row r = r0 + i / nq, q index iq = i % nq (i over nr rows x nq):
-//   S[r][iq] = (shift rsum[r] + sum over splits of Spart[s][i], fixed order)
-//            + (self[r] + sum_k low[r][k] sinc(q r_k))
+// Row r = r0 + i / nq in the order the sums were computed in, q index iq = i % nq (i over
+// nr rows x nq), the caller's row o = perm[r] (perm null: o = r):
+//   S[o][iq] = (shift rsum[r] + sum over splits of Spart[s][i], fixed order)
+//            + (self[o] + sum_k low[o][k] sinc(q r_k))
 // rsum (the q-independent part of the matrix-core sum) may be null; nsplit may be 0.  The
 // second term is the reference's weight rule for the entries at r <= 0.0005 among the first
 // first_n (weight 1, no sinc) and their neighbours below first_n (weight 2 with sinc).
 __global__ void p2_finish_rows_kernel(const double *__restrict__ Spart, int nsplit, int r0, int nr, int nq,
                                       const unsigned long long *__restrict__ rsum, double shift,
                                       const double *__restrict__ self, const double *__restrict__ low, int nlow,
-                                      double rstep, const double *__restrict__ q, double *__restrict__ S)
+                                      double rstep, const double *__restrict__ q, const int *__restrict__ perm,
+                                      double *__restrict__ S)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)nr * nq;
     if (i >= n) return;
     const int r = r0 + (int)(i / nq);
     const int iq = (int)(i - (int64_t)(r - r0) * nq);
+    const int o = perm ? perm[r] : r;
     double acc = rsum ? shift * (double)rsum[r] : 0.0;
     // eight loads in flight, the sum kept in split order
     int s = 0;
@@ -361,16 +373,16 @@ __global__ void p2_finish_rows_kernel(const double *__restrict__ Spart, int nspl
         for (int u = 0; u < 8; u++) acc += v[u];
     }
     for (; s < nsplit; s++) acc += Spart[(int64_t)s * n + i];
-    double lo = self[r];
+    double lo = self[o];
     for (int k = 0; k < nlow; k++) {
-        const double w = low[(int64_t)r * nlow + k];
+        const double w = low[(int64_t)o * nlow + k];
         if (w != 0.0) {
             const double rk = (double)k * rstep;
             const double x = q[iq] * rk;
             lo += rk == 0.0 ? w : w * sin(x) / x;
         }
     }
-    S[(int64_t)r0 * nq + i] = acc + lo;
+    S[(int64_t)o * nq + iq] = acc + lo;
 }
 
 // Absolute error bound of each row's structure sum S[r][q], the same for every q:
@@ -378,15 +390,17 @@ __global__ void p2_finish_rows_kernel(const double *__restrict__ Spart, int nspl
 // rsum[r] = sum of the weights 2 N over the row's compacted bins (exact), e = the error per unit
 // weight of the mode's sum over them (pass2_unit_error); the weight-1 bins are exact integers and
 // each low bin's FP64 sinc term is within 5 u of its value.  rsum null: no compacted bin.
+// r: the order rsum was computed in; perm[r] (perm null: r) the caller's row
 __global__ void p2_row_bounds_kernel(const unsigned long long *__restrict__ rsum, const double *__restrict__ self,
                                      const double *__restrict__ low, int nlow, int n_rows, double e,
-                                     double *__restrict__ B)
+                                     const int *__restrict__ perm, double *__restrict__ B)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n_rows) return;
-    double lo = self[r];
-    for (int k = 0; k < nlow; k++) lo += fabs(low[(int64_t)r * nlow + k]);
-    B[r] = (rsum ? e * (double)rsum[r] : 0.0) + 5.0 * 0x1p-53 * lo;
+    const int o = perm ? perm[r] : r;
+    double lo = self[o];
+    for (int k = 0; k < nlow; k++) lo += fabs(low[(int64_t)o * nlow + k]);
+    B[o] = (rsum ? e * (double)rsum[r] : 0.0) + 5.0 * 0x1p-53 * lo;
 }
 
 // ---------------------------------------------------------------------------------
@@ -397,7 +411,7 @@ __global__ void p2_row_bounds_kernel(const unsigned long long *__restrict__ rsum
 // over 256 steps of 64 bins stays below 2^31 and is exact whatever order the matrix core adds
 // in.  Every 256 steps the classes are folded into FP64, S += sum_s 2^(8s) acc_s / c, and the
 // 3/8 part, (3/8) sum_k A_k, is added per row from the exact integer sum of its counts.  The
-// only approximation is the rounding of c (T - 3/8) to an integer (|err| <= 1/(2c) < 2^-40 per
+// only approximation is the rounding of c (T - 3/8) to an integer (|err| <= 1/(2c) < 2^-48 per
 // bin), with T from an FP64 sine with full argument reduction.
 // ---------------------------------------------------------------------------------
 typedef int i32x4 __attribute__((ext_vector_type(4)));
@@ -469,18 +483,17 @@ __device__ __forceinline__ void rtab_entry(int i, const int *__restrict__ kz, in
     ir[i] = 1.0 / r;
 }
 
-// The inputs of the contraction in one launch: blocks [0, cd_x rows_pad) write the count digits
-// (block b: bins of chunk b % cd_x of row b / cd_x), the next (Kc_pad + 255) / 256 the r table,
-// the rest the row-block bin extents.
+// The inputs of the contraction of one class of rows in one launch: blocks [0, cd_x rows_pad)
+// write the count digits (block b: bins of chunk b % cd_x of row b / cd_x), the next n_rt the r
+// table (n_rt 0: written by another class's launch), the rest the row-block bin extents.
 __global__ __launch_bounds__(kCdThreads) void p2_mf_inputs_kernel(
     const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows, int n_rows,
     const int *__restrict__ kz, int Kc, int Kc_pad, int rows_pad, int I, int cd_x, signed char *__restrict__ cdig,
     unsigned long long *__restrict__ rsum, double rstep, double *__restrict__ rr, double *__restrict__ ir,
-    int n_kblocks, int *__restrict__ kext)
+    int n_rt, int n_kblocks, int *__restrict__ kext)
 {
     __shared__ unsigned long long wsum[kCdThreads / 64];
     const int n_cd = cd_x * rows_pad;
-    const int n_rt = (Kc_pad + kCdThreads - 1) / kCdThreads;
     const int b = blockIdx.x;
     if (b < n_cd) {
         count_digits_block(b % cd_x, b / cd_x, wsum, hist, rows, n_rows, kz, Kc, Kc_pad, rows_pad, I, cdig, rsum);
@@ -506,26 +519,36 @@ __global__ __launch_bounds__(kCdThreads) void p2_mf_inputs_kernel(
 //   - The 32-bit sums are folded into FP64 partial sums in Spart every kMfFold steps.
 // The workgroup's bins end at kext of its first row block: rows are ordered by length,
 // longest first.
-template <int RB, int WQ, int WR, int I>
-__global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
-    const double *__restrict__ rr, const double *__restrict__ ir, int Kc_pad, int k_per_split,
-    const int *__restrict__ kext, const signed char *__restrict__ cdig, int rows_pad,
-    const double *__restrict__ q, int nq, int nrows, double rstep, double *__restrict__ Spart)
+// The tile (q group, split, row group) of a workgroup, q group fastest: the dispatcher deals
+// consecutive workgroups round-robin to the 8 XCDs, so every XCD gets a share of every row group
+// (the longest rows first) and the count digits of a (split, row group) are read through each
+// XCD's L2 by its share of the q groups.  Binding whole (split, row group)s to one XCD would read
+// them once, but rows of unequal length leave the XCDs unequally loaded, and the digit traffic
+// is far below the HBM bandwidth in either case.
+__device__ __forceinline__ int3 mf_tile() { return make_int3(blockIdx.x, blockIdx.y, blockIdx.z); }
+
+// The contraction of one tile with I count digits (the tile's rows need at most I); the LDS ring
+// holds NFA >= I WR RB fragments per slot.  kend: the tile's last bin.
+template <int RB, int WQ, int WR, int NFA, int I>
+__device__ __forceinline__ void mf_tile_sum(const int3 tile, int kend, i32x4 (*sa)[NFA][64], double (*srt)[2 * kMfK],
+                                            i32x4 (*sd)[WR > 1 ? WQ : 1][kMfJ][64], const double *__restrict__ rr,
+                                            const double *__restrict__ ir, int Kc_pad, int k_per_split,
+                                            const signed char *__restrict__ cdig, int rows_pad,
+                                            const double *__restrict__ q, int nq, int nrows, int srows,
+                                            double rstep, double *__restrict__ Spart)
 {
     constexpr int NW = WQ * WR;
     constexpr int NF = I * WR * RB;
+    static_assert(NF <= NFA, "LDS ring smaller than the tile's fragments");
     constexpr int LF = (NF + NW - 1) / NW;
     constexpr int C = I + kMfJ - 1;
     constexpr int V = 16 / WR;
     constexpr int NG = V / 4;
-    __shared__ i32x4 sa[3][NF][64];
-    __shared__ double srt[3][2 * kMfK];
-    __shared__ i32x4 sd[WR > 1 ? 2 : 1][WR > 1 ? WQ : 1][kMfJ][64];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int wq = wave % WQ, wr = wave / WQ;
-    const int q0 = blockIdx.x * (16 * WQ) + wq * 16;
-    const int rowt = blockIdx.z * (WR * RB * 16);
+    const int q0 = tile.x * (16 * WQ) + wq * 16;
+    const int rowt = tile.z * (WR * RB * 16);
     const int col = lane & 15, kq = lane >> 4;
     // generator role: thread g of the q block's WR waves -> fragment lane gl, values gt..gt+V-1
     const int g = wr * 64 + lane;
@@ -539,8 +562,8 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     const double sdQ = sin_cw(Q * rstep);
     const double shQ = sin_cw(0.5 * (Q * rstep));
     const double lamQ = 4.0 * (shQ * shQ);
-    const int kx = (kext[blockIdx.z * (WR * RB)] + kMfK - 1) & ~(kMfK - 1);
-    const int kb = blockIdx.y * k_per_split;
+    const int kx = (kend + kMfK - 1) & ~(kMfK - 1);
+    const int kb = tile.y * k_per_split;
     const int ke = min(min(Kc_pad, kx), kb + k_per_split);
     const int nsteps = ke > kb ? (ke - kb) / kMfK : 0;
     i32x4 acc[RB][C];
@@ -561,7 +584,7 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
             for (int v = 0; v < 4; v++) {
                 const int row = rowt + (wr * RB + rb) * 16 + kq * 4 + v;
                 const bool own = qq < nq && row < nrows;
-                double *p = Spart + ((int64_t)blockIdx.y * nrows + row) * nq + qq;
+                double *p = Spart + ((int64_t)tile.y * srows + row) * nq + qq;
                 double a = (own && !first) ? *p : 0.0;
 #pragma unroll
                 for (int c = 0; c < C; c++) a += __builtin_ldexp((double)acc[rb][c][v], 8 * c);
@@ -732,6 +755,42 @@ __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
     fold(true);
 }
 
+// the tile sum with the smallest instantiated digit count I' >= Irg (Irg in [1, I])
+template <int RB, int WQ, int WR, int NFA, int I, typename... A>
+__device__ __forceinline__ void mf_tile_dispatch(int Irg, A... a)
+{
+    if constexpr (I > 1) {
+        if (Irg < I) {
+            mf_tile_dispatch<RB, WQ, WR, NFA, I - 1>(Irg, a...);
+            return;
+        }
+    }
+    mf_tile_sum<RB, WQ, WR, NFA, I>(a...);
+}
+
+// One workgroup per tile (q group, split, row group).  The row group's bins end at the largest
+// extent of its 16-row blocks; its count digits number bcls of its first block (rows are ordered
+// by digit class, most first, so the first block's is the largest), or IMAX without bcls.
+template <int RB, int WQ, int WR, int IMAX>
+__global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
+    const double *__restrict__ rr, const double *__restrict__ ir, int Kc_pad, int k_per_split,
+    const int *__restrict__ kext, const int *__restrict__ bcls, const signed char *__restrict__ cdig, int rows_pad,
+    const double *__restrict__ q, int nq, int nrows, int srows, double rstep, double *__restrict__ Spart)
+{
+    constexpr int NFA = IMAX * WR * RB;
+    __shared__ i32x4 sa[3][NFA][64];
+    __shared__ double srt[3][2 * kMfK];
+    __shared__ i32x4 sd[WR > 1 ? 2 : 1][WR > 1 ? WQ : 1][kMfJ][64];
+    const int3 tile = mf_tile();  // (q group, split, row group)
+    const int b0 = tile.z * (WR * RB);
+    int kend = 0;
+#pragma unroll
+    for (int b = 0; b < WR * RB; b++) kend = max(kend, kext[b0 + b]);
+    const int Irg = bcls ? bcls[b0] : IMAX;
+    mf_tile_dispatch<RB, WQ, WR, NFA, IMAX>(Irg, tile, kend, sa, srt, sd, rr, ir, Kc_pad, k_per_split, cdig,
+                                            rows_pad, q, nq, nrows, srows, rstep, Spart);
+}
+
 // launch shapes of the matrix-core Debye sum: (RB, WQ, WR) by row count
 struct MfShape {
     int RB, WQ, WR;
@@ -754,13 +813,13 @@ static MfShape mf_shape(int n_rows, int I)
 
 template <int RB, int WQ, int WR>
 static void mf_launch_i(int I, dim3 grid, hipStream_t st, const double *rr, const double *ir, int Kc_pad,
-                        int kper, const int *kext, const signed char *cdig, int rows_pad, const double *q,
-                        int nq, int nrows, double rstep, double *Spart)
+                        int kper, const int *kext, const int *bcls, const signed char *cdig, int rows_pad,
+                        const double *q, int nq, int nrows, int srows, double rstep, double *Spart)
 {
     const dim3 blk(WQ * WR * 64);
 #define DBX_MF(II)                                                                                              \
     hipLaunchKernelGGL((p2_debye_mfma_kernel<RB, WQ, WR, II>), grid, blk, 0, st, rr, ir, Kc_pad, kper, kext, \
-                       cdig, rows_pad, q, nq, nrows, rstep, Spart)
+                       bcls, cdig, rows_pad, q, nq, nrows, srows, rstep, Spart)
     switch (I) {
     case 1: DBX_MF(1); break;
     case 2: DBX_MF(2); break;
@@ -777,15 +836,18 @@ static void mf_launch_i(int I, dim3 grid, hipStream_t st, const double *rr, cons
 }
 
 static void mf_launch(const MfShape &sh, int I, dim3 grid, hipStream_t st, const double *rr, const double *ir,
-                      int Kc_pad, int kper, const int *kext, const signed char *cdig, int rows_pad,
-                      const double *q, int nq, int nrows, double rstep, double *Spart)
+                      int Kc_pad, int kper, const int *kext, const int *bcls, const signed char *cdig, int rows_pad,
+                      const double *q, int nq, int nrows, int srows, double rstep, double *Spart)
 {
     if (sh.RB == 1)
-        mf_launch_i<1, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
+        mf_launch_i<1, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, bcls, cdig, rows_pad, q, nq, nrows, srows,
+                             rstep, Spart);
     else if (sh.RB == 2)
-        mf_launch_i<2, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
+        mf_launch_i<2, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, bcls, cdig, rows_pad, q, nq, nrows, srows,
+                             rstep, Spart);
     else
-        mf_launch_i<4, 4, 2>(I, grid, st, rr, ir, Kc_pad, kper, kext, cdig, rows_pad, q, nq, nrows, rstep, Spart);
+        mf_launch_i<4, 4, 2>(I, grid, st, rr, ir, Kc_pad, kper, kext, bcls, cdig, rows_pad, q, nq, nrows, srows,
+                             rstep, Spart);
 }
 
 // I[dest][q] = sum over the model's rows of ff S, FP64, rows in order;
@@ -862,35 +924,47 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
     const int ksm = self_bin_max(rstep);
     const int nlow = ksm + 1;
     const int64_t nblk = (nb_max + kScanBlock - 1) / kScanBlock;
-    // scratch: rows, block counts, kz, self, low, kext, then the parts that start at zero
-    // (total + max count, row sums, flags), cleared by one launch
-    const int n_rb16 = (n_rows + 15) / 16;
+    // scratch: rows (the caller's order, then the order of the sums), the permutation between
+    // them, block counts, kz, self, low, kext, block digit classes, then the parts that start at
+    // zero (totals, row digit classes, row sums, flags), cleared by one launch
+    const int n_kext = (n_rows + 15) / 16 + 8 * (kMfImax + 1);
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off = (off + b + 255) & ~(size_t)255; return o; };
     const size_t o_rows = take(sizeof(RowDesc) * n_rows);
+    const size_t o_krows = take(sizeof(RowDesc) * n_rows);
+    const size_t o_perm = take(sizeof(int) * n_rows);
     const size_t o_bc = take(sizeof(int) * (nblk + 1));
     const size_t o_kz = take(sizeof(int) * nb_max);
     const size_t o_self = take(sizeof(double) * n_rows);
     const size_t o_low = take(sizeof(double) * (size_t)n_rows * nlow);
-    const size_t o_kext = take(sizeof(int) * (n_rb16 + 8));
+    const size_t o_kext = take(sizeof(int) * n_kext);
+    const size_t o_bcls = take(sizeof(int) * n_kext);
     const size_t o_tot = take(sizeof(unsigned long long) * 2);
+    const size_t o_rcls = take(sizeof(int) * n_rows);
     const size_t o_rsum = take(sizeof(unsigned long long) * n_rows);
     const size_t o_flags = take((size_t)(nblk * kScanBlock));
     char *base = (char *)mem.dev(kSlotPass2, off);
     RowDesc *d_rows = (RowDesc *)(base + o_rows);
+    RowDesc *d_krows = (RowDesc *)(base + o_krows);
+    int *d_perm = (int *)(base + o_perm);
     int *bc = (int *)(base + o_bc);
-    unsigned long long *tot = (unsigned long long *)(base + o_tot);  // [0] Kc, [1] max count
+    unsigned long long *tot = (unsigned long long *)(base + o_tot);  // [0] Kc, [1] largest row digit class
+    int *rcls = (int *)(base + o_rcls);
     int *kz = (int *)(base + o_kz);
     double *self = (double *)(base + o_self);
     double *low = (double *)(base + o_low);
     int *kext = (int *)(base + o_kext);
+    int *d_bcls = (int *)(base + o_bcls);
     unsigned long long *rsum = (unsigned long long *)(base + o_rsum);
     unsigned char *flags = (unsigned char *)(base + o_flags);
     const unsigned long long *h = (const unsigned long long *)d_hist_base;
 
-    DBX_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
+    // host staging: the rows, the permutation and the block classes
+    const size_t stage_bytes = (sizeof(RowDesc) + sizeof(int)) * n_rows + sizeof(int) * n_kext;
+    if (!ctx->p2_refining) DBX_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     if (h_rows) {
-        RowDesc *hrows = (RowDesc *)pinned(ctx, kPinRows, sizeof(RowDesc) * n_rows);
+        RowDesc *hrows = (RowDesc *)mem.staging(stage_bytes);
+        DBX_REQUIRE(hrows != nullptr, DBX_ERR_BUDGET, "pass-2 row list beyond the staging memory");
         std::copy(h_rows, h_rows + n_rows, hrows);
         DBX_HIP(hipMemcpyAsync(d_rows, hrows, sizeof(RowDesc) * n_rows, hipMemcpyHostToDevice, ctx->stream));
     } else {
@@ -901,21 +975,32 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
     hipLaunchKernelGGL(p2_zero_kernel, dim3((unsigned)((zwords + 255) / 256)), dim3(256), 0, ctx->stream,
                        (unsigned int *)tot, zwords);
     hipLaunchKernelGGL(p2_flags_kernel, dim3((unsigned)nblk, (unsigned)((n_rows + kFlagRows - 1) / kFlagRows)),
-                       dim3(kScanBlock), 0, ctx->stream, h, d_rows, n_rows, nb_max, ksm, flags, tot + 1);
+                       dim3(kScanBlock), 0, ctx->stream, h, d_rows, n_rows, nb_max, ksm, flags, rcls);
     hipLaunchKernelGGL(p2_count_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, flags, nb_max, bc);
-    hipLaunchKernelGGL(p2_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, bc, (int)nblk, (int *)tot);
+    hipLaunchKernelGGL(p2_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, bc, (int)nblk, (int *)tot, rcls, n_rows);
     hipLaunchKernelGGL(p2_scatter_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, flags, nb_max,
                        bc, kz);
     hipLaunchKernelGGL(p2_self_kernel, dim3((n_rows + 63) / 64), dim3(64), 0, ctx->stream, h, d_rows, n_rows,
                        ksm, self, low);
-    unsigned long long *hv = mem.readback();
+    // the totals, and with several row blocks each row's digit class when the read-back holds
+    // them and the staging memory the reordered rows
+    const bool mfma = mode != DBX_PASS2_FP64;
+    const size_t cls_words = (size_t)(n_rows + 1) / 2;
+    unsigned long long *hv = nullptr;
+    if (mfma && n_rows > 16 && mem.staging(stage_bytes))
+        hv = mem.readback(2 + cls_words);
+    const bool per_row = hv != nullptr;
+    if (!hv) hv = mem.readback(2);
     DBX_HIP(hipMemcpyAsync(hv, tot, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    if (per_row)
+        DBX_HIP(hipMemcpyAsync(hv + 2, rcls, sizeof(int) * n_rows, hipMemcpyDeviceToHost, ctx->stream));
     DBX_HIP(hipStreamSynchronize(ctx->stream));
     const int Kc = (int)(hv[0] & 0xFFFFFFFFull);
-    const unsigned long long max_count = hv[1];
+    const int Imax = std::max(1, (int)hv[1]);
     ctx->last_p2_bins = Kc;
     double unit_err = 0.0;  // error per unit weight of the compacted bins' sum (pass2_unit_error)
-    if (Kc > 0 && mode == DBX_PASS2_FP64) {
+    const int *perm = nullptr;  // order of the sums -> the caller's rows (null: the same)
+    if (Kc > 0 && !mfma) {
         const int qblocks = (nq + kDsThreads - 1) / kDsThreads;
         int nsplit = std::max(1, std::min((2048 + qblocks - 1) / qblocks, (Kc + 511) / 512));
         const int kper = (Kc + nsplit - 1) / nsplit;
@@ -931,50 +1016,89 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
                                kz, Kc, kper, A, np, rstep, d_q, nq, Spart);
             hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)np * nq + 255) / 256)),
                                dim3(256), 0, ctx->stream, Spart, nsplit, r0, np, nq, nullptr, 0.0, self, low,
-                               nlow, rstep, d_q, d_S);
+                               nlow, rstep, d_q, nullptr, d_S);
         }
         unit_err = pass2_unit_error(mode, kper, nsplit, 1);
     } else if (Kc > 0) {
-        // balanced count digits: 2 N < 127 256^(I - 1)
-        int I = 1;
-        while (I < kMfImax && 2.0 * (double)max_count >= mf_count_limit(I)) I++;
-        DBX_REQUIRE(2.0 * (double)max_count < mf_count_limit(kMfImax), DBX_ERR_RANGE,
-                    "bin count too large for the matrix-core Debye sum");
+        DBX_REQUIRE(Imax <= kMfImax, DBX_ERR_RANGE, "bin count too large for the matrix-core Debye sum");
         const int Kc_pad = (Kc + kMfK - 1) & ~(kMfK - 1);
         const int ksteps = Kc_pad / kMfK;
-        double *rtab = (double *)mem.dev(kSlotRtab, sizeof(double) * 2 * (size_t)Kc_pad);
-        const MfShape sh = mf_shape(n_rows, I);
+        const MfShape sh = mf_shape(n_rows, Imax);
         const int rows_pad = (n_rows + sh.rows() - 1) / sh.rows() * sh.rows();
+        const int nblocks = rows_pad / 16;
+        DBX_REQUIRE(nblocks <= n_kext, DBX_ERR_STATE, "pass-2 row blocks beyond their table");
+        const RowDesc *krows = d_rows;
+        const int *bcls = nullptr;
+        std::vector<int> hbc;
+        if (per_row) {
+            // Rows ordered by digit class, most digits first, each class in the caller's order
+            // (longest first); a row group's matrix instructions then cover only the count digits
+            // of its own rows (bcls: the largest class of each 16-row block).  The sums are the
+            // same in any order: each row's sum depends on its own counts and the splits alone.
+            const int *hc = reinterpret_cast<const int *>(hv + 2);
+            std::vector<int> cls(n_rows), order(n_rows);
+            for (int r = 0; r < n_rows; r++) {
+                cls[r] = std::min(Imax, std::max(1, hc[r]));
+                order[r] = r;
+            }
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cls[a] > cls[b]; });
+            bool moved = false;
+            for (int r = 0; r < n_rows; r++) moved = moved || order[r] != r;
+            hbc.assign(nblocks, 1);
+            for (int r = 0; r < n_rows; r++) hbc[r / 16] = std::max(hbc[r / 16], cls[order[r]]);
+            // (the staging memory may be the read-back memory: the classes have been read)
+            char *st = (char *)mem.staging(stage_bytes);
+            int *sbc = (int *)(st + (sizeof(RowDesc) + sizeof(int)) * n_rows);
+            std::copy(hbc.begin(), hbc.end(), sbc);
+            if (moved) {
+                RowDesc *sr = (RowDesc *)st;
+                int *sp = (int *)(st + sizeof(RowDesc) * n_rows);
+                for (int r = 0; r < n_rows; r++) {
+                    sr[r] = h_rows ? h_rows[order[r]] : RowDesc{(int64_t)order[r] * nb_max, nb_max, uni_first_n, 0};
+                    sp[r] = order[r];
+                }
+                DBX_HIP(hipMemcpyAsync(d_krows, sr, sizeof(RowDesc) * n_rows, hipMemcpyHostToDevice, ctx->stream));
+                DBX_HIP(hipMemcpyAsync(d_perm, sp, sizeof(int) * n_rows, hipMemcpyHostToDevice, ctx->stream));
+                krows = d_krows;
+                perm = d_perm;
+            }
+            DBX_HIP(hipMemcpyAsync(d_bcls, sbc, sizeof(int) * nblocks, hipMemcpyHostToDevice, ctx->stream));
+            bcls = d_bcls;
+        }
+        double *rtab = (double *)mem.dev(kSlotRtab, sizeof(double) * 2 * (size_t)Kc_pad);
         const int qgroups = (nq + sh.qs() - 1) / sh.qs();
         const int rgroups = rows_pad / sh.rows();
-        // about 8 waves per SIMD over the grid, at least 8 steps per split
-        const int target = 2048 * 4 / (sh.WQ * sh.WR);
+        const int target = 2048 * 4 / (sh.WQ * sh.WR);  // about 8 waves per SIMD
         int nsplit = std::max(1, std::min((target + qgroups * rgroups - 1) / (qgroups * rgroups), ksteps / 8));
         const int kper = ((ksteps + nsplit - 1) / nsplit) * kMfK;
         nsplit = (Kc_pad + kper - 1) / kper;
-        signed char *cdig = (signed char *)mem.dev(kSlotDigits, (size_t)I * rows_pad * Kc_pad);
+        signed char *cdig = (signed char *)mem.dev(kSlotDigits, (size_t)Imax * rows_pad * Kc_pad);
         double *Spart = (double *)mem.dev(kSlotPass2b, sizeof(double) * (size_t)nsplit * n_rows * nq);
-        const int nblocks = rows_pad / 16;
         const int cd_x = (Kc_pad + 4 * kCdThreads - 1) / (4 * kCdThreads);
-        const int64_t n_in = (int64_t)cd_x * rows_pad + (Kc_pad + kCdThreads - 1) / kCdThreads +
-                             (nblocks + kCdThreads - 1) / kCdThreads;
+        const int n_rt = (Kc_pad + kCdThreads - 1) / kCdThreads;
+        const int64_t n_in = (int64_t)cd_x * rows_pad + n_rt + (nblocks + kCdThreads - 1) / kCdThreads;
         DBX_REQUIRE(n_in < INT32_MAX, DBX_ERR_ARG, "pass-2 problem too large");
-        hipLaunchKernelGGL(p2_mf_inputs_kernel, dim3((unsigned)n_in), dim3(kCdThreads), 0, ctx->stream, h, d_rows,
-                           n_rows, kz, Kc, Kc_pad, rows_pad, I, cd_x, cdig, rsum, rstep, rtab, rtab + Kc_pad,
-                           nblocks, kext);
+        hipLaunchKernelGGL(p2_mf_inputs_kernel, dim3((unsigned)n_in), dim3(kCdThreads), 0, ctx->stream, h, krows,
+                           n_rows, kz, Kc, Kc_pad, rows_pad, Imax, cd_x, cdig, rsum, rstep, rtab, rtab + Kc_pad,
+                           n_rt, nblocks, kext);
         if (ctx->profiling) DBX_HIP(hipEventRecord(ctx->ev_mf[0], ctx->stream));
-        mf_launch(sh, I, dim3((unsigned)qgroups, (unsigned)nsplit, (unsigned)rgroups), ctx->stream, rtab,
-                  rtab + Kc_pad, Kc_pad, kper, kext, cdig, rows_pad, d_q, nq, n_rows, rstep, Spart);
+        mf_launch(sh, Imax, dim3((unsigned)qgroups, (unsigned)nsplit, (unsigned)rgroups), ctx->stream, rtab,
+                  rtab + Kc_pad, Kc_pad, kper, kext, bcls, cdig, rows_pad, d_q, nq, n_rows, n_rows, rstep, Spart);
         if (ctx->profiling) {
             DBX_HIP(hipEventRecord(ctx->ev_mf[1], ctx->stream));
-            // issued matrix instructions: per workgroup, its steps x waves x RB x I x 5
+            // issued matrix instructions: per workgroup, its steps x waves x RB x its digits x 6
             std::vector<int> hk(nblocks);
             DBX_HIP(hipMemcpyAsync(hk.data(), kext, sizeof(int) * nblocks, hipMemcpyDeviceToHost, ctx->stream));
             DBX_HIP(hipStreamSynchronize(ctx->stream));
-            double steps = 0;
+            const int bpg = sh.rows() / 16;
+            double issued = 0, digits = 0;
             for (int z = 0; z < rgroups; z++) {
-                const int kx = std::min(Kc_pad, (hk[z * (sh.rows() / 16)] + kMfK - 1) & ~(kMfK - 1));
-                steps += (double)(kx / kMfK);
+                int ke = 0;
+                for (int b = 0; b < bpg; b++) ke = std::max(ke, hk[z * bpg + b]);
+                const int kx = std::min(Kc_pad, (ke + kMfK - 1) & ~(kMfK - 1));
+                const int Iz = bcls ? hbc[z * bpg] : Imax;
+                issued += (double)(kx / kMfK) * qgroups * sh.WQ * sh.WR * sh.RB * Iz * kMfJ * 32768.0;
+                digits += (double)Iz * sh.rows();
             }
             // algorithmic work: 2 nQ flops per nonzero bin of every row
             hipLaunchKernelGGL(p2_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, (unsigned int *)tot, (int64_t)2);
@@ -984,25 +1108,26 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
             unsigned long long nnz = 0;
             DBX_HIP(hipMemcpyAsync(&nnz, tot, sizeof(nnz), hipMemcpyDeviceToHost, ctx->stream));
             DBX_HIP(hipStreamSynchronize(ctx->stream));
-            const double waves = (double)qgroups * sh.WQ * sh.WR;
-            ctx->mf_flops = steps * waves * sh.RB * I * kMfJ * 32768.0;
+            ctx->mf_flops = issued;
             ctx->mf_useful = (double)nnz * (double)nq * 2.0;
-            ctx->mf_digits = I;
+            ctx->mf_digits = Imax;
+            ctx->mf_mean_digits = digits / rows_pad;
+            ctx->mf_bins = Kc;
             ctx->mf_timed = true;
         }
         hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
                            dim3(256), 0, ctx->stream, Spart, nsplit, 0, n_rows, nq, rsum, kMfShift, self, low,
-                           nlow, rstep, d_q, d_S);
-        unit_err = pass2_unit_error(mode, kper, nsplit, I + kMfJ - 1);
+                           nlow, rstep, d_q, perm, d_S);
+        unit_err = pass2_unit_error(mode, kper, nsplit, Imax + kMfJ - 1);
     } else {
         // no bin beyond the self-pair entries: S is their part alone
         hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
                            dim3(256), 0, ctx->stream, nullptr, 0, 0, n_rows, nq, nullptr, 0.0, self, low, nlow,
-                           rstep, d_q, d_S);
+                           rstep, d_q, nullptr, d_S);
     }
     if (d_B)
         hipLaunchKernelGGL(p2_row_bounds_kernel, dim3((n_rows + 63) / 64), dim3(64), 0, ctx->stream,
-                           Kc > 0 ? rsum : nullptr, self, low, nlow, n_rows, unit_err, d_B);
+                           Kc > 0 ? rsum : nullptr, self, low, nlow, n_rows, unit_err, perm, d_B);
     DBX_HIP(hipGetLastError());
     DBX_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     ctx->p2_timed = true;
@@ -1013,10 +1138,11 @@ namespace {
 struct CtxMem : P2Mem {
     dbx_ctx *ctx = nullptr;
     void *dev(int slot, size_t bytes) override { return scratch(ctx, slot, bytes); }
-    unsigned long long *readback() override
+    unsigned long long *readback(size_t words) override
     {
-        return (unsigned long long *)pinned(ctx, kPinMisc, sizeof(unsigned long long) * 2);
+        return (unsigned long long *)pinned(ctx, kPinMisc, sizeof(unsigned long long) * words);
     }
+    void *staging(size_t bytes) override { return pinned(ctx, kPinRows, bytes); }
 };
 }  // namespace
 
@@ -1040,6 +1166,28 @@ void debye_structure_rows(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::
     DBX_REQUIRE(rows[0].n_bins < (int64_t)INT32_MAX, DBX_ERR_ARG, "too many bins");
     structure_core(ctx, ctx_pass2_mem(ctx), d_hist_base, rows.data(), n_rows, rows[0].n_bins, 0, rstep, d_q, nq,
                    d_S, mode < 0 ? ctx->pass2_mode : mode, d_B);
+}
+
+void structure_fp64_models(dbx_ctx *ctx, const uint64_t *d_hist_base, const std::vector<RowDesc> &rows,
+                           const std::vector<int2> &models, double rstep, const double *d_q, int nq,
+                           const std::vector<double *> &S_dst, const std::vector<double *> &B_dst)
+{
+    std::vector<RowDesc> sub;
+    for (const int2 &m : models) sub.insert(sub.end(), rows.begin() + m.x, rows.begin() + m.x + m.y);
+    if (sub.empty()) return;
+    const size_t n = sub.size(), nqd = (size_t)nq;
+    double *dS = (double *)scratch(ctx, kSlotRefine, sizeof(double) * n * (nqd + 1));
+    double *dB = dS + n * nqd;
+    P2Refining mark(ctx);
+    debye_structure_rows(ctx, d_hist_base, sub, rstep, d_q, nq, dS, dB, DBX_PASS2_FP64);
+    size_t r0 = 0;
+    for (size_t m = 0; m < models.size(); m++) {
+        const size_t c = (size_t)models[m].y;
+        DBX_HIP(hipMemcpyAsync(S_dst[m], dS + r0 * nqd, sizeof(double) * c * nqd, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        DBX_HIP(hipMemcpyAsync(B_dst[m], dB + r0, sizeof(double) * c, hipMemcpyDeviceToDevice, ctx->stream));
+        r0 += c;
+    }
 }
 
 void debye_structure_mem(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist, int P, int64_t n_bins, double rstep,

@@ -386,6 +386,14 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
         io += order[i]->q.size();
         DBX_HIP(hipMemcpyAsync(hoor + i, order[i]->d_oor, 8, hipMemcpyDeviceToHost, ctx->stream));
     }
+    // each group's rows, kept for the FP64 recomputation after the synchronisation
+    struct GroupRows {
+        std::vector<size_t> grp;
+        std::vector<RowDesc> rows;
+        std::vector<int4> mrows;
+        uint64_t *base;
+    };
+    std::vector<GroupRows> grows;
     for (auto &g : groups) {
         // rows longest first within the group
         std::vector<size_t> grp;
@@ -429,24 +437,41 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
             DBX_HIP(hipMemcpyAsync(r->d_ff, dff + r0 * nq, sizeof(double) * rn, hipMemcpyDeviceToDevice, ctx->stream));
             DBX_HIP(hipMemcpyAsync(r->d_B, dB + r0, sizeof(double) * r->P, hipMemcpyDeviceToDevice, ctx->stream));
         }
+        grows.push_back(GroupRows{std::move(grp), std::move(rows), std::move(mrows), base});
     }
     DBX_HIP(hipStreamSynchronize(ctx->stream));
-    // requests whose bound misses the target: their partials again with FP64 arithmetic
+    // requests whose bound misses the target: their partials again with FP64 arithmetic, one
+    // pass 2 per q grid, into the request's own sums and bounds
     bool refined = false;
     if (ctx->pass2_mode == DBX_PASS2_MFMA) {
-        for (size_t i = 0; i < order.size(); i++) {
-            Request *r = order[i];
-            const int nq = (int)r->q.size();
-            bool bad = false;
-            for (int k = 0; k < nq && !bad; k++) bad = exceeds_target(hrel[I_off[i] + k], ctx->rel_err_target);
-            if (!bad || hoor[i]) continue;
-            debye_structure(ctx, r->d_hist, r->P, r->n_bins, rstep, r->first_n, r->d_q, nq, r->d_S, r->d_B,
-                            DBX_PASS2_FP64);
-            launch_combine(ctx->stream, r->d_ff, r->d_S, r->d_B, nullptr, 1, nq, r->d_I, r->d_rel,
-                           make_int4(0, r->P, 0, 0));
-            DBX_HIP(hipMemcpyAsync(hI + I_off[i], r->d_I, sizeof(double) * nq, hipMemcpyDeviceToHost, ctx->stream));
-            DBX_HIP(hipMemcpyAsync(hrel + I_off[i], r->d_rel, sizeof(double) * nq, hipMemcpyDeviceToHost,
-                                   ctx->stream));
+        for (GroupRows &gr : grows) {
+            std::vector<int2> redo;
+            std::vector<double *> Sd, Bd;
+            std::vector<size_t> who;
+            for (size_t gi = 0; gi < gr.grp.size(); gi++) {
+                const size_t i = gr.grp[gi];
+                Request *r = order[i];
+                bool bad = false;
+                for (size_t k = 0; k < r->q.size() && !bad; k++)
+                    bad = exceeds_target(hrel[I_off[i] + k], ctx->rel_err_target);
+                if (!bad || hoor[i]) continue;
+                redo.push_back(make_int2(gr.mrows[gi].x, r->P));
+                Sd.push_back(r->d_S);
+                Bd.push_back(r->d_B);
+                who.push_back(i);
+            }
+            if (redo.empty()) continue;
+            const int nq = (int)order[who[0]]->q.size();
+            structure_fp64_models(ctx, gr.base, gr.rows, redo, rstep, order[who[0]]->d_q, nq, Sd, Bd);
+            for (size_t i : who) {
+                Request *r = order[i];
+                launch_combine(ctx->stream, r->d_ff, r->d_S, r->d_B, nullptr, 1, nq, r->d_I, r->d_rel,
+                               make_int4(0, r->P, 0, 0));
+                DBX_HIP(hipMemcpyAsync(hI + I_off[i], r->d_I, sizeof(double) * nq, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+                DBX_HIP(hipMemcpyAsync(hrel + I_off[i], r->d_rel, sizeof(double) * nq, hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            }
             refined = true;
         }
         if (refined) DBX_HIP(hipStreamSynchronize(ctx->stream));

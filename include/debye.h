@@ -140,15 +140,15 @@ int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bin
 /* Pass-2 arithmetic of a context (default DBX_PASS2_MFMA):
  *   DBX_PASS2_MFMA: matrix-core contraction on signed 8-bit digits with exact 32-bit integer
  *     accumulation; sinc is evaluated in FP64, shifted by 3/8 and rounded to a multiple of
- *     1 / (3 2^38) (|error| < 2^-40 per bin);
+ *     1 / (3 2^46) (|error| < 2^-48 per bin, below the FP64 sine's own error);
  *   DBX_PASS2_FP64: FP64 vector arithmetic throughout (the reference's debyeRowSum). */
 enum { DBX_PASS2_MFMA = 0, DBX_PASS2_FP64 = 1 };
 int dbx_set_pass2_mode(dbx_ctx *ctx, int mode);
 /* Error target of a context's pass 2 (default 1e-6).  Every I(q) comes with a bound of its
  * relative error: (sum_p |ffoobb_p(q)| B_p + rounding of the final sum) / |I(q)|, where B_p is
  * the sum of the partial's weights 2 N_pk times the largest error per unit weight of the
- * arithmetic (matrix cores: the sinc digits' rounding to 1 / (3 2^39) plus the FP64 sine and
- * sums; FP64: the rounding of the sums).  A model whose bound exceeds the target at any q
+ * arithmetic (matrix cores: the sinc digits' rounding to 1 / (3 2^47) plus the FP64 sine and
+ * sums; FP64: the rounding of the sinc values and sums).  A model whose bound exceeds the target at any q
  * has its partials recomputed with FP64 arithmetic (target <= 0: never). */
 int dbx_set_rel_err_target(dbx_ctx *ctx, double target);
 
@@ -334,7 +334,9 @@ int dbx_selftest_sin(const double *x, int n, double *out);
  * that met an out-of-window bin and were re-run by the checked kernel (performance figures). */
 int dbx_last_pass1_stats(const dbx_ctx *ctx, int64_t *n_jobs, uint64_t *window_misses, uint64_t *retried);
 /* Device times (HIP events on the context's stream) of the last pass-1 histogram kernel
- * and of the last pass-2 structure computation (compaction through the Debye sum), ms. */
+ * and of the last pass-2 structure computation (compaction through the Debye sum), ms.  When
+ * some models were recomputed with FP64 arithmetic to meet the error target, the pass-2 time
+ * runs from the start of the first pass 2 to the end of the recomputation (read-back included). */
 int dbx_last_kernel_times(dbx_ctx *ctx, double *pass1_ms, double *pass2_ms);
 
 #ifdef __cplusplus

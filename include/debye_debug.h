@@ -49,9 +49,10 @@ int dbx_selftest_lds_atomic_rate(double *atomics_per_s);
 int dbx_debug_sin_host(const double *x, int n, double *out);
 int dbx_debug_sincos_host(const double *x, int n, double *s_out, double *c_out);
 
-/* The five signed sinc digits d_0..d_4 (digits[5 i + j]) that the matrix-core Debye sum forms
- * for sinc(q[i] r[i]): sinc = 3/8 + sum_j d_j 256^j / (3 2^38) within 2^-40 (the device code,
- * built for the host; q = 0 is taken as 1e-30).  n must be a multiple of 4. */
+/* The six signed sinc digits d_0..d_5 (digits[6 i + j]) that the matrix-core Debye sum forms
+ * for sinc(q[i] r[i]): sinc = 3/8 + sum_j d_j 256^j / (3 2^46) within 2^-48 plus the error of
+ * the FP64 sine (the device code, built for the host; q = 0 is taken as 1e-30).  n must be a
+ * multiple of 4. */
 int dbx_debug_sinc_digits_host(const double *q, const double *r, int n, signed char *digits);
 
 /* Profiling of the matrix-core Debye sum on ctx (off by default).  When on, each pass 2
@@ -60,10 +61,11 @@ int dbx_debug_sinc_digits_host(const double *q, const double *r, int n, signed c
  * for the last pass 2: the contraction kernel's time (ms), the 8-bit integer matrix operations
  * it issued (16 x 16 x 64 x 2 per v_mfma_i32_16x16x64_i8), the algorithmic flops (2 nQ per nonzero bin
  * of every row: the contraction counted once, whatever the digit count), the compacted bin
- * count and the count digits. */
+ * count, the largest count-digit number and the mean over the row tiles (each tile takes the
+ * digits its own rows need). */
 int dbx_debug_set_profiling(dbx_ctx *ctx, int on);
 int dbx_debug_pass2_profile(dbx_ctx *ctx, double *kernel_ms, double *mfma_flops, double *useful_flops,
-                            int *n_bins_compacted, int *count_digits);
+                            int *n_bins_compacted, int *count_digits, double *mean_digits);
 
 /* Models whose partials the last pass 2 on ctx recomputed with FP64 arithmetic because their
  * matrix-core error bound exceeded the context's target (dbx_set_rel_err_target). */

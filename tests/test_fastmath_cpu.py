@@ -67,10 +67,11 @@ def test_sincos_pair_accuracy():
 
 def test_sinc_digits_encoding():
     """The matrix-core Debye sum's sinc digits (mfdigits.h, host build of the device code):
-    sinc(q r) = 3/8 + sum_j d_j 256^j / (3 2^38) to within half a grid step (2^-40) plus a few
-    ulp, for five signed bytes d_j, against a 50-digit sinc.  Covers q = 0 (sinc exactly 1),
-    tiny q r, the sinc minimum (q r = 4.4934, sinc = -0.21723), and q r up to 2e4: the ends of
-    the range the five digits must hold."""
+    sinc(q r) = 3/8 + sum_j d_j 256^j / (3 2^46) to within half a grid step (below 2^-48) plus
+    the FP64 sine's error, for six signed bytes d_j, against a 50-digit sinc.  Covers q = 0
+    (sinc exactly 1), tiny q r, the sinc minimum (q r = 4.4934, sinc = -0.21723), and q r up to
+    2e4: the ends of the range the six digits must hold.  A dropped or misplaced top digit
+    would be off by 256^4 / c or more."""
     rng = np.random.default_rng(5)
     xmin = 4.493409457909064
     qr = np.concatenate([rng.uniform(0, 3, 400), rng.uniform(0, 2e4, 800), xmin + rng.uniform(-1e-6, 1e-6, 40),
@@ -80,13 +81,13 @@ def test_sinc_digits_encoding():
     q[:8] = 0.0
     n = len(q) // 4 * 4
     q, r = np.ascontiguousarray(q[:n]), np.ascontiguousarray(r[:n])
-    d = np.zeros((n, 5), np.int8)
+    d = np.zeros((n, 6), np.int8)
     fn = cd.lib().dbx_debug_sinc_digits_host
     fn.restype = C.c_int
     p = lambda a: a.ctypes.data_as(C.c_void_p)
     assert fn(p(q), p(r), n, p(d)) == 0
-    c = 3.0 * 2.0 ** 38
-    V = sum(d[:, j].astype(np.int64) * 256 ** j for j in range(5))
+    c = 3.0 * 2.0 ** 46
+    V = sum(d[:, j].astype(np.int64) * 256 ** j for j in range(6))
     got = 0.375 + V / c
     ref = []
     for qi, ri in zip(q, r):
@@ -95,4 +96,4 @@ def test_sinc_digits_encoding():
     ref = np.array(ref)
     assert np.abs(got - ref).max() <= 0.5 / c + 4e-15
     assert np.all(got[:8] == 1.0)
-    assert got.min() < -0.2172 and V.max() > 0.46 * 2.0 ** 40
+    assert got.min() < -0.2172 and V.max() > 0.46 * 2.0 ** 48

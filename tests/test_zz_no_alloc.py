@@ -124,3 +124,30 @@ def test_noalloc_refuses_problems_beyond_its_region():
             assert np.array_equal(I, Iref)
         finally:
             dev.free()
+
+
+def test_noalloc_equals_debye_on_rows_of_mixed_digit_classes():
+    """40 rows whose counts need one to five digits in shuffled order: the fixed region's pinned
+    words hold the row classes and the reordered rows, so the call takes the same launches as
+    dbx_debye and equals it bit for bit."""
+    from types import SimpleNamespace
+    rng = np.random.default_rng(78)
+    nb, nq = 12000, 200
+    caps = np.repeat([2 ** 5, 2 ** 13, 2 ** 21, 2 ** 29, 2 ** 36], 8)
+    rng.shuffle(caps)
+    P = len(caps)
+    H = np.zeros((P, nb), np.uint64)
+    for p in range(P):
+        idx = rng.choice(np.arange(1, nb), 1000, replace=False)
+        H[p, idx] = rng.integers(1, caps[p], 1000).astype(np.uint64)
+    m = SimpleNamespace(q=np.linspace(0.3, 20.0, nq), rstep=1e-3, first_n=8)
+    ff = rng.uniform(0.5, 2.0, (P, nq))
+    dev = Dev()
+    with cd.Context(0) as ctx:
+        ctx.set_rel_err_target(0)  # the matrix-core sums, never recomputed in FP64
+        try:
+            Iref, Sref, rref = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n, with_S=True, with_err=True)
+            I, S, rel = _noalloc(ctx, dev, H, m, ff)
+            assert np.array_equal(I, Iref) and np.array_equal(S, Sref) and np.array_equal(rel, rref)
+        finally:
+            dev.free()
