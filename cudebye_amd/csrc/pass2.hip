@@ -105,19 +105,27 @@ constexpr int kFlagRows = 32;  // rows per thread of the nonzero-flag pass
 // flags[k] = 1 where some row holds a nonzero count at bin k > k_self_max; grid.y over chunks
 // of kFlagRows rows (a bin written by several chunks gets the same value from each).  Also each
 // row's digit class: the count digits the matrix-core sum needs for its largest count beyond the
-// self bins, the smallest I with 2 N < 127 256^(I - 1) (6: more than five), from four ballots per
-// wave and row and one atomic per wave and row that needs more than one digit.
+// self bins, the smallest I with 2 N < 127 256^(I - 1) (6: more than five), from up to five
+// ballots per wave and row, an LDS maximum per workgroup and row, and a global atomic only
+// where the workgroup's class exceeds the one already recorded (the same few addresses are
+// hit by every workgroup: unconditional atomics there serialise).
+// Rows are longest first: a wave stops at the first row that ends before its first bin.
 __global__ __launch_bounds__(kScanBlock) void p2_flags_kernel(
     const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows, int n_rows, int64_t nb_max,
     int k_self_max, unsigned char *__restrict__ flags, int *__restrict__ rowcls)
 {
+    __shared__ int scls[kFlagRows];
     const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
     const int r0 = blockIdx.y * kFlagRows;
     const int r1 = min(n_rows, r0 + kFlagRows);
+    if (threadIdx.x < kFlagRows) scls[threadIdx.x] = 0;
+    __syncthreads();
     const bool kin = k < nb_max && k > k_self_max;
+    const int64_t kw = k - (threadIdx.x & 63);  // the wave's first bin
     bool nz = false;
     for (int r = r0; r < r1; r++) {
         const RowDesc rd = rows[r];
+        if (kw >= rd.n_bins) break;  // rows are longest first: no later one reaches the wave's bins
         const unsigned long long v2 = (kin && k < rd.n_bins) ? 2ull * hist[rd.h_off + k] : 0ull;
         nz |= v2 != 0ull;
         const uint64_t c2 = __ballot(v2 >= (unsigned long long)mf_count_limit(1));
@@ -126,9 +134,16 @@ __global__ __launch_bounds__(kScanBlock) void p2_flags_kernel(
                       (__ballot(v2 >= (unsigned long long)mf_count_limit(3)) != 0) +
                       (__ballot(v2 >= (unsigned long long)mf_count_limit(4)) != 0) +
                       (__ballot(v2 >= (unsigned long long)mf_count_limit(5)) != 0);
-        if ((threadIdx.x & 63) == 0) atomicMax(&rowcls[r], c);
+        if ((threadIdx.x & 63) == 0) atomicMax(&scls[r - r0], c);
     }
     if (nz) flags[k] = 1;
+    __syncthreads();
+    // (a stale read only costs an atomic: the recorded class never decreases)
+    const int t = threadIdx.x;
+    if (t < r1 - r0) {
+        const int c = scls[t];
+        if (c > 1 && c > rowcls[r0 + t]) atomicMax(&rowcls[r0 + t], c);
+    }
 }
 
 // profiling: total nonzero bins over all rows (the bins the reference's per-pair sum contracts)

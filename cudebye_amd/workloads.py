@@ -9,10 +9,13 @@
     c4-batch  256 two-species models (1k-60k atoms) in one dbx_batch_intensity call
     c4-plain  the same 256 models, one dbx_intensity call each
     c5-serve  150 mixed requests (500-200k atoms, 1-4 species) through the server, all at once
+    copy      a 1 GiB device-to-device copy (the HIP runtime's copy kernel): the reference for
+              scaling the HBM traffic counters FETCH_SIZE and WRITE_SIZE to bytes
 
 Each repetition prints one line with the wall time and the library's device times.
 """
 import argparse
+import ctypes as C
 import time
 
 import cudebye_amd as cd
@@ -70,9 +73,30 @@ def _c5(reps, mode):
             print(f"c5-serve rep {r}: {(time.perf_counter() - t) * 1e3:.2f} ms  {srv.stats()}", flush=True)
 
 
+def _copy(reps):
+    cd.lib()  # loads the HIP runtime the library links
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    n = 1 << 30
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0
+    assert hip.hipMemset(a, 1, n) == 0 and hip.hipDeviceSynchronize() == 0
+    for r in range(reps):
+        t = time.perf_counter()
+        assert hip.hipMemcpy(b, a, n, 3) == 0  # device to device
+        assert hip.hipDeviceSynchronize() == 0
+        dt = time.perf_counter() - t
+        print(f"copy rep {r}: {n} bytes in {dt * 1e3:.3f} ms ({2 * n / dt / 1e9:.0f} GB/s read + write)", flush=True)
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("name", choices=["c1", "c2", "c2-fine", "c3", "c4-batch", "c4-plain", "c5-serve"])
+    ap.add_argument("name", choices=["c1", "c2", "c2-fine", "c3", "c4-batch", "c4-plain", "c5-serve", "copy"])
     ap.add_argument("--reps", type=int, default=2)
     ap.add_argument("--fp64", action="store_true", help="FP64 pass 2 instead of the matrix cores")
     a = ap.parse_args()
@@ -89,6 +113,8 @@ def main():
         _c4(a.reps, mode, True)
     elif a.name == "c4-plain":
         _c4(a.reps, mode, False)
+    elif a.name == "copy":
+        _copy(a.reps)
     else:
         _c5(a.reps, mode)
 
