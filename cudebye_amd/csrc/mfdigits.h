@@ -52,11 +52,16 @@ __host__ __device__ inline uint32_t mf_perm(uint32_t hi, uint32_t lo, uint32_t s
 // fma(sv cQ, ir, magic) rounds c T + magic to an integer whose low 48 mantissa bits are
 // W = V + 0x808080808080.  lo: bytes 0..3 of W, each xor 0x80 (= the signed digits d_0..d_3);
 // hi: the upper word of the double, bytes 0 and 1 = bytes 4 and 5 of W.
-__host__ __device__ inline void sinc_word(double sv, double cQ, double ir, uint32_t &lo, uint32_t &hi)
+// svc = sv cQ: the sine already scaled (the kernel's recurrence runs on scaled sines)
+__host__ __device__ inline void sinc_word_scaled(double svc, double ir, uint32_t &lo, uint32_t &hi)
 {
-    const uint64_t u = __builtin_bit_cast(uint64_t, fma(sv * cQ, ir, kMfMagic));
+    const uint64_t u = __builtin_bit_cast(uint64_t, fma(svc, ir, kMfMagic));
     lo = (uint32_t)u ^ 0x80808080u;
     hi = (uint32_t)(u >> 32);
+}
+__host__ __device__ inline void sinc_word(double sv, double cQ, double ir, uint32_t &lo, uint32_t &hi)
+{
+    sinc_word_scaled(sv * cQ, ir, lo, hi);
 }
 
 // Four bins' words -> six digit dwords (dword j: byte b = digit j of bin b): a 4 x 4 byte

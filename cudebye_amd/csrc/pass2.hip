@@ -656,8 +656,14 @@ __device__ __forceinline__ void mf_tile_sum(const int3 tile, int kend, i32x4 (*s
     };
     auto gen = [&](auto rot_tag, int slot, int buf) {
         constexpr bool ROT = decltype(rot_tag)::value;
+        // ROT: the recurrence runs on sines and cosines scaled by cQ (it is linear), which
+        // saves the scaling of every value
         double sv = 0.0, cv = 0.0, dv = 0.0;
-        if constexpr (ROT) sincos_cw(Q * srt[slot][gk], sv, cv);
+        if constexpr (ROT) {
+            sincos_cw(Q * srt[slot][gk], sv, cv);
+            sv *= cQ;
+            cv *= cQ;
+        }
         int dw[NG][kMfJ];
 #pragma unroll
         for (int g4 = 0; g4 < NG; g4++) {
@@ -676,9 +682,9 @@ __device__ __forceinline__ void mf_tile_sum(const int3 tile, int kend, i32x4 (*s
                     }
                     s = sv;
                 } else {
-                    s = sin_cw(Q * srt[slot][gk + t]);
+                    s = sin_cw(Q * srt[slot][gk + t]) * cQ;
                 }
-                sinc_word(s, cQ, srt[slot][kMfK + gk + t], lo[u], hi[u]);
+                sinc_word_scaled(s, srt[slot][kMfK + gk + t], lo[u], hi[u]);
             }
             pack4(lo, hi, dw[g4]);
         }
@@ -914,8 +920,11 @@ __global__ void p2_rows_fill_kernel(RowDesc *__restrict__ rows, int n_rows, int6
 // on the device.
 // Error per unit weight of a row's sum over its compacted bins, sum_k A_k T_k with A_k = 2 N_k:
 //   matrix cores: the sinc digits round c (T - 3/8) to an integer, |dT| <= 1 / (2c); the FP64 sine
-//     and its recurrence add at most 4e-15 (sin_cw alone, below 1e-15), the product forming c T
-//     3 u; the FP64 partial sums hold at most 0.625 sum_k A_k (|T - 3/8| <= 0.625) and take
+//     and its recurrence add at most 4e-15 (sin_cw alone, below 1e-15); forming c T takes 5 u:
+//     c / Q and 1 / r one rounding each, and the scaling of a run's first sine and cosine by
+//     c / Q, which the linear recurrence carries to every bin of the run with a gain of at most
+//     |s| + |c| <= sqrt(2), below 2.9 u of T where Q r >= 0.5 (below u for a lone sine scaled
+//     once); the FP64 partial sums hold at most 0.625 sum_k A_k (|T - 3/8| <= 0.625) and take
 //     nsplit + classes x folds + 2 rounded additions each, and the 3/8 sum_k A_k term one
 //     conversion;
 //   FP64: a sinc value within 4 u (sine, quotient, product), then kper + nsplit + 2 rounded
@@ -929,7 +938,7 @@ static double pass2_unit_error(int mode, int kper_bins, int nsplit, int classes)
         return 4 * u + n * u / (1.0 - n * u);
     }
     const int folds = (kper_bins / kMfK + kMfFold - 1) / kMfFold;
-    return 0.5 / kMfC + 4e-15 + 3 * u + 0.625 * u * (double)(nsplit + classes * folds + 2) + 0.375 * u;
+    return 0.5 / kMfC + 4e-15 + 5 * u + 0.625 * u * (double)(nsplit + classes * folds + 2) + 0.375 * u;
 }
 
 static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base, const RowDesc *h_rows,
