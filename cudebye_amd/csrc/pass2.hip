@@ -112,9 +112,10 @@ constexpr int kFlagRows = 32;  // rows per thread of the nonzero-flag pass
 // Rows are longest first: a wave stops at the first row that ends before its first bin.
 __global__ __launch_bounds__(kScanBlock) void p2_flags_kernel(
     const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows, int n_rows, int64_t nb_max,
-    int k_self_max, unsigned char *__restrict__ flags, int *__restrict__ rowcls)
+    int k_self_max, unsigned char *__restrict__ flags, int *__restrict__ rowcls, int *__restrict__ block_counts)
 {
     __shared__ int scls[kFlagRows];
+    __shared__ int wcnt[kScanBlock / 64];
     const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
     const int r0 = blockIdx.y * kFlagRows;
     const int r1 = min(n_rows, r0 + kFlagRows);
@@ -137,12 +138,22 @@ __global__ __launch_bounds__(kScanBlock) void p2_flags_kernel(
         if ((threadIdx.x & 63) == 0) atomicMax(&scls[r - r0], c);
     }
     if (nz) flags[k] = 1;
+    // one chunk of rows (block_counts given): the block's flagged bins, for the scan
+    if (block_counts) {
+        const uint64_t bnz = __ballot(nz);
+        if ((threadIdx.x & 63) == 0) wcnt[threadIdx.x >> 6] = __popcll(bnz);
+    }
     __syncthreads();
     // (a stale read only costs an atomic: the recorded class never decreases)
     const int t = threadIdx.x;
     if (t < r1 - r0) {
         const int c = scls[t];
         if (c > 1 && c > rowcls[r0 + t]) atomicMax(&rowcls[r0 + t], c);
+    }
+    if (block_counts && t == 0) {
+        int c = 0;
+        for (int w = 0; w < kScanBlock / 64; w++) c += wcnt[w];
+        block_counts[blockIdx.x] = c;
     }
 }
 
@@ -182,48 +193,85 @@ __global__ void p2_count_kernel(const unsigned char *__restrict__ flags, int64_t
 }
 
 // exclusive scan of the block counts, one workgroup; total[0] = the sum (compacted bins),
-// total[2] = the largest row digit class (at least 1)
-__global__ void p2_scan_kernel(int *__restrict__ block_counts, int n_blocks, int *__restrict__ total,
-                               const int *__restrict__ rowcls, int n_rows)
+// total[2] = the largest row digit class (at least 1).  Each thread sums a run of blocks; the
+// runs' totals are scanned by waves (shuffles) and across the 16 waves, with no serial loop.
+__global__ __launch_bounds__(1024) void p2_scan_kernel(int *__restrict__ block_counts, int n_blocks,
+                                                       int *__restrict__ total, const int *__restrict__ rowcls,
+                                                       int n_rows)
 {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
+    __shared__ int wsum[16], wcls[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (n_blocks + 1023) / 1024;
     const int b0 = min(n_blocks, t * per), b1 = min(n_blocks, b0 + per);
     int s = 0;
     for (int i = b0; i < b1; i++) s += block_counts[i];
-    part[t] = s;
     int cls = 1;
     for (int r = t; r < n_rows; r += 1024) cls = max(cls, rowcls[r]);
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int i = 0; i < 1024; i++) {
-            const int c = part[i];
-            part[i] = acc;
-            acc += c;
-        }
-        *total = acc;
+    // inclusive scan of s within the wave, maximum of cls
+    int inc = s;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(inc, off);
+        if (lane >= off) inc += v;
+        cls = max(cls, __shfl_xor(cls, off));
     }
+    if (lane == 63) wsum[w] = inc;
+    if (lane == 0) wcls[w] = cls;
     __syncthreads();
-    int acc = part[t];
+    int base = 0, all = 0, cmax = 1;
+    for (int i = 0; i < 16; i++) {
+        if (i < w) base += wsum[i];
+        all += wsum[i];
+        cmax = max(cmax, wcls[i]);
+    }
+    int acc = base + inc - s;  // exclusive prefix of this thread's run
     for (int i = b0; i < b1; i++) {
         const int c = block_counts[i];
         block_counts[i] = acc;
         acc += c;
     }
-    __syncthreads();
-    part[t] = cls;
-    __syncthreads();
     if (t == 0) {
-        for (int i = 1; i < 1024; i++) cls = max(cls, part[i]);
-        total[2] = cls;
+        total[0] = all;
+        total[2] = cmax;
     }
 }
 
-__global__ void p2_scatter_kernel(const unsigned char *__restrict__ flags, int64_t nb_max,
-                                  const int *__restrict__ block_offsets, int *__restrict__ kz)
+// the self-pair terms of row r: the reference's weight rule for the entries at r <= 0.0005
+// (bins up to k_self_max): the first first_n nonzero ones count once (self), the others twice
+// with sinc (low)
+__device__ __forceinline__ void self_row(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
+                                         int r, int k_self_max, double *__restrict__ self, double *__restrict__ low)
 {
+    const RowDesc rd = rows[r];
+    double s = 0;
+    int rank = 0;
+    for (int k = 0; k <= k_self_max; k++) {
+        const unsigned long long n = k < rd.n_bins ? hist[rd.h_off + k] : 0ull;
+        double lw = 0;
+        if (n) {
+            if (rank < rd.first_n)
+                s += (double)n;
+            else
+                lw = 2.0 * (double)n;
+            rank++;
+        }
+        low[(int64_t)r * (k_self_max + 1) + k] = lw;
+    }
+    self[r] = s;
+}
+
+// blocks [0, nblk): the compacted bin list kz from the flags and the scanned block offsets;
+// blocks from nblk: the self-pair terms of one row per thread
+__global__ void p2_scatter_kernel(const unsigned char *__restrict__ flags, int64_t nb_max,
+                                  const int *__restrict__ block_offsets, int *__restrict__ kz, int nblk,
+                                  const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
+                                  int n_rows, int k_self_max, double *__restrict__ self, double *__restrict__ low)
+{
+    if ((int)blockIdx.x >= nblk) {
+        const int r = (blockIdx.x - nblk) * kScanBlock + threadIdx.x;
+        if (r < n_rows) self_row(hist, rows, r, k_self_max, self, low);
+        return;
+    }
     __shared__ int woff[kScanBlock / 64];
     const int64_t k = (int64_t)blockIdx.x * kScanBlock + threadIdx.x;
     const bool nz = k < nb_max && flags[k];
@@ -290,29 +338,6 @@ __global__ void p2_weights_kernel(const unsigned long long *__restrict__ hist, c
 // weight-1 bins (cuDebye-v15.cu:508-514) and the weight-2 bins at r <= 0.0005 that the
 // index condition excludes; one thread per row.  self[r] = sum of weight-1 counts,
 // low[r * (k_self_max+1) + k] = 2 N for the other low bins (added to S with sinc(q r_k)).
-__global__ void p2_self_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
-                               int n_rows, int k_self_max, double *__restrict__ self, double *__restrict__ low)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_rows) return;
-    const RowDesc rd = rows[r];
-    double s = 0;
-    int rank = 0;
-    for (int k = 0; k <= k_self_max; k++) {
-        const unsigned long long n = k < rd.n_bins ? hist[rd.h_off + k] : 0ull;
-        double lw = 0;
-        if (n) {
-            if (rank < rd.first_n)
-                s += (double)n;
-            else
-                lw = 2.0 * (double)n;
-            rank++;
-        }
-        low[(int64_t)r * (k_self_max + 1) + k] = lw;
-    }
-    self[r] = s;
-}
-
 // ---------------------------------------------------------------------------------
 // Debye sum, FP64: one thread per q, bins split over blockIdx.y and staged through LDS,
 // one sinc per (bin, q) shared by all rows of the pass; split partials reduced in fixed order.
@@ -358,6 +383,22 @@ __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
     }
 }
 
+// Absolute error bound of each row's structure sum S[r][q], the same for every q:
+//   B[r] = e rsum[r] + 5 u (self[r] + sum_k |low[r][k]|)
+// rsum[r] = sum of the weights 2 N over the row's compacted bins (exact), e = the error per unit
+// weight of the mode's sum over them (pass2_unit_error); the weight-1 bins are exact integers and
+// each low bin's FP64 sinc term is within 5 u of its value.  rsum null: no compacted bin.
+// r: the order rsum was computed in; perm[r] (perm null: r) the caller's row
+__device__ __forceinline__ void row_bound(const unsigned long long *__restrict__ rsum,
+                                          const double *__restrict__ self, const double *__restrict__ low, int nlow,
+                                          int r, double e, const int *__restrict__ perm, double *__restrict__ B)
+{
+    const int o = perm ? perm[r] : r;
+    double lo = self[o];
+    for (int k = 0; k < nlow; k++) lo += fabs(low[(int64_t)o * nlow + k]);
+    B[o] = (rsum ? e * (double)rsum[r] : 0.0) + 5.0 * 0x1p-53 * lo;
+}
+
 // Row r = r0 + i / nq in the order the sums were computed in, q index iq = i % nq (i over
 // nr rows x nq), the caller's row o = perm[r] (perm null: o = r):
 //   S[o][iq] = (shift rsum[r] + sum over splits of Spart[s][i], fixed order)
@@ -365,14 +406,23 @@ __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
 // rsum (the q-independent part of the matrix-core sum) may be null; nsplit may be 0.  The
 // second term is the reference's weight rule for the entries at r <= 0.0005 among the first
 // first_n (weight 1, no sinc) and their neighbours below first_n (weight 2 with sinc).
-__global__ void p2_finish_rows_kernel(const double *__restrict__ Spart, int nsplit, int r0, int nr, int nq,
-                                      const unsigned long long *__restrict__ rsum, double shift,
-                                      const double *__restrict__ self, const double *__restrict__ low, int nlow,
-                                      double rstep, const double *__restrict__ q, const int *__restrict__ perm,
-                                      double *__restrict__ S)
+// The blocks past those of the sums write the bounds B of rows r0..r0+nr (brsum: the row sums
+// of the weights, e: the unit error), one row per thread; none when B is null.
+constexpr int kFinishThreads = 256;
+__global__ __launch_bounds__(kFinishThreads) void p2_finish_rows_kernel(
+    const double *__restrict__ Spart, int nsplit, int r0, int nr, int nq, const unsigned long long *__restrict__ rsum,
+    double shift, const double *__restrict__ self, const double *__restrict__ low, int nlow, double rstep,
+    const double *__restrict__ q, const int *__restrict__ perm, double *__restrict__ S,
+    const unsigned long long *__restrict__ brsum, double e, double *__restrict__ B)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n = (int64_t)nr * nq;
+    const int64_t nfin = (n + kFinishThreads - 1) / kFinishThreads;
+    if ((int64_t)blockIdx.x >= nfin) {
+        const int r = r0 + (int)((int64_t)blockIdx.x - nfin) * kFinishThreads + (int)threadIdx.x;
+        if (r < r0 + nr) row_bound(brsum, self, low, nlow, r, e, perm, B);
+        return;
+    }
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int r = r0 + (int)(i / nq);
     const int iq = (int)(i - (int64_t)(r - r0) * nq);
@@ -398,24 +448,6 @@ __global__ void p2_finish_rows_kernel(const double *__restrict__ Spart, int nspl
         }
     }
     S[(int64_t)o * nq + iq] = acc + lo;
-}
-
-// Absolute error bound of each row's structure sum S[r][q], the same for every q:
-//   B[r] = e rsum[r] + 5 u (self[r] + sum_k |low[r][k]|)
-// rsum[r] = sum of the weights 2 N over the row's compacted bins (exact), e = the error per unit
-// weight of the mode's sum over them (pass2_unit_error); the weight-1 bins are exact integers and
-// each low bin's FP64 sinc term is within 5 u of its value.  rsum null: no compacted bin.
-// r: the order rsum was computed in; perm[r] (perm null: r) the caller's row
-__global__ void p2_row_bounds_kernel(const unsigned long long *__restrict__ rsum, const double *__restrict__ self,
-                                     const double *__restrict__ low, int nlow, int n_rows, double e,
-                                     const int *__restrict__ perm, double *__restrict__ B)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_rows) return;
-    const int o = perm ? perm[r] : r;
-    double lo = self[o];
-    for (int k = 0; k < nlow; k++) lo += fabs(low[(int64_t)o * nlow + k]);
-    B[o] = (rsum ? e * (double)rsum[r] : 0.0) + 5.0 * 0x1p-53 * lo;
 }
 
 // ---------------------------------------------------------------------------------
@@ -876,7 +908,7 @@ static void mf_launch(const MfShape &sh, int I, dim3 grid, hipStream_t st, const
 // rel (optional) receives the relative error bound of each I[dest][q] against the exact sum of
 // the same histograms and prefactors:
 //   (sum_r |ff_r| B_r + (P + 8) u sum_r |ff_r S_r|) / |I|
-// with B_r the row bounds (p2_row_bounds_kernel); the second term covers the FP64 rounding of
+// with B_r the row bounds (row_bound); the second term covers the FP64 rounding of
 // this sum, of S_r's last additions and of the prefactors (a few units each).
 __global__ void p2_combine_kernel(const double *__restrict__ ff, const double *__restrict__ S,
                                   const double *__restrict__ B, const int4 *__restrict__ mrows, int4 m0, int nq,
@@ -907,11 +939,14 @@ static int self_bin_max(double rstep)
     return k;
 }
 
-// row r of a single model's P partials: its histogram at r * n_bins, all of equal length
-__global__ void p2_rows_fill_kernel(RowDesc *__restrict__ rows, int n_rows, int64_t n_bins, int first_n)
+// zeroes n_words words at z and, with rows given, writes the descriptors of a single model's
+// P partials: row r's histogram at r * n_bins, all of equal length
+__global__ void p2_prep_kernel(unsigned int *__restrict__ z, int64_t n_words, RowDesc *__restrict__ rows,
+                               int n_rows, int64_t n_bins, int first_n)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r < n_rows) rows[r] = RowDesc{(int64_t)r * n_bins, n_bins, first_n, 0};
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_words) z[i] = 0u;
+    if (rows && i < n_rows) rows[i] = RowDesc{i * n_bins, n_bins, first_n, 0};
 }
 
 // The pass-2 structure sums S[r][q] of n_rows rows (longest first; nb_max = the first row's
@@ -939,6 +974,18 @@ static double pass2_unit_error(int mode, int kper_bins, int nsplit, int classes)
     }
     const int folds = (kper_bins / kMfK + kMfFold - 1) / kMfFold;
     return 0.5 / kMfC + 4e-15 + 5 * u + 0.625 * u * (double)(nsplit + classes * folds + 2) + 0.375 * u;
+}
+
+// the finish launch: sums of rows [r0, r0 + nr) (and their bounds when B is given)
+static void launch_finish(hipStream_t st, const double *Spart, int nsplit, int r0, int nr, int nq,
+                          const unsigned long long *rsum, double shift, const double *self, const double *low,
+                          int nlow, double rstep, const double *q, const int *perm, double *S,
+                          const unsigned long long *brsum, double e, double *B)
+{
+    const int64_t nfin = ((int64_t)nr * nq + kFinishThreads - 1) / kFinishThreads;
+    const int64_t nb = B ? (nr + kFinishThreads - 1) / kFinishThreads : 0;
+    hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(nfin + nb)), dim3(kFinishThreads), 0, st, Spart, nsplit,
+                       r0, nr, nq, rsum, shift, self, low, nlow, rstep, q, perm, S, brsum, e, B);
 }
 
 static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base, const RowDesc *h_rows,
@@ -991,21 +1038,24 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
         DBX_REQUIRE(hrows != nullptr, DBX_ERR_BUDGET, "pass-2 row list beyond the staging memory");
         std::copy(h_rows, h_rows + n_rows, hrows);
         DBX_HIP(hipMemcpyAsync(d_rows, hrows, sizeof(RowDesc) * n_rows, hipMemcpyHostToDevice, ctx->stream));
-    } else {
-        hipLaunchKernelGGL(p2_rows_fill_kernel, dim3((n_rows + 63) / 64), dim3(64), 0, ctx->stream, d_rows, n_rows,
-                           nb_max, uni_first_n);
     }
+    // zeroing (and the rows of one model), flags and row classes (with the block counts when one
+    // chunk of rows covers every row), counts otherwise, the scan, then the scatter with the
+    // self-pair terms in its last blocks
     const int64_t zwords = (int64_t)(off - o_tot) / 4;
-    hipLaunchKernelGGL(p2_zero_kernel, dim3((unsigned)((zwords + 255) / 256)), dim3(256), 0, ctx->stream,
-                       (unsigned int *)tot, zwords);
-    hipLaunchKernelGGL(p2_flags_kernel, dim3((unsigned)nblk, (unsigned)((n_rows + kFlagRows - 1) / kFlagRows)),
-                       dim3(kScanBlock), 0, ctx->stream, h, d_rows, n_rows, nb_max, ksm, flags, rcls);
-    hipLaunchKernelGGL(p2_count_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, flags, nb_max, bc);
+    const int64_t prep_n = std::max<int64_t>(zwords, h_rows ? 0 : n_rows);
+    hipLaunchKernelGGL(p2_prep_kernel, dim3((unsigned)((prep_n + 255) / 256)), dim3(256), 0, ctx->stream,
+                       (unsigned int *)tot, zwords, h_rows ? nullptr : d_rows, n_rows, nb_max, uni_first_n);
+    const int nchunks = (n_rows + kFlagRows - 1) / kFlagRows;
+    hipLaunchKernelGGL(p2_flags_kernel, dim3((unsigned)nblk, (unsigned)nchunks), dim3(kScanBlock), 0, ctx->stream, h,
+                       d_rows, n_rows, nb_max, ksm, flags, rcls, nchunks == 1 ? bc : nullptr);
+    if (nchunks > 1)
+        hipLaunchKernelGGL(p2_count_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, flags, nb_max,
+                           bc);
     hipLaunchKernelGGL(p2_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, bc, (int)nblk, (int *)tot, rcls, n_rows);
-    hipLaunchKernelGGL(p2_scatter_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, flags, nb_max,
-                       bc, kz);
-    hipLaunchKernelGGL(p2_self_kernel, dim3((n_rows + 63) / 64), dim3(64), 0, ctx->stream, h, d_rows, n_rows,
-                       ksm, self, low);
+    hipLaunchKernelGGL(p2_scatter_kernel, dim3((unsigned)(nblk + (n_rows + kScanBlock - 1) / kScanBlock)),
+                       dim3(kScanBlock), 0, ctx->stream, flags, nb_max, bc, kz, (int)nblk, h, d_rows, n_rows, ksm,
+                       self, low);
     // the totals, and with several row blocks each row's digit class when the read-back holds
     // them and the staging memory the reordered rows
     const bool mfma = mode != DBX_PASS2_FP64;
@@ -1030,6 +1080,7 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
         const int kper = (Kc + nsplit - 1) / nsplit;
         nsplit = (Kc + kper - 1) / kper;
         const int npass = std::min(n_rows, kDsPmax);
+        unit_err = pass2_unit_error(mode, kper, nsplit, 1);
         double *A = (double *)mem.dev(kSlotDigits, sizeof(double) * (size_t)npass * Kc);
         double *Spart = (double *)mem.dev(kSlotPass2b, sizeof(double) * (size_t)nsplit * npass * nq);
         for (int r0 = 0; r0 < n_rows; r0 += kDsPmax) {
@@ -1038,11 +1089,9 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
                                r0, np, kz, Kc, A, rsum);
             hipLaunchKernelGGL(p2_debye_fp64_kernel, dim3(qblocks, nsplit), dim3(kDsThreads), 0, ctx->stream,
                                kz, Kc, kper, A, np, rstep, d_q, nq, Spart);
-            hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)np * nq + 255) / 256)),
-                               dim3(256), 0, ctx->stream, Spart, nsplit, r0, np, nq, nullptr, 0.0, self, low,
-                               nlow, rstep, d_q, nullptr, d_S);
+            launch_finish(ctx->stream, Spart, nsplit, r0, np, nq, nullptr, 0.0, self, low, nlow, rstep, d_q, nullptr,
+                          d_S, rsum, unit_err, d_B);
         }
-        unit_err = pass2_unit_error(mode, kper, nsplit, 1);
     } else if (Kc > 0) {
         DBX_REQUIRE(Imax <= kMfImax, DBX_ERR_RANGE, "bin count too large for the matrix-core Debye sum");
         const int Kc_pad = (Kc + kMfK - 1) & ~(kMfK - 1);
@@ -1139,19 +1188,14 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
             ctx->mf_bins = Kc;
             ctx->mf_timed = true;
         }
-        hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
-                           dim3(256), 0, ctx->stream, Spart, nsplit, 0, n_rows, nq, rsum, kMfShift, self, low,
-                           nlow, rstep, d_q, perm, d_S);
         unit_err = pass2_unit_error(mode, kper, nsplit, Imax + kMfJ - 1);
+        launch_finish(ctx->stream, Spart, nsplit, 0, n_rows, nq, rsum, kMfShift, self, low, nlow, rstep, d_q, perm,
+                      d_S, rsum, unit_err, d_B);
     } else {
         // no bin beyond the self-pair entries: S is their part alone
-        hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(((int64_t)n_rows * nq + 255) / 256)),
-                           dim3(256), 0, ctx->stream, nullptr, 0, 0, n_rows, nq, nullptr, 0.0, self, low, nlow,
-                           rstep, d_q, nullptr, d_S);
+        launch_finish(ctx->stream, nullptr, 0, 0, n_rows, nq, nullptr, 0.0, self, low, nlow, rstep, d_q, nullptr, d_S,
+                      nullptr, 0.0, d_B);
     }
-    if (d_B)
-        hipLaunchKernelGGL(p2_row_bounds_kernel, dim3((n_rows + 63) / 64), dim3(64), 0, ctx->stream,
-                           Kc > 0 ? rsum : nullptr, self, low, nlow, n_rows, unit_err, perm, d_B);
     DBX_HIP(hipGetLastError());
     DBX_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
     ctx->p2_timed = true;
