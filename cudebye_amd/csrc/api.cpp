@@ -376,15 +376,19 @@ void free_model(ModelDev &m)
 // The I(q) of one model (rows [row0, row0 + P) of d_S, d_B; destination dest) and its bound to
 // the host (hI, hrel: nq each).  When the bound exceeds the context's target at some q and the
 // rows were summed on the matrix cores, they are recomputed with FP64 arithmetic from the
-// model's histograms d_hist[P][n_bins] and combined again.  Synchronises the stream.
+// model's histograms d_hist[P][n_bins] and combined again.  Synchronises the stream.  dI == hI
+// (and drel == hrel): the combine writes pinned host memory directly, no copy.
 static void finish_model(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bins, double rstep, int first_n,
                          const double *dq, int nq, const double *dff, double *dS, double *dB, int row0, int dest,
                          double *dI, double *drel, double *hI, double *hrel)
 {
     const size_t nqd = (size_t)nq;
     auto fetch = [&] {
-        DBX_HIP(hipMemcpyAsync(hI, dI + dest * nqd, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
-        DBX_HIP(hipMemcpyAsync(hrel, drel + dest * nqd, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
+        if (dI != hI) {
+            DBX_HIP(hipMemcpyAsync(hI, dI + dest * nqd, sizeof(double) * nqd, hipMemcpyDeviceToHost, ctx->stream));
+            DBX_HIP(hipMemcpyAsync(hrel, drel + dest * nqd, sizeof(double) * nqd, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        }
         DBX_HIP(hipStreamSynchronize(ctx->stream));
     };
     fetch();
@@ -656,11 +660,11 @@ int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int
         DBX_HIP(hipMemcpyAsync(dff, ffoobb, sizeof(double) * P * nqd, hipMemcpyHostToDevice,
                                ctx->stream));
         ctx->last_p2_refined = 0;
-        debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS, dB);
-        combine_intensity(ctx, dff, dS, dB, P, nq, dI, drel);
         double *hI = (double *)pinned(ctx, kPinI, sizeof(double) * nqd);
         double *hrel = (double *)pinned(ctx, kPinRel, sizeof(double) * nqd);
-        finish_model(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dff, dS, dB, 0, 0, dI, drel, hI, hrel);
+        debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS, dB);
+        combine_intensity(ctx, dff, dS, dB, P, nq, hI, hrel);  // straight into pinned host memory
+        finish_model(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dff, dS, dB, 0, 0, hI, hrel, hI, hrel);
         std::memcpy(I_out, hI, sizeof(double) * nqd);
         if (rel_err_out) std::memcpy(rel_err_out, hrel, sizeof(double) * nqd);
         if (S_out) {
@@ -806,13 +810,14 @@ int dbx_intensity(dbx_ctx *ctx, dbx_model *m, const dbx_group_params *params, do
         for (int a = 0; a < S; a++)
             for (int b = a; b < S; b++) pairs.push_back(make_group_pair(params[a], params[b]));
         ctx->last_p2_refined = 0;
-        prefactor_rows(ctx, pairs, dq, nq, dff);
-        debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS, dB);
-        combine_intensity(ctx, dff, dS, dB, P, nq, dI, drel);
-        if (hist_out) DBX_HIP(hipMemcpyAsync(hist_out, dh, hb, hipMemcpyDeviceToHost, ctx->stream));
+        // I and its bound land in pinned host memory straight from the combine kernel
         double *hI = (double *)pinned(ctx, kPinI, sizeof(double) * nqd);
         double *hrel = (double *)pinned(ctx, kPinRel, sizeof(double) * nqd);
-        finish_model(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dff, dS, dB, 0, 0, dI, drel, hI, hrel);
+        prefactor_rows(ctx, pairs, dq, nq, dff);
+        debye_structure(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dS, dB);
+        combine_intensity(ctx, dff, dS, dB, P, nq, hI, hrel);
+        if (hist_out) DBX_HIP(hipMemcpyAsync(hist_out, dh, hb, hipMemcpyDeviceToHost, ctx->stream));
+        finish_model(ctx, dh, P, n_bins, rstep, first_n, dq, nq, dff, dS, dB, 0, 0, hI, hrel, hI, hrel);
         std::memcpy(I_out, hI, sizeof(double) * nqd);
         if (rel_err_out) std::memcpy(rel_err_out, hrel, sizeof(double) * nqd);
         pass1_read_stats(ctx, st);

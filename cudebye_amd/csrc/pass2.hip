@@ -64,10 +64,30 @@ GroupPair make_group_pair(const dbx_group_params &ga, const dbx_group_params &gb
     return g;
 }
 
+// up to kPairArgs pairs travel as a kernel argument (no upload)
+constexpr int kPairArgs = 10;
+struct PairArgs {
+    GroupPair g[kPairArgs];
+};
+__global__ void p2_prefactor_args_kernel(PairArgs pa, const double *__restrict__ q, int nq, double *__restrict__ ff)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nq) return;
+    ff[(int64_t)blockIdx.y * nq + i] = prefactor_one(pa.g[blockIdx.y], q[i]);
+}
+
 void prefactor_rows(dbx_ctx *ctx, const std::vector<GroupPair> &rows, const double *d_q, int nq, double *d_ff)
 {
     const int n = (int)rows.size();
     if (n == 0) return;
+    if (n <= kPairArgs) {
+        PairArgs pa;
+        std::copy(rows.begin(), rows.end(), pa.g);
+        hipLaunchKernelGGL(p2_prefactor_args_kernel, dim3((nq + 255) / 256, n), dim3(256), 0, ctx->stream, pa, d_q,
+                           nq, d_ff);
+        DBX_HIP(hipGetLastError());
+        return;
+    }
     GroupPair *hg = (GroupPair *)pinned(ctx, kPinPairs, sizeof(GroupPair) * n);
     std::copy(rows.begin(), rows.end(), hg);
     GroupPair *dg = (GroupPair *)scratch(ctx, kSlotPairs, sizeof(GroupPair) * n);
