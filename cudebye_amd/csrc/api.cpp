@@ -74,9 +74,9 @@ const double *device_q(dbx_ctx *ctx, const double *q, int nq)
 // ---------------------------------------------------------------------------------
 namespace {
 
-struct Pt {
+struct Pt {  // 16 bytes: the median splits move these records
     float x, y, z;
-    int64_t id;
+    int32_t id;  // atom index (models hold fewer than 2^31 atoms)
 };
 
 // One median split of pts[lo, hi) (more than `unit` atoms) along its longest bounding-box
@@ -222,13 +222,20 @@ void tile_model_host(const float *x, const float *y, const float *z, const int64
     m.tiles_h.clear();
     pos_h.clear();
     pos_h.reserve(N);
-    int64_t start = 0;
-    for (int g = 0; g < S; g++) {
-        const int64_t n = group_n[g];
-        std::vector<Pt> pts(n);
+    // the subgroups' bisections are independent: those of 4096 atoms or more run on threads of
+    // their own (up to eight at once, each subgroup's top levels on further threads when few
+    // run); the leaves are the same whatever runs where
+    std::vector<std::vector<Pt>> gpts(S);
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> gblocks(S);
+    std::vector<int64_t> gstart(S);
+    for (int g = 0, st = 0; g < S; st += (int)group_n[g], g++) gstart[g] = st;
+    auto bisect = [&](int g, int depth) {
+        const int64_t n = group_n[g], start = gstart[g];
+        std::vector<Pt> &pts = gpts[g];
+        pts.resize(n);
         float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
         for (int64_t i = 0; i < n; i++) {
-            pts[i] = {x[start + i], y[start + i], z[start + i], start + i};
+            pts[i] = {x[start + i], y[start + i], z[start + i], (int32_t)(start + i)};
             const float v[3] = {pts[i].x, pts[i].y, pts[i].z};
             for (int a = 0; a < 3; a++) {
                 mn[a] = std::min(mn[a], v[a]);
@@ -239,8 +246,36 @@ void tile_model_host(const float *x, const float *y, const float *z, const int64
             m.bbox[6 * g + a] = n ? mn[a] : 0.f;
             m.bbox[6 * g + 3 + a] = n ? mx[a] : 0.f;
         }
-        std::vector<std::pair<int64_t, int64_t>> blocks;
-        if (n) kd_split_parallel(pts, 0, n, kTileAtoms, 3, blocks);
+        if (n) kd_split_parallel(pts, 0, n, kTileAtoms, depth, gblocks[g]);
+    };
+    {
+        std::vector<int> big;
+        for (int g = 0; g < S; g++)
+            if (group_n[g] >= 4096) big.push_back(g);
+        const int depth = big.size() <= 1 ? 3 : big.size() <= 2 ? 2 : big.size() <= 4 ? 1 : 0;
+        std::vector<std::exception_ptr> errs(S);
+        for (size_t b0 = 0; b0 < big.size(); b0 += 8) {
+            std::vector<std::thread> th;
+            for (size_t b = b0; b < std::min(big.size(), b0 + 8); b++) {
+                const int g = big[b];
+                th.emplace_back([&, g] {
+                    try {
+                        bisect(g, depth);
+                    } catch (...) {
+                        errs[g] = std::current_exception();
+                    }
+                });
+            }
+            for (auto &t : th) t.join();
+        }
+        for (int g = 0; g < S; g++)
+            if (group_n[g] < 4096) bisect(g, 0);
+        for (auto &e : errs)
+            if (e) std::rethrow_exception(e);
+    }
+    for (int g = 0; g < S; g++) {
+        const std::vector<Pt> &pts = gpts[g];
+        const std::vector<std::pair<int64_t, int64_t>> &blocks = gblocks[g];
         // blocks in ascending bounding-box diagonal
         std::vector<std::pair<double, size_t>> key(blocks.size());
         for (size_t b = 0; b < blocks.size(); b++) {
@@ -256,7 +291,7 @@ void tile_model_host(const float *x, const float *y, const float *z, const int64
             for (int64_t i = lo; i < hi; i++) pos_h.push_back(make_float4(pts[i].x, pts[i].y, pts[i].z, 0.f));
         }
         m.group_tile1[g] = (int32_t)m.tiles_h.size();
-        start += n;
+        std::vector<Pt>().swap(gpts[g]);
     }
 }
 
