@@ -69,7 +69,7 @@ struct Request {
     // device buffers (arena)
     std::vector<void *> dev;
     uint64_t *d_hist = nullptr;
-    uint64_t *d_oor = nullptr;
+    // DBX_REQ_KEEP requests only (reweighting): sums, prefactors, I, bounds, pair parameters
     double *d_S = nullptr;  // [P][nq]
     double *d_q = nullptr;
     double *d_ff = nullptr;  // [P][nq]: the step's prefactors, then the last reweight's
@@ -148,9 +148,11 @@ static void build_image(Request &r)
 static size_t request_bytes(const Request &r)
 {
     const size_t nq = r.q.size(), P = (size_t)r.P;
-    return al256(r.image.size()) + al256(sizeof(uint64_t) * P * (size_t)r.n_bins) + al256(8) +
-           2 * al256(sizeof(double) * P * nq) + 2 * al256(sizeof(double) * nq) + al256(sizeof(double) * P) +
-           al256(sizeof(GroupPair) * P) + al256(sizeof(int4));
+    const size_t kept = (r.flags & DBX_REQ_KEEP) ? 2 * al256(sizeof(double) * P * nq) + 2 * al256(sizeof(double) * nq) +
+                                                       al256(sizeof(double) * P) + al256(sizeof(GroupPair) * P) +
+                                                       al256(sizeof(int4))
+                                                 : 0;
+    return al256(r.image.size()) + al256(sizeof(uint64_t) * P * (size_t)r.n_bins) + kept;
 }
 
 // every device buffer of the request from the arena; false (nothing kept) when one does not fit
@@ -175,14 +177,15 @@ static bool alloc_request(DevArena &A, Request &r)
         r.d_q = (double *)(img + r.off[6]);
     }
     r.d_hist = (uint64_t *)need(take(sizeof(uint64_t) * P * (size_t)r.n_bins));
-    r.d_oor = (uint64_t *)need(take(8));
-    r.d_S = (double *)need(take(sizeof(double) * P * nq));
-    r.d_ff = (double *)need(take(sizeof(double) * P * nq));
-    r.d_I = (double *)need(take(sizeof(double) * nq));
-    r.d_rel = (double *)need(take(sizeof(double) * nq));
-    r.d_B = (double *)need(take(sizeof(double) * P));
-    r.d_pairs = (GroupPair *)need(take(sizeof(GroupPair) * P));
-    r.d_mrows = (int4 *)need(take(sizeof(int4)));
+    if (r.flags & DBX_REQ_KEEP) {
+        r.d_S = (double *)need(take(sizeof(double) * P * nq));
+        r.d_ff = (double *)need(take(sizeof(double) * P * nq));
+        r.d_I = (double *)need(take(sizeof(double) * nq));
+        r.d_rel = (double *)need(take(sizeof(double) * nq));
+        r.d_B = (double *)need(take(sizeof(double) * P));
+        r.d_pairs = (GroupPair *)need(take(sizeof(GroupPair) * P));
+        r.d_mrows = (int4 *)need(take(sizeof(int4)));
+    }
     if (!ok) {
         for (void *p : r.dev) A.free(p);
         r.dev.clear();
@@ -344,10 +347,13 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
         m.plan_cu = ph.cu;
         m.plan_rstep = ph.rstep;
     }
+    // the requests' out-of-range counts: one array for the step, zeroed by pass 1's first launch
+    uint64_t *d_oor = (uint64_t *)scratch(ctx, kSlotOor, sizeof(uint64_t) * batch.size());
     std::vector<P1Item> items;
-    for (Request *r : batch) {
+    for (size_t i = 0; i < batch.size(); i++) {
+        Request *r = batch[i];
         P1Item it{&r->m, r->n_bins, r->d_hist};
-        it.d_oor = r->d_oor;
+        it.d_oor = d_oor + i;
         items.push_back(it);
     }
     pass1_batch(ctx, items, rstep, false);
@@ -360,129 +366,129 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
         std::vector<Tile>().swap(r->m.tiles_h);
     }
     // pass 2: one contraction per q grid, rows longest first
-    std::vector<Request *> order(batch);
-    std::stable_sort(order.begin(), order.end(), [](Request *a, Request *b) {
-        if (a->q.size() != b->q.size()) return a->q.size() < b->q.size();
-        return a->n_bins > b->n_bins;
+    std::vector<size_t> order(batch.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+        if (batch[a]->q.size() != batch[b]->q.size()) return batch[a]->q.size() < batch[b]->q.size();
+        return batch[a]->n_bins > batch[b]->n_bins;
     });
-    std::vector<std::pair<size_t, size_t>> groups;
-    for (size_t g0 = 0; g0 < order.size();) {
-        size_t g1 = g0 + 1;
-        while (g1 < order.size() && order[g1]->q == order[g0]->q) g1++;
-        groups.push_back({g0, g1});
-        g0 = g1;
-    }
-    // every group's intensities, their bounds and the out-of-range counts land in one pinned
-    // block each, read after the step's synchronisation
-    size_t nI = 0;
-    for (Request *r : batch) nI += r->q.size();
-    double *hI = (double *)pinned(ctx, kPinStepI, sizeof(double) * nI);
-    double *hrel = (double *)pinned(ctx, kPinStepRel, sizeof(double) * nI);
-    uint64_t *hoor = (uint64_t *)pinned(ctx, kPinOor, sizeof(uint64_t) * batch.size());
-    std::vector<size_t> I_off(order.size());
-    size_t io = 0;
-    for (size_t i = 0; i < order.size(); i++) {
-        I_off[i] = io;
-        io += order[i]->q.size();
-        DBX_HIP(hipMemcpyAsync(hoor + i, order[i]->d_oor, 8, hipMemcpyDeviceToHost, ctx->stream));
-    }
-    // each group's rows, kept for the FP64 recomputation after the synchronisation
-    struct GroupRows {
-        std::vector<size_t> grp;
-        std::vector<RowDesc> rows;
-        std::vector<int4> mrows;
-        uint64_t *base;
-    };
-    std::vector<GroupRows> grows;
-    for (auto &g : groups) {
-        // rows longest first within the group
-        std::vector<size_t> grp;
-        for (size_t i = g.first; i < g.second; i++) grp.push_back(i);
-        std::stable_sort(grp.begin(), grp.end(),
-                         [&](size_t a, size_t b) { return order[a]->n_bins > order[b]->n_bins; });
-        const int nq = (int)order[grp[0]]->q.size();
-        uint64_t *base = order[grp[0]]->d_hist;
-        for (size_t i : grp) base = std::min(base, order[i]->d_hist);
+    // each group (requests of one q grid, longest first) and its rows
+    struct Group {
+        std::vector<size_t> req;  // batch indices
         std::vector<RowDesc> rows;
         std::vector<GroupPair> pairs;
-        std::vector<int4> mrows;
-        for (size_t gi = 0; gi < grp.size(); gi++) {
-            Request *r = order[grp[gi]];
+        std::vector<int4> mrows;  // (first row, rows, dest, -)
+        uint64_t *base;
+        int nq;
+        size_t buf_off, I_base;
+    };
+    std::vector<Group> groups;
+    size_t buf_total = 0, nI = 0;
+    for (size_t g0 = 0; g0 < order.size();) {
+        size_t g1 = g0 + 1;
+        while (g1 < order.size() && batch[order[g1]]->q == batch[order[g0]]->q) g1++;
+        Group G;
+        for (size_t i = g0; i < g1; i++) G.req.push_back(order[i]);
+        G.nq = (int)batch[G.req[0]]->q.size();
+        G.base = batch[G.req[0]]->d_hist;
+        for (size_t i : G.req) G.base = std::min(G.base, batch[i]->d_hist);
+        for (size_t gi = 0; gi < G.req.size(); gi++) {
+            Request *r = batch[G.req[gi]];
             const int S = (int)r->group_n.size();
-            mrows.push_back(make_int4((int)rows.size(), r->P, (int)gi, 0));
+            G.mrows.push_back(make_int4((int)G.rows.size(), r->P, (int)gi, 0));
             int p = 0;
             for (int a = 0; a < S; a++)
                 for (int b = a; b < S; b++, p++) {
-                    rows.push_back(RowDesc{(int64_t)(r->d_hist - base) + (int64_t)p * r->n_bins, r->n_bins,
-                                           r->first_n, 0});
-                    pairs.push_back(make_group_pair(r->params[a], r->params[b]));
+                    G.rows.push_back(RowDesc{(int64_t)(r->d_hist - G.base) + (int64_t)p * r->n_bins, r->n_bins,
+                                             r->first_n, 0});
+                    G.pairs.push_back(make_group_pair(r->params[a], r->params[b]));
                 }
         }
-        const size_t R = rows.size(), G = grp.size();
-        double *buf = (double *)scratch(ctx, kSlotQ, sizeof(double) * ((size_t)nq * (2 * R + 2 * G) + R));
-        double *dff = buf, *dS = buf + R * nq, *dI = dS + R * nq, *drel = dI + G * nq, *dB = drel + G * nq;
-        const double *dq = order[grp[0]]->d_q;
-        prefactor_rows(ctx, pairs, dq, nq, dff);
-        debye_structure_rows(ctx, base, rows, rstep, dq, nq, dS, dB);
-        combine_models(ctx, dff, dS, dB, mrows, nq, dI, drel);
-        for (size_t gi = 0; gi < G; gi++) {
-            Request *r = order[grp[gi]];
-            const size_t rn = (size_t)r->P * nq, r0 = (size_t)mrows[gi].x;
-            DBX_HIP(hipMemcpyAsync(hI + I_off[grp[gi]], dI + gi * (size_t)nq, sizeof(double) * (size_t)nq,
-                                   hipMemcpyDeviceToHost, ctx->stream));
-            DBX_HIP(hipMemcpyAsync(hrel + I_off[grp[gi]], drel + gi * (size_t)nq, sizeof(double) * (size_t)nq,
-                                   hipMemcpyDeviceToHost, ctx->stream));
-            // the request keeps its own S_p, prefactors and bounds (reweighting, FP64 recomputation)
+        const size_t R = G.rows.size();
+        G.buf_off = buf_total;
+        buf_total += (size_t)G.nq * 2 * R + R;  // prefactors, sums, row bounds
+        G.I_base = nI;
+        nI += G.req.size() * (size_t)G.nq;
+        groups.push_back(std::move(G));
+        g0 = g1;
+    }
+    // every group's intensities and bounds land in pinned host memory straight from its combine
+    // (row gi of a group at I_base + gi nq), the out-of-range counts in one copy; all read after
+    // the step's synchronisation
+    double *hI = (double *)pinned(ctx, kPinStepI, sizeof(double) * nI);
+    double *hrel = (double *)pinned(ctx, kPinStepRel, sizeof(double) * nI);
+    uint64_t *hoor = (uint64_t *)pinned(ctx, kPinOor, sizeof(uint64_t) * batch.size());
+    DBX_HIP(hipMemcpyAsync(hoor, d_oor, sizeof(uint64_t) * batch.size(), hipMemcpyDeviceToHost, ctx->stream));
+    double *buf = (double *)scratch(ctx, kSlotQ, sizeof(double) * buf_total);
+    for (Group &G : groups) {
+        const size_t R = G.rows.size(), nq = (size_t)G.nq;
+        double *dff = buf + G.buf_off, *dS = dff + R * nq, *dB = dS + R * nq;
+        const double *dq = batch[G.req[0]]->d_q;
+        prefactor_rows(ctx, G.pairs, dq, G.nq, dff);
+        debye_structure_rows(ctx, G.base, G.rows, rstep, dq, G.nq, dS, dB);
+        combine_models(ctx, dff, dS, dB, G.mrows, G.nq, hI + G.I_base, hrel + G.I_base);
+        // a kept request keeps its own S_p, prefactors and bounds (reweighting)
+        for (size_t gi = 0; gi < G.req.size(); gi++) {
+            Request *r = batch[G.req[gi]];
+            if (!(r->flags & DBX_REQ_KEEP)) continue;
+            const size_t rn = (size_t)r->P * nq, r0 = (size_t)G.mrows[gi].x;
             DBX_HIP(hipMemcpyAsync(r->d_S, dS + r0 * nq, sizeof(double) * rn, hipMemcpyDeviceToDevice, ctx->stream));
             DBX_HIP(hipMemcpyAsync(r->d_ff, dff + r0 * nq, sizeof(double) * rn, hipMemcpyDeviceToDevice, ctx->stream));
             DBX_HIP(hipMemcpyAsync(r->d_B, dB + r0, sizeof(double) * r->P, hipMemcpyDeviceToDevice, ctx->stream));
         }
-        grows.push_back(GroupRows{std::move(grp), std::move(rows), std::move(mrows), base});
     }
     DBX_HIP(hipStreamSynchronize(ctx->stream));
     // requests whose bound misses the target: their partials again with FP64 arithmetic, one
-    // pass 2 per q grid, into the request's own sums and bounds
+    // pass 2 per q grid, over their rows of the group's sums and bounds, then combined again
     bool refined = false;
     if (ctx->pass2_mode == DBX_PASS2_MFMA) {
-        for (GroupRows &gr : grows) {
+        for (Group &G : groups) {
+            const size_t R = G.rows.size(), nq = (size_t)G.nq;
+            double *dff = buf + G.buf_off, *dS = dff + R * nq, *dB = dS + R * nq;
             std::vector<int2> redo;
             std::vector<double *> Sd, Bd;
             std::vector<size_t> who;
-            for (size_t gi = 0; gi < gr.grp.size(); gi++) {
-                const size_t i = gr.grp[gi];
-                Request *r = order[i];
+            for (size_t gi = 0; gi < G.req.size(); gi++) {
+                const size_t i = G.req[gi];
+                Request *r = batch[i];
+                const double *rel = hrel + G.I_base + gi * nq;
                 bool bad = false;
-                for (size_t k = 0; k < r->q.size() && !bad; k++)
-                    bad = exceeds_target(hrel[I_off[i] + k], ctx->rel_err_target);
+                for (size_t k = 0; k < nq && !bad; k++) bad = exceeds_target(rel[k], ctx->rel_err_target);
                 if (!bad || hoor[i]) continue;
-                redo.push_back(make_int2(gr.mrows[gi].x, r->P));
-                Sd.push_back(r->d_S);
-                Bd.push_back(r->d_B);
-                who.push_back(i);
+                const int r0 = G.mrows[gi].x;
+                redo.push_back(make_int2(r0, r->P));
+                Sd.push_back(dS + (size_t)r0 * nq);
+                Bd.push_back(dB + r0);
+                who.push_back(gi);
             }
             if (redo.empty()) continue;
-            const int nq = (int)order[who[0]]->q.size();
-            structure_fp64_models(ctx, gr.base, gr.rows, redo, rstep, order[who[0]]->d_q, nq, Sd, Bd);
-            for (size_t i : who) {
-                Request *r = order[i];
-                launch_combine(ctx->stream, r->d_ff, r->d_S, r->d_B, nullptr, 1, nq, r->d_I, r->d_rel,
-                               make_int4(0, r->P, 0, 0));
-                DBX_HIP(hipMemcpyAsync(hI + I_off[i], r->d_I, sizeof(double) * nq, hipMemcpyDeviceToHost,
-                                       ctx->stream));
-                DBX_HIP(hipMemcpyAsync(hrel + I_off[i], r->d_rel, sizeof(double) * nq, hipMemcpyDeviceToHost,
-                                       ctx->stream));
+            structure_fp64_models(ctx, G.base, G.rows, redo, rstep, batch[G.req[0]]->d_q, G.nq, Sd, Bd);
+            for (size_t gi : who) {
+                Request *r = batch[G.req[gi]];
+                const int r0 = G.mrows[gi].x;
+                launch_combine(ctx->stream, dff, dS, dB, nullptr, 1, G.nq, hI + G.I_base + gi * nq,
+                               hrel + G.I_base + gi * nq, make_int4(r0, r->P, 0, 0));
+                if (r->flags & DBX_REQ_KEEP) {
+                    DBX_HIP(hipMemcpyAsync(r->d_S, dS + (size_t)r0 * nq, sizeof(double) * r->P * nq,
+                                           hipMemcpyDeviceToDevice, ctx->stream));
+                    DBX_HIP(hipMemcpyAsync(r->d_B, dB + r0, sizeof(double) * r->P, hipMemcpyDeviceToDevice,
+                                           ctx->stream));
+                }
             }
             refined = true;
         }
         if (refined) DBX_HIP(hipStreamSynchronize(ctx->stream));
     }
-    for (size_t i = 0; i < order.size(); i++) {
-        Request *r = order[i];
-        r->I.assign(hI + I_off[i], hI + I_off[i] + r->q.size());
-        r->rel.assign(hrel + I_off[i], hrel + I_off[i] + r->q.size());
-        r->oor = hoor[i];
-        std::vector<char>().swap(r->image);  // its upload is complete
-    }
+    for (Group &G : groups)
+        for (size_t gi = 0; gi < G.req.size(); gi++) {
+            const size_t i = G.req[gi];
+            Request *r = batch[i];
+            const double *I = hI + G.I_base + gi * (size_t)G.nq, *rel = hrel + G.I_base + gi * (size_t)G.nq;
+            r->I.assign(I, I + G.nq);
+            r->rel.assign(rel, rel + G.nq);
+            r->oor = hoor[i];
+            std::vector<char>().swap(r->image);  // its upload is complete
+        }
 }
 
 // pair distances of a request: N(N+1)/2
