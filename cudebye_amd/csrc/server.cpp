@@ -11,8 +11,9 @@
 // A request's device memory comes from one arena sized by the HBM budget; a request waits
 // in the ready queue until its buffers fit.  Prepared requests are admitted in submission
 // order, except that one may pass earlier requests still being prepared or waiting for
-// memory while the oldest waiting request is younger than kAdmitHoldMs; past that, nothing
-// passes it, so the memory that finishing steps return goes to it.  Memory is returned at
+// memory while the oldest waiting request is younger than kAdmitHoldMs and the arena keeps
+// room for those earlier requests beside it; past that, nothing passes the oldest, so the
+// memory that finishing steps return goes to it.  Memory is returned at
 // completion (or at dbx_release for DBX_REQ_KEEP requests, whose histograms and S_p(q) stay
 // resident).
 #include <hip/hip_runtime.h>
@@ -79,6 +80,7 @@ struct Request {
     GroupPair *d_pairs = nullptr;
     int4 *d_mrows = nullptr;
     size_t bytes = 0;
+    size_t est_bytes = 0;  // lower bound of bytes, known at submission (before preparation)
     // results
     std::vector<double> I, rel;
     uint64_t oor = 0;
@@ -153,6 +155,19 @@ static size_t request_bytes(const Request &r)
                                                        al256(sizeof(int4))
                                                  : 0;
     return al256(r.image.size()) + al256(sizeof(uint64_t) * P * (size_t)r.n_bins) + kept;
+}
+
+// a lower bound of request_bytes from what submission knows: positions, histograms, q grid
+// and the kept buffers (tiles and plan are sized by preparation)
+static size_t request_bytes_lower(const Request &r)
+{
+    Request t;
+    t.q.resize(r.q.size());
+    t.P = r.P;
+    t.n_bins = r.n_bins;
+    t.flags = r.flags;
+    t.image.resize(sizeof(float4) * (size_t)r.n_atoms + sizeof(double) * r.q.size());
+    return request_bytes(t);
 }
 
 // every device buffer of the request from the arena; false (nothing kept) when one does not fit
@@ -279,8 +294,12 @@ static void prep_loop(dbx_server *s)
 
 // admission (caller holds s->mu): prepared requests in submission order while their buffers
 // fit.  While the oldest waiting request (being prepared, or prepared and waiting for memory)
-// is younger than kAdmitHoldMs, later prepared requests that fit may pass it; after that,
-// none does.  Returns the number admitted.
+// is younger than kAdmitHoldMs, a later prepared request may pass the earlier waiting ones
+// if the arena keeps room for all of them beside it (their sizes, or for those still being
+// prepared a lower bound known at submission): passing never takes memory that an earlier
+// request needs, so under memory pressure (DBX_REQ_KEEP requests resident) requests are
+// admitted in submission order.  Past kAdmitHoldMs nothing passes the oldest.  Returns the
+// number admitted.
 static int admit(dbx_server *s)
 {
     const double now = now_ms();
@@ -295,12 +314,20 @@ static int admit(dbx_server *s)
             hold = it != s->reqs.end() && now - it->second->t_submit > kAdmitHoldMs;
         }
         bool held_in = false;
+        auto prep = s->preparing.begin();
+        size_t owed = 0;  // bytes of the earlier requests still waiting
         for (auto it = s->ready.begin(); it != s->ready.end();) {
             // a request held in line: nothing after it is admitted before it
             if (hold && it->first != oldest) break;
             Request *r = it->second;
-            if (!alloc_request(*s->arena, *r)) {
+            for (; prep != s->preparing.end() && *prep < it->first; ++prep) {
+                auto pr = s->reqs.find(*prep);
+                if (pr != s->reqs.end()) owed += pr->second->est_bytes;
+            }
+            const size_t left = s->arena->capacity() - s->arena->used();
+            if ((owed > 0 && (r->bytes > left || left - r->bytes < owed)) || !alloc_request(*s->arena, *r)) {
                 if (hold) break;
+                owed += r->bytes;
                 ++it;
                 continue;
             }
@@ -747,6 +774,7 @@ int dbx_submit(dbx_server *s, const dbx_request *q, uint64_t *ticket)
     r->flags = q->flags;
     r->P = q->n_groups * (q->n_groups + 1) / 2;
     r->n_atoms = n;
+    r->est_bytes = request_bytes_lower(*r);
     // histograms alone beyond the budget: refuse now
     if (sizeof(uint64_t) * (size_t)r->P * (size_t)r->n_bins + 16 * (size_t)n > s->arena->capacity()) {
         std::lock_guard<std::mutex> lk(s->mu);
