@@ -55,13 +55,38 @@ void *pinned(dbx_ctx *ctx, int slot, size_t bytes)
     return ctx->pin_ptr[slot];
 }
 
+void *stage(dbx_ctx *ctx, size_t bytes)
+{
+    const size_t a = (std::max<size_t>(bytes, 1) + 255) & ~(size_t)255;
+    if (ctx->stage_used + a > ctx->stage_cap) {
+        // every earlier region's upload has run once the stream is idle
+        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        ctx->stage_used = 0;
+        if (a > ctx->stage_cap) {
+            if (ctx->stage_buf) DBX_HIP(hipHostFree(ctx->stage_buf));
+            ctx->stage_buf = nullptr;
+            ctx->stage_cap = 0;
+            const size_t want = std::max<size_t>(a + a / 2, (size_t)1 << 20);
+            void *p = nullptr;
+            DBX_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+            ctx->stage_buf = (char *)p;
+            ctx->stage_cap = want;
+        }
+    }
+    char *p = ctx->stage_buf + ctx->stage_used;
+    ctx->stage_used += a;
+    return p;
+}
+
+void stage_reset(dbx_ctx *ctx) { ctx->stage_used = 0; }
+
 const double *device_q(dbx_ctx *ctx, const double *q, int nq)
 {
     const size_t b = sizeof(double) * (size_t)nq;
     double *d = (double *)scratch(ctx, kSlotQin, b);
     if (d == ctx->q_dev && ctx->q_host.size() == (size_t)nq && std::memcmp(ctx->q_host.data(), q, b) == 0)
         return d;
-    double *h = (double *)pinned(ctx, kPinQ, b);
+    double *h = (double *)stage(ctx, b);
     std::memcpy(h, q, b);
     DBX_HIP(hipMemcpyAsync(d, h, b, hipMemcpyHostToDevice, ctx->stream));
     ctx->q_host.assign(q, q + nq);
@@ -424,7 +449,7 @@ static void finish_model(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_
             DBX_HIP(hipMemcpyAsync(hrel, drel + dest * nqd, sizeof(double) * nqd, hipMemcpyDeviceToHost,
                                    ctx->stream));
         }
-        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        sync_pass2(ctx);
     };
     fetch();
     if (ctx->pass2_mode != DBX_PASS2_MFMA) return;
@@ -467,7 +492,9 @@ int dbx_open(int device, size_t hbm_budget, dbx_ctx **out)
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&c->counters, 64) != hipSuccess ||
         hipHostMalloc((void **)&c->counters_h, 64, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&c->p2_readback_h, kReadbackBytes, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&c->p2_readback_h, kReadbackBytes, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&c->p2_status_h, 2 * sizeof(uint64_t) * kP2StatusMax, hipHostMallocDefault) !=
+            hipSuccess) {
         dbx_close(c);
         return DBX_ERR_HIP;
     }
@@ -506,6 +533,8 @@ int dbx_close(dbx_ctx *ctx)
         if (ctx->ev_mf[i]) (void)hipEventDestroy(ctx->ev_mf[i]);
     if (ctx->counters_h) (void)hipHostFree(ctx->counters_h);
     if (ctx->p2_readback_h) (void)hipHostFree(ctx->p2_readback_h);
+    if (ctx->p2_status_h) (void)hipHostFree(ctx->p2_status_h);
+    if (ctx->stage_buf) (void)hipHostFree(ctx->stage_buf);
     for (int i = 0; i < 4; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -782,7 +811,7 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
                 }
         uint64_t *hoor = (uint64_t *)pinned(ctx, kPinOor, sizeof(uint64_t) * (size_t)n_items);
         DBX_HIP(hipMemcpyAsync(hoor, door, sizeof(uint64_t) * (size_t)n_items, hipMemcpyDeviceToHost, ctx->stream));
-        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        sync_pass2(ctx);
         // models whose bound misses the target: their partials again with FP64 arithmetic
         // (all of them in one FP64 pass 2, written over their rows of dS and dB)
         if (ctx->pass2_mode == DBX_PASS2_MFMA) {
@@ -804,7 +833,7 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
                 combine_models(ctx, dff, dS, dB, redo, nq, dI, drel);
                 ctx->last_p2_refined = (int)redo.size();
                 fetch();
-                DBX_HIP(hipStreamSynchronize(ctx->stream));
+                sync_pass2(ctx);
             }
         }
         std::memcpy(I_out, hI, sizeof(double) * nqd * n_items);

@@ -166,10 +166,13 @@ struct dbx_ctx {
     std::recursive_mutex mu;
     std::set<dbx_model *> models;  // live models, released by dbx_close if still there
     // grow-only device scratch, one buffer per slot (dbx::kSlot*)
-    void *slot_ptr[14] = {};
-    size_t slot_bytes[14] = {};
-    void *pin_ptr[12] = {};
-    size_t pin_bytes[12] = {};
+    void *slot_ptr[15] = {};
+    size_t slot_bytes[15] = {};
+    void *pin_ptr[6] = {};
+    size_t pin_bytes[6] = {};
+    // pinned staging of uploads (dbx::stage): a grow-only buffer handed out in consecutive regions
+    char *stage_buf = nullptr;
+    size_t stage_cap = 0, stage_used = 0;
     // the q grid last uploaded to slot kSlotQin (host copy and device address)
     std::vector<double> q_host;
     const void *q_dev = nullptr;
@@ -190,8 +193,13 @@ struct dbx_ctx {
     int pass2_mode = DBX_PASS2_MFMA;  // DBX_PASS2_MFMA or DBX_PASS2_FP64
     double rel_err_target = 1e-6;     // models whose I bound exceeds it are redone in FP64 (0: never)
     int last_p2_refined = 0;          // models redone in FP64 by the last call
-    int last_p2_bins = 0;             // compacted bins of the last pass 2
+    int last_p2_bins = 0;             // compacted bins of the last pass 2 (known after sync_pass2)
     uint64_t *p2_readback_h = nullptr;  // kReadbackBytes of pinned host memory for pass 2, from dbx_open
+    // the compacted-bin count and largest row digit class of each pass 2 enqueued since the last
+    // sync_pass2 (pinned, 2 words each, from dbx_open), and the mode of each
+    uint64_t *p2_status_h = nullptr;
+    int p2_pending = 0;
+    int p2_pending_mode[64] = {};
     // profiling of the matrix-core contraction (dbx_debug_set_profiling)
     bool profiling = false;
     hipEvent_t ev_mf[2] = {};
@@ -205,14 +213,19 @@ namespace dbx {
 
 enum { kSlotPass1 = 0, kSlotPass2 = 1, kSlotPass2b = 2, kSlotHist = 3, kSlotQ = 4, kSlotDigits = 5,
        kSlotRtab = 6, kSlotBatch = 7, kSlotRows = 8, kSlotPairs = 9, kSlotBatchHist = 10,
-       kSlotOor = 11, kSlotQin = 12, kSlotRefine = 13 };
+       kSlotOor = 11, kSlotQin = 12, kSlotRefine = 13, kSlotProf = 14 };
 // device buffer of at least `bytes` for `slot`; contents are not preserved when it grows
 void *scratch(dbx_ctx *ctx, int slot, size_t bytes);
-// pinned host buffer of at least `bytes` for `slot` (staging of per-call uploads; every call
-// synchronises its stream before returning, so a slot is free again at the next call)
-enum { kPinRefs = 0, kPinList = 1, kPinRows = 2, kPinPairs = 3, kPinMisc = 4, kPinModels = 5, kPinOor = 6,
-       kPinStepI = 7, kPinQ = 8, kPinI = 9, kPinRel = 10, kPinStepRel = 11 };
+// pinned host buffer of at least `bytes` for `slot`: destinations of downloads, read after the
+// caller's wait (uploads take their staging from stage())
+enum { kPinMisc = 0, kPinOor = 1, kPinStepI = 2, kPinI = 3, kPinRel = 4, kPinStepRel = 5 };
 void *pinned(dbx_ctx *ctx, int slot, size_t bytes);
+// Pinned host memory for one upload enqueued on the context's stream, a region of its own:
+// uploads enqueued one after another with no wait between them never share staging memory.
+// Regions are handed out again once the stream has been waited for (stage_reset after a
+// synchronisation; stage itself waits when the buffer is full).
+void *stage(dbx_ctx *ctx, size_t bytes);
+void stage_reset(dbx_ctx *ctx);
 // the q grid on the device (slot kSlotQin), uploaded through pinned memory unless the slot
 // already holds the same values
 const double *device_q(dbx_ctx *ctx, const double *q, int nq);
@@ -286,19 +299,24 @@ void prefactor_rows(dbx_ctx *ctx, const std::vector<GroupPair> &rows, const doub
 // Where pass 2's device buffers and its read-back words come from: the context's grow-only
 // slots (ctx_pass2_mem), or a fixed device region (the allocation-free entry point).  dev()
 // returns a buffer of at least `bytes` that no other dev() call of the same pass 2 overlaps.
-// pinned host memory that dbx_open allocates for the allocation-free pass 2's read-backs and
-// uploads: the row digit classes and the reordered row list of up to 2340 rows
+// pinned host memory that dbx_open allocates for the allocation-free pass 2's read-backs
 constexpr size_t kReadbackBytes = 64 << 10;
+// pass-2 status words a context holds (passes 2 enqueued between two sync_pass2)
+constexpr int kP2StatusMax = 64;
 struct P2Mem {
     virtual ~P2Mem() = default;
     virtual void *dev(int slot, size_t bytes) = 0;
-    // `words` words of pinned host memory for read-backs, or null when this provider has fewer
-    // (two always fit)
-    virtual unsigned long long *readback(size_t words) = 0;
-    // pinned host memory for uploads, or null when this provider has less; may be the read-back
-    // memory
+    // bytes that dev() can still hand out in this pass 2 (SIZE_MAX: as many as the device holds)
+    virtual size_t room() const { return SIZE_MAX; }
+    // pinned host memory for uploads, or null when this provider has less
     virtual void *staging(size_t bytes) = 0;
 };
+// Pass 2 leaves its compacted-bin count and largest row digit class in pinned status words
+// (pass2_status_slot, at most kP2StatusMax between waits); sync_pass2 waits for the context's
+// stream and checks them: a count too large for the matrix cores' digits throws DBX_ERR_RANGE.
+// Every caller waits through sync_pass2 before it uses pass 2's results.
+unsigned long long *pass2_status_slot(dbx_ctx *ctx, int mode);
+void sync_pass2(dbx_ctx *ctx);
 // marks an FP64 recomputation of pass 2 for its lifetime
 struct P2Refining {
     dbx_ctx *ctx;
@@ -349,6 +367,7 @@ int guarded(dbx_ctx *ctx, F &&f)
     std::lock_guard<std::recursive_mutex> lk(ctx->mu);
     try {
         DBX_HIP(hipSetDevice(ctx->device));
+        ctx->p2_pending = 0;  // status words of a call that failed before its wait are stale
         f();
         ctx->err.clear();
         return DBX_OK;

@@ -45,7 +45,8 @@ namespace {
 
 // pass 2's buffers carved out of the fixed region in order, 256-byte aligned.  A problem whose
 // buffers do not fit is refused with DBX_ERR_BUDGET at the first carve that exceeds the region
-// (the later buffers are sized by the compacted bin count, which pass 2 reads back mid-call).
+// (pass 2 sizes its largest buffers from upper bounds when they fit the room left, else from
+// the compacted-bin count it reads back first).
 struct FixedMem : P2Mem {
     char *base;
     size_t used = 0;
@@ -62,10 +63,7 @@ struct FixedMem : P2Mem {
         used += a;
         return p;
     }
-    unsigned long long *readback(size_t words) override
-    {
-        return words * sizeof(unsigned long long) <= kReadbackBytes ? rb : nullptr;
-    }
+    size_t room() const override { return kNoAllocBytes - used; }
     void *staging(size_t bytes) override { return bytes <= kReadbackBytes ? rb : nullptr; }
 };
 
@@ -103,7 +101,7 @@ int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bin
             launch_max_bits(ctx->stream, rel, nq, worst);
             DBX_HIP(hipMemcpyAsync(ctx->p2_readback_h, worst, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                                    ctx->stream));
-            DBX_HIP(hipStreamSynchronize(ctx->stream));
+            sync_pass2(ctx);
             double w;
             std::memcpy(&w, ctx->p2_readback_h, sizeof(w));
             if (exceeds_target(w, ctx->rel_err_target)) {
@@ -114,7 +112,7 @@ int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bin
                 ctx->last_p2_refined = 1;
             }
         }
-        DBX_HIP(hipStreamSynchronize(ctx->stream));
+        sync_pass2(ctx);
     });
 }
 

@@ -963,7 +963,7 @@ P1Stats pass1_batch(dbx_ctx *ctx, const std::vector<P1Item> &items, double rstep
     P1Ref *d_refs = (P1Ref *)buf;
     int4 *d_retry = (int4 *)(buf + ref_bytes + list_bytes);
     if (items.size() != 1) {
-        int4 *hl = (int4 *)pinned(ctx, kPinList, sizeof(int4) * std::max(1, st.n_jobs));
+        int4 *hl = (int4 *)stage(ctx, sizeof(int4) * std::max(1, st.n_jobs));
         order_entries(fast, checked, hl);
         d_list = (int4 *)(buf + ref_bytes);
         DBX_HIP(hipMemcpyAsync((void *)d_list, hl, sizeof(int4) * st.n_jobs, hipMemcpyHostToDevice,
@@ -974,7 +974,7 @@ P1Stats pass1_batch(dbx_ctx *ctx, const std::vector<P1Item> &items, double rstep
     if (items.size() == 1) {
         d_refs = nullptr;
     } else {
-        P1Ref *hr = (P1Ref *)pinned(ctx, kPinRefs, sizeof(P1Ref) * refs.size());
+        P1Ref *hr = (P1Ref *)stage(ctx, sizeof(P1Ref) * refs.size());
         std::memcpy(hr, refs.data(), sizeof(P1Ref) * refs.size());
         DBX_HIP(hipMemcpyAsync(d_refs, hr, sizeof(P1Ref) * refs.size(), hipMemcpyHostToDevice, ctx->stream));
     }

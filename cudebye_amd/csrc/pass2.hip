@@ -88,7 +88,7 @@ void prefactor_rows(dbx_ctx *ctx, const std::vector<GroupPair> &rows, const doub
         DBX_HIP(hipGetLastError());
         return;
     }
-    GroupPair *hg = (GroupPair *)pinned(ctx, kPinPairs, sizeof(GroupPair) * n);
+    GroupPair *hg = (GroupPair *)stage(ctx, sizeof(GroupPair) * n);
     std::copy(rows.begin(), rows.end(), hg);
     GroupPair *dg = (GroupPair *)scratch(ctx, kSlotPairs, sizeof(GroupPair) * n);
     DBX_HIP(hipMemcpyAsync(dg, hg, sizeof(GroupPair) * n, hipMemcpyHostToDevice, ctx->stream));
@@ -212,14 +212,30 @@ __global__ void p2_count_kernel(const unsigned char *__restrict__ flags, int64_t
     }
 }
 
+// a row's digit class as the flags kernel recorded it (0: never raised), in [1, 6]
+__device__ __forceinline__ int row_class(const int *__restrict__ rowcls, int r)
+{
+    return min(6, max(1, rowcls[r]));
+}
+
 // exclusive scan of the block counts, one workgroup; total[0] = the sum (compacted bins),
 // total[2] = the largest row digit class (at least 1).  Each thread sums a run of blocks; the
 // runs' totals are scanned by waves (shuffles) and across the 16 waves, with no serial loop.
+// With krows given, the rows are also ordered for the matrix-core sum: by digit class, most
+// digits first, each class in the caller's order (a stable counting sort):
+//   krows[p] = rows[perm[p]], and bcls[b] = the class of row 16 b of that order, the largest
+//   of its 16-row block (1 past the rows), for n_blocks blocks.
+// A row's sum does not depend on its place in this order: it depends on its own counts and on
+// the bin splits alone.
 __global__ __launch_bounds__(1024) void p2_scan_kernel(int *__restrict__ block_counts, int n_blocks,
                                                        int *__restrict__ total, const int *__restrict__ rowcls,
-                                                       int n_rows)
+                                                       int n_rows, const RowDesc *__restrict__ rows,
+                                                       RowDesc *__restrict__ krows, int *__restrict__ perm,
+                                                       int *__restrict__ bcls, int n_rblocks)
 {
     __shared__ int wsum[16], wcls[16];
+    __shared__ int ccount[8], cstart[8], cnext[8];
+    __shared__ int wrank[16][8];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (n_blocks + 1023) / 1024;
     const int b0 = min(n_blocks, t * per), b1 = min(n_blocks, b0 + per);
@@ -237,6 +253,7 @@ __global__ __launch_bounds__(1024) void p2_scan_kernel(int *__restrict__ block_c
     }
     if (lane == 63) wsum[w] = inc;
     if (lane == 0) wcls[w] = cls;
+    if (t < 8) ccount[t] = 0;
     __syncthreads();
     int base = 0, all = 0, cmax = 1;
     for (int i = 0; i < 16; i++) {
@@ -253,6 +270,54 @@ __global__ __launch_bounds__(1024) void p2_scan_kernel(int *__restrict__ block_c
     if (t == 0) {
         total[0] = all;
         total[2] = cmax;
+    }
+    if (!krows) return;
+    // rows per class, the first place of each class (class 6 first, then 5, ..., 1)
+    for (int r = t; r < n_rows; r += 1024) atomicAdd(&ccount[row_class(rowcls, r)], 1);
+    __syncthreads();
+    if (t == 0) {
+        int p = 0;
+        for (int c = 6; c >= 1; c--) {
+            cstart[c] = p;
+            cnext[c] = p;
+            p += ccount[c];
+        }
+    }
+    __syncthreads();
+    // chunks of 1024 rows in order: a row's place is its class's next place plus the rows of its
+    // class before it in the chunk (earlier waves, then earlier lanes of its wave)
+    for (int r0 = 0; r0 < n_rows; r0 += 1024) {
+        const int r = r0 + t;
+        const int c = r < n_rows ? row_class(rowcls, r) : 0;
+        int below = 0;
+        for (int cc = 1; cc <= 6; cc++) {
+            const uint64_t m = __ballot(c == cc);
+            if (lane == 0) wrank[w][cc] = __popcll(m);
+            if (c == cc)
+                below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0));
+        }
+        __syncthreads();
+        if (r < n_rows) {
+            int p = cnext[c] + below;
+            for (int i = 0; i < w; i++) p += wrank[i][c];
+            krows[p] = rows[r];
+            perm[p] = r;
+        }
+        __syncthreads();
+        if (t >= 1 && t <= 6) {
+            int n = 0;
+            for (int i = 0; i < 16; i++) n += wrank[i][t];
+            cnext[t] += n;
+        }
+        __syncthreads();
+    }
+    for (int b = t; b < n_rblocks; b += 1024) {
+        const int p = 16 * b;
+        int c = 1;
+        if (p < n_rows)
+            for (int cc = 6; cc >= 1; cc--)
+                if (p >= cstart[cc] && p < cstart[cc] + ccount[cc]) c = cc;
+        bcls[b] = c;
     }
 }
 
@@ -337,18 +402,20 @@ __device__ __forceinline__ void kext_entry(int b, const int *__restrict__ kz, in
     kext[b] = lo;
 }
 
-// weights of the compacted bins, FP64: A[r][i] = 2 N_r,kz[i]; rsum[r0 + t] += the row's sum of
-// them (one atomic per wave and row)
+// weights of the compacted bins, FP64: A[r][i] = 2 N_r,kz[i] at A + r ld + i; rsum[r0 + t] += the
+// row's sum of them (one atomic per wave and row).  The grid covers ld >= Kc bins; Kc = dyn[0].
 __global__ void p2_weights_kernel(const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows,
-                                  int r0, int nr, const int *__restrict__ kz, int Kc, double *__restrict__ A,
-                                  unsigned long long *__restrict__ rsum)
+                                  int r0, int nr, const int *__restrict__ kz, const int *__restrict__ dyn,
+                                  int64_t ld, double *__restrict__ A, unsigned long long *__restrict__ rsum)
 {
+    const int Kc = dyn[0];
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.x * blockDim.x >= Kc) return;  // a whole block past the compacted bins
     const int64_t k = i < Kc ? kz[i] : -1;
     for (int t = 0; t < nr; t++) {
         const RowDesc rd = rows[r0 + t];
         const unsigned long long n = (k >= 0 && k < rd.n_bins) ? 2ull * hist[rd.h_off + k] : 0ull;
-        if (i < Kc) A[(int64_t)t * Kc + i] = (double)n;
+        if (i < Kc) A[(int64_t)t * ld + i] = (double)n;
         unsigned long long w = n;
         for (int off = 32; off > 0; off >>= 1) w += __shfl_xor(w, off);
         if ((threadIdx.x & 63) == 0 && w) atomicAdd(&rsum[r0 + t], w);
@@ -366,12 +433,16 @@ constexpr int kDsThreads = 256;
 constexpr int kDsPmax = 10;  // rows per pass
 constexpr int kDsChunk = 256;
 
+// Kc = dyn[0] compacted bins in nsplit splits of ceil(Kc / nsplit) (the last ones possibly
+// empty); A's rows ld apart
 __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
-    const int *__restrict__ kz, int Kc, int k_per_split, const double *__restrict__ A, int np, double rstep,
-    const double *__restrict__ q, int nq, double *__restrict__ Spart)
+    const int *__restrict__ kz, const int *__restrict__ dyn, int nsplit, int64_t ld, const double *__restrict__ A,
+    int np, double rstep, const double *__restrict__ q, int nq, double *__restrict__ Spart)
 {
     __shared__ double sr[kDsChunk];
     __shared__ double sa[kDsPmax][kDsChunk];
+    const int Kc = dyn[0];
+    const int k_per_split = (Kc + nsplit - 1) / nsplit;
     const int iq = blockIdx.x * kDsThreads + threadIdx.x;
     const double Q = iq < nq ? q[iq] : 1.0;
     const int kb = blockIdx.y * k_per_split;
@@ -384,7 +455,7 @@ __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += kDsThreads) {
             sr[i] = (double)kz[c0 + i] * rstep;
-            for (int t = 0; t < np; t++) sa[t][i] = A[(int64_t)t * Kc + c0 + i];
+            for (int t = 0; t < np; t++) sa[t][i] = A[(int64_t)t * ld + c0 + i];
         }
         __syncthreads();
         for (int i = 0; i < n; i++) {
@@ -401,6 +472,50 @@ __global__ __launch_bounds__(kDsThreads) void p2_debye_fp64_kernel(
         for (int t = 0; t < kDsPmax; t++)
             if (t < np) dst[(int64_t)t * nq + iq] = acc[t];
     }
+}
+
+constexpr int kMfK = 64;       // bins per step of the matrix-core sum: the K of one matrix instruction
+constexpr int kMfFold = 256;   // steps between folds of its 32-bit classes
+
+// The compacted-bin count and the largest row digit class, as the scan kernel left them in
+// device memory (dyn[0], dyn[2]): kernels sized on the host from upper bounds read the actual
+// values here, so no host read-back stands between the compaction and the sums.
+__device__ __forceinline__ int dyn_bins(const int *dyn) { return dyn[0]; }
+__device__ __forceinline__ int dyn_class(const int *dyn) { return max(1, dyn[2]); }
+__host__ __device__ constexpr int pad_k(int kc) { return (kc + kMfK - 1) & ~(kMfK - 1); }
+
+// Error per unit weight of a row's sum over its compacted bins, sum_k A_k T_k with A_k = 2 N_k:
+//   matrix cores: the sinc digits round c (T - 3/8) to an integer, |dT| <= 1 / (2c); the FP64 sine
+//     and its recurrence add at most 4e-15 (sin_cw alone, below 1e-15); forming c T takes 5 u:
+//     c / Q and 1 / r one rounding each, and the scaling of a run's first sine and cosine by
+//     c / Q, which the linear recurrence carries to every bin of the run with a gain of at most
+//     |s| + |c| <= sqrt(2), below 2.9 u of T where Q r >= 0.5 (below u for a lone sine scaled
+//     once); the FP64 partial sums hold at most 0.625 sum_k A_k (|T - 3/8| <= 0.625) and take
+//     nsplit + classes x folds + 2 rounded additions each, and the 3/8 sum_k A_k term one
+//     conversion;
+//   FP64: a sinc value within 4 u (sine, quotient, product), then kper + nsplit + 2 rounded
+//     additions of partial sums below sum_k A_k (the classical gamma_n bound).
+// u = 2^-53.  Every term is an upper bound, whatever the signs of the errors.
+__host__ __device__ inline double pass2_unit_error(int mode, int kper_bins, int nsplit, int classes)
+{
+    const double u = 0x1p-53;
+    if (mode == DBX_PASS2_FP64) {
+        const double n = (double)kper_bins + nsplit + 2;
+        return 4 * u + n * u / (1.0 - n * u);
+    }
+    const int folds = (kper_bins / kMfK + kMfFold - 1) / kMfFold;
+    return 0.5 / kMfC + 4e-15 + 5 * u + 0.625 * u * (double)(nsplit + classes * folds + 2) + 0.375 * u;
+}
+
+// the unit error of the sums of one pass 2 split nsplit ways, from its compacted-bin count and
+// largest row class in dyn (the matrix-core sums' splits: ceil(Kc_pad / 64 / nsplit) steps; the
+// FP64 sums': ceil(Kc / nsplit) bins)
+__device__ inline double unit_error_dyn(int mode, const int *__restrict__ dyn, int nsplit)
+{
+    const int Kc = dyn_bins(dyn);
+    if (mode == DBX_PASS2_FP64) return pass2_unit_error(mode, (Kc + nsplit - 1) / nsplit, nsplit, 1);
+    const int kper = (pad_k(Kc) / kMfK + nsplit - 1) / nsplit * kMfK;
+    return pass2_unit_error(mode, kper, nsplit, min(kMfImax, dyn_class(dyn)) + kMfJ - 1);
 }
 
 // Absolute error bound of each row's structure sum S[r][q], the same for every q:
@@ -427,19 +542,20 @@ __device__ __forceinline__ void row_bound(const unsigned long long *__restrict__
 // second term is the reference's weight rule for the entries at r <= 0.0005 among the first
 // first_n (weight 1, no sinc) and their neighbours below first_n (weight 2 with sinc).
 // The blocks past those of the sums write the bounds B of rows r0..r0+nr (brsum: the row sums
-// of the weights, e: the unit error), one row per thread; none when B is null.
+// of the weights, with the unit error of this pass 2: unit_error_dyn of mode, dyn and nsplit),
+// one row per thread; none when B is null.
 constexpr int kFinishThreads = 256;
 __global__ __launch_bounds__(kFinishThreads) void p2_finish_rows_kernel(
     const double *__restrict__ Spart, int nsplit, int r0, int nr, int nq, const unsigned long long *__restrict__ rsum,
     double shift, const double *__restrict__ self, const double *__restrict__ low, int nlow, double rstep,
     const double *__restrict__ q, const int *__restrict__ perm, double *__restrict__ S,
-    const unsigned long long *__restrict__ brsum, double e, double *__restrict__ B)
+    const unsigned long long *__restrict__ brsum, int mode, const int *__restrict__ dyn, double *__restrict__ B)
 {
     const int64_t n = (int64_t)nr * nq;
     const int64_t nfin = (n + kFinishThreads - 1) / kFinishThreads;
     if ((int64_t)blockIdx.x >= nfin) {
         const int r = r0 + (int)((int64_t)blockIdx.x - nfin) * kFinishThreads + (int)threadIdx.x;
-        if (r < r0 + nr) row_bound(brsum, self, low, nlow, r, e, perm, B);
+        if (r < r0 + nr) row_bound(brsum, self, low, nlow, r, unit_error_dyn(mode, dyn, nsplit), perm, B);
         return;
     }
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -484,19 +600,18 @@ __global__ __launch_bounds__(kFinishThreads) void p2_finish_rows_kernel(
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef int i32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kMfK = 64;       // bins per step: the K of one matrix instruction
-constexpr int kMfFold = 256;   // steps between folds of the 32-bit classes
 
-// balanced count digits as bytes: cdig[i][row][kk] = digit i of 2 N_row,kz[kk] (0 past Kc, past
-// the row's bins or past n_rows), and rsum[row] += the row's sum of 2 N over the compacted bins.
-// Block (bx, row) of (Kc_pad / 1024 rounded up) x rows_pad, 256 threads, four consecutive bins
-// per thread (Kc_pad is a multiple of 64, so a thread's four bins are all inside or all outside
-// it); one atomic per block.
+// balanced count digits as bytes: cdig[i][row][kk] = digit i of 2 N_row,kz[kk] at
+// cdig + (i rows_pad + row) ld + kk (0 past Kc, past the row's bins or past n_rows), and
+// rsum[row] += the row's sum of 2 N over the compacted bins.  Block (bx, row) of
+// (ld / 1024 rounded up) x rows_pad, 256 threads, four consecutive bins per thread (Kc_pad and
+// ld are multiples of 64, so a thread's four bins are all inside or all outside Kc_pad); one
+// atomic per block.
 constexpr int kCdThreads = 256;
 __device__ __forceinline__ void count_digits_block(
     int bx, int row, unsigned long long *wsum, const unsigned long long *__restrict__ hist,
-    const RowDesc *__restrict__ rows, int n_rows, const int *__restrict__ kz, int Kc, int Kc_pad, int rows_pad,
-    int I, signed char *__restrict__ cdig, unsigned long long *__restrict__ rsum)
+    const RowDesc *__restrict__ rows, int n_rows, const int *__restrict__ kz, int Kc, int Kc_pad, int64_t ld,
+    int rows_pad, int I, signed char *__restrict__ cdig, unsigned long long *__restrict__ rsum)
 {
     const int kk0 = (bx * kCdThreads + threadIdx.x) * 4;
     long long v[4] = {0, 0, 0, 0};
@@ -533,7 +648,7 @@ __device__ __forceinline__ void count_digits_block(
             }
             packed |= (uint32_t)(uint8_t)(signed char)e << (8 * u);
         }
-        *reinterpret_cast<uint32_t *>(cdig + ((int64_t)i * rows_pad + row) * Kc_pad + kk0) = packed;
+        *reinterpret_cast<uint32_t *>(cdig + ((int64_t)i * rows_pad + row) * ld + kk0) = packed;
     }
 }
 
@@ -550,20 +665,24 @@ __device__ __forceinline__ void rtab_entry(int i, const int *__restrict__ kz, in
     ir[i] = 1.0 / r;
 }
 
-// The inputs of the contraction of one class of rows in one launch: blocks [0, cd_x rows_pad)
-// write the count digits (block b: bins of chunk b % cd_x of row b / cd_x), the next n_rt the r
-// table (n_rt 0: written by another class's launch), the rest the row-block bin extents.
+// The inputs of the contraction: blocks [0, cd_x rows_pad) write the count digits (block b:
+// bins of chunk b % cd_x of row b / cd_x; as many digits as the largest row class needs, at
+// most kMfImax), the next n_rt the r table, the rest the row-block bin extents.  cd_x and n_rt
+// cover ld bins, an upper bound of the compacted-bin count; blocks past the actual count write
+// nothing.
 __global__ __launch_bounds__(kCdThreads) void p2_mf_inputs_kernel(
     const unsigned long long *__restrict__ hist, const RowDesc *__restrict__ rows, int n_rows,
-    const int *__restrict__ kz, int Kc, int Kc_pad, int rows_pad, int I, int cd_x, signed char *__restrict__ cdig,
-    unsigned long long *__restrict__ rsum, double rstep, double *__restrict__ rr, double *__restrict__ ir,
-    int n_rt, int n_kblocks, int *__restrict__ kext)
+    const int *__restrict__ kz, const int *__restrict__ dyn, int64_t ld, int rows_pad, int cd_x,
+    signed char *__restrict__ cdig, unsigned long long *__restrict__ rsum, double rstep, double *__restrict__ rr,
+    double *__restrict__ ir, int n_rt, int n_kblocks, int *__restrict__ kext)
 {
     __shared__ unsigned long long wsum[kCdThreads / 64];
+    const int Kc = dyn_bins(dyn), Kc_pad = pad_k(Kc);
+    const int I = min(kMfImax, dyn_class(dyn));
     const int n_cd = cd_x * rows_pad;
     const int b = blockIdx.x;
     if (b < n_cd) {
-        count_digits_block(b % cd_x, b / cd_x, wsum, hist, rows, n_rows, kz, Kc, Kc_pad, rows_pad, I, cdig, rsum);
+        count_digits_block(b % cd_x, b / cd_x, wsum, hist, rows, n_rows, kz, Kc, Kc_pad, ld, rows_pad, I, cdig, rsum);
     } else if (b < n_cd + n_rt) {
         rtab_entry((b - n_cd) * kCdThreads + threadIdx.x, kz, Kc, Kc_pad, rstep, rr, ir);
     } else {
@@ -599,7 +718,7 @@ __device__ __forceinline__ int3 mf_tile() { return make_int3(blockIdx.x, blockId
 template <int RB, int WQ, int WR, int NFA, int I>
 __device__ __forceinline__ void mf_tile_sum(const int3 tile, int kend, i32x4 (*sa)[NFA][64], double (*srt)[2 * kMfK],
                                             i32x4 (*sd)[WR > 1 ? WQ : 1][kMfJ][64], const double *__restrict__ rr,
-                                            const double *__restrict__ ir, int Kc_pad, int k_per_split,
+                                            const double *__restrict__ ir, int Kc_pad, int64_t ld, int k_per_split,
                                             const signed char *__restrict__ cdig, int rows_pad,
                                             const double *__restrict__ q, int nq, int nrows, int srows,
                                             double rstep, double *__restrict__ Spart)
@@ -674,7 +793,7 @@ __device__ __forceinline__ void mf_tile_sum(const int3 tile, int kend, i32x4 (*s
             const int f = min(wave + m * NW, NF - 1);
             const int i = f / (WR * RB), blk = f - i * (WR * RB);
             pf[m] = *reinterpret_cast<const i32x4 *>(
-                cdig + ((int64_t)i * rows_pad + rowt + blk * 16 + col) * Kc_pad + k0 + kq * 16);
+                cdig + ((int64_t)i * rows_pad + rowt + blk * 16 + col) * ld + k0 + kq * 16);
         }
     };
     auto store_a = [&](int slot) {
@@ -843,24 +962,33 @@ __device__ __forceinline__ void mf_tile_dispatch(int Irg, A... a)
 
 // One workgroup per tile (q group, split, row group).  The row group's bins end at the largest
 // extent of its 16-row blocks; its count digits number bcls of its first block (rows are ordered
-// by digit class, most first, so the first block's is the largest), or IMAX without bcls.
+// by digit class, most first, so the first block's is the largest).  The compacted-bin count
+// and the largest class come from device memory (dyn): a launch runs only when that class lies
+// in [band_lo, band_hi] (the launches of one contraction cover disjoint bands, so exactly one
+// runs), and the nsplit splits share the ceil(Kc_pad / 64 / nsplit) steps of bins, the last
+// ones possibly empty.  Digit plane rows are ld bytes apart.
 template <int RB, int WQ, int WR, int IMAX>
 __global__ __launch_bounds__(WQ * WR * 64) void p2_debye_mfma_kernel(
-    const double *__restrict__ rr, const double *__restrict__ ir, int Kc_pad, int k_per_split,
-    const int *__restrict__ kext, const int *__restrict__ bcls, const signed char *__restrict__ cdig, int rows_pad,
-    const double *__restrict__ q, int nq, int nrows, int srows, double rstep, double *__restrict__ Spart)
+    const double *__restrict__ rr, const double *__restrict__ ir, const int *__restrict__ dyn, int64_t ld,
+    int nsplit, int band_lo, int band_hi, const int *__restrict__ kext, const int *__restrict__ bcls,
+    const signed char *__restrict__ cdig, int rows_pad, const double *__restrict__ q, int nq, int nrows, int srows,
+    double rstep, double *__restrict__ Spart)
 {
     constexpr int NFA = IMAX * WR * RB;
     __shared__ i32x4 sa[3][NFA][64];
     __shared__ double srt[3][2 * kMfK];
     __shared__ i32x4 sd[WR > 1 ? 2 : 1][WR > 1 ? WQ : 1][kMfJ][64];
+    const int imax = dyn_class(dyn);
+    if (imax < band_lo || imax > band_hi) return;
+    const int Kc_pad = pad_k(dyn_bins(dyn));
+    const int k_per_split = (Kc_pad / kMfK + nsplit - 1) / nsplit * kMfK;
     const int3 tile = mf_tile();  // (q group, split, row group)
     const int b0 = tile.z * (WR * RB);
     int kend = 0;
 #pragma unroll
     for (int b = 0; b < WR * RB; b++) kend = max(kend, kext[b0 + b]);
-    const int Irg = bcls ? bcls[b0] : IMAX;
-    mf_tile_dispatch<RB, WQ, WR, NFA, IMAX>(Irg, tile, kend, sa, srt, sd, rr, ir, Kc_pad, k_per_split, cdig,
+    const int Irg = min(IMAX, bcls[b0]);
+    mf_tile_dispatch<RB, WQ, WR, NFA, IMAX>(Irg, tile, kend, sa, srt, sd, rr, ir, Kc_pad, ld, k_per_split, cdig,
                                             rows_pad, q, nq, nrows, srows, rstep, Spart);
 }
 
@@ -884,15 +1012,36 @@ static MfShape mf_shape(int n_rows, int I)
     return {4, 4, 2};
 }
 
+// One launch of the contraction: shape, instantiated digit count I (bodies for 1..I, chosen per
+// tile at run time) and the band of largest row classes for which it runs.
+struct MfLaunch {
+    MfShape sh;
+    int I, lo, hi;
+};
+
+// The launches of a contraction of n_rows rows.  With the largest row class known (imax > 0):
+// one launch of its shape.  Unknown (imax 0, the class is read from device memory): one
+// instantiation for five digits where the shape does not depend on the class; otherwise the
+// 128-row tile for classes up to 3 and for class 4 and the 32-row tile for class 5, of which
+// exactly one runs.
+static std::vector<MfLaunch> mf_launches(int n_rows, int imax)
+{
+    if (imax > 0) return {MfLaunch{mf_shape(n_rows, imax), imax, 1, kMfImax}};
+    const MfShape s4 = mf_shape(n_rows, 4), s5 = mf_shape(n_rows, kMfImax);
+    if (s4.RB == s5.RB) return {MfLaunch{s5, kMfImax, 1, kMfImax}};
+    return {MfLaunch{s4, 3, 1, 3}, MfLaunch{s4, 4, 4, 4}, MfLaunch{s5, kMfImax, kMfImax, kMfImax}};
+}
+
 template <int RB, int WQ, int WR>
-static void mf_launch_i(int I, dim3 grid, hipStream_t st, const double *rr, const double *ir, int Kc_pad,
-                        int kper, const int *kext, const int *bcls, const signed char *cdig, int rows_pad,
-                        const double *q, int nq, int nrows, int srows, double rstep, double *Spart)
+static void mf_launch_i(int I, dim3 grid, hipStream_t st, const double *rr, const double *ir, const int *dyn,
+                        int64_t ld, int nsplit, int lo, int hi, const int *kext, const int *bcls,
+                        const signed char *cdig, int rows_pad, const double *q, int nq, int nrows, int srows,
+                        double rstep, double *Spart)
 {
     const dim3 blk(WQ * WR * 64);
-#define DBX_MF(II)                                                                                              \
-    hipLaunchKernelGGL((p2_debye_mfma_kernel<RB, WQ, WR, II>), grid, blk, 0, st, rr, ir, Kc_pad, kper, kext, \
-                       bcls, cdig, rows_pad, q, nq, nrows, srows, rstep, Spart)
+#define DBX_MF(II)                                                                                               \
+    hipLaunchKernelGGL((p2_debye_mfma_kernel<RB, WQ, WR, II>), grid, blk, 0, st, rr, ir, dyn, ld, nsplit, lo, hi, \
+                       kext, bcls, cdig, rows_pad, q, nq, nrows, srows, rstep, Spart)
     switch (I) {
     case 1: DBX_MF(1); break;
     case 2: DBX_MF(2); break;
@@ -908,19 +1057,22 @@ static void mf_launch_i(int I, dim3 grid, hipStream_t st, const double *rr, cons
 #undef DBX_MF
 }
 
-static void mf_launch(const MfShape &sh, int I, dim3 grid, hipStream_t st, const double *rr, const double *ir,
-                      int Kc_pad, int kper, const int *kext, const int *bcls, const signed char *cdig, int rows_pad,
-                      const double *q, int nq, int nrows, int srows, double rstep, double *Spart)
+// grid: (q groups, nsplit, row groups of the launch's shape)
+static void mf_launch(const MfLaunch &L, int qgroups, int nsplit, int rows_pad, hipStream_t st, const double *rr,
+                      const double *ir, const int *dyn, int64_t ld, const int *kext, const int *bcls,
+                      const signed char *cdig, const double *q, int nq, int nrows, int srows, double rstep,
+                      double *Spart)
 {
-    if (sh.RB == 1)
-        mf_launch_i<1, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, bcls, cdig, rows_pad, q, nq, nrows, srows,
-                             rstep, Spart);
-    else if (sh.RB == 2)
-        mf_launch_i<2, 4, 1>(I, grid, st, rr, ir, Kc_pad, kper, kext, bcls, cdig, rows_pad, q, nq, nrows, srows,
-                             rstep, Spart);
+    const dim3 grid((unsigned)qgroups, (unsigned)nsplit, (unsigned)(rows_pad / L.sh.rows()));
+    if (L.sh.RB == 1)
+        mf_launch_i<1, 4, 1>(L.I, grid, st, rr, ir, dyn, ld, nsplit, L.lo, L.hi, kext, bcls, cdig, rows_pad, q, nq,
+                             nrows, srows, rstep, Spart);
+    else if (L.sh.RB == 2)
+        mf_launch_i<2, 4, 1>(L.I, grid, st, rr, ir, dyn, ld, nsplit, L.lo, L.hi, kext, bcls, cdig, rows_pad, q, nq,
+                             nrows, srows, rstep, Spart);
     else
-        mf_launch_i<4, 4, 2>(I, grid, st, rr, ir, Kc_pad, kper, kext, bcls, cdig, rows_pad, q, nq, nrows, srows,
-                             rstep, Spart);
+        mf_launch_i<4, 4, 2>(L.I, grid, st, rr, ir, dyn, ld, nsplit, L.lo, L.hi, kext, bcls, cdig, rows_pad, q, nq,
+                             nrows, srows, rstep, Spart);
 }
 
 // I[dest][q] = sum over the model's rows of ff S, FP64, rows in order;
@@ -969,45 +1121,108 @@ __global__ void p2_prep_kernel(unsigned int *__restrict__ z, int64_t n_words, Ro
     if (rows && i < n_rows) rows[i] = RowDesc{i * n_bins, n_bins, first_n, 0};
 }
 
-// The pass-2 structure sums S[r][q] of n_rows rows (longest first; nb_max = the first row's
-// length), buffers from mem.  h_rows: the row descriptors on the host (uploaded through pinned
-// memory), or null for the P equal rows of one model (nb_max bins each, uni_first_n), written
-// on the device.
-// Error per unit weight of a row's sum over its compacted bins, sum_k A_k T_k with A_k = 2 N_k:
-//   matrix cores: the sinc digits round c (T - 3/8) to an integer, |dT| <= 1 / (2c); the FP64 sine
-//     and its recurrence add at most 4e-15 (sin_cw alone, below 1e-15); forming c T takes 5 u:
-//     c / Q and 1 / r one rounding each, and the scaling of a run's first sine and cosine by
-//     c / Q, which the linear recurrence carries to every bin of the run with a gain of at most
-//     |s| + |c| <= sqrt(2), below 2.9 u of T where Q r >= 0.5 (below u for a lone sine scaled
-//     once); the FP64 partial sums hold at most 0.625 sum_k A_k (|T - 3/8| <= 0.625) and take
-//     nsplit + classes x folds + 2 rounded additions each, and the 3/8 sum_k A_k term one
-//     conversion;
-//   FP64: a sinc value within 4 u (sine, quotient, product), then kper + nsplit + 2 rounded
-//     additions of partial sums below sum_k A_k (the classical gamma_n bound).
-// u = 2^-53.  Every term is an upper bound, whatever the signs of the errors.
-static double pass2_unit_error(int mode, int kper_bins, int nsplit, int classes)
-{
-    const double u = 0x1p-53;
-    if (mode == DBX_PASS2_FP64) {
-        const double n = (double)kper_bins + nsplit + 2;
-        return 4 * u + n * u / (1.0 - n * u);
-    }
-    const int folds = (kper_bins / kMfK + kMfFold - 1) / kMfFold;
-    return 0.5 / kMfC + 4e-15 + 5 * u + 0.625 * u * (double)(nsplit + classes * folds + 2) + 0.375 * u;
-}
-
 // the finish launch: sums of rows [r0, r0 + nr) (and their bounds when B is given)
 static void launch_finish(hipStream_t st, const double *Spart, int nsplit, int r0, int nr, int nq,
                           const unsigned long long *rsum, double shift, const double *self, const double *low,
                           int nlow, double rstep, const double *q, const int *perm, double *S,
-                          const unsigned long long *brsum, double e, double *B)
+                          const unsigned long long *brsum, int mode, const int *dyn, double *B)
 {
     const int64_t nfin = ((int64_t)nr * nq + kFinishThreads - 1) / kFinishThreads;
     const int64_t nb = B ? (nr + kFinishThreads - 1) / kFinishThreads : 0;
     hipLaunchKernelGGL(p2_finish_rows_kernel, dim3((unsigned)(nfin + nb)), dim3(kFinishThreads), 0, st, Spart, nsplit,
-                       r0, nr, nq, rsum, shift, self, low, nlow, rstep, q, perm, S, brsum, e, B);
+                       r0, nr, nq, rsum, shift, self, low, nlow, rstep, q, perm, S, brsum, mode, dyn, B);
 }
 
+static_assert(sizeof(((dbx_ctx *)nullptr)->p2_pending_mode) == sizeof(int) * kP2StatusMax,
+              "one mode per status slot");
+
+unsigned long long *pass2_status_slot(dbx_ctx *ctx, int mode)
+{
+    if (ctx->p2_pending == kP2StatusMax) sync_pass2(ctx);
+    const int i = ctx->p2_pending++;
+    ctx->p2_pending_mode[i] = mode;
+    return (unsigned long long *)ctx->p2_status_h + 2 * i;
+}
+
+void sync_pass2(dbx_ctx *ctx)
+{
+    DBX_HIP(hipStreamSynchronize(ctx->stream));
+    stage_reset(ctx);
+    const int n = ctx->p2_pending;
+    ctx->p2_pending = 0;
+    bool too_large = false;
+    for (int i = 0; i < n; i++) {
+        const unsigned long long *w = (const unsigned long long *)ctx->p2_status_h + 2 * i;
+        ctx->last_p2_bins = (int)(w[0] & 0xFFFFFFFFull);
+        too_large = too_large || (ctx->p2_pending_mode[i] != DBX_PASS2_FP64 && (int)(w[1] & 0xFFFFFFFFull) > kMfImax);
+    }
+    DBX_REQUIRE(!too_large, DBX_ERR_RANGE, "bin count too large for the matrix-core Debye sum");
+}
+
+// Profiling (dbx_debug_set_profiling): the matrix instructions the contraction issued, from
+// the launch that ran, its geometry and the row blocks' bin extents and digit classes (read
+// back after a wait), and the algorithmic work, 2 nq flops per nonzero bin of every row.
+static void profile_contraction(dbx_ctx *ctx, const unsigned long long *h, const RowDesc *d_rows, int n_rows,
+                                int64_t nb_max, int nq, int rows_pad, int qgroups, const std::vector<MfLaunch> &Ls,
+                                const unsigned long long *tot, const int *kext, const int *bcls)
+{
+    const int nblocks = rows_pad / 16;
+    unsigned long long *hv = (unsigned long long *)pinned(ctx, kPinMisc, sizeof(unsigned long long) * 3 +
+                                                                          2 * sizeof(int) * nblocks);
+    int *hk = (int *)(hv + 3), *hb = hk + nblocks;
+    DBX_HIP(hipMemcpyAsync(hv, tot, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    DBX_HIP(hipMemcpyAsync(hk, kext, sizeof(int) * nblocks, hipMemcpyDeviceToHost, ctx->stream));
+    DBX_HIP(hipMemcpyAsync(hb, bcls, sizeof(int) * nblocks, hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long *cnt = (unsigned long long *)scratch(ctx, kSlotProf, sizeof(unsigned long long));
+    hipLaunchKernelGGL(p2_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, (unsigned int *)cnt, (int64_t)2);
+    hipLaunchKernelGGL(p2_row_nnz_kernel,
+                       dim3((unsigned)((nb_max + kScanBlock - 1) / kScanBlock), (unsigned)((n_rows + kFlagRows - 1) / kFlagRows)),
+                       dim3(kScanBlock), 0, ctx->stream, h, d_rows, n_rows, nb_max, cnt);
+    DBX_HIP(hipMemcpyAsync(hv + 2, cnt, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream));
+    sync_pass2(ctx);
+    const int Kc = (int)(hv[0] & 0xFFFFFFFFull);
+    const int Imax = std::max(1, (int)(hv[1] & 0xFFFFFFFFull));
+    const MfLaunch *run = nullptr;
+    for (const MfLaunch &L : Ls)
+        if (Imax >= L.lo && Imax <= L.hi) run = &L;
+    if (!run) return;
+    const MfShape &sh = run->sh;
+    const int bpg = sh.rows() / 16;
+    double issued = 0, digits = 0;
+    for (int z = 0; z < rows_pad / sh.rows(); z++) {
+        int ke = 0;
+        for (int b = 0; b < bpg; b++) ke = std::max(ke, hk[z * bpg + b]);
+        const int kx = std::min(pad_k(Kc), pad_k(ke));
+        const int Iz = std::min(run->I, hb[z * bpg]);
+        issued += (double)(kx / kMfK) * qgroups * sh.WQ * sh.WR * sh.RB * Iz * kMfJ * 32768.0;
+        digits += (double)Iz * sh.rows();
+    }
+    ctx->mf_flops = issued;
+    ctx->mf_useful = (double)hv[2] * (double)nq * 2.0;
+    ctx->mf_digits = Imax;
+    ctx->mf_mean_digits = digits / rows_pad;
+    ctx->mf_bins = Kc;
+    ctx->mf_timed = true;
+}
+
+// Buffers of one pass 2 sized from upper bounds stay below this; a larger problem reads its
+// compacted-bin count and digit class back first and sizes them exactly.
+constexpr size_t kP2BoundBytes = (size_t)8 << 30;
+
+// The pass-2 structure sums S[r][q] of n_rows rows (longest first; nb_max = the first row's
+// length), buffers from mem.  h_rows: the row descriptors on the host (uploaded through pinned
+// memory), or null for the P equal rows of one model (nb_max bins each, uni_first_n), written
+// on the device.
+//
+// No host round trip: every launch and buffer is sized before the compaction has run, from
+// upper bounds (the compacted bins from the bins of the longest row past the self bins, the
+// count digits from kMfImax planes), and the kernels read the actual compacted-bin count and
+// largest digit class from device memory (dyn).  Only when buffers sized that way would exceed
+// kP2BoundBytes or the provider's room does the host read the two values back first and size
+// the buffers exactly.  The splits of the sums are chosen from the bounds either way, so both
+// give the same sums bit for bit.  The two values are also copied to the context's pinned
+// status words, which sync_pass2 checks after the caller's wait (a count beyond the matrix
+// cores' digits is reported there as DBX_ERR_RANGE).
 static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base, const RowDesc *h_rows,
                            int n_rows, int64_t nb_max, int uni_first_n, double rstep, const double *d_q, int nq,
                            double *d_S, int mode, double *d_B)
@@ -1015,10 +1230,30 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
     const int ksm = self_bin_max(rstep);
     const int nlow = ksm + 1;
     const int64_t nblk = (nb_max + kScanBlock - 1) / kScanBlock;
+    const bool mfma = mode != DBX_PASS2_FP64;
+    // upper bound of the compacted bins: every bin of the longest row past the self bins
+    const int kc_cap = (int)std::max<int64_t>(0, nb_max - nlow);
+    // the contraction's launches for an unknown digit class; rows padded for each of their shapes
+    const std::vector<MfLaunch> Lb = mf_launches(n_rows, 0);
+    int tile_rows = 16;
+    for (const MfLaunch &L : Lb) tile_rows = std::max(tile_rows, L.sh.rows());
+    const int rows_pad = (n_rows + tile_rows - 1) / tile_rows * tile_rows;
+    const int nblocks = rows_pad / 16;
+    // the splits of the sums, from the bounds
+    const int qgroups = (nq + Lb[0].sh.qs() - 1) / Lb[0].sh.qs();
+    const int qblocks = (nq + kDsThreads - 1) / kDsThreads;
+    int nsplit;
+    if (mfma) {
+        const int target = 2048 * 4 / (Lb[0].sh.WQ * Lb[0].sh.WR);  // about 8 waves per SIMD
+        const int rg = rows_pad / Lb[0].sh.rows();
+        nsplit = std::max(1, std::min((target + qgroups * rg - 1) / (qgroups * rg), pad_k(kc_cap) / kMfK / 8));
+    } else {
+        nsplit = std::max(1, std::min((2048 + qblocks - 1) / qblocks, (kc_cap + 511) / 512));
+    }
+    const int npass = std::min(n_rows, kDsPmax);
     // scratch: rows (the caller's order, then the order of the sums), the permutation between
     // them, block counts, kz, self, low, kext, block digit classes, then the parts that start at
     // zero (totals, row digit classes, row sums, flags), cleared by one launch
-    const int n_kext = (n_rows + 15) / 16 + 8 * (kMfImax + 1);
     size_t off = 0;
     auto take = [&](size_t b) { size_t o = off; off = (off + b + 255) & ~(size_t)255; return o; };
     const size_t o_rows = take(sizeof(RowDesc) * n_rows);
@@ -1028,8 +1263,8 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
     const size_t o_kz = take(sizeof(int) * nb_max);
     const size_t o_self = take(sizeof(double) * n_rows);
     const size_t o_low = take(sizeof(double) * (size_t)n_rows * nlow);
-    const size_t o_kext = take(sizeof(int) * n_kext);
-    const size_t o_bcls = take(sizeof(int) * n_kext);
+    const size_t o_kext = take(sizeof(int) * nblocks);
+    const size_t o_bcls = take(sizeof(int) * nblocks);
     const size_t o_tot = take(sizeof(unsigned long long) * 2);
     const size_t o_rcls = take(sizeof(int) * n_rows);
     const size_t o_rsum = take(sizeof(unsigned long long) * n_rows);
@@ -1040,6 +1275,7 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
     int *d_perm = (int *)(base + o_perm);
     int *bc = (int *)(base + o_bc);
     unsigned long long *tot = (unsigned long long *)(base + o_tot);  // [0] Kc, [1] largest row digit class
+    const int *dyn = (const int *)tot;
     int *rcls = (int *)(base + o_rcls);
     int *kz = (int *)(base + o_kz);
     double *self = (double *)(base + o_self);
@@ -1050,18 +1286,16 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
     unsigned char *flags = (unsigned char *)(base + o_flags);
     const unsigned long long *h = (const unsigned long long *)d_hist_base;
 
-    // host staging: the rows, the permutation and the block classes
-    const size_t stage_bytes = (sizeof(RowDesc) + sizeof(int)) * n_rows + sizeof(int) * n_kext;
     if (!ctx->p2_refining) DBX_HIP(hipEventRecord(ctx->ev[2], ctx->stream));
     if (h_rows) {
-        RowDesc *hrows = (RowDesc *)mem.staging(stage_bytes);
+        RowDesc *hrows = (RowDesc *)mem.staging(sizeof(RowDesc) * n_rows);
         DBX_REQUIRE(hrows != nullptr, DBX_ERR_BUDGET, "pass-2 row list beyond the staging memory");
         std::copy(h_rows, h_rows + n_rows, hrows);
         DBX_HIP(hipMemcpyAsync(d_rows, hrows, sizeof(RowDesc) * n_rows, hipMemcpyHostToDevice, ctx->stream));
     }
     // zeroing (and the rows of one model), flags and row classes (with the block counts when one
-    // chunk of rows covers every row), counts otherwise, the scan, then the scatter with the
-    // self-pair terms in its last blocks
+    // chunk of rows covers every row), counts otherwise, the scan with the rows ordered by digit
+    // class, then the scatter with the self-pair terms in its last blocks
     const int64_t zwords = (int64_t)(off - o_tot) / 4;
     const int64_t prep_n = std::max<int64_t>(zwords, h_rows ? 0 : n_rows);
     hipLaunchKernelGGL(p2_prep_kernel, dim3((unsigned)((prep_n + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -1072,149 +1306,66 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
     if (nchunks > 1)
         hipLaunchKernelGGL(p2_count_kernel, dim3((unsigned)nblk), dim3(kScanBlock), 0, ctx->stream, flags, nb_max,
                            bc);
-    hipLaunchKernelGGL(p2_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, bc, (int)nblk, (int *)tot, rcls, n_rows);
+    hipLaunchKernelGGL(p2_scan_kernel, dim3(1), dim3(1024), 0, ctx->stream, bc, (int)nblk, (int *)tot, rcls, n_rows,
+                       d_rows, mfma ? d_krows : nullptr, d_perm, d_bcls, nblocks);
     hipLaunchKernelGGL(p2_scatter_kernel, dim3((unsigned)(nblk + (n_rows + kScanBlock - 1) / kScanBlock)),
                        dim3(kScanBlock), 0, ctx->stream, flags, nb_max, bc, kz, (int)nblk, h, d_rows, n_rows, ksm,
                        self, low);
-    // the totals, and with several row blocks each row's digit class when the read-back holds
-    // them and the staging memory the reordered rows
-    const bool mfma = mode != DBX_PASS2_FP64;
-    const size_t cls_words = (size_t)(n_rows + 1) / 2;
-    unsigned long long *hv = nullptr;
-    if (mfma && n_rows > 16 && mem.staging(stage_bytes))
-        hv = mem.readback(2 + cls_words);
-    const bool per_row = hv != nullptr;
-    if (!hv) hv = mem.readback(2);
-    DBX_HIP(hipMemcpyAsync(hv, tot, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, ctx->stream));
-    if (per_row)
-        DBX_HIP(hipMemcpyAsync(hv + 2, rcls, sizeof(int) * n_rows, hipMemcpyDeviceToHost, ctx->stream));
-    DBX_HIP(hipStreamSynchronize(ctx->stream));
-    const int Kc = (int)(hv[0] & 0xFFFFFFFFull);
-    const int Imax = std::max(1, (int)hv[1]);
-    ctx->last_p2_bins = Kc;
-    double unit_err = 0.0;  // error per unit weight of the compacted bins' sum (pass2_unit_error)
-    const int *perm = nullptr;  // order of the sums -> the caller's rows (null: the same)
-    if (Kc > 0 && !mfma) {
-        const int qblocks = (nq + kDsThreads - 1) / kDsThreads;
-        int nsplit = std::max(1, std::min((2048 + qblocks - 1) / qblocks, (Kc + 511) / 512));
-        const int kper = (Kc + nsplit - 1) / nsplit;
-        nsplit = (Kc + kper - 1) / kper;
-        const int npass = std::min(n_rows, kDsPmax);
-        unit_err = pass2_unit_error(mode, kper, nsplit, 1);
-        double *A = (double *)mem.dev(kSlotDigits, sizeof(double) * (size_t)npass * Kc);
+    // the compacted-bin count and largest class to the status words (checked after the caller's
+    // wait); sized from the bounds unless those buffers would not fit
+    unsigned long long *hs = pass2_status_slot(ctx, mode);
+    DBX_HIP(hipMemcpyAsync(hs, tot, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    int64_t ld = mfma ? pad_k(std::max(kc_cap, 1)) : std::max(kc_cap, 1);
+    int planes = kMfImax;
+    std::vector<MfLaunch> Ls = Lb;
+    // the buffers sized by the compacted bins, and the partial sums after them (with alignment)
+    const size_t spart_bytes = sizeof(double) * (size_t)nsplit * (mfma ? n_rows : npass) * nq + 1024;
+    const size_t bound_bytes = spart_bytes + (mfma ? (size_t)planes * rows_pad * ld + 2 * sizeof(double) * ld
+                                                   : sizeof(double) * (size_t)npass * ld);
+    if (bound_bytes > std::min(kP2BoundBytes, mem.room())) {
+        sync_pass2(ctx);  // throws for a count beyond the matrix cores' digits
+        const int Kc = (int)(hs[0] & 0xFFFFFFFFull);
+        const int Imax = std::max(1, (int)(hs[1] & 0xFFFFFFFFull));
+        ld = mfma ? pad_k(std::max(Kc, 1)) : std::max(Kc, 1);
+        planes = Imax;
+        if (mfma) {
+            Ls = mf_launches(n_rows, Imax);
+            DBX_REQUIRE(rows_pad % Ls[0].sh.rows() == 0, DBX_ERR_STATE, "pass-2 rows padded for another tile");
+        }
+    }
+    if (!mfma) {
+        double *A = (double *)mem.dev(kSlotDigits, sizeof(double) * (size_t)npass * ld);
         double *Spart = (double *)mem.dev(kSlotPass2b, sizeof(double) * (size_t)nsplit * npass * nq);
         for (int r0 = 0; r0 < n_rows; r0 += kDsPmax) {
             const int np = std::min(kDsPmax, n_rows - r0);
-            hipLaunchKernelGGL(p2_weights_kernel, dim3((Kc + 255) / 256), dim3(256), 0, ctx->stream, h, d_rows,
-                               r0, np, kz, Kc, A, rsum);
-            hipLaunchKernelGGL(p2_debye_fp64_kernel, dim3(qblocks, nsplit), dim3(kDsThreads), 0, ctx->stream,
-                               kz, Kc, kper, A, np, rstep, d_q, nq, Spart);
+            hipLaunchKernelGGL(p2_weights_kernel, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, ctx->stream, h,
+                               d_rows, r0, np, kz, dyn, ld, A, rsum);
+            hipLaunchKernelGGL(p2_debye_fp64_kernel, dim3(qblocks, nsplit), dim3(kDsThreads), 0, ctx->stream, kz,
+                               dyn, nsplit, ld, A, np, rstep, d_q, nq, Spart);
             launch_finish(ctx->stream, Spart, nsplit, r0, np, nq, nullptr, 0.0, self, low, nlow, rstep, d_q, nullptr,
-                          d_S, rsum, unit_err, d_B);
+                          d_S, rsum, mode, dyn, d_B);
         }
-    } else if (Kc > 0) {
-        DBX_REQUIRE(Imax <= kMfImax, DBX_ERR_RANGE, "bin count too large for the matrix-core Debye sum");
-        const int Kc_pad = (Kc + kMfK - 1) & ~(kMfK - 1);
-        const int ksteps = Kc_pad / kMfK;
-        const MfShape sh = mf_shape(n_rows, Imax);
-        const int rows_pad = (n_rows + sh.rows() - 1) / sh.rows() * sh.rows();
-        const int nblocks = rows_pad / 16;
-        DBX_REQUIRE(nblocks <= n_kext, DBX_ERR_STATE, "pass-2 row blocks beyond their table");
-        const RowDesc *krows = d_rows;
-        const int *bcls = nullptr;
-        std::vector<int> hbc;
-        if (per_row) {
-            // Rows ordered by digit class, most digits first, each class in the caller's order
-            // (longest first); a row group's matrix instructions then cover only the count digits
-            // of its own rows (bcls: the largest class of each 16-row block).  The sums are the
-            // same in any order: each row's sum depends on its own counts and the splits alone.
-            const int *hc = reinterpret_cast<const int *>(hv + 2);
-            std::vector<int> cls(n_rows), order(n_rows);
-            for (int r = 0; r < n_rows; r++) {
-                cls[r] = std::min(Imax, std::max(1, hc[r]));
-                order[r] = r;
-            }
-            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cls[a] > cls[b]; });
-            bool moved = false;
-            for (int r = 0; r < n_rows; r++) moved = moved || order[r] != r;
-            hbc.assign(nblocks, 1);
-            for (int r = 0; r < n_rows; r++) hbc[r / 16] = std::max(hbc[r / 16], cls[order[r]]);
-            // (the staging memory may be the read-back memory: the classes have been read)
-            char *st = (char *)mem.staging(stage_bytes);
-            int *sbc = (int *)(st + (sizeof(RowDesc) + sizeof(int)) * n_rows);
-            std::copy(hbc.begin(), hbc.end(), sbc);
-            if (moved) {
-                RowDesc *sr = (RowDesc *)st;
-                int *sp = (int *)(st + sizeof(RowDesc) * n_rows);
-                for (int r = 0; r < n_rows; r++) {
-                    sr[r] = h_rows ? h_rows[order[r]] : RowDesc{(int64_t)order[r] * nb_max, nb_max, uni_first_n, 0};
-                    sp[r] = order[r];
-                }
-                DBX_HIP(hipMemcpyAsync(d_krows, sr, sizeof(RowDesc) * n_rows, hipMemcpyHostToDevice, ctx->stream));
-                DBX_HIP(hipMemcpyAsync(d_perm, sp, sizeof(int) * n_rows, hipMemcpyHostToDevice, ctx->stream));
-                krows = d_krows;
-                perm = d_perm;
-            }
-            DBX_HIP(hipMemcpyAsync(d_bcls, sbc, sizeof(int) * nblocks, hipMemcpyHostToDevice, ctx->stream));
-            bcls = d_bcls;
-        }
-        double *rtab = (double *)mem.dev(kSlotRtab, sizeof(double) * 2 * (size_t)Kc_pad);
-        const int qgroups = (nq + sh.qs() - 1) / sh.qs();
-        const int rgroups = rows_pad / sh.rows();
-        const int target = 2048 * 4 / (sh.WQ * sh.WR);  // about 8 waves per SIMD
-        int nsplit = std::max(1, std::min((target + qgroups * rgroups - 1) / (qgroups * rgroups), ksteps / 8));
-        const int kper = ((ksteps + nsplit - 1) / nsplit) * kMfK;
-        nsplit = (Kc_pad + kper - 1) / kper;
-        signed char *cdig = (signed char *)mem.dev(kSlotDigits, (size_t)Imax * rows_pad * Kc_pad);
+    } else {
+        double *rtab = (double *)mem.dev(kSlotRtab, sizeof(double) * 2 * (size_t)ld);
+        signed char *cdig = (signed char *)mem.dev(kSlotDigits, (size_t)planes * rows_pad * ld);
         double *Spart = (double *)mem.dev(kSlotPass2b, sizeof(double) * (size_t)nsplit * n_rows * nq);
-        const int cd_x = (Kc_pad + 4 * kCdThreads - 1) / (4 * kCdThreads);
-        const int n_rt = (Kc_pad + kCdThreads - 1) / kCdThreads;
+        const int cd_x = (int)((ld + 4 * kCdThreads - 1) / (4 * kCdThreads));
+        const int n_rt = (int)((ld + kCdThreads - 1) / kCdThreads);
         const int64_t n_in = (int64_t)cd_x * rows_pad + n_rt + (nblocks + kCdThreads - 1) / kCdThreads;
         DBX_REQUIRE(n_in < INT32_MAX, DBX_ERR_ARG, "pass-2 problem too large");
-        hipLaunchKernelGGL(p2_mf_inputs_kernel, dim3((unsigned)n_in), dim3(kCdThreads), 0, ctx->stream, h, krows,
-                           n_rows, kz, Kc, Kc_pad, rows_pad, Imax, cd_x, cdig, rsum, rstep, rtab, rtab + Kc_pad,
-                           n_rt, nblocks, kext);
+        hipLaunchKernelGGL(p2_mf_inputs_kernel, dim3((unsigned)n_in), dim3(kCdThreads), 0, ctx->stream, h, d_krows,
+                           n_rows, kz, dyn, ld, rows_pad, cd_x, cdig, rsum, rstep, rtab, rtab + ld, n_rt, nblocks,
+                           kext);
         if (ctx->profiling) DBX_HIP(hipEventRecord(ctx->ev_mf[0], ctx->stream));
-        mf_launch(sh, Imax, dim3((unsigned)qgroups, (unsigned)nsplit, (unsigned)rgroups), ctx->stream, rtab,
-                  rtab + Kc_pad, Kc_pad, kper, kext, bcls, cdig, rows_pad, d_q, nq, n_rows, n_rows, rstep, Spart);
+        for (const MfLaunch &L : Ls)
+            mf_launch(L, qgroups, nsplit, rows_pad, ctx->stream, rtab, rtab + ld, dyn, ld, kext, d_bcls, cdig, d_q,
+                      nq, n_rows, n_rows, rstep, Spart);
         if (ctx->profiling) {
             DBX_HIP(hipEventRecord(ctx->ev_mf[1], ctx->stream));
-            // issued matrix instructions: per workgroup, its steps x waves x RB x its digits x 6
-            std::vector<int> hk(nblocks);
-            DBX_HIP(hipMemcpyAsync(hk.data(), kext, sizeof(int) * nblocks, hipMemcpyDeviceToHost, ctx->stream));
-            DBX_HIP(hipStreamSynchronize(ctx->stream));
-            const int bpg = sh.rows() / 16;
-            double issued = 0, digits = 0;
-            for (int z = 0; z < rgroups; z++) {
-                int ke = 0;
-                for (int b = 0; b < bpg; b++) ke = std::max(ke, hk[z * bpg + b]);
-                const int kx = std::min(Kc_pad, (ke + kMfK - 1) & ~(kMfK - 1));
-                const int Iz = bcls ? hbc[z * bpg] : Imax;
-                issued += (double)(kx / kMfK) * qgroups * sh.WQ * sh.WR * sh.RB * Iz * kMfJ * 32768.0;
-                digits += (double)Iz * sh.rows();
-            }
-            // algorithmic work: 2 nQ flops per nonzero bin of every row
-            hipLaunchKernelGGL(p2_zero_kernel, dim3(1), dim3(64), 0, ctx->stream, (unsigned int *)tot, (int64_t)2);
-            hipLaunchKernelGGL(p2_row_nnz_kernel,
-                               dim3((unsigned)nblk, (unsigned)((n_rows + kFlagRows - 1) / kFlagRows)),
-                               dim3(kScanBlock), 0, ctx->stream, h, d_rows, n_rows, nb_max, tot);
-            unsigned long long nnz = 0;
-            DBX_HIP(hipMemcpyAsync(&nnz, tot, sizeof(nnz), hipMemcpyDeviceToHost, ctx->stream));
-            DBX_HIP(hipStreamSynchronize(ctx->stream));
-            ctx->mf_flops = issued;
-            ctx->mf_useful = (double)nnz * (double)nq * 2.0;
-            ctx->mf_digits = Imax;
-            ctx->mf_mean_digits = digits / rows_pad;
-            ctx->mf_bins = Kc;
-            ctx->mf_timed = true;
+            profile_contraction(ctx, h, d_rows, n_rows, nb_max, nq, rows_pad, qgroups, Ls, tot, kext, d_bcls);
         }
-        unit_err = pass2_unit_error(mode, kper, nsplit, Imax + kMfJ - 1);
-        launch_finish(ctx->stream, Spart, nsplit, 0, n_rows, nq, rsum, kMfShift, self, low, nlow, rstep, d_q, perm,
-                      d_S, rsum, unit_err, d_B);
-    } else {
-        // no bin beyond the self-pair entries: S is their part alone
-        launch_finish(ctx->stream, nullptr, 0, 0, n_rows, nq, nullptr, 0.0, self, low, nlow, rstep, d_q, nullptr, d_S,
-                      nullptr, 0.0, d_B);
+        launch_finish(ctx->stream, Spart, nsplit, 0, n_rows, nq, rsum, kMfShift, self, low, nlow, rstep, d_q, d_perm,
+                      d_S, rsum, mode, dyn, d_B);
     }
     DBX_HIP(hipGetLastError());
     DBX_HIP(hipEventRecord(ctx->ev[3], ctx->stream));
@@ -1226,11 +1377,7 @@ namespace {
 struct CtxMem : P2Mem {
     dbx_ctx *ctx = nullptr;
     void *dev(int slot, size_t bytes) override { return scratch(ctx, slot, bytes); }
-    unsigned long long *readback(size_t words) override
-    {
-        return (unsigned long long *)pinned(ctx, kPinMisc, sizeof(unsigned long long) * words);
-    }
-    void *staging(size_t bytes) override { return pinned(ctx, kPinRows, bytes); }
+    void *staging(size_t bytes) override { return stage(ctx, bytes); }
 };
 }  // namespace
 
@@ -1304,7 +1451,7 @@ void combine_models(dbx_ctx *ctx, const double *d_ff, const double *d_S, const d
         DBX_HIP(hipGetLastError());
         return;
     }
-    int4 *hm = (int4 *)pinned(ctx, kPinModels, sizeof(int4) * nm);
+    int4 *hm = (int4 *)stage(ctx, sizeof(int4) * nm);
     std::copy(mrows.begin(), mrows.end(), hm);
     int4 *dm = (int4 *)scratch(ctx, kSlotRows, sizeof(int4) * nm);
     DBX_HIP(hipMemcpyAsync(dm, hm, sizeof(int4) * nm, hipMemcpyHostToDevice, ctx->stream));

@@ -355,10 +355,11 @@ static bool admission_stalled(dbx_server *s, const Request *r)
 }
 
 // one step on a lane: uploads, pass 1 over all requests, pass 2 per q grid, results.  The
-// host waits for the device once, at the end (pass 2 sizes its contraction from values it
-// reads back, see debye_structure_rows).
+// host waits for the device once, at the end: pass 2 sizes its launches from upper bounds and
+// its kernels read the compacted-bin counts from device memory (see structure_core).
 static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
 {
+    ctx->p2_pending = 0;  // status words of a step that failed before its wait are stale
     const double rstep = batch[0]->rstep;
     for (Request *r : batch) {
         // the model's device arrays, its plan and its q grid: one copy of the request's image
@@ -464,7 +465,7 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
             DBX_HIP(hipMemcpyAsync(r->d_B, dB + r0, sizeof(double) * r->P, hipMemcpyDeviceToDevice, ctx->stream));
         }
     }
-    DBX_HIP(hipStreamSynchronize(ctx->stream));
+    sync_pass2(ctx);
     // requests whose bound misses the target: their partials again with FP64 arithmetic, one
     // pass 2 per q grid, over their rows of the group's sums and bounds, then combined again
     bool refined = false;
@@ -504,7 +505,7 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
             }
             refined = true;
         }
-        if (refined) DBX_HIP(hipStreamSynchronize(ctx->stream));
+        if (refined) sync_pass2(ctx);
     }
     for (Group &G : groups)
         for (size_t gi = 0; gi < G.req.size(); gi++) {
