@@ -212,19 +212,23 @@ def stream_c4(cd, synth, np):
     passes = []
     with cd.Server(0) as srv:
         for rep in range(4):
-            steps0 = srv.stats()["steps"]
+            st0 = srv.stats()
+            steps0 = st0["steps"]
             t0 = time.perf_counter()
             ts = [srv.submit(m.x, m.y, m.z, m.group_n, pa, m.rstep, m.n_bins, m.q, first_n=m.first_n)
                   for m, pa in zip(models, params)]
             res = [srv.wait(t) for t in ts]
             t_pass = time.perf_counter() - t0
-            steps = srv.stats()["steps"] - steps0
+            st1 = srv.stats()
+            steps = st1["steps"] - steps0
+            host = ((st1["step_enqueue_ms"] - st0["step_enqueue_ms"]) / max(1, steps),
+                    (st1["step_wait_ms"] - st0["step_wait_ms"]) / max(1, steps))
             for t in ts:
                 srv.release(t)
             if rep:
-                passes.append((t_pass, res, steps))
+                passes.append((t_pass, res, steps, host))
     passes.sort(key=lambda p: p[0])
-    t_srv, res, steps = passes[len(passes) // 2]
+    t_srv, res, steps, host = passes[len(passes) // 2]
     lats = np.array([r[1] for r in res])
     rel = max(float(np.max(np.abs(r[0] - ip) / np.abs(ip))) for r, ip in zip(res, I_plain))
     n = len(models)
@@ -240,55 +244,57 @@ def stream_c4(cd, synth, np):
         "server_p50_ms": round(float(np.percentile(lats, 50)), 2),
         "server_p99_ms": round(float(np.percentile(lats, 99)), 2),
         "server_steps": int(steps),
+        "step_enqueue_ms_mean": round(host[0], 3),
+        "step_wait_ms_mean": round(host[1], 3),
         "max_rel_I_server_vs_plain": rel,
         "plain": "per model: dbx_model_create (host tiling + upload) + dbx_intensity, one at a time",
         "batch_call": batch_call,
     }
 
 
-def _server_burst(cd, models, params):
-    with cd.Server(0) as srv:
-        m0 = models[0]
-        t = srv.submit(m0.x, m0.y, m0.z, m0.group_n, params[0], m0.rstep, m0.n_bins, m0.q, first_n=m0.first_n)
+def _server_burst(srv, models, params):
+    """Models/s of the server with every request submitted at once (after one warm-up request)."""
+    m0 = models[0]
+    t = srv.submit(m0.x, m0.y, m0.z, m0.group_n, params[0], m0.rstep, m0.n_bins, m0.q, first_n=m0.first_n)
+    srv.wait(t)
+    srv.release(t)
+    t0 = time.perf_counter()
+    ts = [srv.submit(m.x, m.y, m.z, m.group_n, pa, m.rstep, m.n_bins, m.q, first_n=m.first_n)
+          for m, pa in zip(models, params)]
+    for t in ts:
         srv.wait(t)
         srv.release(t)
-        t0 = time.perf_counter()
-        ts = [srv.submit(m.x, m.y, m.z, m.group_n, pa, m.rstep, m.n_bins, m.q, first_n=m.first_n)
-              for m, pa in zip(models, params)]
-        for t in ts:
-            srv.wait(t)
-            srv.release(t)
-        return len(models) / (time.perf_counter() - t0)
+    return len(models) / (time.perf_counter() - t0)
 
 
 def stream_c5(cd, synth, np):
     models = [synth.serve_request(i) for i in range(SERVE_N)]
     params = [[_gp(cd, p) for p in m.params] for m in models]
-    capacity = _server_burst(cd, models, params)
-    rate = SERVE_LOAD * capacity
-    rng = np.random.default_rng(5150)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(models)))
-    # plain: a FIFO worker serving one model per call at the same arrival times
-    with cd.Context(0) as ctx:
-        m0 = models[0]
-        mod = ctx.model(m0.x, m0.y, m0.z, m0.group_n)
-        ctx.intensity(mod, params[0], m0.rstep, m0.n_bins, m0.q, first_n=m0.first_n)
-        mod.close()
-        t0 = time.perf_counter()
-        lat_plain, I_plain = [], []
-        for m, pa, a in zip(models, params, arrivals):
-            wait = t0 + a - time.perf_counter()
-            if wait > 0:
-                time.sleep(wait)
-            mod = ctx.model(m.x, m.y, m.z, m.group_n)
-            I_plain.append(ctx.intensity(mod, pa, m.rstep, m.n_bins, m.q, first_n=m.first_n))
-            mod.close()
-            lat_plain.append((time.perf_counter() - t0 - a) * 1e3)
-        t_plain = time.perf_counter() - t0
+    m0 = models[0]
+    # one server for the capacity burst and the timed stream: the stream meets lanes whose
+    # scratch memory the burst has grown (steady-state serving)
     with cd.Server(0) as srv:
-        t = srv.submit(m0.x, m0.y, m0.z, m0.group_n, params[0], m0.rstep, m0.n_bins, m0.q, first_n=m0.first_n)
-        srv.wait(t)
-        srv.release(t)
+        capacity = _server_burst(srv, models, params)
+        rate = SERVE_LOAD * capacity
+        rng = np.random.default_rng(5150)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate, len(models)))
+        # plain: a FIFO worker serving one model per call at the same arrival times
+        with cd.Context(0) as ctx:
+            mod = ctx.model(m0.x, m0.y, m0.z, m0.group_n)
+            ctx.intensity(mod, params[0], m0.rstep, m0.n_bins, m0.q, first_n=m0.first_n)
+            mod.close()
+            t0 = time.perf_counter()
+            lat_plain, I_plain = [], []
+            for m, pa, a in zip(models, params, arrivals):
+                wait = t0 + a - time.perf_counter()
+                if wait > 0:
+                    time.sleep(wait)
+                mod = ctx.model(m.x, m.y, m.z, m.group_n)
+                I_plain.append(ctx.intensity(mod, pa, m.rstep, m.n_bins, m.q, first_n=m.first_n))
+                mod.close()
+                lat_plain.append((time.perf_counter() - t0 - a) * 1e3)
+            t_plain = time.perf_counter() - t0
+        st0 = srv.stats()
         ts = [None] * len(models)
         t0 = time.perf_counter()
 
@@ -305,7 +311,9 @@ def stream_c5(cd, synth, np):
         res = [srv.wait(t, with_err=True) for t in ts]
         t_srv = time.perf_counter() - t0
         lat_srv = [r[1] for r in res]
-        st = srv.stats()
+        st1 = srv.stats()
+        st = {k: st1[k] - st0[k] for k in ("steps", "step_enqueue_ms", "step_wait_ms")}
+        st["max_step_qgrids"] = st1["max_step_qgrids"]
         for t in ts:
             srv.release(t)
     n = len(models)
@@ -318,7 +326,11 @@ def stream_c5(cd, synth, np):
         "server": {"models_per_s": round(n / t_srv, 1),
                    "p50_ms": round(float(np.percentile(lat_srv, 50)), 2),
                    "p99_ms": round(float(np.percentile(lat_srv, 99)), 2),
-                   "steps": int(st["steps"])},
+                   "steps": int(st["steps"]),
+                   # host time per step: issuing its work (one wait per step, nothing read back
+                   # before it) and waiting for it; the other lane's step runs meanwhile
+                   "step_enqueue_ms_mean": round(st["step_enqueue_ms"] / max(1, st["steps"]), 3),
+                   "step_wait_ms_mean": round(st["step_wait_ms"] / max(1, st["steps"]), 3)},
         "plain": {"models_per_s": round(n / t_plain, 1),
                   "p50_ms": round(float(np.percentile(lat_plain, 50)), 2),
                   "p99_ms": round(float(np.percentile(lat_plain, 99)), 2)},

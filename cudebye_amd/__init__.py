@@ -69,7 +69,8 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
                 ("steps", C.c_uint64), ("pairs", C.c_double), ("resident_bytes", C.c_uint64),
                 ("budget_bytes", C.c_uint64), ("max_step_requests", C.c_double),
-                ("max_step_qgrids", C.c_double)]
+                ("max_step_qgrids", C.c_double), ("step_enqueue_ms", C.c_double),
+                ("step_wait_ms", C.c_double)]
 
 
 DBX_REQ_KEEP = 1

@@ -21,15 +21,22 @@ void *scratch(dbx_ctx *ctx, int slot, size_t bytes)
 {
     if (bytes == 0) bytes = 256;
     if (ctx->slot_bytes[slot] < bytes) {
+        // grows at least twofold: a growth waits for the stream and reallocates, so calls of
+        // slowly rising size (a server lane's steps) pay it a few times only
+        const size_t old = ctx->slot_bytes[slot];
         if (ctx->slot_ptr[slot]) {
             DBX_HIP(hipStreamSynchronize(ctx->stream));
             DBX_HIP(hipFree(ctx->slot_ptr[slot]));
             ctx->slot_ptr[slot] = nullptr;
             ctx->slot_bytes[slot] = 0;
         }
-        size_t want = bytes + bytes / 4;
+        size_t want = std::max(bytes + bytes / 4, 2 * old);
         void *p = nullptr;
-        DBX_HIP(hipMalloc(&p, want));
+        if (hipMalloc(&p, want) != hipSuccess) {
+            (void)hipGetLastError();  // the doubled size does not fit: the size asked for
+            want = bytes + bytes / 4;
+            DBX_HIP(hipMalloc(&p, want));
+        }
         ctx->slot_ptr[slot] = p;
         ctx->slot_bytes[slot] = want;
     }
@@ -40,13 +47,14 @@ void *pinned(dbx_ctx *ctx, int slot, size_t bytes)
 {
     if (bytes == 0) bytes = 64;
     if (ctx->pin_bytes[slot] < bytes) {
+        const size_t old = ctx->pin_bytes[slot];
         if (ctx->pin_ptr[slot]) {
             DBX_HIP(hipStreamSynchronize(ctx->stream));
             DBX_HIP(hipHostFree(ctx->pin_ptr[slot]));
             ctx->pin_ptr[slot] = nullptr;
             ctx->pin_bytes[slot] = 0;
         }
-        const size_t want = bytes + bytes / 2;
+        const size_t want = std::max(bytes + bytes / 2, 2 * old);
         void *p = nullptr;
         DBX_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
         ctx->pin_ptr[slot] = p;

@@ -39,6 +39,79 @@ namespace dbx {
 
 enum { kQueued = 0, kReady = 1, kAdmitted = 2, kRunning = 3, kDone = 4, kFailed = 5 };
 
+// Pinned host memory for the requests' upload images, which the prep threads write: an upload
+// from pinned memory is one DMA that the lane's stream runs on its own, where one from pageable
+// memory is staged through the runtime's buffers on the calling thread (measured ~16 us per
+// request on the host per step, most of a step's host time).  Chunks of kPinChunk bytes (or one
+// image's size) are allocated on demand up to kPinCap in all, each handed out by an offset
+// allocator; an image that finds no room stays in pageable memory.  Thread-safe.
+class PinnedPool {
+public:
+    static constexpr size_t kPinChunk = (size_t)64 << 20;
+    static constexpr size_t kPinCap = (size_t)1 << 30;
+    ~PinnedPool()
+    {
+        for (Chunk &c : chunks_) (void)hipHostFree(c.base);
+    }
+    char *alloc(size_t bytes)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (Chunk &c : chunks_)
+            if (void *p = c.arena->alloc(bytes)) return (char *)p;
+        const size_t want = std::max(kPinChunk, (bytes + 255) & ~(size_t)255);
+        if (total_ + want > kPinCap) return nullptr;
+        void *b = nullptr;
+        if (hipHostMalloc(&b, want, hipHostMallocDefault) != hipSuccess) return nullptr;
+        chunks_.push_back(Chunk{(char *)b, want, std::unique_ptr<DevArena>(new DevArena((char *)b, want))});
+        total_ += want;
+        return (char *)chunks_.back().arena->alloc(bytes);
+    }
+    void free(char *p)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (Chunk &c : chunks_)
+            if (p >= c.base && p < c.base + c.bytes) {
+                c.arena->free(p);
+                return;
+            }
+    }
+
+private:
+    struct Chunk {
+        char *base;
+        size_t bytes;
+        std::unique_ptr<DevArena> arena;
+    };
+    std::mutex mu_;
+    std::vector<Chunk> chunks_;
+    size_t total_ = 0;
+};
+
+// a request's upload image: pinned from the pool when it has room, else pageable
+struct Image {
+    PinnedPool *pool = nullptr;
+    char *pinned = nullptr;
+    std::vector<char> pageable;
+    size_t bytes = 0;
+    char *data() { return pinned ? pinned : pageable.data(); }
+    void assign(PinnedPool *p, size_t n)
+    {
+        release();
+        bytes = n;
+        pool = p;
+        pinned = p ? p->alloc(n) : nullptr;
+        if (!pinned) pageable.assign(n, 0);
+    }
+    void release()
+    {
+        if (pinned) pool->free(pinned);
+        pinned = nullptr;
+        std::vector<char>().swap(pageable);
+        bytes = 0;
+    }
+    ~Image() { release(); }
+};
+
 // how long the oldest waiting request may be passed by later ones at admission
 constexpr double kAdmitHoldMs = 50.0;
 
@@ -65,7 +138,7 @@ struct Request {
     std::vector<float4> pos_h;
     PlanHost plan;
     // everything uploaded, in its device layout: one copy per request
-    std::vector<char> image;
+    Image image;
     size_t off[7] = {};  // pos, tiles, jobs, edge2, list, alist, q
     // device buffers (arena)
     std::vector<void *> dev;
@@ -104,6 +177,7 @@ struct dbx_server {
     std::unique_ptr<dbx::DevArena> arena;
     std::mutex mu;
     std::condition_variable cv_prep, cv_sched, cv_done;
+    dbx::PinnedPool pins;  // upload images (outlives every request below)
     std::deque<dbx::Request *> prep_q, admitted_q;
     std::map<uint64_t, dbx::Request *> ready;  // prepared, by submission order
     std::set<uint64_t> preparing;              // submitted, not yet prepared
@@ -126,7 +200,7 @@ namespace dbx {
 static size_t al256(size_t b) { return (std::max<size_t>(b, 1) + 255) & ~(size_t)255; }
 
 // the request's uploads in their device layout (positions, tiles, plan, q grid), 256-byte aligned
-static void build_image(Request &r)
+static void build_image(Request &r, PinnedPool *pool)
 {
     const void *src[7] = {r.pos_h.data(), r.m.tiles_h.data(), r.plan.jobs.data(), r.plan.edge2.data(),
                           r.plan.list.data(), r.plan.alist.data(), r.q.data()};
@@ -139,7 +213,7 @@ static void build_image(Request &r)
         r.off[i] = off;
         off += al256(len[i]);
     }
-    r.image.assign(off, 0);
+    r.image.assign(pool, off);
     for (int i = 0; i < 7; i++)
         if (len[i]) std::memcpy(r.image.data() + r.off[i], src[i], len[i]);
     // the tile-ordered positions now live in the image only
@@ -154,7 +228,7 @@ static size_t request_bytes(const Request &r)
                                                        al256(sizeof(double) * P) + al256(sizeof(GroupPair) * P) +
                                                        al256(sizeof(int4))
                                                  : 0;
-    return al256(r.image.size()) + al256(sizeof(uint64_t) * P * (size_t)r.n_bins) + kept;
+    return al256(r.image.bytes) + al256(sizeof(uint64_t) * P * (size_t)r.n_bins) + kept;
 }
 
 // a lower bound of request_bytes from what submission knows: positions, histograms, q grid
@@ -166,7 +240,7 @@ static size_t request_bytes_lower(const Request &r)
     t.P = r.P;
     t.n_bins = r.n_bins;
     t.flags = r.flags;
-    t.image.resize(sizeof(float4) * (size_t)r.n_atoms + sizeof(double) * r.q.size());
+    t.image.bytes = sizeof(float4) * (size_t)r.n_atoms + sizeof(double) * r.q.size();
     return request_bytes(t);
 }
 
@@ -181,7 +255,7 @@ static bool alloc_request(DevArena &A, Request &r)
     };
     bool ok = true;
     auto need = [&](void *p) { ok = ok && p != nullptr; return p; };
-    char *img = (char *)need(take(r.image.size()));
+    char *img = (char *)need(take(r.image.bytes));
     if (img) {
         r.m.pos = (float4 *)(img + r.off[0]);
         r.m.tiles = (Tile *)(img + r.off[1]);
@@ -262,7 +336,7 @@ static void prep_loop(dbx_server *s)
             std::vector<float>().swap(r->x);
             std::vector<float>().swap(r->y);
             std::vector<float>().swap(r->z);
-            build_image(*r);
+            build_image(*r, &s->pins);
             r->bytes = request_bytes(*r);
         } catch (const Error &e) {
             code = e.code;
@@ -359,13 +433,14 @@ static bool admission_stalled(dbx_server *s, const Request *r)
 // its kernels read the compacted-bin counts from device memory (see structure_core).
 static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
 {
+    const double t_start = now_ms();
     ctx->p2_pending = 0;  // status words of a step that failed before its wait are stale
     const double rstep = batch[0]->rstep;
     for (Request *r : batch) {
         // the model's device arrays, its plan and its q grid: one copy of the request's image
         ModelDev &m = r->m;
         PlanHost &ph = r->plan;
-        DBX_HIP(hipMemcpyAsync(m.pos, r->image.data(), r->image.size(), hipMemcpyHostToDevice, ctx->stream));
+        DBX_HIP(hipMemcpyAsync(m.pos, r->image.data(), r->image.bytes, hipMemcpyHostToDevice, ctx->stream));
         m.plan_h = std::move(ph.jobs);
         m.plan_cost = std::move(ph.cost);
         m.plan_jobs = (int)m.plan_h.size();
@@ -465,7 +540,13 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
             DBX_HIP(hipMemcpyAsync(r->d_B, dB + r0, sizeof(double) * r->P, hipMemcpyDeviceToDevice, ctx->stream));
         }
     }
+    const double t_issued = now_ms();
     sync_pass2(ctx);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->st.step_enqueue_ms += t_issued - t_start;
+        s->st.step_wait_ms += now_ms() - t_issued;
+    }
     // requests whose bound misses the target: their partials again with FP64 arithmetic, one
     // pass 2 per q grid, over their rows of the group's sums and bounds, then combined again
     bool refined = false;
@@ -515,7 +596,7 @@ static void run_step(dbx_server *s, dbx_ctx *ctx, std::vector<Request *> &batch)
             r->I.assign(I, I + G.nq);
             r->rel.assign(rel, rel + G.nq);
             r->oor = hoor[i];
-            std::vector<char>().swap(r->image);  // its upload is complete
+            r->image.release();  // its upload is complete
         }
 }
 

@@ -231,6 +231,8 @@ typedef struct dbx_server_stats {
     uint64_t resident_bytes, budget_bytes;
     double max_step_requests;     /* most requests in one step */
     double max_step_qgrids;       /* most distinct q grids in one step */
+    double step_enqueue_ms;       /* host time the lanes spent issuing their steps' work (sum) */
+    double step_wait_ms;          /* host time the lanes waited for their steps to end (sum) */
 } dbx_server_stats;
 int dbx_server_open(int device, const dbx_server_config *cfg, dbx_server **out);
 int dbx_server_close(dbx_server *s);
