@@ -127,9 +127,9 @@ def test_noalloc_refuses_problems_beyond_its_region():
 
 
 def test_noalloc_equals_debye_on_rows_of_mixed_digit_classes():
-    """40 rows whose counts need one to five digits in shuffled order: the fixed region's pinned
-    words hold the row classes and the reordered rows, so the call takes the same launches as
-    dbx_debye and equals it bit for bit."""
+    """40 rows whose counts need one to five digits in shuffled order: the rows are ordered by
+    class on the device and both calls size their launches alike, so the call equals dbx_debye
+    bit for bit."""
     from types import SimpleNamespace
     rng = np.random.default_rng(78)
     nb, nq = 12000, 200
@@ -149,5 +149,35 @@ def test_noalloc_equals_debye_on_rows_of_mixed_digit_classes():
             Iref, Sref, rref = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n, with_S=True, with_err=True)
             I, S, rel = _noalloc(ctx, dev, H, m, ff)
             assert np.array_equal(I, Iref) and np.array_equal(S, Sref) and np.array_equal(rel, rref)
+        finally:
+            dev.free()
+
+
+@pytest.mark.parametrize("mode", [cd.DBX_PASS2_MFMA, cd.DBX_PASS2_FP64], ids=["mfma", "fp64"])
+def test_noalloc_sized_from_the_compacted_bins_equals_debye(mode):
+    """Six sparse rows of 1.5M bins (20,000 occupied each): buffers sized from the bins' upper
+    bound (five digit planes or FP64 weights over 1.5M bins) exceed the fixed region, so the
+    call reads the compacted-bin count back and sizes them exactly, while dbx_debye sizes them
+    from the bound with no read-back; the splits are chosen from the bound in both, so the sums
+    are the same bit for bit."""
+    from types import SimpleNamespace
+    rng = np.random.default_rng(79)
+    P, nb, nq = 6, 1_500_000, 500
+    H = np.zeros((P, nb), np.uint64)
+    for p in range(P):
+        idx = rng.choice(np.arange(1, nb), 20000, replace=False)
+        H[p, idx] = rng.integers(1, [2 ** 6, 2 ** 14, 2 ** 22][p % 3], 20000).astype(np.uint64)
+        H[p, 0] = 1000
+    m = SimpleNamespace(q=np.linspace(0.3, 20.0, nq), rstep=1e-4, first_n=8)
+    ff = rng.uniform(0.5, 2.0, (P, nq))
+    dev = Dev()
+    with cd.Context(0) as ctx:
+        ctx.set_pass2_mode(mode)
+        ctx.set_rel_err_target(0)  # the mode's own sums, never recomputed
+        try:
+            Iref, Sref, rref = ctx.debye(H, m.rstep, m.q, ff, first_n=m.first_n, with_S=True, with_err=True)
+            I, S, rel = _noalloc(ctx, dev, H, m, ff)
+            assert np.array_equal(S, Sref), np.max(np.abs(S - Sref))
+            assert np.array_equal(I, Iref) and np.array_equal(rel, rref)
         finally:
             dev.free()
