@@ -223,7 +223,8 @@ void *pinned(dbx_ctx *ctx, int slot, size_t bytes);
 // Pinned host memory for one upload enqueued on the context's stream, a region of its own:
 // uploads enqueued one after another with no wait between them never share staging memory.
 // Regions are handed out again once the stream has been waited for (stage_reset after a
-// synchronisation; stage itself waits when the buffer is full).
+// synchronisation; stage itself waits when the buffer is full, and may then move it): fill a
+// region and enqueue its copy before the next stage call on the context.
 void *stage(dbx_ctx *ctx, size_t bytes);
 void stage_reset(dbx_ctx *ctx);
 // the q grid on the device (slot kSlotQin), uploaded through pinned memory unless the slot
