@@ -86,6 +86,14 @@ int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bin
         char *base = g_scratch[ctx->device];
         DBX_REQUIRE(base != nullptr, DBX_ERR_STATE, "fixed scratch region not set up on this device");
         FixedMem mem(base, (unsigned long long *)ctx->p2_readback_h);
+        // the contraction profile (a debugging aid) reads back through the context's growing
+        // slots: off for this call
+        struct NoProfile {
+            dbx_ctx *c;
+            bool was;
+            ~NoProfile() { c->profiling = was; }
+        } np{ctx, ctx->profiling};
+        ctx->profiling = false;
         // the structure sums S[p][q], their bounds and the bound of I: the caller's buffers, or
         // the region's
         double *S = d_S ? d_S : (double *)mem.dev(kSlotQ, sizeof(double) * (size_t)P * (size_t)nq);
