@@ -1002,13 +1002,13 @@ struct MfShape {
 
 // 32 rows per workgroup (2 row blocks per wave) or 128 (4 row blocks, two waves per q block):
 // the larger tile shares each sinc digit among more rows, at about 6.6 against 7.6 us per row
-// (measured, 4096 q), but pads the rows to a multiple of 128.  Five count digits overfill the
-// larger tile's LDS.
+// (measured, 4096 q), but pads the rows to a multiple of 128.  The larger tile takes up to
+// three count digits: five overfill its LDS, and four spill its registers.
 static MfShape mf_shape(int n_rows, int I)
 {
     if (n_rows <= 16) return {1, 4, 1};
     const int64_t pad32 = (n_rows + 31) / 32 * 32, pad128 = (n_rows + 127) / 128 * 128;
-    if (I >= 5 || pad32 * 76 <= pad128 * 66) return {2, 4, 1};
+    if (I >= 4 || pad32 * 76 <= pad128 * 66) return {2, 4, 1};
     return {4, 4, 2};
 }
 
@@ -1022,14 +1022,14 @@ struct MfLaunch {
 // The launches of a contraction of n_rows rows.  With the largest row class known (imax > 0):
 // one launch of its shape.  Unknown (imax 0, the class is read from device memory): one
 // instantiation for five digits where the shape does not depend on the class; otherwise the
-// 128-row tile for classes up to 3 and for class 4 and the 32-row tile for class 5, of which
-// exactly one runs.
+// 128-row tile for classes up to 3 and the 32-row tile for classes 4 and 5, of which exactly
+// one runs (the other exits at once: 5 us for the 32-row tile's grid on the 256-model batch).
 static std::vector<MfLaunch> mf_launches(int n_rows, int imax)
 {
     if (imax > 0) return {MfLaunch{mf_shape(n_rows, imax), imax, 1, kMfImax}};
-    const MfShape s4 = mf_shape(n_rows, 4), s5 = mf_shape(n_rows, kMfImax);
-    if (s4.RB == s5.RB) return {MfLaunch{s5, kMfImax, 1, kMfImax}};
-    return {MfLaunch{s4, 3, 1, 3}, MfLaunch{s4, 4, 4, 4}, MfLaunch{s5, kMfImax, kMfImax, kMfImax}};
+    const MfShape s3 = mf_shape(n_rows, 3), s5 = mf_shape(n_rows, kMfImax);
+    if (s3.RB == s5.RB) return {MfLaunch{s5, kMfImax, 1, kMfImax}};
+    return {MfLaunch{s3, 3, 1, 3}, MfLaunch{s5, kMfImax, 4, kMfImax}};
 }
 
 template <int RB, int WQ, int WR>
@@ -1046,12 +1046,15 @@ static void mf_launch_i(int I, dim3 grid, hipStream_t st, const double *rr, cons
     case 1: DBX_MF(1); break;
     case 2: DBX_MF(2); break;
     case 3: DBX_MF(3); break;
-    case 4: DBX_MF(4); break;
     default:
-        if constexpr (WR == 1)
-            DBX_MF(5);
-        else
-            throw Error(DBX_ERR_ARG, "five count digits need a one-row-wave shape");
+        if constexpr (WR == 1) {
+            if (I == 4)
+                DBX_MF(4);
+            else
+                DBX_MF(5);
+        } else {
+            throw Error(DBX_ERR_ARG, "four or five count digits need a one-row-wave shape");
+        }
         break;
     }
 #undef DBX_MF
@@ -1318,11 +1321,34 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
     int64_t ld = mfma ? pad_k(std::max(kc_cap, 1)) : std::max(kc_cap, 1);
     int planes = kMfImax;
     std::vector<MfLaunch> Ls = Lb;
-    // the buffers sized by the compacted bins, and the partial sums after them (with alignment)
+    // the buffers sized by the compacted bins (digit planes and r table, or FP64 weights), and
+    // the partial sums after them
+    double *A = nullptr, *rtab = nullptr, *Spart = nullptr;
+    signed char *cdig = nullptr;
+    auto take_sized = [&] {
+        if (mfma) {
+            rtab = (double *)mem.dev(kSlotRtab, sizeof(double) * 2 * (size_t)ld);
+            cdig = (signed char *)mem.dev(kSlotDigits, (size_t)planes * rows_pad * ld);
+        } else {
+            A = (double *)mem.dev(kSlotDigits, sizeof(double) * (size_t)npass * ld);
+        }
+        Spart = (double *)mem.dev(kSlotPass2b, sizeof(double) * (size_t)nsplit * (mfma ? n_rows : npass) * nq);
+    };
     const size_t spart_bytes = sizeof(double) * (size_t)nsplit * (mfma ? n_rows : npass) * nq + 1024;
     const size_t bound_bytes = spart_bytes + (mfma ? (size_t)planes * rows_pad * ld + 2 * sizeof(double) * ld
                                                    : sizeof(double) * (size_t)npass * ld);
-    if (bound_bytes > std::min(kP2BoundBytes, mem.room())) {
+    bool sized = false;
+    if (bound_bytes <= std::min(kP2BoundBytes, mem.room())) {
+        try {
+            take_sized();
+            sized = true;
+        } catch (const Error &e) {
+            // the device has no room for them now: size them exactly instead
+            if (e.code != DBX_ERR_HIP && e.code != DBX_ERR_BUDGET) throw;
+            (void)hipGetLastError();
+        }
+    }
+    if (!sized) {
         sync_pass2(ctx);  // throws for a count beyond the matrix cores' digits
         const int Kc = (int)(hs[0] & 0xFFFFFFFFull);
         const int Imax = std::max(1, (int)(hs[1] & 0xFFFFFFFFull));
@@ -1332,10 +1358,9 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
             Ls = mf_launches(n_rows, Imax);
             DBX_REQUIRE(rows_pad % Ls[0].sh.rows() == 0, DBX_ERR_STATE, "pass-2 rows padded for another tile");
         }
+        take_sized();
     }
     if (!mfma) {
-        double *A = (double *)mem.dev(kSlotDigits, sizeof(double) * (size_t)npass * ld);
-        double *Spart = (double *)mem.dev(kSlotPass2b, sizeof(double) * (size_t)nsplit * npass * nq);
         for (int r0 = 0; r0 < n_rows; r0 += kDsPmax) {
             const int np = std::min(kDsPmax, n_rows - r0);
             hipLaunchKernelGGL(p2_weights_kernel, dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, ctx->stream, h,
@@ -1346,9 +1371,6 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
                           d_S, rsum, mode, dyn, d_B);
         }
     } else {
-        double *rtab = (double *)mem.dev(kSlotRtab, sizeof(double) * 2 * (size_t)ld);
-        signed char *cdig = (signed char *)mem.dev(kSlotDigits, (size_t)planes * rows_pad * ld);
-        double *Spart = (double *)mem.dev(kSlotPass2b, sizeof(double) * (size_t)nsplit * n_rows * nq);
         const int cd_x = (int)((ld + 4 * kCdThreads - 1) / (4 * kCdThreads));
         const int n_rt = (int)((ld + kCdThreads - 1) / kCdThreads);
         const int64_t n_in = (int64_t)cd_x * rows_pad + n_rt + (nblocks + kCdThreads - 1) / kCdThreads;

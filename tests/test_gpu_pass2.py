@@ -288,10 +288,11 @@ def test_error_target_in_a_batch_recomputes_only_the_models_that_miss_it(ctx):
 def test_rows_of_mixed_digit_classes(ctx, top):
     """200 rows whose largest counts need 1 to `top` count digits, in shuffled order: the rows
     are ordered by digit class on the device and each row tile's matrix instructions cover only
-    the digits of its own rows.  200 rows take the 128-row tile for classes up to 4 and the
-    32-row tile for class 5, so each of the contraction's three launches is the one that runs
-    for one value of `top`.  Every row's sum lands in the caller's row within the quantisation
-    bound of the oracle's FP64 sum, and the bound of I covers its error."""
+    the digits of its own rows.  200 rows take the 128-row tile for classes up to 3 and the
+    32-row tile for classes 4 and 5, so each of the contraction's two launches runs for some
+    value of `top` and exits at once for the others.  Every row's sum lands in the caller's row
+    within the quantisation bound of the oracle's FP64 sum, and the bound of I covers its
+    error."""
     rng = np.random.default_rng(77 + top)
     nb, nq = 20000, 150
     caps = np.repeat([2 ** 13, 2 ** 5, 2 ** 21, 2 ** 29, 2 ** 36][:top], [100, 40, 30, 20, 10][:top])
