@@ -49,6 +49,11 @@ class BatchItem(C.Structure):
                 ("first_n", C.c_int)]
 
 
+# dbx_batch_item as a NumPy record, so that a batch's items are packed column by column
+_BATCH_ITEM = np.dtype([("model", "<u8"), ("params", "<u8"), ("n_bins", "<i8"), ("first_n", "<i4"),
+                        ("pad", "<i4")])
+
+
 class ServerConfig(C.Structure):
     """dbx_server_config."""
     _fields_ = [("hbm_budget", C.c_size_t), ("max_batch", C.c_int), ("max_step_pairs", C.c_double),
@@ -479,23 +484,24 @@ class Context:
         (+ list of histograms, + out-of-range counts per item)."""
         n = len(items)
         q = np.ascontiguousarray(q, np.float64)
-        keep = []
-        arr = (BatchItem * n)()
-        hists = []
-        for i, (m, params, nb, first_n) in enumerate(items):
-            pa = (GroupParams * m.n_groups)(*params)
-            keep.append(pa)
-            arr[i].model = m.handle.value
-            arr[i].params = C.cast(pa, C.c_void_p).value
-            arr[i].n_bins = int(nb)
-            arr[i].first_n = int(first_n)
-            if with_hist:
-                hists.append(np.zeros((m.n_pairs, int(nb)), np.uint64))
+        assert _BATCH_ITEM.itemsize == C.sizeof(BatchItem)
+        for m, params, _, _ in items:
+            if len(params) != m.n_groups:
+                raise ValueError("one GroupParams per subgroup of each model")
+        # every item's parameters in one buffer, the items as one record array
+        pbuf = np.frombuffer(b"".join(bytes(g) for _, params, _, _ in items for g in params), np.uint8).copy()
+        ng = np.fromiter((len(params) for _, params, _, _ in items), np.int64, n)
+        arr = np.zeros(n, _BATCH_ITEM)
+        arr["model"] = np.fromiter((m.handle.value for m, _, _, _ in items), np.uint64, n)
+        arr["params"] = pbuf.ctypes.data + (np.cumsum(ng) - ng) * C.sizeof(GroupParams)
+        arr["n_bins"] = np.fromiter((int(nb) for _, _, nb, _ in items), np.int64, n)
+        arr["first_n"] = np.fromiter((int(f) for _, _, _, f in items), np.int32, n)
+        hists = [np.zeros((m.n_pairs, int(nb)), np.uint64) for m, _, nb, _ in items] if with_hist else []
         I = np.zeros((n, len(q)))
         oor = np.zeros(n, np.uint64)
         hp = (C.c_void_p * n)(*[h.ctypes.data for h in hists]) if with_hist else None
         rel = np.zeros((n, len(q))) if with_err else None
-        self._check(lib().dbx_batch_intensity(self.handle, arr, n, rstep, _p(q), len(q), _p(I), _p(oor),
+        self._check(lib().dbx_batch_intensity(self.handle, _p(arr), n, rstep, _p(q), len(q), _p(I), _p(oor),
                                               hp, None if rel is None else _p(rel)), "batch_intensity",
                     allow=(DBX_ERR_RANGE,) if return_oor else ())
         return _results(I, (hists, with_hist), ([int(v) for v in oor], return_oor), (rel, with_err))

@@ -88,6 +88,33 @@ void *stage(dbx_ctx *ctx, size_t bytes)
 
 void stage_reset(dbx_ctx *ctx) { ctx->stage_used = 0; }
 
+// host copy of a call's results into the caller's (pageable) memory: large ones split over
+// four threads (one thread copies ~10 GB/s, so a 256-model batch's 8.4 MB of I took 0.7 ms
+// with the device idle)
+static void copy_out(void *dst, const void *src, size_t bytes)
+{
+    constexpr size_t kSplit = (size_t)4 << 20;
+    if (bytes < kSplit) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    constexpr int kThreads = 4;
+    const size_t part = (bytes / kThreads + 63) & ~(size_t)63;
+    std::vector<std::thread> th;
+    for (int t = 1; t < kThreads; t++) {
+        const size_t o = part * t;
+        if (o >= bytes) break;
+        auto piece = [=] { std::memcpy((char *)dst + o, (const char *)src + o, std::min(part, bytes - o)); };
+        try {
+            th.emplace_back(piece);
+        } catch (const std::exception &) {
+            piece();  // no thread to spare: this one copies it
+        }
+    }
+    std::memcpy(dst, src, std::min(part, bytes));
+    for (auto &t : th) t.join();
+}
+
 const double *device_q(dbx_ctx *ctx, const double *q, int nq)
 {
     const size_t b = sizeof(double) * (size_t)nq;
@@ -844,8 +871,8 @@ int dbx_batch_intensity(dbx_ctx *ctx, const dbx_batch_item *items, int n_items, 
                 sync_pass2(ctx);
             }
         }
-        std::memcpy(I_out, hI, sizeof(double) * nqd * n_items);
-        if (rel_err_out) std::memcpy(rel_err_out, hrel, sizeof(double) * nqd * n_items);
+        copy_out(I_out, hI, sizeof(double) * nqd * n_items);
+        if (rel_err_out) copy_out(rel_err_out, hrel, sizeof(double) * nqd * n_items);
         P1Stats st;
         pass1_read_stats(ctx, st);
         int bad = 0;

@@ -815,17 +815,34 @@ static double plan_total(const ModelDev &m)
     return t;
 }
 
+// one part of a launch list, largest cost first: a counting sort on the cost's exponent and top
+// five mantissa bits (buckets of 1/32 octave; a non-negative double orders like its bits), in
+// insertion order within a bucket.  The order only balances the launch (the counts do not
+// depend on it), and a 256-model batch's 20,000 pieces take a fraction of what a comparison
+// sort of them took (0.6 ms on the host, the device idle meanwhile).
+static void order_part(const EntryList &e, int4 *out)
+{
+    if (e.empty()) return;
+    std::vector<uint32_t> key(e.size());
+    uint32_t kmin = UINT32_MAX, kmax = 0;
+    for (size_t i = 0; i < e.size(); i++) {
+        uint64_t b;
+        std::memcpy(&b, &e[i].first, sizeof(b));
+        key[i] = (uint32_t)(b >> 47);
+        kmin = std::min(kmin, key[i]);
+        kmax = std::max(kmax, key[i]);
+    }
+    std::vector<size_t> at((size_t)(kmax - kmin) + 2, 0);
+    for (uint32_t k : key) at[kmax - k + 1]++;
+    for (size_t d = 1; d < at.size(); d++) at[d] += at[d - 1];
+    for (size_t i = 0; i < e.size(); i++) out[at[kmax - key[i]]++] = e[i].second;
+}
+
 // exact-window entries then checked ones, each part largest first
 static void order_entries(EntryList &fast, EntryList &checked, int4 *out)
 {
-    auto by_cost = [](const std::pair<double, int4> &a, const std::pair<double, int4> &b) {
-        return a.first > b.first;
-    };
-    std::stable_sort(fast.begin(), fast.end(), by_cost);
-    std::stable_sort(checked.begin(), checked.end(), by_cost);
-    size_t n = 0;
-    for (auto &e : fast) out[n++] = e.second;
-    for (auto &e : checked) out[n++] = e.second;
+    order_part(fast, out);
+    order_part(checked, out + fast.size());
 }
 
 // The pass-1 plan of a model for one bin width, built on the host (no device needed): the
