@@ -1012,6 +1012,34 @@ static MfShape mf_shape(int n_rows, int I)
     return {4, 4, 2};
 }
 
+// Splits of the contraction along the bins, for `tiles` (q group, row group) tiles of shape sh:
+// the launch holds 8 waves per CU (every shape's register use allows 8), so a round of
+// resident workgroups is cu_count 8 / (waves per workgroup).  At least eight rounds, and the
+// fewest splits past that whose last round is nearly full.  A half-empty last round leaves
+// half the CUs idle at the end, and row groups of unequal length balance better in smaller
+// pieces; measured (contraction kernel, 4096 q): the 256-model batch's 128-row tiles 4.26-4.29
+// ms at 4.5 rounds, 4.04-4.07 at 6, 3.82-3.91 at 9; 192 rows on the 32-row tile 1.13 ms at 4.5
+// rounds, 1.08 at 6 and at 9; 48 rows 0.44 ms at 4 rounds, 0.45 at 8; one model's 3 rows the
+// same at 4 and 8.  At most max_splits.
+static int mf_splits(int tiles, const MfShape &sh, int cu_count, int max_splits)
+{
+    const int64_t round = (int64_t)std::max(1, cu_count) * 8 / (sh.WQ * sh.WR);
+    const int nmax = std::max(1, max_splits);
+    const int nlo = (int)std::min<int64_t>(nmax, std::max<int64_t>(1, (8 * round + tiles - 1) / tiles));
+    int best = nlo;
+    double best_fill = 0;
+    for (int n = nlo; n <= std::min(nmax, 4 * nlo); n++) {
+        const int64_t wg = (int64_t)tiles * n;
+        const double fill = (double)wg / (double)(round * ((wg + round - 1) / round));
+        if (fill >= 0.97) return n;
+        if (fill > best_fill) {
+            best_fill = fill;
+            best = n;
+        }
+    }
+    return best;
+}
+
 // One launch of the contraction: shape, instantiated digit count I (bodies for 1..I, chosen per
 // tile at run time) and the band of largest row classes for which it runs.
 struct MfLaunch {
@@ -1247,9 +1275,8 @@ static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base
     const int qblocks = (nq + kDsThreads - 1) / kDsThreads;
     int nsplit;
     if (mfma) {
-        const int target = 2048 * 4 / (Lb[0].sh.WQ * Lb[0].sh.WR);  // about 8 waves per SIMD
         const int rg = rows_pad / Lb[0].sh.rows();
-        nsplit = std::max(1, std::min((target + qgroups * rg - 1) / (qgroups * rg), pad_k(kc_cap) / kMfK / 8));
+        nsplit = mf_splits(qgroups * rg, Lb[0].sh, ctx->cu_count, pad_k(kc_cap) / kMfK / 32);  // >= 32 steps each
     } else {
         nsplit = std::max(1, std::min((2048 + qblocks - 1) / qblocks, (kc_cap + 511) / 512));
     }
