@@ -74,6 +74,7 @@ def primary(args, cd, synth, np):
                                     with_err=True)
         refined = ctx.pass2_refined()
         n_jobs, misses = ctx.pass1_stats()
+        evals = ctx.pass1_evaluations()
         ctx.set_profiling(True)
         ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n)
         mf_ms, mf_flops, mf_useful, kc, digits, mean_digits = ctx.pass2_profile()
@@ -122,6 +123,7 @@ def primary(args, cd, synth, np):
         "sinc_gelements_per_s": round(kc * len(m.q) / (mf_ms * 1e-3) / 1e9, 2),
         "pass1_jobs": n_jobs,
         "pass1_window_misses": misses,
+        "pass1_evaluations_per_pair": round(evals / pairs, 3),
         "roofline": {
             "bound": "valu",
             "kernel": "p1_histogram_kernel",
@@ -394,11 +396,15 @@ def workload_legs(cd, synth, np):
     params = [_gp(cd, p) for p in m.params]
     with cd.Context(0) as ctx:
         wall, p1, p2, _ = _resident_steps(cd, ctx, m, params)
+        evals = ctx.pass1_evaluations()
     out["spinel_sphere_1e-4"] = {
         "workload": "the primary spinel-like sphere at bin 1e-4 A (the reference's default), 4096 Q",
         "bins": m.n_bins, "ms_per_step": round(wall, 2), "gpairs_per_s": round(m.pair_count / wall / 1e6, 1),
         "pass1_kernel_ms": round(p1, 2), "pass1_gpairs_per_s": round(m.pair_count / p1 / 1e6, 1),
         "pass2_ms": round(p2, 3),
+        # pair evaluations per counted pair: shells wider than one LDS window are counted window
+        # by window, each job evaluating every pair its staged atoms can form
+        "pass1_evaluations_per_pair": round(evals / m.pair_count, 3),
     }
     # the primary model cold: host tiling, upload, pass-1 planning and the step, per model
     m = synth.spinel_sphere()

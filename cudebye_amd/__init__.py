@@ -128,6 +128,7 @@ def lib():
             "dbx_set_rel_err_target": (i32, [vp, f64]),
             "dbx_view_deb": (i32, [C.c_char_p, vp, C.c_char_p, C.c_char_p, C.c_char_p, P(i64)]),
             "dbx_debug_pass2_refined": (i32, [vp, P(i32)]),
+            "dbx_debug_pass1_evaluations": (i32, [vp, P(u64)]),
             "dbx_run_deb": (i32, [C.c_char_p, i32]),
             "dbx_run_folder": (i32, [C.c_char_p, i32]),
             "dbx_read_debx": (i32, [C.c_char_p, f64, i64, vp, P(i64)]),
@@ -410,6 +411,13 @@ class Context:
         if with_retried:
             return int(nj.value), int(miss.value), int(rt.value)
         return int(nj.value), int(miss.value)
+
+    def pass1_evaluations(self):
+        """Pair evaluations of the last pass 1 (staged B atoms times A-tile atoms, in or out of
+        their job's window): about the pairs counted where one window holds each shell."""
+        v = C.c_uint64(0)
+        self._check(lib().dbx_debug_pass1_evaluations(self.handle, C.byref(v)), "pass1_evaluations")
+        return int(v.value)
 
     def set_pass2_mode(self, mode):
         """DBX_PASS2_MFMA (8-bit integer digits on the matrix cores, default) or DBX_PASS2_FP64."""

@@ -960,6 +960,13 @@ int64_t dbx_debug_model_positions(dbx_ctx *ctx, const dbx_model *m, float *out4,
     return n;
 }
 
+int dbx_debug_pass1_evaluations(const dbx_ctx *ctx, uint64_t *evaluated)
+{
+    if (!ctx || !evaluated) return DBX_ERR_ARG;
+    *evaluated = ctx->last_p1_evals;
+    return DBX_OK;
+}
+
 int dbx_debug_pass1_plan(const dbx_model *m, double rstep, void *jobs, int cap_jobs, float *edge2,
                          int cap_edge2, int *n_jobs, int *n_edge2)
 {

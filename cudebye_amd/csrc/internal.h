@@ -177,7 +177,7 @@ struct dbx_ctx {
     std::vector<double> q_host;
     const void *q_dev = nullptr;
     // [0] out-of-range pairs, [1] pairs outside the LDS window (checked kernel),
-    // [2] fast jobs queued for the checked kernel; then the retry list (int32)
+    // [2] fast jobs queued for the checked kernel, [3] pair evaluations; then the retry list
     uint64_t *counters = nullptr;
     uint64_t *counters_h = nullptr;  // pinned host copy of counters[0..3]
     bool p1_timed = false;
@@ -185,6 +185,7 @@ struct dbx_ctx {
     int64_t last_p1_jobs = 0;
     uint64_t last_p1_misses = 0;
     uint64_t last_p1_retried = 0;
+    uint64_t last_p1_evals = 0;   // pair evaluations of the last pass 1 (in- and out-of-window)
     float last_p1_kernel_ms = 0.f;  // hipEvent time of the last pass-1 histogram kernel
     float last_p2_ms = 0.f;         // hipEvent time of the last pass-2 structure computation
     hipEvent_t ev[4] = {};

@@ -331,6 +331,9 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
     }
 
     int cur_ia = -1;
+    // pair evaluations of this wave (staged B atoms times the A tile's atoms, whether their bin
+    // falls in this job's window or not), summed into counters[3] at the job's end
+    unsigned long long evals = 0;
     // squared distances from the A tile's centre of the B atoms that can pair in this job: the
     // shell, narrowed to the window's reach in multi-window jobs
     float s_lo2 = e_lo, s_hi2 = e_hi;
@@ -416,6 +419,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
                     // stage the shell's B atoms (with their atom index) in the wave's LDS area; the
                     // lanes then read them back as broadcasts, R per step
                     const int nb = __popcll(bm);
+                    evals += (unsigned long long)nb * (unsigned long long)ta.count;
                     if (in_shell) {
                         const int below = __builtin_amdgcn_mbcnt_hi((uint32_t)(bm >> 32),
                                                                      __builtin_amdgcn_mbcnt_lo((uint32_t)bm, 0));
@@ -485,6 +489,7 @@ __global__ __launch_bounds__(kP1Threads) void p1_histogram_kernel(
             }
             __syncthreads();
         }
+        if (lane == 0 && evals) atomicAdd(&counters[3], evals);
     };
     if (CHECKED) {
         CountChecked cnt{h, k_lo, job.win, width, complement, n_bins, hrow, 0u, 0u};
@@ -1038,6 +1043,7 @@ void pass1_read_stats(dbx_ctx *ctx, P1Stats &st)
     st.out_of_range = c[0];
     st.window_misses = c[1];
     st.retried = c[2];
+    ctx->last_p1_evals = c[3];
     ctx->last_p1_misses = c[1];
     ctx->last_p1_retried = c[2];
     if (ctx->p1_timed) DBX_HIP(hipEventElapsedTime(&ctx->last_p1_kernel_ms, ctx->ev[0], ctx->ev[1]));

@@ -25,6 +25,13 @@ int64_t dbx_debug_model_positions(dbx_ctx *ctx, const dbx_model *m, float *out4,
 int dbx_debug_pass1_plan(const dbx_model *m, double rstep, void *jobs, int cap_jobs, float *edge2,
                          int cap_edge2, int *n_jobs, int *n_edge2);
 
+/* Pair evaluations of the context's last pass 1 (kernel counter): for every (A tile, B tile)
+ * candidate, its B atoms inside the job's shell times the A tile's atoms, whether the pair's
+ * bin falls in the job's LDS window or not.  Against the pairs counted (N(N+1)/2 per
+ * subgroup, N_a N_b per pair of subgroups) this is the work spent per counted pair: about one
+ * where one window holds a shell, more where fine bins split shells over several windows. */
+int dbx_debug_pass1_evaluations(const dbx_ctx *ctx, uint64_t *evaluated);
+
 /* The same tiling and plan computed on the host from arrays (no device call): tiles (all
  * 64-atom blocks of every subgroup), the tile-ordered positions
  * (float4), work items, shell edges and the A-tile index list the work items point into,
