@@ -1249,11 +1249,11 @@ constexpr size_t kP2BoundBytes = (size_t)8 << 30;
 // upper bounds (the compacted bins from the bins of the longest row past the self bins, the
 // count digits from kMfImax planes), and the kernels read the actual compacted-bin count and
 // largest digit class from device memory (dyn).  Only when buffers sized that way would exceed
-// kP2BoundBytes or the provider's room does the host read the two values back first and size
-// the buffers exactly.  The splits of the sums are chosen from the bounds either way, so both
-// give the same sums bit for bit.  The two values are also copied to the context's pinned
-// status words, which sync_pass2 checks after the caller's wait (a count beyond the matrix
-// cores' digits is reported there as DBX_ERR_RANGE).
+// kP2BoundBytes or the provider's room, or cannot be allocated, does the host read the two
+// values back first and size the buffers exactly.  The splits of the sums are chosen from the
+// bounds either way, so both give the same sums bit for bit.  The two values are also copied
+// to the context's pinned status words, which sync_pass2 checks after the caller's wait (a
+// count beyond the matrix cores' digits is reported there as DBX_ERR_RANGE).
 static void structure_core(dbx_ctx *ctx, P2Mem &mem, const uint64_t *d_hist_base, const RowDesc *h_rows,
                            int n_rows, int64_t nb_max, int uni_first_n, double rstep, const double *d_q, int nq,
                            double *d_S, int mode, double *d_B)
