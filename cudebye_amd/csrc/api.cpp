@@ -17,50 +17,63 @@
 
 namespace dbx {
 
+// Slots start at kSlotMin bytes and grow at least twofold.  A growth does not wait: work
+// already enqueued on the stream may still use the old buffer, which is retired (freed by
+// release_retired once the stream is idle) rather than freed, so a growth inside a server step
+// costs one allocation, not a wait for that step's earlier kernels (and a device-wide one in
+// hipFree).  Only when the device has no room for the new buffer beside the old ones does it
+// wait, free them and allocate the size asked for.
+constexpr size_t kSlotMin = (size_t)64 << 10;
+
 void *scratch(dbx_ctx *ctx, int slot, size_t bytes)
 {
-    if (bytes == 0) bytes = 256;
-    if (ctx->slot_bytes[slot] < bytes) {
-        // grows at least twofold: a growth waits for the stream and reallocates, so calls of
-        // slowly rising size (a server lane's steps) pay it a few times only
+    if (ctx->slot_bytes[slot] < bytes || !ctx->slot_ptr[slot]) {
         const size_t old = ctx->slot_bytes[slot];
-        if (ctx->slot_ptr[slot]) {
-            DBX_HIP(hipStreamSynchronize(ctx->stream));
-            DBX_HIP(hipFree(ctx->slot_ptr[slot]));
-            ctx->slot_ptr[slot] = nullptr;
-            ctx->slot_bytes[slot] = 0;
-        }
-        size_t want = std::max(bytes + bytes / 4, 2 * old);
+        size_t want = std::max({bytes + bytes / 4, 2 * old, kSlotMin});
         void *p = nullptr;
         if (hipMalloc(&p, want) != hipSuccess) {
-            (void)hipGetLastError();  // the doubled size does not fit: the size asked for
-            want = bytes + bytes / 4;
-            DBX_HIP(hipMalloc(&p, want));
+            (void)hipGetLastError();
+            DBX_HIP(hipStreamSynchronize(ctx->stream));
+            release_retired(ctx);
+            if (ctx->slot_ptr[slot]) DBX_HIP(hipFree(ctx->slot_ptr[slot]));
+            ctx->slot_ptr[slot] = nullptr;
+            ctx->slot_bytes[slot] = 0;
+            want = std::max(bytes + bytes / 4, kSlotMin);
+            if (hipMalloc(&p, want) != hipSuccess) {
+                (void)hipGetLastError();
+                want = std::max<size_t>(bytes, 256);
+                DBX_HIP(hipMalloc(&p, want));
+            }
         }
+        if (ctx->slot_ptr[slot]) ctx->retired_dev.push_back(ctx->slot_ptr[slot]);
         ctx->slot_ptr[slot] = p;
         ctx->slot_bytes[slot] = want;
+        // a new buffer holds no q grid, even at an address an earlier one had
+        if (slot == kSlotQin) ctx->q_dev = nullptr;
     }
     return ctx->slot_ptr[slot];
 }
 
 void *pinned(dbx_ctx *ctx, int slot, size_t bytes)
 {
-    if (bytes == 0) bytes = 64;
-    if (ctx->pin_bytes[slot] < bytes) {
-        const size_t old = ctx->pin_bytes[slot];
-        if (ctx->pin_ptr[slot]) {
-            DBX_HIP(hipStreamSynchronize(ctx->stream));
-            DBX_HIP(hipHostFree(ctx->pin_ptr[slot]));
-            ctx->pin_ptr[slot] = nullptr;
-            ctx->pin_bytes[slot] = 0;
-        }
-        const size_t want = std::max(bytes + bytes / 2, 2 * old);
+    if (ctx->pin_bytes[slot] < bytes || !ctx->pin_ptr[slot]) {
+        // as scratch: the old buffer may be the destination of a download still enqueued
+        const size_t want = std::max({bytes + bytes / 2, 2 * ctx->pin_bytes[slot], kSlotMin});
         void *p = nullptr;
         DBX_HIP(hipHostMalloc(&p, want, hipHostMallocDefault));
+        if (ctx->pin_ptr[slot]) ctx->retired_pin.push_back(ctx->pin_ptr[slot]);
         ctx->pin_ptr[slot] = p;
         ctx->pin_bytes[slot] = want;
     }
     return ctx->pin_ptr[slot];
+}
+
+void release_retired(dbx_ctx *ctx)
+{
+    for (void *p : ctx->retired_dev) (void)hipFree(p);
+    for (void *p : ctx->retired_pin) (void)hipHostFree(p);
+    ctx->retired_dev.clear();
+    ctx->retired_pin.clear();
 }
 
 void *stage(dbx_ctx *ctx, size_t bytes)
@@ -559,6 +572,7 @@ int dbx_close(dbx_ctx *ctx)
         delete m;
     }
     ctx->models.clear();
+    release_retired(ctx);
     for (void *p : ctx->slot_ptr)
         if (p) (void)hipFree(p);
     for (void *p : ctx->pin_ptr)

@@ -170,6 +170,9 @@ struct dbx_ctx {
     size_t slot_bytes[15] = {};
     void *pin_ptr[6] = {};
     size_t pin_bytes[6] = {};
+    // buffers that growths replaced while enqueued work may still use them (release_retired)
+    std::vector<void *> retired_dev, retired_pin;
+    bool no_alloc = false;  // inside dbx_debye_noalloc: waits leave the retired buffers alone
     // pinned staging of uploads (dbx::stage): a grow-only buffer handed out in consecutive regions
     char *stage_buf = nullptr;
     size_t stage_cap = 0, stage_used = 0;
@@ -215,12 +218,15 @@ namespace dbx {
 enum { kSlotPass1 = 0, kSlotPass2 = 1, kSlotPass2b = 2, kSlotHist = 3, kSlotQ = 4, kSlotDigits = 5,
        kSlotRtab = 6, kSlotBatch = 7, kSlotRows = 8, kSlotPairs = 9, kSlotBatchHist = 10,
        kSlotOor = 11, kSlotQin = 12, kSlotRefine = 13, kSlotProf = 14 };
-// device buffer of at least `bytes` for `slot`; contents are not preserved when it grows
+// device buffer of at least `bytes` for `slot`; contents are not preserved when it grows (the
+// old buffer stays valid for work already enqueued until release_retired)
 void *scratch(dbx_ctx *ctx, int slot, size_t bytes);
 // pinned host buffer of at least `bytes` for `slot`: destinations of downloads, read after the
-// caller's wait (uploads take their staging from stage())
+// caller's wait (uploads take their staging from stage()); grows like scratch
 enum { kPinMisc = 0, kPinOor = 1, kPinStepI = 2, kPinI = 3, kPinRel = 4, kPinStepRel = 5 };
 void *pinned(dbx_ctx *ctx, int slot, size_t bytes);
+// frees the buffers that growths of scratch and pinned replaced; the stream must be idle
+void release_retired(dbx_ctx *ctx);
 // Pinned host memory for one upload enqueued on the context's stream, a region of its own:
 // uploads enqueued one after another with no wait between them never share staging memory.
 // Regions are handed out again once the stream has been waited for (stage_reset after a

@@ -87,13 +87,18 @@ int dbx_debye_noalloc(dbx_ctx *ctx, const uint64_t *d_hist, int P, int64_t n_bin
         DBX_REQUIRE(base != nullptr, DBX_ERR_STATE, "fixed scratch region not set up on this device");
         FixedMem mem(base, (unsigned long long *)ctx->p2_readback_h);
         // the contraction profile (a debugging aid) reads back through the context's growing
-        // slots: off for this call
-        struct NoProfile {
+        // slots: off for this call; and its waits free no buffer that an earlier call retired
+        struct NoAllocScope {
             dbx_ctx *c;
             bool was;
-            ~NoProfile() { c->profiling = was; }
+            ~NoAllocScope()
+            {
+                c->profiling = was;
+                c->no_alloc = false;
+            }
         } np{ctx, ctx->profiling};
         ctx->profiling = false;
+        ctx->no_alloc = true;
         // the structure sums S[p][q], their bounds and the bound of I: the caller's buffers, or
         // the region's
         double *S = d_S ? d_S : (double *)mem.dev(kSlotQ, sizeof(double) * (size_t)P * (size_t)nq);

@@ -1179,6 +1179,7 @@ void sync_pass2(dbx_ctx *ctx)
 {
     DBX_HIP(hipStreamSynchronize(ctx->stream));
     stage_reset(ctx);
+    if (!ctx->no_alloc && (!ctx->retired_dev.empty() || !ctx->retired_pin.empty())) release_retired(ctx);
     const int n = ctx->p2_pending;
     ctx->p2_pending = 0;
     bool too_large = false;

@@ -40,6 +40,8 @@ def test_sqrt_correctly_rounded_exhaustive():
     square root of pass 1 is correctly rounded and equals the compiler's sqrtf."""
     bad, mism = cd.selftest_sqrt(0x0F800000, 0x7F800000)
     assert bad == 0 and mism == 0
+    # s == 0 (self pairs, coincident atoms) gives +0
+    assert cd.selftest_sqrt(0, 1)[0] == 0
 
 
 def test_round_half_up_exhaustive():
