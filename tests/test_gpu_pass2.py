@@ -6,6 +6,7 @@ import pytest
 import cudebye_amd as cd
 from cudebye_amd import synth
 from oracle import oracle as O
+from tests.sweep_models import SWEEP, sweep_model
 from tests.conftest import read_debi
 
 pytestmark = pytest.mark.gpu
@@ -321,3 +322,37 @@ def test_rows_of_mixed_digit_classes(ctx, top):
         err = np.abs(S[p] - Sref)
         assert err.max() <= bound, (p, caps[p], err.max(), bound)
     assert np.all(np.abs(I - Iref) / np.abs(Iref) <= rel)
+
+
+# the sweep's q grids: nq and range per model (fewer points where the oracle's sums are long)
+_SWEEP_Q = {"hollow_shell_fine": (17, 0.5, 12.0), "fine_bins_wide": (5, 0.3, 9.0), "very_coarse_bins": (1, 1.7, 1.7),
+            "plane": (257, 0.2, 30.0)}
+
+
+@pytest.mark.parametrize("kind", SWEEP)
+def test_geometry_and_bin_width_sweep_within_bound(ctx, kind):
+    """The pass-1 sweep's models through pass 2 against the oracle's FP64 sums: on the matrix
+    cores (no FP64 recomputation) and in FP64, every I(q) within its reported bound, and the
+    default call (recomputation at 1e-6) within 1e-6."""
+    m = sweep_model(kind, 900 + SWEEP.index(kind))
+    nq, q0, q1 = _SWEEP_Q.get(kind, (97, 0.3, 25.0))
+    m.q = np.linspace(q0, q1, nq)
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    params = [_gp(p) for p in m.params]
+    H = ctx.histogram(mod, m.rstep, m.n_bins)
+    Iref, _ = _oracle_I(m, H, m.first_n)
+    try:
+        ctx.set_rel_err_target(0)
+        for mode in MODES:
+            ctx.set_pass2_mode(mode)
+            I, rel = ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n, with_err=True)
+            err = np.abs(I - Iref) / np.abs(Iref)
+            assert np.all(err <= rel + 1e-14), (mode, err.max(), rel.max())
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+        ctx.set_rel_err_target(1e-6)
+        I = ctx.intensity(mod, params, m.rstep, m.n_bins, m.q, first_n=m.first_n)
+        assert np.max(np.abs(I - Iref) / np.abs(Iref)) < 1e-6
+    finally:
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+        ctx.set_rel_err_target(1e-6)
+    mod.close()
