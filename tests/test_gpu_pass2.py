@@ -324,6 +324,38 @@ def test_rows_of_mixed_digit_classes(ctx, top):
     assert np.all(np.abs(I - Iref) / np.abs(Iref) <= rel)
 
 
+
+@pytest.mark.parametrize("top", [3, 5])
+def test_more_rows_than_one_ordering_pass(ctx, top):
+    """2,500 rows, more than the 1,024 rows that the device's ordering by digit class handles
+    per pass, with largest counts needing 1 to `top` count digits in shuffled order (top 3: the
+    128-row tile; top 5: the 32-row tile).  Every row's sum lands in the caller's row within
+    the quantisation bound of the oracle's FP64 sum, and the bound of I covers its error."""
+    rng = np.random.default_rng(700 + top)
+    P, nb, nz, nq = 2500, 4000, 300, 40
+    caps = np.array([2 ** 5, 2 ** 13, 2 ** 21, 2 ** 29, 2 ** 36][:top])[rng.integers(0, top, P)]
+    H = np.zeros((P, nb), np.uint64)
+    for p in range(P):
+        idx = rng.choice(np.arange(1, nb), nz, replace=False)
+        H[p, idx] = rng.integers(1, caps[p], nz).astype(np.uint64)
+    K = np.linspace(0.3, 20.0, nq)
+    ff = np.ones((P, nq))
+    ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+    ctx.set_rel_err_target(0)
+    try:
+        I, S, rel = ctx.debye(H, 1e-3, K, ff, first_n=8, with_S=True, with_err=True)
+    finally:
+        ctx.set_rel_err_target(1e-6)
+    Iref = np.zeros(nq)
+    for p in range(P):
+        idx = np.nonzero(H[p])[0]
+        Sref = np.zeros(nq)
+        O.debye_sum(idx, H[p][idx].astype(float), 1e-3, K, np.ones(nq), s_out=Sref, first_n=8)
+        Iref += Sref
+        err = np.abs(S[p] - Sref)
+        assert err.max() <= _quant_bound(H[p], Sref), (p, caps[p], err.max())
+    assert np.all(np.abs(I - Iref) / np.abs(Iref) <= rel)
+
 # the sweep's q grids: nq and range per model (fewer points where the oracle's sums are long)
 _SWEEP_Q = {"hollow_shell_fine": (17, 0.5, 12.0), "fine_bins_wide": (5, 0.3, 9.0), "very_coarse_bins": (1, 1.7, 1.7),
             "plane": (257, 0.2, 30.0)}
