@@ -1,5 +1,7 @@
 """Pass 2 on the GPU against the FP64 oracle: prefactors, the Debye sum with the reference
 weight rule, and I(Q) (relative error bound written in each test)."""
+import copy
+
 import numpy as np
 import pytest
 
@@ -387,4 +389,33 @@ def test_geometry_and_bin_width_sweep_within_bound(ctx, kind):
     finally:
         ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
         ctx.set_rel_err_target(1e-6)
+    mod.close()
+
+
+def test_q_zero_and_tiny_q_give_the_square_of_the_total_scattering(ctx):
+    """At Q = 0 every sinc is 1 (the reference's sin(x)/x would be 0/0 there): I(0) is
+    (sum_i occ_i f_i(0))^2 when every pair is counted, and I(1e-7) agrees with it; the oracle
+    pins the other points.  Both modes, within the reported bounds."""
+    m = synth.small_random(900, 3, 61, spread=9.0)
+    q = np.array([0.0, 1e-7, 1e-3, 0.3, 5.0])
+    mod = ctx.model(m.x, m.y, m.z, m.group_n)
+    params = [_gp(p) for p in m.params]
+    f0 = np.array([p["c"] + sum(p["a"]) for p in m.params])
+    occ = np.array([p["occ"] for p in m.params])
+    tot = float((m.group_n * occ * f0).sum())
+    m2 = copy.copy(m)
+    m2.q = q[1:]
+    H = ctx.histogram(mod, m.rstep, m.n_bins)
+    Iref, _ = _oracle_I(m2, H, m.first_n)
+    try:
+        for mode in MODES:
+            ctx.set_pass2_mode(mode)
+            I, rel = ctx.intensity(mod, params, m.rstep, m.n_bins, q, first_n=m.first_n, with_err=True)
+            assert np.all(np.isfinite(I)) and np.all(np.isfinite(rel))
+            assert abs(I[0] - tot ** 2) <= max(rel[0], 1e-12) * tot ** 2, (mode, I[0], tot ** 2)
+            assert abs(I[1] - tot ** 2) <= 1e-9 * tot ** 2
+            err = np.abs(I[1:] - Iref) / np.abs(Iref)
+            assert np.all(err <= rel[1:] + 1e-14), (mode, err, rel)
+    finally:
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
     mod.close()
