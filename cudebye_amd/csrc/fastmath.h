@@ -13,6 +13,9 @@
 
 namespace dbx {
 
+// the arguments sin_cw and sincos_cw hold for: |x| below this
+constexpr double kSinArgMax = 0x1p20;
+
 __host__ __device__ inline double sin_cw(double x)
 {
     const double inv_pi = 0x1.45f306dc9c883p-2;

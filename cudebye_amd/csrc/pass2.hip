@@ -752,6 +752,22 @@ __device__ __forceinline__ void mf_tile_sum(const int3 tile, int kend, i32x4 (*s
     const int kb = tile.y * k_per_split;
     const int ke = min(min(Kc_pad, kx), kb + k_per_split);
     const int nsteps = ke > kb ? (ke - kb) / kMfK : 0;
+    const int qq = q0 + col;
+    // The sines hold for Q r < kSinArgMax.  A tile whose largest Q r (its last bin, at most 63
+    // padding bins past the last compacted one) reaches that writes NaN sums instead: a call
+    // with an error target recomputes those models in FP64 (whose sines take any argument),
+    // one without returns NaN.
+    if (__syncthreads_or(nsteps > 0 && Q * rr[ke - 1] >= kSinArgMax)) {
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = rowt + (wr * RB + rb) * 16 + kq * 4 + v;
+                if (qq < nq && row < nrows)
+                    Spart[((int64_t)tile.y * srows + row) * nq + qq] = __builtin_nan("");
+            }
+        return;
+    }
     i32x4 acc[RB][C];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++)
@@ -760,7 +776,6 @@ __device__ __forceinline__ void mf_tile_sum(const int3 tile, int kend, i32x4 (*s
     // The 32-bit sums (digit c weighs 2^(8c)) are folded into the split's FP64 sums, which live
     // in Spart rather than in registers: each (row, q) has one owner lane, the first fold
     // stores, later ones add, and the last one scales by 1 / c.
-    const int qq = q0 + col;
     bool first = true;
     auto fold = [&](bool last) {
         constexpr double inv_c = 1.0 / kMfC;
@@ -1130,7 +1145,10 @@ __global__ void p2_combine_kernel(const double *__restrict__ ff, const double *_
         if (B) bnd += fabs(f) * B[r];
     }
     I[(int64_t)rr.z * nq + iq] = acc;
-    if (rel) rel[(int64_t)rr.z * nq + iq] = (bnd + (double)(rr.y + 8) * 0x1p-53 * mag) / fabs(acc);
+    // an I of exactly 0 from terms of exactly 0 (form factors that underflow at large Q) has
+    // relative error 0, not 0 / 0
+    const double num = bnd + (double)(rr.y + 8) * 0x1p-53 * mag;
+    if (rel) rel[(int64_t)rr.z * nq + iq] = num == 0.0 ? 0.0 : num / fabs(acc);
 }
 
 // largest k with double(k)*rstep <= 0.0005 (the reference's test r[k] <= 0.0005)

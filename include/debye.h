@@ -119,7 +119,9 @@ int dbx_prefactor(dbx_ctx *ctx, const dbx_group_params *ga, const dbx_group_para
  * host FP64.  I_out[nq] host FP64 (overwritten).  S_out ([P][nq], optional) receives S_p.
  * rel_err_out ([nq], optional) receives for every I(q) a bound of its relative error against
  * the exact (FP64-oracle) sum over the same histograms and prefactors, derived from the
- * arithmetic used (see dbx_set_rel_err_target); every I returned here is within that bound. */
+ * arithmetic used (see dbx_set_rel_err_target); every I returned here is within that bound.
+ * An I of exactly 0 from terms of exactly 0 has bound 0.  The matrix-core sines hold for
+ * q r < 2^20: past that their I and bound are NaN, which an error target recomputes in FP64. */
 int dbx_debye(dbx_ctx *ctx, const uint64_t *hist, int hist_is_device, int P, int64_t n_bins,
               double rstep, int first_n, const double *q, int nq, const double *ffoobb,
               double *I_out, double *S_out, double *rel_err_out);

@@ -419,3 +419,67 @@ def test_q_zero_and_tiny_q_give_the_square_of_the_total_scattering(ctx):
     finally:
         ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
     mod.close()
+
+
+def test_q_r_beyond_the_sine_range_recomputed_in_fp64(ctx):
+    """Two clusters 4000 A apart and Q up to 400 A^-1: Q r passes 2^20, past which the
+    matrix-core path's sines do not hold.  Its tiles there return NaN, which the error target
+    turns into an FP64 recomputation (libm sines), equal to the FP64 mode and to the oracle;
+    with the target off the NaN reaches the caller instead of a wrong value."""
+    rng = np.random.default_rng(4000)
+    a = rng.uniform(-3, 3, (10, 3))
+    b = rng.uniform(-3, 3, (10, 3)) + [4000.0, 0, 0]
+    p = np.concatenate([a, b]).astype(np.float32)
+    gn = np.array([12, 8], np.int64)
+    rstep = 0.01
+    nb = O.channels(4020.0, rstep)
+    q = np.array([0.5, 50.0, 400.0])
+    gps = [synth.cromer_mann(rng) for _ in range(2)]
+    for g in gps:  # form factors that do not underflow at 400 A^-1
+        g["b"] = g["b"] * 1e-3
+        g["beq"] = 0.0
+    params = [_gp(g) for g in gps]
+    mod = ctx.model(p[:, 0].copy(), p[:, 1].copy(), p[:, 2].copy(), gn)
+    try:
+        ctx.set_pass2_mode(cd.DBX_PASS2_FP64)
+        I64, rel64 = ctx.intensity(mod, params, rstep, nb, q, first_n=8, with_err=True)
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+        I, rel = ctx.intensity(mod, params, rstep, nb, q, first_n=8, with_err=True)
+        assert ctx.pass2_refined() == 1
+        assert np.array_equal(I, I64) and np.array_equal(rel, rel64)
+        ctx.set_rel_err_target(0)
+        Inan = ctx.intensity(mod, params, rstep, nb, q, first_n=8)
+        assert np.isnan(Inan).all()
+    finally:
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+        ctx.set_rel_err_target(1e-6)
+    H = ctx.histogram(mod, rstep, nb)
+    mod.close()
+
+    class M:
+        pass
+    M.group_n, M.q, M.rstep = gn, q, rstep
+    M.params = gps
+    Iref, _ = _oracle_I(M, H, 8)
+    assert np.all(np.abs(I - Iref) / np.abs(Iref) <= rel + 1e-14)
+
+
+def test_exact_zero_intensity_has_zero_bound(ctx):
+    """Form factors of exactly 0 at one q (they underflow at large Q): I is exactly 0 there and
+    its relative error bound 0, not 0 / 0; the other q keep their bounds.  Both modes."""
+    rng = np.random.default_rng(12)
+    H = np.zeros((3, 5000), np.uint64)
+    for p in range(3):
+        idx = rng.choice(np.arange(1, 5000), 400, replace=False)
+        H[p, idx] = rng.integers(1, 1000, 400).astype(np.uint64)
+    K = np.array([0.7, 3.0, 11.0])
+    ff = rng.uniform(0.5, 2.0, (3, 3))
+    ff[:, 1] = 0.0
+    try:
+        for mode in MODES:
+            ctx.set_pass2_mode(mode)
+            I, rel = ctx.debye(H, 1e-3, K, ff, first_n=8, with_err=True)
+            assert I[1] == 0.0 and rel[1] == 0.0, (mode, I, rel)
+            assert np.all(np.isfinite(rel)) and rel[0] > 0 and rel[2] > 0
+    finally:
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
