@@ -954,7 +954,8 @@ P1Stats pass1_batch(dbx_ctx *ctx, const std::vector<P1Item> &items, double rstep
     int64_t max_words = 0;
     for (size_t i = 0; i < items.size(); i++) {
         const P1Item &it = items[i];
-        DBX_REQUIRE(it.n_bins > 0 && it.n_bins < (int64_t)1 << 40, DBX_ERR_ARG, "bad n_bins");
+        // bins are 32-bit integers on the device (a scaled distance converts with saturation)
+        DBX_REQUIRE(it.n_bins > 0 && it.n_bins < (int64_t)INT32_MAX, DBX_ERR_ARG, "bad n_bins");
         ensure_plan(ctx, *it.m, rstep);
         refs[i] = make_ref(*it.m, it.n_bins, it.d_hist, rstep);
         refs[i].oor = it.d_oor ? (unsigned long long *)it.d_oor : (unsigned long long *)ctx->counters;

@@ -93,7 +93,8 @@ int dbx_model_info(const dbx_model *m, int64_t *n_atoms, int *n_groups, int *n_p
  *   dx,dy,dz FP32 differences; s = (dx*dx + dy*dy) + dz*dz, each operation rounded;
  *   k = (int)roundf(sqrtf(s) * (float)(1/rstep)), sqrtf correctly rounded.
  * Same subgroup: pairs col <= row including self pairs (bin 0 += n); cross: n_a*n_b.
- * out: uint64[P][n_bins] (host memory, or device memory when out_is_device != 0).
+ * out: uint64[P][n_bins] (host memory, or device memory when out_is_device != 0);
+ * 0 < n_bins < 2^31 - 1, else DBX_ERR_ARG.
  * Returns DBX_ERR_RANGE (histogram still written, offending pairs not counted) if any
  * pair falls at k >= n_bins; *n_out_of_range (optional) receives their number.
  * The model keeps its pass-1 work plan for the last rstep (host planning runs once per
