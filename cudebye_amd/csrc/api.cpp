@@ -632,7 +632,7 @@ int64_t dbx_channels(double rmax, double rstep)
 int dbx_model_create(dbx_ctx *ctx, const float *x, const float *y, const float *z,
                      const int64_t *group_n, int n_groups, dbx_model **out)
 {
-    if (!out || !group_n || (!x && n_groups > 0)) return DBX_ERR_ARG;
+    if (!out || !group_n || (n_groups > 0 && (!x || !y || !z))) return DBX_ERR_ARG;
     *out = nullptr;
     return guarded(ctx, [&] {
         dbx_model *m = new dbx_model;
