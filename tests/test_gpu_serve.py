@@ -409,3 +409,28 @@ def test_serve_error_target_recomputes_in_fp64(ctx):
         assert _rel(Ir, I0) < 1e-13 and np.allclose(br, b0, rtol=1e-9, atol=0)
         for t in ts:
             srv.release(t)
+
+
+def test_serve_mixed_bin_widths(ctx):
+    """Requests of three bin widths (1e-3, 5e-4 and 2e-3 A) submitted interleaved: a step takes
+    the head's bin width only, so they run in separate steps; every request's histograms are
+    bit-exact against the model alone at its own width and its I(q) within the mix tolerance."""
+    base = _models(12, 17)
+    widths = [1e-3, 5e-4, 2e-3]
+    models = []
+    for i, m in enumerate(base):
+        w = widths[i % 3]
+        models.append(synth.small_random(int(m.group_n.sum()), len(m.group_n), 17000 + i,
+                                         spread=float(np.ptp(m.x)) / 2 + 1.0, rstep=w, nq=len(m.q)))
+    alone = [_alone(ctx, m) for m in models]
+    with cd.Server(0) as srv:
+        tickets = [_submit(srv, m) for m in models]
+        for i, m in enumerate(models):
+            I, _, rc = _finish(srv, tickets[i])
+            assert rc == cd.DBX_OK
+            H = srv.histogram(tickets[i])
+            assert np.array_equal(H, alone[i][1]), f"request {i} (bin {m.rstep})"
+            assert _rel(I, alone[i][0]) < MIX_TOL, f"request {i}"
+            srv.release(tickets[i])
+        st = srv.stats()
+        assert st["completed"] == len(models) and st["steps"] >= 3 and st["resident_bytes"] == 0
