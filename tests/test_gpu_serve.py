@@ -434,3 +434,30 @@ def test_serve_mixed_bin_widths(ctx):
             srv.release(tickets[i])
         st = srv.stats()
         assert st["completed"] == len(models) and st["steps"] >= 3 and st["resident_bytes"] == 0
+
+
+def test_serve_fp64_mode(ctx):
+    """A server configured for FP64 pass 2 (the reference's arithmetic): every request's
+    histograms bit-exact against the model alone and its I(q) within 1e-11 of the FP64 mode's
+    result alone (the same sums over another split of the bins), and within both bounds."""
+    models = _models(6, 53, nqs=(80, 120))
+    ctx.set_pass2_mode(cd.DBX_PASS2_FP64)
+    try:
+        ref = []
+        for m in models:
+            mod = ctx.model(m.x, m.y, m.z, m.group_n)
+            ref.append(ctx.intensity(mod, [_gp(p) for p in m.params], m.rstep, m.n_bins, m.q, first_n=m.first_n,
+                                     with_hist=True, with_err=True))
+            mod.close()
+    finally:
+        ctx.set_pass2_mode(cd.DBX_PASS2_MFMA)
+    with cd.Server(0, pass2_mode=cd.DBX_PASS2_FP64) as srv:
+        ts = [_submit(srv, m) for m in models]
+        for t, (I64, H64, b64) in zip(ts, ref):
+            _finish(srv, t)
+            I, _, rc, bound = srv.wait(t, with_err=True)
+            assert rc == cd.DBX_OK
+            assert np.array_equal(srv.histogram(t), H64)
+            assert _rel(I, I64) < 1e-11
+            assert np.all(np.abs(I - I64) / np.abs(I64) <= bound + b64)
+            srv.release(t)
